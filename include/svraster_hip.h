@@ -1,0 +1,161 @@
+/*
+ * svraster_hip.h — C ABI of the MI355X sparse-voxel rasterizer (libsvraster_hip.so).
+ *
+ * The entry points replace the pybind11 functions of the reference backend
+ * module `new_svraster_cuda._C` (backends/new_cuda/binding.cpp:24-57) for the
+ * rasterize path.  Conventions:
+ *   - every pointer is a device pointer unless its name ends in `_host`;
+ *   - matrices (w2c, c2w) are 16 floats, row-major 4x4, on the device;
+ *   - images are planar float [C, H, W];
+ *   - every call is asynchronous on `stream` unless documented otherwise;
+ *   - the library never allocates device memory: callers size the opaque state
+ *     buffers with the *_size() queries and pass them in;
+ *   - the return value is an svr_status; on failure svr_last_error() holds a
+ *     message for the calling thread.
+ */
+#ifndef SVRASTER_HIP_H_
+#define SVRASTER_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* svr_stream_t; /* == hipStream_t */
+
+typedef enum svr_status {
+    SVR_OK = 0,
+    SVR_ERR_INVALID = 1,     /* bad argument (maps to the reference's Exception / AT_ERROR) */
+    SVR_ERR_HIP = 2,         /* a HIP runtime call or launch failed */
+    SVR_ERR_UNSUPPORTED = 3, /* option not implemented by this build */
+    SVR_ERR_CAPACITY = 4     /* a preallocated buffer is too small */
+} svr_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* svr_last_error(void);
+
+/* Read-only constants exported by the reference module (binding.cpp:51-57; config.h:14-34). */
+typedef struct svr_constants {
+    int32_t max_num_levels;       /* MAX_NUM_LEVELS = 21 */
+    float step_sz_scale;          /* STEP_SZ_SCALE = 100 */
+    int32_t max_n_samp;           /* MAX_N_SAMP = 3 */
+    int32_t max_sort_key_bits;    /* MAX_SORT_KEY_BITS = 128 */
+    int32_t max_packed_image_dim; /* MAX_PACKED_IMAGE_DIM = 65535 */
+    int64_t max_render_tiles;     /* MAX_RENDER_TILES */
+    int32_t tile_x, tile_y;       /* BLOCK_X, BLOCK_Y = 16 */
+} svr_constants;
+int svr_get_constants(svr_constants* out_host);
+
+/* ---- state-buffer sizes (bytes) ---------------------------------------- */
+/* Geometry state of `rasterize_preprocess` (raster_state.h:43-59). */
+size_t svr_geom_state_size(int64_t n_voxels);
+/* Per-octree rank-order table (no reference counterpart: replaces the per-frame
+ * 63-bit order-rank part of the 128-bit sort key, forward.cu:666-730). */
+size_t svr_order_table_size(int64_t n_voxels);
+size_t svr_order_table_scratch_size(int64_t n_voxels);
+/* Binning state of `rasterize_voxels` (raster_state.h:70-82) for R entries. */
+size_t svr_binning_state_size(int64_t num_rendered);
+/* Image state (ranges, tile_last, n_contrib) of an W x H image (raster_state.h:61-68). */
+size_t svr_image_state_size(int32_t width, int32_t height);
+
+/* ---- forward path ------------------------------------------------------- */
+/* Replaces `_C.rasterize_preprocess` (preprocess.cu:136-199): per voxel the
+ * near-sphere cull, the 8-corner projection, the packed pixel bbox, the
+ * camera-quadrant bitset and n_duplicates = tiles x popcount(quadrants).
+ * Writes out_n_duplicates[N] (int32) and the geometry state. */
+int svr_rasterize_preprocess(int32_t width, int32_t height, float tanfovx, float tanfovy, float cx, float cy,
+                             const float* w2c, const float* c2w, float near_plane, int64_t n_voxels,
+                             const float* vox_centers, const float* vox_lengths, int32_t* out_n_duplicates,
+                             void* geom_state, svr_stream_t stream);
+
+/* Inclusive scan of n_duplicates and the number of rendered entries R
+ * (forward.cu:649-663).  Synchronises `stream` to return R to the host, as
+ * the reference does before sizing its binning buffer. */
+int svr_num_rendered(int64_t n_voxels, void* geom_state, svr_stream_t stream, int64_t* num_rendered_host);
+
+/* Builds the rank-order table of an octree: the position of every
+ * (voxel, quadrant) pair in the order of the reference's 63-bit order rank
+ * octpath ^ repeat(q) (auxiliary.h:57-60), ties by (voxel, quadrant).  Valid
+ * until octree_paths changes.  `scratch` has svr_order_table_scratch_size()
+ * bytes.  Synchronises `stream` once (reads the finest used level). */
+int svr_build_order_table(int64_t n_voxels, const int64_t* octree_paths, void* order_table, void* scratch,
+                          svr_stream_t stream);
+
+/* Replaces `_C.rasterize_voxels` (forward.cu:792-879) for n_samp in [1,3]
+ * without depth / normal / distortion / max_w: key duplication, sort, tile
+ * ranges and the front-to-back compositor.  Outputs out_color [3,H,W] and
+ * out_T [1,H,W]; fills the binning and image states used by the backward. */
+int svr_rasterize_forward(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy, float cx,
+                          float cy, const float* c2w, float bg_color, int64_t n_voxels, const float* vox_centers,
+                          const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,
+                          const void* order_table, int64_t num_rendered, void* binning_state, void* image_state,
+                          float* out_color, float* out_T, svr_stream_t stream);
+
+/* Replaces `_C.rasterize_voxels_backward` (backward.cu:654-772): reverse
+ * traversal with T recovered from the stored final T and n_contrib.  Writes
+ * (after zeroing) dL_dgeos [N,8], dL_drgbs [N,3], subdiv_p_bw [N,1]. */
+int svr_rasterize_backward(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy, float cx,
+                           float cy, const float* c2w, float bg_color, int64_t n_voxels, const float* vox_centers,
+                           const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,
+                           int64_t num_rendered, const void* binning_state, const void* image_state,
+                           const float* out_T, const float* dL_dout_color, const float* dL_dout_T, float* dL_dgeos,
+                           float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
+
+/* Replaces `_C.unpack_ImageState` (raster_state.cu:150-205): copies ranges
+ * [T,2], tile_last [T] and n_contrib [H,W] (int32, device) out of the image state. */
+int svr_unpack_image_state(int32_t width, int32_t height, const void* image_state, int32_t* ranges,
+                           int32_t* tile_last, int32_t* n_contrib, svr_stream_t stream);
+
+/* Parity export: the sorted keys in the reference's 128-bit form
+ * hi = tile >> 1, lo = (tile << 63) | (octpath ^ repeat(q)) (auxiliary.h:67-76),
+ * and the sorted values (q << 29) | vox_id. */
+int svr_export_sort_keys(int32_t width, int32_t height, int64_t num_rendered, const void* binning_state,
+                         const void* order_table, const int64_t* octree_paths, int64_t n_voxels, uint64_t* keys_hi,
+                         uint64_t* keys_lo, uint32_t* values, svr_stream_t stream);
+
+/* Parity export of the geometry state: n_duplicates [N] (u32), packed bboxes
+ * [N,2] (x << 16 | y for min and max) and camera-quadrant bitsets [N]
+ * (raster_state.h:43-59; the reference only reads them on the device). */
+int svr_unpack_geom_state(int64_t n_voxels, const void* geom_state, uint32_t* n_dup, uint32_t* bbox,
+                          uint32_t* qbits, svr_stream_t stream);
+
+/* ---- colour ----------------------------------------------------------- */
+/* Replaces `_C.sh_compute` (sh_compute.cu:244-270): rgbs [N,3] = SH(deg) of
+ * the direction cam_pos -> centre, +0.5, clamped at 0; rows outside idx are
+ * zero.  n_idx == 0 evaluates all voxels.  M = 1 + shs.shape[1]. */
+int svr_sh_compute(int32_t deg, int64_t n_idx, const int64_t* idx, int64_t n_voxels, const float* vox_centers,
+                   const float* cam_pos, const float* sh0, const float* shs, int32_t M, float* rgbs,
+                   svr_stream_t stream);
+/* Replaces `_C.sh_compute_bw` (sh_compute.cu:272-299), no clamp mask. */
+int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx, int64_t n_voxels,
+                      const float* vox_centers, const float* cam_pos, const float* dL_drgbs, float* dL_dsh0,
+                      float* dL_dshs, svr_stream_t stream);
+
+/* ---- optimizer -------------------------------------------------------- */
+/* Replaces `_C.unbiased_adam_step` (adam_step.cu:106-137); sparse skips grad == 0. */
+int svr_unbiased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
+                           float* exp_avg_sq, double step, double lr, double beta1, double beta2, float eps,
+                           svr_stream_t stream);
+/* Replaces `_C.biased_adam_step` (adam_step.cu:139-162). */
+int svr_biased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
+                         float* exp_avg_sq, float lr, float beta1, float beta2, float eps, svr_stream_t stream);
+
+/* ---- octree utilities (utils.cu:106-189) ---------------------------- */
+int svr_voxel_order_rank(int64_t n_voxels, const int64_t* octree_paths, int64_t* order_ranks /* [8,N] */,
+                         svr_stream_t stream);
+int svr_ijk_2_octpath(int64_t n, const int64_t* ijk, const int8_t* octlevel, int64_t* octpath,
+                      svr_stream_t stream);
+int svr_octpath_2_ijk(int64_t n, const int64_t* octpath, const int8_t* octlevel, int64_t* ijk,
+                      svr_stream_t stream);
+
+/* Waits for `stream` and reports any pending device error (the reference's
+ * CHECK_CUDA(debug), auxiliary.h:394-401). */
+int svr_debug_sync(svr_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVRASTER_HIP_H_ */

@@ -1,0 +1,638 @@
+// ORACLE — test infrastructure, not product code.
+//
+// A scalar CPU restatement of the sparse-voxel rasterizer hot path of the
+// reference backend (backends/new_cuda).  It exists to check the HIP kernels
+// and to give the CPU baseline timing; only tests/, the smoke check and the
+// benchmark's cpu_baseline leg load it.  Nothing in sv_raster_amd/ links it.
+//
+// Every function cites the reference lines it restates.  Floating-point
+// contract shared with the HIP kernels (both sides compile with
+// -ffp-contract=off, so nothing is fused unless written as fma here):
+//   * explicit fma at: trilinear density sum, sample point, colour
+//     accumulation, backward dot / last_dL_dT update, Adam lerp;
+//   * exp() is svr_ref_expf below (Cody-Waite reduction + degree-7 Taylor),
+//     the same method and constants as the device exp, so that integer
+//     outputs (bboxes, n_contrib, tile_last) are bit-identical;
+//   * divisions and square roots are IEEE correctly rounded on both sides.
+//
+// Templated on the real type: float is the parity path, double is used by
+// the finite-difference gradient checks.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// NC/src/config.h:14-34
+constexpr int BLOCK_X = 16;
+constexpr int BLOCK_Y = 16;
+constexpr int MAX_NUM_LEVELS = 21;
+constexpr float MAX_ALPHA = 0.99999f;
+constexpr float MIN_ALPHA = 0.00001f;
+constexpr float EARLY_STOP_T = 0.0001f;
+constexpr float STEP_SZ_SCALE = 100.f;
+
+// exp(x) by Cody & Waite argument reduction (x = k*ln2 + r, ln2 split in a
+// high part with trailing zero bits and a low correction) followed by the
+// degree-7 Taylor polynomial of e^r on |r| <= ln2/2 in Horner form, and an
+// exact scaling by 2^k assembled from the exponent bits.  Error <= 2 ulp.
+inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+float svr_ref_expf(float x)
+{
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return u2f(0x7f800000u);
+    if (x < -103.97208404541015625f) return 0.f;
+    const float kf = std::rint(x * 1.44269502162933349609375f);
+    float r = std::fma(-kf, 0.693145751953125f, x);
+    r = std::fma(-kf, 1.428606765330187045e-06f, r);
+    float p = 1.98412701976485550403594970703125e-04f;      // 1/5040
+    p = std::fma(p, r, 1.388888922519981861114501953125e-03f); // 1/720
+    p = std::fma(p, r, 8.33333376795053482055664062500e-03f);  // 1/120
+    p = std::fma(p, r, 4.16666679084300994873046875000e-02f);  // 1/24
+    p = std::fma(p, r, 1.66666671633720397949218750000e-01f);  // 1/6
+    p = std::fma(p, r, 0.5f);
+    p = std::fma(p, r, 1.0f);
+    p = std::fma(p, r, 1.0f);
+    int k = (int)kf;
+    // 2^k in two factors so that k in [-150, 128] never leaves the normal range
+    const int k1 = k / 2;
+    const int k2 = k - k1;
+    const float s1 = u2f((uint32_t)(k1 + 127) << 23);
+    const float s2 = u2f((uint32_t)(k2 + 127) << 23);
+    return (p * s1) * s2;
+}
+
+template <typename R> inline R rexp(R x);
+template <> inline float rexp<float>(float x) { return svr_ref_expf(x); }
+template <> inline double rexp<double>(double x) { return std::exp(x); }
+
+template <typename R> inline R rfma(R a, R b, R c) { return std::fma(a, b, c); }
+
+// NC/src/auxiliary.h:104-112: the ternary min/max (NaN-order sensitive)
+template <typename R> inline R smin(R a, R b) { return (a < b) ? a : b; }
+template <typename R> inline R smax(R a, R b) { return (a > b) ? a : b; }
+
+// NC/src/auxiliary.h:33-60: octant order tables (q repeated 21 times)
+inline uint64_t order_table(uint32_t q)
+{
+    uint64_t v = 0;
+    for (int i = 0; i < MAX_NUM_LEVELS; ++i) v = (v << 3) | q;
+    return v;
+}
+
+// NC/src/auxiliary.h:150-170
+inline void tile_rect(uint32_t bx, uint32_t by, int gx, int gy,
+                      int& txmin, int& tymin, int& txmax, int& tymax)
+{
+    const uint32_t xmin = bx >> 16, ymin = bx & 0xFFFFu;
+    const uint32_t xmax = by >> 16, ymax = by & 0xFFFFu;
+    txmin = std::min((int)(xmin / BLOCK_X), gx - 1);
+    tymin = std::min((int)(ymin / BLOCK_Y), gy - 1);
+    txmax = std::min((int)(xmax / BLOCK_X), gx - 1);
+    tymax = std::min((int)(ymax / BLOCK_Y), gy - 1);
+}
+
+// NC/src/auxiliary.h:172-179
+inline bool pix_in_bbox(uint32_t px, uint32_t py, uint32_t bx, uint32_t by)
+{
+    return px >= (bx >> 16) && py >= (bx & 0xFFFFu) && px <= (by >> 16) && py <= (by & 0xFFFFu);
+}
+
+template <typename R>
+struct Ray
+{
+    R ro[3], rd[3], rd_inv[3];
+    uint32_t pq;
+};
+
+// NC/src/forward.cu:102-113 (ray set-up), auxiliary.h:140-148, 294-302
+template <typename R>
+Ray<R> make_ray(int px, int py, int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w)
+{
+    Ray<R> ray;
+    const R pfx = (R)px, pfy = (R)py;
+    const R dx = (pfx + R(0.5f) - cx) * R(2) * tanfovx / (R)W;
+    const R dy = (pfy + R(0.5f) - cy) * R(2) * tanfovy / (R)H;
+    const R dz = R(1);
+    const R nrm = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const R rx = c2w[0] * dx + c2w[1] * dy + c2w[2] * dz;
+    const R ry = c2w[4] * dx + c2w[5] * dy + c2w[6] * dz;
+    const R rz = c2w[8] * dx + c2w[9] * dy + c2w[10] * dz;
+    const R ninv = R(1) / nrm;
+    ray.ro[0] = c2w[3]; ray.ro[1] = c2w[7]; ray.ro[2] = c2w[11];
+    ray.rd[0] = rx * ninv; ray.rd[1] = ry * ninv; ray.rd[2] = rz * ninv;
+    for (int i = 0; i < 3; ++i) ray.rd_inv[i] = R(1) / ray.rd[i];
+    ray.pq = ((ray.rd[0] < 0) << 2) | ((ray.rd[1] < 0) << 1) | (ray.rd[2] < 0);
+    return ray;
+}
+
+// NC/src/auxiliary.h:310-323
+template <typename R>
+inline void ray_aabb(const R* c, R l, const Ray<R>& ray, R& a, R& b)
+{
+    const R r = R(0.5f) * l;
+    R lo[3], hi[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        const R dir = c[i] - ray.ro[i];
+        const R c0 = (dir - r) * ray.rd_inv[i];
+        const R c1 = (dir + r) * ray.rd_inv[i];
+        lo[i] = smin(c0, c1);
+        hi[i] = smax(c0, c1);
+    }
+    a = smax(smax(lo[0], lo[1]), lo[2]);
+    b = smin(smin(hi[0], hi[1]), hi[2]);
+}
+
+// NC/src/auxiliary.h:348-356
+template <typename R> inline R exp_linear_11(R x)
+{
+    return (x > R(1.1f)) ? x : rexp<R>(rfma<R>(R(0.909090909091f), x, R(-0.904689820196f)));
+}
+template <typename R> inline R exp_linear_11_bw(R x)
+{
+    return (x > R(1.1f)) ? R(1) : R(0.909090909091f) * rexp<R>(rfma<R>(R(0.909090909091f), x, R(-0.904689820196f)));
+}
+
+// NC/src/auxiliary.h:378-391 (corner order: bit2 = x, bit1 = y, bit0 = z)
+template <typename R> inline void tri_w(const R* qt, R* w)
+{
+    const R wx[2] = {R(1) - qt[0], qt[0]};
+    const R wy[2] = {R(1) - qt[1], qt[1]};
+    const R wz[2] = {R(1) - qt[2], qt[2]};
+    for (int i = 0; i < 8; ++i) w[i] = (wx[(i >> 2) & 1] * wy[(i >> 1) & 1]) * wz[i & 1];
+}
+
+template <typename R> inline R inv_nsamp(int n)
+{
+    return n == 1 ? R(1) : (n == 2 ? R(0.5f) : R(1.f / 3.f));
+}
+
+// Per-entry volume integral (NC/src/forward.cu:224-253; backward.cu:283-328).
+// Returns vol; fills dI/dgeo if requested.
+template <typename R>
+inline R vox_vol(const R* c, R l, const R* geo, R a, R b, const Ray<R>& ray, int n_samp, R* dI)
+{
+    const R l_inv = R(1) / l;
+    const R step_sz = (b - a) * inv_nsamp<R>(n_samp);
+    const R t0 = rfma<R>(R(0.5f), step_sz, a);
+    const R r = R(0.5f) * l;
+    R qt[3], qs[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        const R pt = rfma<R>(t0, ray.rd[i], ray.ro[i]);
+        qt[i] = (pt - (c[i] - r)) * l_inv;
+        qs[i] = (step_sz * ray.rd[i]) * l_inv;
+    }
+    if (dI) for (int i = 0; i < 8; ++i) dI[i] = R(0);
+    R vol = R(0);
+    const R scale = R(STEP_SZ_SCALE) * step_sz;
+    for (int k = 0; k < n_samp; ++k)
+    {
+        R w[8];
+        tri_w<R>(qt, w);
+        R d = geo[0] * w[0];
+        for (int i = 1; i < 8; ++i) d = rfma<R>(geo[i], w[i], d);
+        vol = vol + scale * exp_linear_11<R>(d);
+        if (dI)
+        {
+            const R dd = scale * exp_linear_11_bw<R>(d);
+            for (int i = 0; i < 8; ++i) dI[i] = rfma<R>(dd, w[i], dI[i]);
+        }
+        for (int i = 0; i < 3; ++i) qt[i] = qt[i] + qs[i];
+    }
+    return vol;
+}
+
+template <typename R>
+void render_fwd(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R bg, int n_samp,
+                const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox,
+                const R* centers, const R* lengths, const R* geos, const R* rgbs,
+                R* out_color, R* out_T, uint32_t* n_contrib, uint32_t* tile_last)
+{
+    const int gx = (W + BLOCK_X - 1) / BLOCK_X, gy = (H + BLOCK_Y - 1) / BLOCK_Y;
+    const int T_tiles = gx * gy;
+    const size_t HW = (size_t)H * W;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < T_tiles; ++t)
+    {
+        const int tx = t % gx, ty = t / gx;
+        const uint32_t rx = ranges[2 * t], ry = ranges[2 * t + 1];
+        uint32_t tl = rx;  // NC/src/forward.cu:126-127, 392
+        for (int py = ty * BLOCK_Y; py < std::min((ty + 1) * BLOCK_Y, H); ++py)
+            for (int px = tx * BLOCK_X; px < std::min((tx + 1) * BLOCK_X, W); ++px)
+            {
+                const Ray<R> ray = make_ray<R>(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+                R T = R(1), C[3] = {R(0), R(0), R(0)};
+                uint32_t last = 0;
+                // NC/src/forward.cu:156-327, flattened over batches
+                for (uint32_t j = rx; j < ry; ++j)
+                {
+                    const uint32_t val = vox_list[j];
+                    const uint32_t v = val & 0x1FFFFFFFu, q = val >> 29;
+                    if (!pix_in_bbox(px, py, bbox[2 * v], bbox[2 * v + 1]) || q != ray.pq) continue;
+                    const R* c = centers + 3 * (size_t)v;
+                    R a, b;
+                    ray_aabb<R>(c, lengths[v], ray, a, b);
+                    if (a > b) continue;
+                    const R vol = vox_vol<R>(c, lengths[v], geos + 8 * (size_t)v, a, b, ray, n_samp, nullptr);
+                    const R alpha = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-vol));
+                    if (alpha < R(MIN_ALPHA)) continue;
+                    const R pt_w = T * alpha;
+                    for (int i = 0; i < 3; ++i) C[i] = rfma<R>(pt_w, rgbs[3 * (size_t)v + i], C[i]);
+                    T = T * (R(1) - alpha);
+                    last = j - rx + 1;
+                    if (T < R(EARLY_STOP_T)) break;
+                }
+                const size_t pid = (size_t)py * W + px;
+                n_contrib[pid] = last;
+                for (int i = 0; i < 3; ++i) out_color[i * HW + pid] = C[i] + T * bg;
+                out_T[pid] = T;
+                tl = std::max(tl, rx + last);
+            }
+        tile_last[t] = tl;
+    }
+}
+
+template <typename R>
+inline void atomic_add(double* p, double v)
+{
+#pragma omp atomic
+    *p += v;
+}
+
+// NC/src/backward.cu:105-527 (no depth / normal / distortion / regularizers)
+template <typename R>
+void render_bwd(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R bg, int n_samp,
+                const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox,
+                const R* centers, const R* lengths, const R* geos, const R* rgbs,
+                const R* out_T, const uint32_t* n_contrib, const uint32_t* tile_last,
+                const R* dL_dcolor, const R* dL_dT, int N, double* acc /* N*12, zeroed */)
+{
+    const int gx = (W + BLOCK_X - 1) / BLOCK_X, gy = (H + BLOCK_Y - 1) / BLOCK_Y;
+    const int T_tiles = gx * gy;
+    const size_t HW = (size_t)H * W;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < T_tiles; ++t)
+    {
+        const int tx = t % gx, ty = t / gx;
+        const uint32_t rx = ranges[2 * t], rend = tile_last[t];
+        for (int py = ty * BLOCK_Y; py < std::min((ty + 1) * BLOCK_Y, H); ++py)
+            for (int px = tx * BLOCK_X; px < std::min((tx + 1) * BLOCK_X, W); ++px)
+            {
+                const Ray<R> ray = make_ray<R>(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+                const size_t pid = (size_t)py * W + px;
+                R T = out_T[pid];
+                const uint32_t nc = n_contrib[pid];
+                const R dpix[3] = {dL_dcolor[pid], dL_dcolor[HW + pid], dL_dcolor[2 * HW + pid]};
+                R last = dL_dT[pid] + bg * (dpix[0] + dpix[1] + dpix[2]);
+                for (uint32_t jj = rend; jj > rx; --jj)
+                {
+                    const uint32_t j = jj - 1;
+                    if (j - rx >= nc) continue;
+                    const uint32_t val = vox_list[j];
+                    const uint32_t v = val & 0x1FFFFFFFu, q = val >> 29;
+                    if (!pix_in_bbox(px, py, bbox[2 * v], bbox[2 * v + 1]) || q != ray.pq) continue;
+                    const R* c = centers + 3 * (size_t)v;
+                    R a, b;
+                    ray_aabb<R>(c, lengths[v], ray, a, b);
+                    if (a > b) continue;
+                    R dI[8];
+                    const R vol = vox_vol<R>(c, lengths[v], geos + 8 * (size_t)v, a, b, ray, n_samp, dI);
+                    const R e = rexp<R>(-vol);
+                    const R alpha = std::min(R(MAX_ALPHA), R(1) - e);
+                    if (alpha < R(MIN_ALPHA)) continue;
+                    T = T / (R(1) - alpha);
+                    const R pt_w = alpha * T;
+                    const R* col = rgbs + 3 * (size_t)v;
+                    const R dpt = rfma<R>(dpix[2], col[2], rfma<R>(dpix[1], col[1], dpix[0] * col[0]));
+                    const R diff = dpt - last;
+                    const R dalpha = T * diff;
+                    last = rfma<R>(alpha, diff, last);
+                    const R dI_scale = dalpha * e;
+                    double* g = acc + 12 * (size_t)v;
+                    for (int i = 0; i < 8; ++i) atomic_add<R>(g + i, (double)(dI_scale * dI[i]));
+                    for (int i = 0; i < 3; ++i) atomic_add<R>(g + 8 + i, (double)(pt_w * dpix[i]));
+                    atomic_add<R>(g + 11, (double)std::fabs(dalpha * alpha));
+                }
+            }
+    }
+}
+
+// NC/src/sh_compute.cu:33-50
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+constexpr float SH_C2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                            -1.0925484305920792f, 0.5462742152960396f};
+constexpr float SH_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                            0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                            -0.5900435899266435f};
+
+// basis values b[0..15] for the direction (deg 0..3); NC/src/sh_compute.cu:55-176
+inline int sh_basis(int deg, float x, float y, float z, float* b)
+{
+    b[0] = SH_C0;
+    if (deg < 1) return 1;
+    b[1] = -SH_C1 * y; b[2] = SH_C1 * z; b[3] = -SH_C1 * x;
+    if (deg < 2) return 4;
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    b[4] = SH_C2[0] * xy;
+    b[5] = SH_C2[1] * yz;
+    b[6] = SH_C2[2] * (2.f * zz - xx - yy);
+    b[7] = SH_C2[3] * xz;
+    b[8] = SH_C2[4] * (xx - yy);
+    if (deg < 3) return 9;
+    b[9] = SH_C3[0] * y * (3.f * xx - yy);
+    b[10] = SH_C3[1] * xy * z;
+    b[11] = SH_C3[2] * y * (4.f * zz - xx - yy);
+    b[12] = SH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
+    b[13] = SH_C3[4] * x * (4.f * zz - xx - yy);
+    b[14] = SH_C3[5] * z * (xx - yy);
+    b[15] = SH_C3[6] * x * (xx - 3.f * yy);
+    return 16;
+}
+
+inline void sh_dir(const float* c, const float* cam, float& x, float& y, float& z)
+{
+    const float dx = c[0] - cam[0], dy = c[1] - cam[1], dz = c[2] - cam[2];
+    const float rn = 1.f / std::sqrt(dx * dx + dy * dy + dz * dz);
+    x = dx * rn; y = dy * rn; z = dz * rn;
+}
+
+}  // namespace
+
+extern "C" {
+
+float oracle_expf(float x) { return svr_ref_expf(x); }
+
+void oracle_expf_array(int n, const float* x, float* y)
+{
+    for (int i = 0; i < n; ++i) y[i] = svr_ref_expf(x[i]);
+}
+
+int oracle_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void oracle_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+// NC/src/preprocess.cu:24-132 (kernel) and :171-174 (focal, tile grid)
+void oracle_preprocess(int W, int H, float tanfovx, float tanfovy, float cx, float cy,
+                       const float* w2c, const float* c2w, float near_, int N,
+                       const float* centers, const float* lengths,
+                       uint32_t* n_dup, uint32_t* bbox, uint32_t* qbits)
+{
+    const int gx = (W + BLOCK_X - 1) / BLOCK_X, gy = (H + BLOCK_Y - 1) / BLOCK_Y;
+    const float focal_x = 0.5f * (float)W / tanfovx;
+    const float focal_y = 0.5f * (float)H / tanfovy;
+    const float ro[3] = {c2w[3], c2w[7], c2w[11]};
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < N; ++v)
+    {
+        n_dup[v] = 0; bbox[2 * v] = 0; bbox[2 * v + 1] = 0; qbits[v] = 0;
+        const float* c = centers + 3 * (size_t)v;
+        const float r = 0.5f * lengths[v];
+        const float rel[3] = {c[0] - ro[0], c[1] - ro[1], c[2] - ro[2]};
+        if (rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2] < near_ * near_) continue;
+        uint32_t bits = 0;
+        float umin = 1e9f, vmin = 1e9f, umax = -1e9f, vmax = -1e9f;
+        for (int i = 0; i < 8; ++i)
+        {
+            const float sx = (float)(((i & 4) >> 2) * 2 - 1);
+            const float sy = (float)(((i & 2) >> 1) * 2 - 1);
+            const float sz = (float)((i & 1) * 2 - 1);
+            const float p[3] = {c[0] + r * sx, c[1] + r * sy, c[2] + r * sz};
+            const float camx = w2c[0] * p[0] + w2c[1] * p[1] + w2c[2] * p[2] + w2c[3];
+            const float camy = w2c[4] * p[0] + w2c[5] * p[1] + w2c[6] * p[2] + w2c[7];
+            const float camz = w2c[8] * p[0] + w2c[9] * p[1] + w2c[10] * p[2] + w2c[11];
+            if (camz < near_) continue;
+            const float iz = 1.0f / camz;
+            const float u = camx * iz, vv = camy * iz;
+            const uint32_t q = ((p[0] < ro[0]) << 2) | ((p[1] < ro[1]) << 1) | (p[2] < ro[2]);
+            umin = smin(umin, u); vmin = smin(vmin, vv);
+            umax = smax(umax, u); vmax = smax(vmax, vv);
+            bits |= 1u << q;
+        }
+        const float cxh = cx - 0.5f, cyh = cy - 0.5f;
+        const float bminx = std::fmax(focal_x * umin + cxh, 0.0f);
+        const float bminy = std::fmax(focal_y * vmin + cyh, 0.0f);
+        const float bmaxx = std::fmin(focal_x * umax + cxh, (float)W);
+        const float bmaxy = std::fmin(focal_y * vmax + cyh, (float)H);
+        if (bminx > bmaxx || bminy > bmaxy) continue;
+        const uint32_t bx = (((uint32_t)std::lrint(bminx)) << 16) | ((uint32_t)std::lrint(bminy));
+        const uint32_t by = (((uint32_t)std::lrint(bmaxx)) << 16) | ((uint32_t)std::lrint(bmaxy));
+        int txmin, tymin, txmax, tymax;
+        tile_rect(bx, by, gx, gy, txmin, tymin, txmax, tymax);
+        const int tiles = (1 + tymax - tymin) * (1 + txmax - txmin);
+        n_dup[v] = (uint32_t)(tiles * __builtin_popcount(bits));
+        bbox[2 * v] = bx; bbox[2 * v + 1] = by;
+        qbits[v] = bits;
+    }
+}
+
+// NC/src/forward.cu:523-577 (duplicateWithKeys), :649-663 (scan, R),
+// :703-708 (stable sort of the 128-bit key), :582-604 + :733 (ranges).
+// Outputs the sorted canonical keys (hi, lo), the sorted values and ranges.
+void oracle_binning(int W, int H, int N, const int64_t* octpaths, const uint32_t* n_dup,
+                    const uint32_t* bbox, const uint32_t* qbits, int64_t R,
+                    uint64_t* keys_hi, uint64_t* keys_lo, uint32_t* vox_list, uint32_t* ranges)
+{
+    const int gx = (W + BLOCK_X - 1) / BLOCK_X, gy = (H + BLOCK_Y - 1) / BLOCK_Y;
+    const int T_tiles = gx * gy;
+    struct E { uint64_t hi, lo; uint32_t val; };
+    std::vector<E> ent;
+    ent.reserve((size_t)R);
+    for (int v = 0; v < N; ++v)
+    {
+        if (n_dup[v] == 0) continue;
+        int txmin, tymin, txmax, tymax;
+        tile_rect(bbox[2 * v], bbox[2 * v + 1], gx, gy, txmin, tymin, txmax, tymax);
+        const uint64_t path = (uint64_t)octpaths[v];
+        for (uint32_t q = 0; q < 8; ++q)
+        {
+            if (!(qbits[v] & (1u << q))) continue;
+            const uint64_t rank = path ^ order_table(q);
+            for (int y = tymin; y <= tymax; ++y)
+                for (int x = txmin; x <= txmax; ++x)
+                {
+                    const uint64_t tile = (uint64_t)((uint32_t)y * (uint32_t)gx + (uint32_t)x);
+                    ent.push_back({tile >> 1, (tile << 63) | rank, (q << 29) | (uint32_t)v});
+                }
+        }
+    }
+    std::stable_sort(ent.begin(), ent.end(), [](const E& a, const E& b) {
+        return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+    });
+    std::memset(ranges, 0, sizeof(uint32_t) * 2 * (size_t)T_tiles);
+    const size_t L = ent.size();
+    for (size_t i = 0; i < L; ++i)
+    {
+        keys_hi[i] = ent[i].hi; keys_lo[i] = ent[i].lo; vox_list[i] = ent[i].val;
+        const uint32_t cur = (uint32_t)((ent[i].hi << 1) | (ent[i].lo >> 63));
+        if (i == 0) ranges[2 * cur] = 0;
+        else
+        {
+            const uint32_t prev = (uint32_t)((ent[i - 1].hi << 1) | (ent[i - 1].lo >> 63));
+            if (cur != prev) { ranges[2 * prev + 1] = (uint32_t)i; ranges[2 * cur] = (uint32_t)i; }
+        }
+        if (i == L - 1) ranges[2 * cur + 1] = (uint32_t)L;
+    }
+}
+
+void oracle_render_fwd_f32(int W, int H, float tanfovx, float tanfovy, float cx, float cy,
+                           const float* c2w, float bg, int n_samp,
+                           const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox,
+                           const float* centers, const float* lengths, const float* geos, const float* rgbs,
+                           float* out_color, float* out_T, uint32_t* n_contrib, uint32_t* tile_last)
+{
+    render_fwd<float>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, n_samp, ranges, vox_list, bbox,
+                      centers, lengths, geos, rgbs, out_color, out_T, n_contrib, tile_last);
+}
+
+void oracle_render_fwd_f64(int W, int H, double tanfovx, double tanfovy, double cx, double cy,
+                           const double* c2w, double bg, int n_samp,
+                           const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox,
+                           const double* centers, const double* lengths, const double* geos, const double* rgbs,
+                           double* out_color, double* out_T, uint32_t* n_contrib, uint32_t* tile_last)
+{
+    render_fwd<double>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, n_samp, ranges, vox_list, bbox,
+                       centers, lengths, geos, rgbs, out_color, out_T, n_contrib, tile_last);
+}
+
+void oracle_render_bwd_f32(int W, int H, float tanfovx, float tanfovy, float cx, float cy,
+                           const float* c2w, float bg, int n_samp,
+                           const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox,
+                           const float* centers, const float* lengths, const float* geos, const float* rgbs,
+                           const float* out_T, const uint32_t* n_contrib, const uint32_t* tile_last,
+                           const float* dL_dcolor, const float* dL_dT, int N, double* acc)
+{
+    render_bwd<float>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, n_samp, ranges, vox_list, bbox,
+                      centers, lengths, geos, rgbs, out_T, n_contrib, tile_last, dL_dcolor, dL_dT, N, acc);
+}
+
+void oracle_render_bwd_f64(int W, int H, double tanfovx, double tanfovy, double cx, double cy,
+                           const double* c2w, double bg, int n_samp,
+                           const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox,
+                           const double* centers, const double* lengths, const double* geos, const double* rgbs,
+                           const double* out_T, const uint32_t* n_contrib, const uint32_t* tile_last,
+                           const double* dL_dcolor, const double* dL_dT, int N, double* acc)
+{
+    render_bwd<double>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, n_samp, ranges, vox_list, bbox,
+                       centers, lengths, geos, rgbs, out_T, n_contrib, tile_last, dL_dcolor, dL_dT, N, acc);
+}
+
+// NC/src/sh_compute.cu:55-106, 180-208.  idx == nullptr (K == 0) means all voxels.
+void oracle_sh_fwd(int deg, int K, const int64_t* idx, int N, const float* centers, const float* cam,
+                   const float* sh0, const float* shs, int M, float* rgbs)
+{
+    std::memset(rgbs, 0, sizeof(float) * 3 * (size_t)N);
+    const int cnt = K ? K : N;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < cnt; ++t)
+    {
+        const int64_t v = K ? idx[t] : t;
+        float x, y, z, b[16];
+        sh_dir(centers + 3 * v, cam, x, y, z);
+        const int nb = sh_basis(deg, x, y, z, b);
+        const float* s0 = sh0 + 3 * v;
+        const float* s = shs + (size_t)v * (M - 1) * 3;
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            // NC/src/sh_compute.cu:66-99: result accumulated term by term
+            float res = SH_C0 * s0[ch];
+            if (nb > 1)
+            {
+                res = res - (SH_C1 * y) * s[0 * 3 + ch] + (SH_C1 * z) * s[1 * 3 + ch] - (SH_C1 * x) * s[2 * 3 + ch];
+            }
+            for (int k = 4; k < nb; ++k) res = res + b[k] * s[(k - 1) * 3 + ch];
+            res = res + 0.5f;
+            rgbs[3 * v + ch] = res > 0.f ? res : 0.f;
+        }
+    }
+}
+
+// NC/src/sh_compute.cu:110-176, 212-240 (no clamp mask, :118-121)
+void oracle_sh_bwd(int deg, int K, const int64_t* idx, int N, const float* centers, const float* cam,
+                   const float* dL_drgbs, int M, float* dL_dsh0, float* dL_dshs)
+{
+    std::memset(dL_dsh0, 0, sizeof(float) * 3 * (size_t)N);
+    std::memset(dL_dshs, 0, sizeof(float) * 3 * (size_t)N * (M - 1));
+    const int cnt = K ? K : N;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < cnt; ++t)
+    {
+        const int64_t v = K ? idx[t] : t;
+        float x, y, z, b[16];
+        sh_dir(centers + 3 * v, cam, x, y, z);
+        const int nb = sh_basis(deg, x, y, z, b);
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            const float g = dL_drgbs[3 * v + ch];
+            dL_dsh0[3 * v + ch] = SH_C0 * g;
+            for (int k = 1; k < nb; ++k) dL_dshs[((size_t)v * (M - 1) + (k - 1)) * 3 + ch] = b[k] * g;
+        }
+    }
+}
+
+// NC/src/adam_step.cu:21-64, 106-137 (unbiased); :67-102, 139-162 (biased)
+void oracle_adam_step(int biased, int sparse, int64_t numel, float* p, const float* g, float* m, float* v,
+                      double step, double lr, double beta1, double beta2, float eps)
+{
+    float step_size, rb2;
+    if (!biased)
+    {
+        const double bc1 = 1.0 - std::pow(beta1, step);
+        const double bc2 = 1.0 - std::pow(beta2, step);
+        step_size = (float)(lr / bc1);
+        rb2 = (float)(1.0 / std::sqrt(bc2));
+    }
+    const float b1 = (float)beta1, b2 = (float)beta2, lrf = (float)lr;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < numel; ++i)
+    {
+        const float gv = g[i];
+        if (sparse && gv == 0.0f) continue;
+        const float mv = std::fma(b1, m[i], std::fma(-b1, gv, gv));
+        const float vv = std::fma(b2, v[i], std::fma(-b2, gv * gv, gv * gv));
+        if (!biased)
+        {
+            const float denom = std::fma(std::sqrt(vv), rb2, eps);
+            p[i] -= step_size * (mv / denom);
+        }
+        else
+        {
+            const float denom = std::sqrt(vv) + eps;
+            p[i] -= lrf * (mv / denom);
+        }
+        m[i] = mv; v[i] = vv;
+    }
+}
+
+// NC/src/utils.cu:106-127
+void oracle_voxel_order_rank(int N, const int64_t* octpaths, int64_t* ranks /* 8*N */)
+{
+    for (int q = 0; q < 8; ++q)
+        for (int v = 0; v < N; ++v) ranks[(size_t)q * N + v] = (int64_t)((uint64_t)octpaths[v] ^ order_table(q));
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+"""Tensor-level adapter over the C ABI, with the function names, argument order
+and return tuples of the reference extension module `new_svraster_cuda._C`
+(backends/new_cuda/binding.cpp:24-57), so that renderer.py / sparse_adam.py /
+utils.py read like their reference counterparts.
+
+torch is used only for device memory and the current stream; every compute
+step runs in libsvraster_hip.so.
+"""
+import ctypes
+import weakref
+
+import torch
+
+from . import _lib
+
+_L = _lib.load()
+
+_consts = _lib.SvrConstants()
+_lib.check(_L.svr_get_constants(ctypes.byref(_consts)), "svr_get_constants")
+MAX_NUM_LEVELS = int(_consts.max_num_levels)
+STEP_SZ_SCALE = float(_consts.step_sz_scale)
+MAX_N_SAMP = int(_consts.max_n_samp)
+MAX_SORT_KEY_BITS = int(_consts.max_sort_key_bits)
+MAX_PACKED_IMAGE_DIM = int(_consts.max_packed_image_dim)
+MAX_RENDER_TILES = int(_consts.max_render_tiles)
+TILE = int(_consts.tile_x)
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def _f32(t):
+    return t.contiguous() if t.dtype == torch.float32 else t.float().contiguous()
+
+
+def _f32_a16(t):
+    t = _f32(t)
+    return t if t.data_ptr() % 16 == 0 else t.clone()
+
+
+def _require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and torch.is_tensor(t) and t.numel() > 0 and not t.is_cuda:
+            raise RuntimeError("sv_raster_amd needs its tensors on a HIP device (found a CPU tensor)")
+
+
+def _debug(debug, device, what):
+    if debug:
+        _lib.check(_L.svr_debug_sync(_stream(device)), what)
+
+
+def tile_grid(W, H):
+    return (W + TILE - 1) // TILE, (H + TILE - 1) // TILE
+
+
+# ---------------------------------------------------------------------------
+# per-octree rank-order table cache
+
+class _OrderTableCache:
+    """Caches the rank-order table of the last few octrees, keyed by the
+    octree_paths tensor (identity + in-place version counter + storage)."""
+
+    def __init__(self, capacity=4):
+        self.capacity = capacity
+        self.entries = []
+
+    def clear(self):
+        self.entries = []
+
+    def get(self, octree_paths):
+        for ref, ver, ptr, n, table in self.entries:
+            if ref() is octree_paths and ver == octree_paths._version and ptr == octree_paths.data_ptr() \
+                    and n == octree_paths.numel():
+                return table
+        table = build_order_table(octree_paths)
+        self.entries.insert(0, (weakref.ref(octree_paths), octree_paths._version, octree_paths.data_ptr(),
+                                octree_paths.numel(), table))
+        del self.entries[self.capacity:]
+        return table
+
+
+order_table_cache = _OrderTableCache()
+
+
+def build_order_table(octree_paths):
+    paths = octree_paths.reshape(-1).contiguous()
+    if paths.dtype != torch.int64:
+        raise RuntimeError("octree_paths must be int64")
+    N = paths.numel()
+    dev = paths.device
+    table = torch.empty(int(_L.svr_order_table_size(N)), dtype=torch.uint8, device=dev)
+    if N == 0:
+        return table
+    scratch = torch.empty(int(_L.svr_order_table_scratch_size(N)), dtype=torch.uint8, device=dev)
+    _lib.check(_L.svr_build_order_table(N, _ptr(paths), _ptr(table), _ptr(scratch), _stream(dev)),
+               "build_order_table")
+    del scratch
+    return table
+
+
+# ---------------------------------------------------------------------------
+# binding.cpp:25 -> preprocess.cu:136-199
+
+def rasterize_preprocess(image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix, c2w_matrix, near,
+                         octree_paths, vox_centers, vox_lengths, debug=False):
+    if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
+        raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
+    _require_cuda(w2c_matrix, c2w_matrix, vox_centers, vox_lengths)
+    dev = vox_centers.device
+    N = vox_centers.size(0)
+    out_n_dup = torch.zeros(N, dtype=torch.int32, device=dev)
+    geom = torch.empty(int(_L.svr_geom_state_size(N)) if N > 0 else 0, dtype=torch.uint8, device=dev)
+    if N == 0:
+        return out_n_dup, geom
+    centers, lengths = _f32(vox_centers), _f32(vox_lengths)
+    w2c, c2w = _f32(w2c_matrix), _f32(c2w_matrix)
+    _lib.check(_L.svr_rasterize_preprocess(int(image_width), int(image_height), float(tan_fovx), float(tan_fovy),
+                                           float(cx), float(cy), _ptr(w2c), _ptr(c2w), float(near), N,
+                                           _ptr(centers), _ptr(lengths), _ptr(out_n_dup), _ptr(geom),
+                                           _stream(dev)), "rasterize_preprocess")
+    _debug(debug, dev, "rasterize_preprocess")
+    return out_n_dup, geom
+
+
+def _unsupported(need_depth, need_distortion, need_normal, track_max_w):
+    flags = [n for n, f in (("need_depth", need_depth), ("need_distortion", need_distortion),
+                            ("need_normal", need_normal), ("track_max_w", track_max_w)) if f]
+    if flags:
+        raise NotImplementedError(f"sv_raster_amd does not implement {', '.join(flags)} yet")
+
+
+# binding.cpp:26 -> forward.cu:792-879
+def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix, c2w_matrix,
+                     bg_color, need_depth, need_distortion, need_normal, track_max_w, octree_paths, vox_centers,
+                     vox_lengths, geos, rgbs, geomBuffer, debug=False):
+    if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
+        raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
+    if rgbs.ndimension() != 2 or rgbs.size(1) != 3:
+        raise RuntimeError("rgbs should be either (num_points, 3)")
+    if vox_centers.size(0) != rgbs.size(0):
+        raise RuntimeError("size mismatch")
+    _unsupported(need_depth, need_distortion, need_normal, track_max_w)
+    _require_cuda(octree_paths, vox_centers, vox_lengths, geos, rgbs, c2w_matrix)
+    dev = vox_centers.device
+    N = vox_centers.size(0)
+    W, H = int(image_width), int(image_height)
+    out_color = torch.empty((3, H, W), dtype=torch.float32, device=dev)
+    out_T = torch.empty((1, H, W), dtype=torch.float32, device=dev)
+    empty = torch.empty(0, dtype=torch.float32, device=dev)
+    img = torch.empty(int(_L.svr_image_state_size(W, H)), dtype=torch.uint8, device=dev)
+    R = ctypes.c_int64(0)
+    if N > 0:
+        _lib.check(_L.svr_num_rendered(N, _ptr(geomBuffer), _stream(dev), ctypes.byref(R)), "num_rendered")
+    R = int(R.value)
+    binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
+    table = order_table_cache.get(octree_paths) if R > 0 else None
+    centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
+    c2w = _f32(c2w_matrix)
+    _lib.check(_L.svr_rasterize_forward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
+                                        float(cy), _ptr(c2w), float(bg_color), N, _ptr(centers), _ptr(lengths),
+                                        _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), _ptr(table), R,
+                                        _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), _stream(dev)),
+               "rasterize_voxels")
+    _debug(debug, dev, "rasterize_voxels")
+    return R, binning, img, out_color, empty, empty, out_T, empty
+
+
+# binding.cpp:27 -> backward.cu:654-772
+def rasterize_voxels_backward(R, n_samp_per_vox, image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix,
+                              c2w_matrix, bg_color, octree_paths, vox_centers, vox_lengths, geos, rgbs, geomBuffer,
+                              binningBuffer, imageBuffer, out_T, dL_dout_color, dL_dout_depth, dL_dout_normal,
+                              dL_dout_T, lambda_R_concen, gt_color, lambda_ascending, lambda_dist, need_depth,
+                              need_normal, out_D, out_N, debug=False):
+    if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
+        raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
+    if lambda_R_concen > 0 or lambda_ascending > 0 or lambda_dist > 0:
+        raise NotImplementedError("sv_raster_amd does not implement the R_concen / ascending / distortion "
+                                  "regularizer gradients yet")
+    _unsupported(need_depth, False, need_normal, False)
+    dev = vox_centers.device
+    N = vox_centers.size(0)
+    if N == 0:
+        e = torch.empty(0, device=dev)
+        return e, e, e
+    W, H = int(image_width), int(image_height)
+    dgeos = torch.empty((N, geos.size(1)), dtype=torch.float32, device=dev)
+    drgbs = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    dsub = torch.empty((N, 1), dtype=torch.float32, device=dev)
+    centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
+    dcol = _f32(dL_dout_color)
+    dT = _f32(dL_dout_T) if dL_dout_T is not None else torch.zeros((1, H, W), device=dev)
+    _lib.check(_L.svr_rasterize_backward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
+                                         float(cy), _ptr(_f32(c2w_matrix)), float(bg_color), N, _ptr(centers),
+                                         _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), int(R),
+                                         _ptr(binningBuffer), _ptr(imageBuffer), _ptr(_f32(out_T)), _ptr(dcol),
+                                         _ptr(dT), _ptr(dgeos), _ptr(drgbs), _ptr(dsub), _stream(dev)),
+               "rasterize_voxels_backward")
+    _debug(debug, dev, "rasterize_voxels_backward")
+    return dgeos, drgbs, dsub
+
+
+# binding.cpp:29 -> raster_state.cu:185-205
+def unpack_ImageState(image_width, image_height, imageBuffer):
+    W, H = int(image_width), int(image_height)
+    gx, gy = tile_grid(W, H)
+    dev = imageBuffer.device
+    ranges = torch.zeros((gy, gx, 2), dtype=torch.int32, device=dev)
+    tile_last = torch.zeros((gy, gx), dtype=torch.int32, device=dev)
+    n_contrib = torch.zeros((H, W), dtype=torch.int32, device=dev)
+    _lib.check(_L.svr_unpack_image_state(W, H, _ptr(imageBuffer), _ptr(ranges), _ptr(tile_last), _ptr(n_contrib),
+                                         _stream(dev)), "unpack_ImageState")
+    return ranges, tile_last, n_contrib
+
+
+def unpack_geom_state(N, geomBuffer):
+    """(n_duplicates, bbox [N,2], quadrant bitsets) of a preprocess call, as int32 bit patterns."""
+    dev = geomBuffer.device
+    n_dup = torch.zeros(N, dtype=torch.int32, device=dev)
+    bbox = torch.zeros((N, 2), dtype=torch.int32, device=dev)
+    qbits = torch.zeros(N, dtype=torch.int32, device=dev)
+    if N > 0:
+        _lib.check(_L.svr_unpack_geom_state(int(N), _ptr(geomBuffer), _ptr(n_dup), _ptr(bbox), _ptr(qbits),
+                                            _stream(dev)), "unpack_geom_state")
+    return n_dup, bbox, qbits
+
+
+def export_sort_keys(image_width, image_height, R, binningBuffer, octree_paths):
+    """Sorted keys in the reference's 128-bit form (hi, lo as int64 bit patterns) and values."""
+    dev = binningBuffer.device
+    hi = torch.zeros(R, dtype=torch.int64, device=dev)
+    lo = torch.zeros(R, dtype=torch.int64, device=dev)
+    vals = torch.zeros(R, dtype=torch.int32, device=dev)
+    paths = octree_paths.reshape(-1).contiguous()
+    if R > 0:
+        _lib.check(_L.svr_export_sort_keys(int(image_width), int(image_height), int(R), _ptr(binningBuffer), None,
+                                           _ptr(paths), paths.numel(), _ptr(hi), _ptr(lo), _ptr(vals),
+                                           _stream(dev)), "export_sort_keys")
+    return hi, lo, vals
+
+
+# binding.cpp:36-37 -> sh_compute.cu:244-299
+def sh_compute(D, indices, vox_centers, cam_pos, sh0, shs):
+    _require_cuda(vox_centers, cam_pos, sh0, shs)
+    dev = vox_centers.device
+    N = vox_centers.size(0)
+    M = 1 + shs.size(1)
+    rgbs = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    idx = indices.to(device=dev, dtype=torch.int64).contiguous() if indices is not None else None
+    n_idx = idx.numel() if idx is not None else 0
+    _lib.check(_L.svr_sh_compute(int(D), n_idx, _ptr(idx), N, _ptr(_f32(vox_centers)), _ptr(_f32(cam_pos)),
+                                 _ptr(_f32(sh0)), _ptr(_f32(shs)), M, _ptr(rgbs), _stream(dev)), "sh_compute")
+    return rgbs
+
+
+def sh_compute_bw(D, M, indices, vox_centers, cam_pos, rgbs, dL_drgbs):
+    dev = vox_centers.device
+    N = vox_centers.size(0)
+    dsh0 = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    dshs = torch.empty((N, M - 1, 3), dtype=torch.float32, device=dev)
+    idx = indices.to(device=dev, dtype=torch.int64).contiguous() if indices is not None else None
+    n_idx = idx.numel() if idx is not None else 0
+    _lib.check(_L.svr_sh_compute_bw(int(D), int(M), n_idx, _ptr(idx), N, _ptr(_f32(vox_centers)),
+                                    _ptr(_f32(cam_pos)), _ptr(_f32(dL_drgbs)), _ptr(dsh0), _ptr(dshs),
+                                    _stream(dev)), "sh_compute_bw")
+    return dsh0, dshs
+
+
+def _check_adam(param, grad, exp_avg, exp_avg_sq):
+    for t in (param, grad, exp_avg, exp_avg_sq):
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise RuntimeError("adam step expects contiguous float32 tensors")
+    _require_cuda(param, grad, exp_avg, exp_avg_sq)
+
+
+# binding.cpp:48-49 -> adam_step.cu:106-162
+def unbiased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, step, lr, beta1, beta2, eps):
+    _check_adam(param, grad, exp_avg, exp_avg_sq)
+    _lib.check(_L.svr_unbiased_adam_step(int(bool(sparse)), param.numel(), _ptr(param), _ptr(grad), _ptr(exp_avg),
+                                         _ptr(exp_avg_sq), float(step), float(lr), float(beta1), float(beta2),
+                                         float(eps), _stream(param.device)), "unbiased_adam_step")
+
+
+def biased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, lr, beta1, beta2, eps):
+    _check_adam(param, grad, exp_avg, exp_avg_sq)
+    _lib.check(_L.svr_biased_adam_step(int(bool(sparse)), param.numel(), _ptr(param), _ptr(grad), _ptr(exp_avg),
+                                       _ptr(exp_avg_sq), float(lr), float(beta1), float(beta2), float(eps),
+                                       _stream(param.device)), "biased_adam_step")
+
+
+# binding.cpp:44-46 -> utils.cu:251-298
+def voxel_order_rank(octree_paths):
+    paths = octree_paths.reshape(-1).contiguous()
+    out = torch.empty((8, paths.numel()), dtype=torch.int64, device=paths.device)
+    _lib.check(_L.svr_voxel_order_rank(paths.numel(), _ptr(paths), _ptr(out), _stream(paths.device)),
+               "voxel_order_rank")
+    return out
+
+
+def ijk_2_octpath(ijk, octlevel):
+    ijk = ijk.contiguous()
+    lv = octlevel.reshape(-1).contiguous()
+    out = torch.empty((lv.numel(), 1), dtype=torch.int64, device=ijk.device)
+    _lib.check(_L.svr_ijk_2_octpath(lv.numel(), _ptr(ijk), _ptr(lv), _ptr(out), _stream(ijk.device)),
+               "ijk_2_octpath")
+    return out
+
+
+def octpath_2_ijk(octpath, octlevel):
+    paths = octpath.reshape(-1).contiguous()
+    lv = octlevel.reshape(-1).contiguous()
+    out = torch.empty((lv.numel(), 3), dtype=torch.int64, device=paths.device)
+    _lib.check(_L.svr_octpath_2_ijk(lv.numel(), _ptr(paths), _ptr(lv), _ptr(out), _stream(paths.device)),
+               "octpath_2_ijk")
+    return out

@@ -1,0 +1,86 @@
+"""ctypes binding of libsvraster_hip.so (declarations: include/svraster_hip.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvraster_hip.so")
+
+c_i32, c_i64, c_f32, c_f64, c_p, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double,
+                                          ctypes.c_void_p, ctypes.c_size_t)
+
+
+class SvrConstants(ctypes.Structure):
+    _fields_ = [("max_num_levels", ctypes.c_int32), ("step_sz_scale", ctypes.c_float),
+                ("max_n_samp", ctypes.c_int32), ("max_sort_key_bits", ctypes.c_int32),
+                ("max_packed_image_dim", ctypes.c_int32), ("max_render_tiles", ctypes.c_int64),
+                ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "svr_last_error": (ctypes.c_char_p, []),
+    "svr_get_constants": (c_i32, [ctypes.POINTER(SvrConstants)]),
+    "svr_geom_state_size": (c_sz, [c_i64]),
+    "svr_order_table_size": (c_sz, [c_i64]),
+    "svr_order_table_scratch_size": (c_sz, [c_i64]),
+    "svr_binning_state_size": (c_sz, [c_i64]),
+    "svr_image_state_size": (c_sz, [c_i32, c_i32]),
+    "svr_rasterize_preprocess": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_p, c_f32, c_i64,
+                                         c_p, c_p, c_p, c_p, c_p]),
+    "svr_num_rendered": (c_i32, [c_i64, c_p, c_p, ctypes.POINTER(c_i64)]),
+    "svr_build_order_table": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
+    "svr_rasterize_forward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
+                                      c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_rasterize_backward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
+                                       c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                       c_p]),
+    "svr_unpack_image_state": (c_i32, [c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
+    "svr_unpack_geom_state": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_export_sort_keys": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p]),
+    "svr_sh_compute": (c_i32, [c_i32, c_i64, c_p, c_i64, c_p, c_p, c_p, c_p, c_i32, c_p, c_p]),
+    "svr_sh_compute_bw": (c_i32, [c_i32, c_i32, c_i64, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_unbiased_adam_step": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_f64, c_f64, c_f64, c_f64, c_f32,
+                                       c_p]),
+    "svr_biased_adam_step": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_f32, c_f32, c_f32, c_f32, c_p]),
+    "svr_voxel_order_rank": (c_i32, [c_i64, c_p, c_p, c_p]),
+    "svr_ijk_2_octpath": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
+    "svr_octpath_2_ijk": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
+    "svr_debug_sync": (c_i32, [c_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python sv_raster_amd/build.py` "
+            "(hipcc --offload-arch=gfx950); sv_raster_amd has no CPU fallback.")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = load().svr_last_error().decode(errors="replace")
+        if status == 1:
+            raise Exception(f"{what}: {msg}" if what else msg)
+        if status == 3:
+            raise NotImplementedError(f"{what}: {msg}")
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")

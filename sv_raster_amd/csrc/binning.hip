@@ -1,0 +1,492 @@
+// Binning: per-octree rank-order table, key duplication, LSD radix sort,
+// tile ranges and the sorted entry list.
+//
+// Reference: backends/new_cuda/src/forward.cu:507-604 (duplicateWithKeys,
+// identifyTileRanges), :666-753 (128-bit CUB sort + ranges),
+// auxiliary.h:33-127 (order rank, key and value encodings).
+//
+// Design: the reference sorts 76..77-bit keys (tile << 63 | order_rank).  The
+// order rank octpath ^ repeat(q) depends only on the octree, so its order is
+// computed once per octree as a table pos[v*8+q] (the rank position of the
+// pair, ties by (v, q) exactly as the reference's stable sort orders them).
+// Per frame the key is (tile << pos_bits) | pos, which sorts into exactly the
+// reference order with 30..40 significant bits instead of 76.
+#include "svr_device.h"
+#include "svr_host.h"
+
+namespace svr {
+
+constexpr uint32_t BIG_DUP = 32;  // voxels with more entries are expanded by a whole workgroup
+
+__device__ inline uint64_t match_digit(uint32_t d, uint64_t active)
+{
+    uint64_t m = active;
+#pragma unroll
+    for (int b = 0; b < RADIX_BITS; ++b)
+    {
+        const uint32_t bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void radix_upsweep(const uint64_t* __restrict__ keys, int64_t n,
+                                                              int shift, uint32_t dmask, uint32_t* __restrict__ counts,
+                                                              int64_t nb)
+{
+    __shared__ uint32_t hist[SORT_THREADS / 64][RADIX];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int i = 0; i < SORT_THREADS / 64; ++i) hist[i][t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)w * (SORT_ITEMS * 64);
+    for (int r = 0; r < SORT_ITEMS; ++r)
+    {
+        const int64_t idx = base + r * 64 + lane;
+        const bool valid = idx < n;
+        const uint64_t active = __ballot(valid);
+        if (active == 0) break;
+        const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & dmask : 0u;
+        const uint64_t m = match_digit(d, active);
+        if (valid && lane == 63 - __clzll(m)) hist[w][d] += (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < SORT_THREADS / 64; ++i) c += hist[i][t];
+    counts[(int64_t)t * nb + blockIdx.x] = c;
+}
+
+// Exclusive scan of each digit row of counts[RADIX][nb]; row totals -> counts[RADIX*nb + d].
+__global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* __restrict__ counts, int64_t nb)
+{
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t carry_s;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t* row = counts + (int64_t)blockIdx.x * nb;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t off = 0; off < nb; off += 256)
+    {
+        const int64_t i = off + t;
+        const uint32_t x = i < nb ? row[i] : 0u;
+        uint32_t s = x;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane == 63) lds[w] = s;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int k = 0; k < w; ++k) pre += lds[k];
+        const uint32_t tot = lds[0] + lds[1] + lds[2] + lds[3];
+        const uint32_t carry = carry_s;
+        if (i < nb) row[i] = carry + pre + s - x;
+        __syncthreads();
+        if (t == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (t == 0) counts[(int64_t)RADIX * nb + blockIdx.x] = carry_s;
+}
+
+template <bool HAS_VALS>
+__global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
+    const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ vals_out, int64_t n, int shift, uint32_t dmask, const uint32_t* __restrict__ counts,
+    int64_t nb)
+{
+    constexpr int NW = SORT_THREADS / 64;
+    __shared__ uint32_t wcnt[NW][RADIX];
+    __shared__ uint32_t gbase[RADIX];
+    __shared__ uint32_t wsum[NW];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+
+    // digit base = exclusive prefix over digit totals + this block's row offset
+    {
+        const uint32_t tot = counts[(int64_t)RADIX * nb + t];
+        uint32_t s = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane == 63) wsum[w] = s;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int k = 0; k < w; ++k) pre += wsum[k];
+        gbase[t] = pre + s - tot + counts[(int64_t)t * nb + blockIdx.x];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) wcnt[i][t] = 0;
+    }
+    __syncthreads();
+
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)w * (SORT_ITEMS * 64);
+    uint64_t k[SORT_ITEMS];
+    uint32_t v[SORT_ITEMS];
+    uint32_t rk[SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r)
+    {
+        const int64_t idx = base + r * 64 + lane;
+        const bool valid = idx < n;
+        k[r] = valid ? keys_in[idx] : 0ull;
+        if (HAS_VALS) v[r] = valid ? vals_in[idx] : 0u;
+        const uint64_t active = __ballot(valid);
+        const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
+        const uint64_t m = match_digit(d, active);
+        const uint32_t before = (uint32_t)__popcll(m & lanemask_lt(lane));
+        uint32_t cnt = 0;
+        if (valid) cnt = wcnt[w][d];
+        rk[r] = cnt + before;
+        if (valid && lane == 63 - __clzll(m)) wcnt[w][d] = cnt + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    {
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i)
+        {
+            const uint32_t c = wcnt[i][t];
+            wcnt[i][t] = s;
+            s += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r)
+    {
+        const int64_t idx = base + r * 64 + lane;
+        if (idx < n)
+        {
+            const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
+            const uint32_t dst = gbase[d] + wcnt[w][d] + rk[r];
+            keys_out[dst] = k[r];
+            if (HAS_VALS) vals_out[dst] = v[r];
+        }
+    }
+}
+
+// Stable LSD radix sort over bits [0, nbits) in 8-bit digits.  The sorted
+// data ends in the buffers returned through out_keys / out_vals.
+int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
+                  uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals)
+{
+    *out_keys = keys;
+    if (out_vals) *out_vals = vals;
+    if (n <= 1 || nbits <= 0) return SVR_OK;
+    const int64_t nb = sort_blocks(n);
+    SVR_REQUIRE(nb < (1ll << 31), SVR_ERR_INVALID, "radix sort of %lld keys too large", (long long)n);
+    uint64_t* kin = keys; uint64_t* kout = keys_alt;
+    uint32_t* vin = vals; uint32_t* vout = vals_alt;
+    for (int shift = 0; shift < nbits; shift += RADIX_BITS)
+    {
+        const int bits = (nbits - shift) < RADIX_BITS ? (nbits - shift) : RADIX_BITS;
+        const uint32_t dmask = (1u << bits) - 1u;
+        radix_upsweep<<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, n, shift, dmask, counts, nb);
+        SVR_CHECK_LAUNCH();
+        radix_scan_rows<<<RADIX, 256, 0, stream>>>(counts, nb);
+        SVR_CHECK_LAUNCH();
+        if (vals)
+            radix_downsweep<true><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, vin, vout, n, shift, dmask,
+                                                                             counts, nb);
+        else
+            radix_downsweep<false><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, nullptr, nullptr, n, shift,
+                                                                              dmask, counts, nb);
+        SVR_CHECK_LAUNCH();
+        uint64_t* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+    }
+    *out_keys = kin;
+    if (out_vals) *out_vals = vin;
+    return SVR_OK;
+}
+
+// ---- order table -----------------------------------------------------
+
+__global__ void or_reduce_kernel(const int64_t* __restrict__ paths, int64_t n, uint32_t* __restrict__ out2)
+{
+    uint64_t acc = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) acc |= (uint64_t)paths[i];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+    {
+        const uint32_t lo = __shfl_xor((uint32_t)acc, o, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(acc >> 32), o, 64);
+        acc |= ((uint64_t)hi << 32) | lo;
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicOr(out2 + 0, (uint32_t)acc);
+        atomicOr(out2 + 1, (uint32_t)(acc >> 32));
+    }
+}
+
+__global__ void order_keys_kernel(const int64_t* __restrict__ paths, int64_t n8, int lshift, int with_q,
+                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n8) return;
+    const uint32_t q = (uint32_t)(i & 7);
+    const uint64_t rank = (uint64_t)paths[i >> 3] ^ order_table(q);
+    keys[i] = with_q ? (((rank >> lshift) << 3) | q) : rank;
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void order_scatter_kernel(const uint32_t* __restrict__ sorted_pairs, int64_t n8, uint32_t* __restrict__ pos,
+                                     uint32_t* __restrict__ pair)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n8) return;
+    const uint32_t pr = sorted_pairs[p];
+    pos[pr] = (uint32_t)p;
+    pair[p] = pr;
+}
+
+struct OrderScratch
+{
+    uint64_t *keys, *keys_alt;
+    uint32_t *vals, *vals_alt, *counts, *misc;
+    static OrderScratch carve(void* p, int64_t N, size_t* bytes)
+    {
+        Carver c(p);
+        OrderScratch s;
+        const int64_t n8 = 8 * N;
+        s.keys = c.take<uint64_t>(n8);
+        s.keys_alt = c.take<uint64_t>(n8);
+        s.vals = c.take<uint32_t>(n8);
+        s.vals_alt = c.take<uint32_t>(n8);
+        s.counts = c.take<uint32_t>(RADIX * sort_blocks(n8 > 0 ? n8 : 1) + RADIX);
+        s.misc = c.take<uint32_t>(4);
+        if (bytes) *bytes = c.size();
+        return s;
+    }
+};
+
+// ---- per-frame duplication, ranges and key export --------------------
+
+__device__ inline void rect_of(uint2 bb, int gx, int gy, int& txmin, int& tymin, int& txmax, int& tymax)
+{
+    const uint32_t xmin = bb.x >> 16, ymin = bb.x & 0xFFFFu, xmax = bb.y >> 16, ymax = bb.y & 0xFFFFu;
+    txmin = min((int)(xmin / TILE_X), gx - 1);
+    tymin = min((int)(ymin / TILE_Y), gy - 1);
+    txmax = min((int)(xmax / TILE_X), gx - 1);
+    tymax = min((int)(ymax / TILE_Y), gy - 1);
+}
+
+// forward.cu:523-577, emitting (tile << pos_bits | pos) keys.
+__global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int gy, int pos_bits,
+                                                        const uint32_t* __restrict__ n_dup,
+                                                        const uint32_t* __restrict__ scan,
+                                                        const uint2* __restrict__ bboxes,
+                                                        const uint32_t* __restrict__ qbits,
+                                                        const uint32_t* __restrict__ order_pos,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ big_list,
+                                                        uint32_t* __restrict__ big_count)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const uint32_t nd = n_dup[v];
+    if (nd == 0) return;
+    if (nd > BIG_DUP)
+    {
+        big_list[atomicAdd(big_count, 1u)] = (uint32_t)v;
+        return;
+    }
+    uint32_t off = scan[v] - nd;
+    int txmin, tymin, txmax, tymax;
+    rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
+    const uint32_t bits = qbits[v];
+    for (uint32_t q = 0; q < 8; ++q)
+    {
+        if (!(bits & (1u << q))) continue;
+        const uint64_t pos = order_pos[8 * v + q];
+        for (int y = tymin; y <= tymax; ++y)
+            for (int x = txmin; x <= txmax; ++x)
+                keys[off++] = ((uint64_t)((uint32_t)y * (uint32_t)gx + (uint32_t)x) << pos_bits) | pos;
+    }
+}
+
+__global__ __launch_bounds__(256) void duplicate_big_kernel(int gx, int gy, int pos_bits,
+                                                            const uint32_t* __restrict__ n_dup,
+                                                            const uint32_t* __restrict__ scan,
+                                                            const uint2* __restrict__ bboxes,
+                                                            const uint32_t* __restrict__ qbits,
+                                                            const uint32_t* __restrict__ order_pos,
+                                                            uint64_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ big_list,
+                                                            const uint32_t* __restrict__ big_count)
+{
+    const uint32_t count = *big_count;
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    {
+        const uint32_t v = big_list[i];
+        const uint32_t nd = n_dup[v];
+        const uint32_t off = scan[v] - nd;
+        int txmin, tymin, txmax, tymax;
+        rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
+        const uint32_t w = (uint32_t)(txmax - txmin + 1);
+        const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
+        const uint32_t bits = qbits[v];
+        for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x)
+        {
+            const uint32_t qi = e / tiles, rem = e - qi * tiles;
+            // the qi-th set bit of bits
+            uint32_t b = bits;
+            for (uint32_t s = 0; s < qi; ++s) b &= b - 1u;
+            const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
+            const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
+            keys[off + e] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | (uint64_t)order_pos[8 * v + q];
+        }
+    }
+}
+
+// forward.cu:582-604 plus the value decode: vox_list[i] = (q << 29) | vox_id.
+__global__ __launch_bounds__(256) void ranges_kernel(const uint64_t* __restrict__ keys, int64_t R, int pos_bits,
+                                                     const uint32_t* __restrict__ order_pair,
+                                                     uint2* __restrict__ ranges, uint32_t* __restrict__ vox_list)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    const uint64_t k = keys[i];
+    const uint32_t tile = (uint32_t)(k >> pos_bits);
+    const uint32_t pair = order_pair[k & ((1ull << pos_bits) - 1ull)];
+    vox_list[i] = ((pair & 7u) << 29) | (pair >> 3);
+    uint32_t* rg = (uint32_t*)ranges;
+    if (i == 0)
+        rg[2 * tile] = 0;
+    else
+    {
+        const uint32_t prev = (uint32_t)(keys[i - 1] >> pos_bits);
+        if (prev != tile)
+        {
+            rg[2 * prev + 1] = (uint32_t)i;
+            rg[2 * tile] = (uint32_t)i;
+        }
+    }
+    if (i == R - 1) rg[2 * tile + 1] = (uint32_t)R;
+}
+
+__global__ void export_keys_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vox_list,
+                                   int64_t R, int pos_bits, const int64_t* __restrict__ paths,
+                                   uint64_t* __restrict__ hi, uint64_t* __restrict__ lo, uint32_t* __restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    const uint64_t tile = keys[i] >> pos_bits;
+    const uint32_t val = vox_list[i];
+    const uint64_t rank = (uint64_t)paths[val & VOX_ID_MASK] ^ order_table(val >> 29);
+    hi[i] = tile >> 1;
+    lo[i] = (tile << 63) | rank;
+    vals[i] = val;
+}
+
+int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
+                    BinningState& b, ImageState& img, hipStream_t stream)
+{
+    const int gx = tiles_x(W), gy = tiles_y(H);
+    const int64_t T = (int64_t)gx * gy;
+    SVR_REQUIRE(fill_u32(img.ranges, 0u, 2 * T, stream) == SVR_OK, SVR_ERR_HIP, "fill ranges failed");
+    if (R == 0) return SVR_OK;
+    const int pos_bits = (int)required_bits((uint64_t)8 * N);
+    const int tile_bits = (int)required_bits((uint64_t)T);
+    SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
+    duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, gx, gy, pos_bits, g.n_dup, g.scan, g.bbox,
+                                                                      g.qbits, ot.pos, b.keys, g.big_list,
+                                                                      g.counters + 2);
+    SVR_CHECK_LAUNCH();
+    duplicate_big_kernel<<<1024, 256, 0, stream>>>(gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos,
+                                                   b.keys, g.big_list, g.counters + 2);
+    SVR_CHECK_LAUNCH();
+    uint64_t* sorted = nullptr;
+    int rc = radix_sort_ex(b.keys, b.keys_alt, nullptr, nullptr, R, pos_bits + tile_bits, b.counts, stream, &sorted,
+                           nullptr);
+    if (rc != SVR_OK) return rc;
+    if (sorted != b.keys)
+    {
+        SVR_CHECK_HIP(hipMemcpyAsync(b.keys, sorted, sizeof(uint64_t) * R, hipMemcpyDeviceToDevice, stream));
+    }
+    ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(b.keys, R, pos_bits, ot.pair, img.ranges,
+                                                                   b.vox_list);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+}  // namespace svr
+
+using namespace svr;
+
+extern "C" {
+
+size_t svr_order_table_scratch_size(int64_t N)
+{
+    size_t bytes = 0;
+    OrderScratch::carve(nullptr, N < 0 ? 0 : N, &bytes);
+    return bytes;
+}
+
+int svr_build_order_table(int64_t N, const int64_t* paths, void* order_table_buf, void* scratch,
+                          svr_stream_t stream)
+{
+    SVR_REQUIRE(N >= 0 && N <= (int64_t)VOX_ID_MASK + 1, SVR_ERR_INVALID, "voxel count out of range");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(paths && order_table_buf && scratch, SVR_ERR_INVALID, "svr_build_order_table: null pointer");
+    OrderTable ot = OrderTable::carve(order_table_buf, N, nullptr);
+    OrderScratch s = OrderScratch::carve(scratch, N, nullptr);
+    const int64_t n8 = 8 * N;
+    if (fill_u32(s.misc, 0u, 4, stream) != SVR_OK) return SVR_ERR_HIP;
+    or_reduce_kernel<<<256, 256, 0, stream>>>(paths, N, s.misc);
+    SVR_CHECK_LAUNCH();
+    uint32_t orv[2] = {0, 0};
+    SVR_CHECK_HIP(hipMemcpyAsync(orv, s.misc, sizeof(orv), hipMemcpyDeviceToHost, stream));
+    SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    const uint64_t all = ((uint64_t)orv[1] << 32) | orv[0];
+    // finest level any octpath uses: below it every rank bit is q repeated
+    const int finest = all ? (MAX_NUM_LEVELS - (__builtin_ctzll(all) / 3)) : 0;
+    const int with_q = finest < MAX_NUM_LEVELS;
+    const int lshift = 3 * (MAX_NUM_LEVELS - finest);
+    const int key_bits = with_q ? 3 * finest + 3 : ORDER_RANK_BITS;
+    order_keys_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, stream>>>(paths, n8, lshift, with_q, s.keys, s.vals);
+    SVR_CHECK_LAUNCH();
+    uint64_t* ko = nullptr;
+    uint32_t* vo = nullptr;
+    int rc = radix_sort_ex(s.keys, s.keys_alt, s.vals, s.vals_alt, n8, key_bits, s.counts, stream, &ko, &vo);
+    if (rc != SVR_OK) return rc;
+    order_scatter_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, stream>>>(vo, n8, ot.pos, ot.pair);
+    SVR_CHECK_LAUNCH();
+    const uint32_t meta[4] = {required_bits((uint64_t)n8), (uint32_t)key_bits, (uint32_t)finest, 0u};
+    SVR_CHECK_HIP(hipMemcpyAsync(ot.meta, meta, sizeof(meta), hipMemcpyHostToDevice, stream));
+    SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    return SVR_OK;
+}
+
+int svr_unpack_geom_state(int64_t N, const void* geom_state, uint32_t* n_dup, uint32_t* bbox, uint32_t* qbits,
+                          svr_stream_t stream)
+{
+    if (N == 0) return SVR_OK;
+    GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    SVR_CHECK_HIP(hipMemcpyAsync(n_dup, g.n_dup, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, stream));
+    SVR_CHECK_HIP(hipMemcpyAsync(bbox, g.bbox, sizeof(uint2) * N, hipMemcpyDeviceToDevice, stream));
+    SVR_CHECK_HIP(hipMemcpyAsync(qbits, g.qbits, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, stream));
+    return SVR_OK;
+}
+
+int svr_export_sort_keys(int32_t W, int32_t H, int64_t R, const void* binning_state, const void* order_table_buf,
+                         const int64_t* paths, int64_t N, uint64_t* hi, uint64_t* lo, uint32_t* vals,
+                         svr_stream_t stream)
+{
+    (void)W; (void)H; (void)order_table_buf;
+    if (R == 0) return SVR_OK;
+    BinningState b = BinningState::carve((void*)binning_state, R, nullptr);
+    const int pos_bits = (int)required_bits((uint64_t)8 * N);
+    export_keys_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(b.keys, b.vox_list, R, pos_bits, paths, hi,
+                                                                        lo, vals);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+}  // extern "C"

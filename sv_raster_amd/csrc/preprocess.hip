@@ -1,0 +1,238 @@
+// Voxel preprocess (screen bbox, camera quadrants, entry counts) and the
+// prefix scan of the entry counts.
+//
+// Reference: backends/new_cuda/src/preprocess.cu:24-199 (kernel + host),
+// forward.cu:649-663 (scan + reading the number of rendered entries).
+#include "svr_device.h"
+#include "svr_host.h"
+
+namespace svr {
+
+// preprocess.cu:24-132.  One thread per voxel; the arithmetic order matches
+// the oracle exactly (no contraction), so the packed bboxes are bit-identical.
+__global__ __launch_bounds__(256) void preprocess_kernel(
+    int64_t N, int W, int H, int gx, int gy, float focal_x, float focal_y, float cx, float cy,
+    const float* __restrict__ w2c, const float* __restrict__ c2w, float near_plane,
+    const float* __restrict__ centers, const float* __restrict__ lengths,
+    int32_t* __restrict__ out_n_dup, uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes,
+    uint32_t* __restrict__ qbits_out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+
+    const float c0 = centers[3 * v + 0], c1 = centers[3 * v + 1], c2 = centers[3 * v + 2];
+    const float r = 0.5f * lengths[v];
+    const float ro0 = c2w[3], ro1 = c2w[7], ro2 = c2w[11];
+    float m[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = w2c[i];
+
+    uint32_t nd = 0, bits = 0;
+    uint2 bb = make_uint2(0u, 0u);
+    const float rel0 = c0 - ro0, rel1 = c1 - ro1, rel2 = c2 - ro2;
+    if (!(rel0 * rel0 + rel1 * rel1 + rel2 * rel2 < near_plane * near_plane))
+    {
+        float umin = 1e9f, vmin = 1e9f, umax = -1e9f, vmax = -1e9f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+        {
+            const float sx = (float)(((i & 4) >> 2) * 2 - 1);
+            const float sy = (float)(((i & 2) >> 1) * 2 - 1);
+            const float sz = (float)((i & 1) * 2 - 1);
+            const float p0 = c0 + r * sx, p1 = c1 + r * sy, p2 = c2 + r * sz;
+            const float camx = m[0] * p0 + m[1] * p1 + m[2] * p2 + m[3];
+            const float camy = m[4] * p0 + m[5] * p1 + m[6] * p2 + m[7];
+            const float camz = m[8] * p0 + m[9] * p1 + m[10] * p2 + m[11];
+            if (camz < near_plane) continue;
+            const float iz = 1.0f / camz;
+            const float u = camx * iz, vv = camy * iz;
+            const uint32_t q = ((p0 < ro0) << 2) | ((p1 < ro1) << 1) | (p2 < ro2);
+            umin = smin(umin, u); vmin = smin(vmin, vv);
+            umax = smax(umax, u); vmax = smax(vmax, vv);
+            bits |= 1u << q;
+        }
+        const float cxh = cx - 0.5f, cyh = cy - 0.5f;
+        const float bminx = fmaxf(focal_x * umin + cxh, 0.0f);
+        const float bminy = fmaxf(focal_y * vmin + cyh, 0.0f);
+        const float bmaxx = fminf(focal_x * umax + cxh, (float)W);
+        const float bmaxy = fminf(focal_y * vmax + cyh, (float)H);
+        if (!(bminx > bmaxx || bminy > bmaxy))
+        {
+            const uint32_t xmin = (uint32_t)(int)__builtin_rintf(bminx);
+            const uint32_t ymin = (uint32_t)(int)__builtin_rintf(bminy);
+            const uint32_t xmax = (uint32_t)(int)__builtin_rintf(bmaxx);
+            const uint32_t ymax = (uint32_t)(int)__builtin_rintf(bmaxy);
+            bb = make_uint2((xmin << 16) | ymin, (xmax << 16) | ymax);
+            // auxiliary.h:150-170
+            const int txmin = min((int)(xmin / TILE_X), gx - 1), tymin = min((int)(ymin / TILE_Y), gy - 1);
+            const int txmax = min((int)(xmax / TILE_X), gx - 1), tymax = min((int)(ymax / TILE_Y), gy - 1);
+            nd = (uint32_t)((1 + tymax - tymin) * (1 + txmax - txmin)) * (uint32_t)__popc(bits);
+        }
+        else
+        {
+            bits = 0;
+        }
+    }
+    else
+    {
+        bits = 0;
+    }
+    out_n_dup[v] = (int32_t)nd;
+    n_dup[v] = nd;
+    bboxes[v] = bb;
+    qbits_out[v] = bits;
+}
+
+// ---- reduce-then-scan over n_dup -------------------------------------
+
+__device__ inline uint32_t wave_incl_scan(uint32_t x, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Inclusive block scan for SCAN_BLOCK threads; returns the block total in *total.
+__device__ inline uint32_t block_incl_scan(uint32_t x, uint32_t* lds, uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int NW = SCAN_BLOCK / 64;
+    x = wave_incl_scan(x, lane);
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    if (wid == 0)
+    {
+        uint32_t s = lane < NW ? lds[lane] : 0u;
+        s = wave_incl_scan(s, lane);
+        if (lane < NW) lds[lane] = s;
+    }
+    __syncthreads();
+    const uint32_t prefix = wid > 0 ? lds[wid - 1] : 0u;
+    *total = lds[NW - 1];
+    __syncthreads();
+    return x + prefix;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                 uint32_t* __restrict__ block_sums)
+{
+    __shared__ uint32_t lds[SCAN_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (base + i < n) s += in[base + i];
+    uint32_t total;
+    block_incl_scan(s, lds, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block sums in place (one workgroup); total -> counters[0..1] (u64).
+__global__ __launch_bounds__(SCAN_BLOCK) void scan_sums_kernel(uint32_t* __restrict__ block_sums, int64_t nb,
+                                                               uint32_t* __restrict__ counters)
+{
+    __shared__ uint32_t lds[SCAN_BLOCK / 64];
+    __shared__ uint64_t carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int64_t off = 0; off < nb; off += SCAN_BLOCK)
+    {
+        const int64_t i = off + threadIdx.x;
+        const uint32_t x = i < nb ? block_sums[i] : 0u;
+        uint32_t total;
+        const uint32_t incl = block_incl_scan(x, lds, &total);
+        const uint64_t carry = carry_s;
+        if (i < nb) block_sums[i] = (uint32_t)(carry + incl - x);
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        const uint64_t R = carry_s;
+        counters[0] = (uint32_t)R;
+        counters[1] = (uint32_t)(R >> 32);
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                const uint32_t* __restrict__ block_sums,
+                                                                uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t lds[SCAN_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+    {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t total;
+    uint32_t run = block_incl_scan(s, lds, &total) - s + block_sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+    {
+        run += v[i];
+        if (base + i < n) out[base + i] = run;
+    }
+}
+
+}  // namespace svr
+
+using namespace svr;
+
+extern "C" {
+
+int svr_rasterize_preprocess(int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,
+                             const float* w2c, const float* c2w, float near_plane, int64_t N,
+                             const float* centers, const float* lengths, int32_t* out_n_dup, void* geom_state,
+                             svr_stream_t stream)
+{
+    SVR_REQUIRE(W > 0 && H > 0 && W <= MAX_PACKED_IMAGE_DIM && H <= MAX_PACKED_IMAGE_DIM, SVR_ERR_INVALID,
+                "image size %dx%d outside [1, %d]", W, H, MAX_PACKED_IMAGE_DIM);
+    SVR_REQUIRE(N >= 0 && N <= (int64_t)VOX_ID_MASK + 1, SVR_ERR_INVALID, "voxel count %lld out of range",
+                (long long)N);
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(geom_state && centers && lengths && out_n_dup && w2c && c2w, SVR_ERR_INVALID,
+                "svr_rasterize_preprocess: null pointer");
+    GeomState g = GeomState::carve(geom_state, N, nullptr);
+    const float focal_x = 0.5f * (float)W / tanfovx;
+    const float focal_y = 0.5f * (float)H / tanfovy;
+    const int64_t blocks = (N + 255) / 256;
+    preprocess_kernel<<<(unsigned)blocks, 256, 0, stream>>>(N, W, H, tiles_x(W), tiles_y(H), focal_x, focal_y, cx,
+                                                            cy, w2c, c2w, near_plane, centers, lengths, out_n_dup,
+                                                            g.n_dup, g.bbox, g.qbits);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* R_host)
+{
+    SVR_REQUIRE(R_host != nullptr, SVR_ERR_INVALID, "svr_num_rendered: null output");
+    *R_host = 0;
+    if (N == 0) return SVR_OK;
+    GeomState g = GeomState::carve(geom_state, N, nullptr);
+    const int64_t nb = (N + SCAN_TILE - 1) / SCAN_TILE;
+    scan_reduce_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums);
+    SVR_CHECK_LAUNCH();
+    scan_sums_kernel<<<1, SCAN_BLOCK, 0, stream>>>(g.block_sums, nb, g.counters);
+    SVR_CHECK_LAUNCH();
+    scan_apply_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums, g.scan);
+    SVR_CHECK_LAUNCH();
+    uint32_t cnt[2] = {0, 0};
+    SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+    SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    const uint64_t R = ((uint64_t)cnt[1] << 32) | cnt[0];
+    SVR_REQUIRE(R < (1ull << 32), SVR_ERR_CAPACITY, "number of rendered entries %llu exceeds 2^32 - 1",
+                (unsigned long long)R);
+    *R_host = (int64_t)R;
+    return SVR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,420 @@
+// Per-tile front-to-back compositor and its reverse-order adjoint.
+//
+// Reference: backends/new_cuda/src/forward.cu:43-394 (renderCUDA),
+// backward.cu:37-530 (renderCUDA backward), host sides forward.cu:607-879 and
+// backward.cu:654-772.
+//
+// Mapping: one 256-thread workgroup per 16x16 tile; each of its four waves owns
+// one 8x8 sub-tile (lane = 8 * row + column) and walks the tile's entry list on
+// its own, 64 entries per step, with no workgroup barrier inside the loop:
+//   1. lane l loads entry base+l, tests its packed bbox against the sub-tile
+//      and its quadrant against the quadrants of the still-active pixels,
+//      producing a 64-bit pixel mask (row of a 64x64 bit matrix), and stages
+//      the entry's payload in LDS at slot l;
+//   2. the bit matrix is transposed across the wave (6 shuffle steps), giving
+//      each pixel lane the mask of entries whose bbox covers it;
+//   3. every lane walks its own mask in list order (lock-step over hits, not
+//      over entries), doing the ray-box test and the compositing.
+// Per-pixel results do not depend on this mapping: each pixel visits exactly
+// the entries the reference visits, in the same order, with the same
+// arithmetic.  The backward accumulates the 12 per-entry gradients of a step
+// in LDS (ds_add) and flushes one global atomic per (entry, component, wave).
+#include "svr_device.h"
+#include "svr_host.h"
+
+namespace svr {
+
+constexpr int NWAVE = 4;
+
+struct Stage
+{
+    float c[3][NWAVE][64];
+    float l[NWAVE][64];
+    float g[8][NWAVE][64];
+    float rgb[3][NWAVE][64];
+};
+
+// Pixel mask (bit = 8 * row + column of the wave's 8x8 sub-tile) covered by a
+// packed bbox (inclusive), intersected with qsel.
+__device__ inline uint64_t bbox_pixel_mask(uint2 bb, int sx0, int sy0, int sx1, int sy1, uint64_t qsel)
+{
+    const int bxmin = (int)(bb.x >> 16), bymin = (int)(bb.x & 0xFFFFu);
+    const int bxmax = (int)(bb.y >> 16), bymax = (int)(bb.y & 0xFFFFu);
+    const int x0 = max(bxmin, sx0) - sx0, x1 = min(bxmax, sx1) - sx0;
+    const int y0 = max(bymin, sy0) - sy0, y1 = min(bymax, sy1) - sy0;
+    if (x0 > x1 || y0 > y1) return 0ull;
+    const uint64_t col = (uint64_t)(((2u << x1) - 1u) & ~((1u << x0) - 1u));
+    const uint64_t rows_hi = (y1 >= 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
+    const uint64_t rows = rows_hi & ~((1ull << (8 * y0)) - 1ull);
+    return (col * 0x0101010101010101ull) & rows & qsel;
+}
+
+__device__ inline uint64_t select_q(const uint64_t* qpix, uint32_t q)
+{
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) s = (q == i) ? qpix[i] : s;
+    return s;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = max(x, (uint32_t)__shfl_xor(x, o, 64));
+    return x;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void render_fwd_kernel(
+    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
+    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
+    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
+{
+    __shared__ Stage st;
+    __shared__ uint32_t s_last[NWAVE];
+    const int tile = xcd_tile(blockIdx.x, per_xcd);
+    if (tile >= T_tiles) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tx = tile % gx, ty = tile / gx;
+    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
+    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
+    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+    const bool inside = px < W && py < H;
+    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    const uint2 range = ranges[tile];
+
+    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+    uint32_t last = 0;
+    bool done = !inside;
+
+    for (uint32_t base = range.x; base < range.y; base += 64)
+    {
+        uint64_t qpix[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
+        if (__ballot(!done) == 0ull) break;
+
+        const uint32_t j = base + (uint32_t)lane;
+        uint64_t pm = 0ull;
+        if (j < range.y)
+        {
+            const uint32_t val = vox_list[j];
+            const uint32_t v = val & VOX_ID_MASK, q = val >> 29;
+            pm = bbox_pixel_mask(bboxes[v], sx0, sy0, sx1, sy1, select_q(qpix, q));
+            if (pm)
+            {
+                st.c[0][w][lane] = centers[3 * (size_t)v + 0];
+                st.c[1][w][lane] = centers[3 * (size_t)v + 1];
+                st.c[2][w][lane] = centers[3 * (size_t)v + 2];
+                st.l[w][lane] = lengths[v];
+                const float4 g0 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v);
+                const float4 g1 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v + 4);
+                st.g[0][w][lane] = g0.x; st.g[1][w][lane] = g0.y; st.g[2][w][lane] = g0.z; st.g[3][w][lane] = g0.w;
+                st.g[4][w][lane] = g1.x; st.g[5][w][lane] = g1.y; st.g[6][w][lane] = g1.z; st.g[7][w][lane] = g1.w;
+                st.rgb[0][w][lane] = rgbs[3 * (size_t)v + 0];
+                st.rgb[1][w][lane] = rgbs[3 * (size_t)v + 1];
+                st.rgb[2][w][lane] = rgbs[3 * (size_t)v + 2];
+            }
+        }
+        wave_sync();
+        uint64_t hm = wave_transpose64(pm, lane);
+
+        while (hm)
+        {
+            const int l = __builtin_ctzll(hm);
+            hm &= hm - 1ull;
+            const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
+            float a, b;
+            ray_aabb(c0, c1, c2, len, ray, a, b);
+            if (a > b) continue;
+            float g[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
+            const float vol = vox_vol<NS, false>(c0, c1, c2, len, g, a, b, ray, nullptr);
+            const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
+            if (alpha < MIN_ALPHA) continue;
+            const float pt_w = T * alpha;
+            C0 = __builtin_fmaf(pt_w, st.rgb[0][w][l], C0);
+            C1 = __builtin_fmaf(pt_w, st.rgb[1][w][l], C1);
+            C2 = __builtin_fmaf(pt_w, st.rgb[2][w][l], C2);
+            T = T * (1.f - alpha);
+            last = base + (uint32_t)l - range.x + 1u;
+            if (T < EARLY_STOP_T)
+            {
+                done = true;
+                break;
+            }
+        }
+        wave_sync();
+    }
+
+    if (inside)
+    {
+        const size_t pid = (size_t)py * W + px;
+        const size_t HW = (size_t)H * W;
+        n_contrib[pid] = last;
+        out_color[pid] = C0 + T * bg;
+        out_color[HW + pid] = C1 + T * bg;
+        out_color[2 * HW + pid] = C2 + T * bg;
+        out_T[pid] = T;
+    }
+    const uint32_t wm = wave_max_u32(inside ? last : 0u);
+    if (lane == 0) s_last[w] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        tile_last[tile] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void render_bwd_kernel(
+    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
+    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
+    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+    float* __restrict__ dL_dgeos, float* __restrict__ dL_drgbs, float* __restrict__ dL_dsub)
+{
+    __shared__ Stage st;
+    __shared__ float acc[12][NWAVE][64];
+    const int tile = xcd_tile(blockIdx.x, per_xcd);
+    if (tile >= T_tiles) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tx = tile % gx, ty = tile / gx;
+    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
+    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
+    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+    const bool inside = px < W && py < H;
+    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    const uint2 range = ranges[tile];
+    const size_t HW = (size_t)H * W;
+    const size_t pid = inside ? (size_t)py * W + px : 0;
+
+    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
+    uint32_t nc = 0;
+    if (inside)
+    {
+        T = out_T[pid];
+        nc = n_contrib[pid];
+        dp0 = dL_dcolor[pid];
+        dp1 = dL_dcolor[HW + pid];
+        dp2 = dL_dcolor[2 * HW + pid];
+        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k][w][lane] = 0.f;
+    uint64_t qpix[8];
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(inside && nc > 0 && ray.pq == q);
+    const int64_t wave_end = (int64_t)range.x + wave_max_u32(nc);
+
+    for (int64_t end = wave_end; end > (int64_t)range.x; end -= 64)
+    {
+        const int64_t jl = end - 1 - lane;
+        uint64_t pm = 0ull;
+        uint32_t v = 0;
+        if (jl >= (int64_t)range.x)
+        {
+            const uint32_t val = vox_list[jl];
+            v = val & VOX_ID_MASK;
+            const uint32_t q = val >> 29;
+            pm = bbox_pixel_mask(bboxes[v], sx0, sy0, sx1, sy1, select_q(qpix, q));
+            if (pm)
+            {
+                st.c[0][w][lane] = centers[3 * (size_t)v + 0];
+                st.c[1][w][lane] = centers[3 * (size_t)v + 1];
+                st.c[2][w][lane] = centers[3 * (size_t)v + 2];
+                st.l[w][lane] = lengths[v];
+                const float4 g0 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v);
+                const float4 g1 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v + 4);
+                st.g[0][w][lane] = g0.x; st.g[1][w][lane] = g0.y; st.g[2][w][lane] = g0.z; st.g[3][w][lane] = g0.w;
+                st.g[4][w][lane] = g1.x; st.g[5][w][lane] = g1.y; st.g[6][w][lane] = g1.z; st.g[7][w][lane] = g1.w;
+                st.rgb[0][w][lane] = rgbs[3 * (size_t)v + 0];
+                st.rgb[1][w][lane] = rgbs[3 * (size_t)v + 1];
+                st.rgb[2][w][lane] = rgbs[3 * (size_t)v + 2];
+            }
+        }
+        wave_sync();
+        uint64_t hm = wave_transpose64(pm, lane);
+        // entry end-1-l is behind this pixel's last contributor when
+        // end-1-l >= range.x + nc, i.e. l < end - range.x - nc
+        const int64_t cut = end - (int64_t)range.x - (int64_t)nc;
+        if (cut >= 64) hm = 0ull;
+        else if (cut > 0) hm &= ~((1ull << cut) - 1ull);
+
+        while (hm)
+        {
+            const int l = __builtin_ctzll(hm);
+            hm &= hm - 1ull;
+            const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
+            float a, b;
+            ray_aabb(c0, c1, c2, len, ray, a, b);
+            if (a > b) continue;
+            float g[8], dI[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
+            const float vol = vox_vol<NS, true>(c0, c1, c2, len, g, a, b, ray, dI);
+            const float e = svr_expf(-vol);
+            const float alpha = fminf(MAX_ALPHA, 1.f - e);
+            if (alpha < MIN_ALPHA) continue;
+            T = T / (1.f - alpha);
+            const float pt_w = alpha * T;
+            const float r0 = st.rgb[0][w][l], r1 = st.rgb[1][w][l], r2 = st.rgb[2][w][l];
+            const float dpt = __builtin_fmaf(dp2, r2, __builtin_fmaf(dp1, r1, dp0 * r0));
+            const float diff = dpt - last_dT;
+            const float dalpha = T * diff;
+            last_dT = __builtin_fmaf(alpha, diff, last_dT);
+            const float dIs = dalpha * e;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) atomicAdd(&acc[i][w][l], dIs * dI[i]);
+            atomicAdd(&acc[8][w][l], pt_w * dp0);
+            atomicAdd(&acc[9][w][l], pt_w * dp1);
+            atomicAdd(&acc[10][w][l], pt_w * dp2);
+            atomicAdd(&acc[11][w][l], fabsf(dalpha * alpha));
+        }
+        wave_sync();
+        if (pm)
+        {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+            {
+                const float a = acc[i][w][lane];
+                if (a != 0.f) unsafeAtomicAdd(dL_dgeos + 8 * (size_t)v + i, a);
+                acc[i][w][lane] = 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                const float a = acc[8 + i][w][lane];
+                if (a != 0.f) unsafeAtomicAdd(dL_drgbs + 3 * (size_t)v + i, a);
+                acc[8 + i][w][lane] = 0.f;
+            }
+            const float a = acc[11][w][lane];
+            if (a != 0.f) unsafeAtomicAdd(dL_dsub + v, a);
+            acc[11][w][lane] = 0.f;
+        }
+        wave_sync();
+    }
+}
+
+template <int NS>
+int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, const float* c2w, float bg,
+               const GeomState& g, const BinningState& b, const ImageState& img, const float* centers,
+               const float* lengths, const float* geos, const float* rgbs, float* out_color, float* out_T,
+               hipStream_t stream)
+{
+    const int gx = tiles_x(W), gy = tiles_y(H);
+    const int T = gx * gy;
+    const int per = (T + 7) / 8;
+    render_fwd_kernel<NS><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
+                                                                   img.ranges, b.vox_list, g.bbox, centers, lengths,
+                                                                   geos, rgbs, img.tile_last, img.n_contrib,
+                                                                   out_color, out_T);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+template <int NS>
+int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, const float* c2w, float bg,
+               const GeomState& g, const BinningState& b, const ImageState& img, const float* centers,
+               const float* lengths, const float* geos, const float* rgbs, const float* out_T, const float* dL_dcolor,
+               const float* dL_dT, float* dgeos, float* drgbs, float* dsub, hipStream_t stream)
+{
+    const int gx = tiles_x(W), gy = tiles_y(H);
+    const int T = gx * gy;
+    const int per = (T + 7) / 8;
+    render_bwd_kernel<NS><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
+                                                                   img.ranges, b.vox_list, g.bbox, centers, lengths,
+                                                                   geos, rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT,
+                                                                   dgeos, drgbs, dsub);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+}  // namespace svr
+
+using namespace svr;
+
+extern "C" {
+
+int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,
+                          const float* c2w, float bg, int64_t N, const float* centers, const float* lengths,
+                          const float* geos, const float* rgbs, const void* geom_state, const void* order_table,
+                          int64_t R, void* binning_state, void* image_state, float* out_color, float* out_T,
+                          svr_stream_t stream)
+{
+    SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
+                MAX_N_SAMP);
+    SVR_REQUIRE(W > 0 && H > 0 && W <= MAX_PACKED_IMAGE_DIM && H <= MAX_PACKED_IMAGE_DIM, SVR_ERR_INVALID,
+                "image size %dx%d outside [1, %d]", W, H, MAX_PACKED_IMAGE_DIM);
+    SVR_REQUIRE(R >= 0 && R < (1ll << 32), SVR_ERR_INVALID, "num_rendered out of range");
+    SVR_REQUIRE(image_state && out_color && out_T && c2w, SVR_ERR_INVALID, "svr_rasterize_forward: null pointer");
+    SVR_REQUIRE(N == 0 || (geom_state && order_table && centers && lengths && geos && rgbs), SVR_ERR_INVALID,
+                "svr_rasterize_forward: null pointer");
+    SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_rasterize_forward: null binning state");
+    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
+    const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
+    GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
+    BinningState b = BinningState::carve(binning_state, R, nullptr);
+    ImageState img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr);
+    int rc = binning_forward(W, H, N, g, ot, R, b, img, stream);
+    if (rc != SVR_OK) return rc;
+    switch (n_samp)
+    {
+    case 1: return launch_fwd<1>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
+                                 out_color, out_T, stream);
+    case 2: return launch_fwd<2>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
+                                 out_color, out_T, stream);
+    default: return launch_fwd<3>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
+                                  out_color, out_T, stream);
+    }
+}
+
+int svr_rasterize_backward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,
+                           const float* c2w, float bg, int64_t N, const float* centers, const float* lengths,
+                           const float* geos, const float* rgbs, const void* geom_state, int64_t R,
+                           const void* binning_state, const void* image_state, const float* out_T,
+                           const float* dL_dcolor, const float* dL_dT, float* dgeos, float* drgbs, float* dsub,
+                           svr_stream_t stream)
+{
+    SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
+                MAX_N_SAMP);
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub && c2w,
+                SVR_ERR_INVALID, "svr_rasterize_backward: null pointer");
+    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
+    if (fill_u32(dgeos, 0u, 8 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+    if (fill_u32(drgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+    if (fill_u32(dsub, 0u, N, stream) != SVR_OK) return SVR_ERR_HIP;
+    if (R == 0) return SVR_OK;
+    SVR_REQUIRE(binning_state, SVR_ERR_INVALID, "svr_rasterize_backward: null binning state");
+    const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
+    GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    BinningState b = BinningState::carve((void*)binning_state, R, nullptr);
+    ImageState img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr);
+    switch (n_samp)
+    {
+    case 1: return launch_bwd<1>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
+                                 out_T, dL_dcolor, dL_dT, dgeos, drgbs, dsub, stream);
+    case 2: return launch_bwd<2>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
+                                 out_T, dL_dcolor, dL_dT, dgeos, drgbs, dsub, stream);
+    default: return launch_bwd<3>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
+                                  out_T, dL_dcolor, dL_dT, dgeos, drgbs, dsub, stream);
+    }
+}
+
+int svr_unpack_image_state(int32_t W, int32_t H, const void* image_state, int32_t* ranges, int32_t* tile_last,
+                           int32_t* n_contrib, svr_stream_t stream)
+{
+    const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
+    ImageState img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr);
+    SVR_CHECK_HIP(hipMemcpyAsync(ranges, img.ranges, sizeof(uint32_t) * 2 * T, hipMemcpyDeviceToDevice, stream));
+    SVR_CHECK_HIP(hipMemcpyAsync(tile_last, img.tile_last, sizeof(uint32_t) * T, hipMemcpyDeviceToDevice, stream));
+    SVR_CHECK_HIP(hipMemcpyAsync(n_contrib, img.n_contrib, sizeof(uint32_t) * W * H, hipMemcpyDeviceToDevice,
+                                 stream));
+    return SVR_OK;
+}
+
+}  // extern "C"

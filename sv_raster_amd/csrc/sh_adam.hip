@@ -1,0 +1,264 @@
+// View-dependent colour (real SH, degree <= 3), the Adam update and the octree
+// codec utilities.
+//
+// Reference: backends/new_cuda/src/sh_compute.cu:33-299, adam_step.cu:21-164,
+// utils.cu:106-189.
+#include <math.h>
+
+#include "svr_device.h"
+#include "svr_host.h"
+
+namespace svr {
+
+// sh_compute.cu:33-50 (real SH normalisations with their signs)
+__constant__ float SH_C2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                               -1.0925484305920792f, 0.5462742152960396f};
+__constant__ float SH_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                               0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                               -0.5900435899266435f};
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+
+__device__ inline int sh_basis(int deg, float x, float y, float z, float* b)
+{
+    b[0] = SH_C0;
+    if (deg < 1) return 1;
+    b[1] = -SH_C1 * y; b[2] = SH_C1 * z; b[3] = -SH_C1 * x;
+    if (deg < 2) return 4;
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    b[4] = SH_C2[0] * xy;
+    b[5] = SH_C2[1] * yz;
+    b[6] = SH_C2[2] * (2.f * zz - xx - yy);
+    b[7] = SH_C2[3] * xz;
+    b[8] = SH_C2[4] * (xx - yy);
+    if (deg < 3) return 9;
+    b[9] = SH_C3[0] * y * (3.f * xx - yy);
+    b[10] = SH_C3[1] * xy * z;
+    b[11] = SH_C3[2] * y * (4.f * zz - xx - yy);
+    b[12] = SH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
+    b[13] = SH_C3[4] * x * (4.f * zz - xx - yy);
+    b[14] = SH_C3[5] * z * (xx - yy);
+    b[15] = SH_C3[6] * x * (xx - 3.f * yy);
+    return 16;
+}
+
+__device__ inline void sh_dir(const float* __restrict__ centers, int64_t v, const float* __restrict__ cam, float& x,
+                              float& y, float& z)
+{
+    const float dx = centers[3 * v] - cam[0], dy = centers[3 * v + 1] - cam[1], dz = centers[3 * v + 2] - cam[2];
+    const float rn = 1.f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    x = dx * rn; y = dy * rn; z = dz * rn;
+}
+
+// sh_compute.cu:55-106, 180-208
+__global__ __launch_bounds__(256) void sh_fwd_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,
+                                                     const float* __restrict__ centers, const float* __restrict__ cam,
+                                                     const float* __restrict__ sh0, const float* __restrict__ shs,
+                                                     int M, float* __restrict__ rgbs)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int64_t v = idx ? idx[t] : t;
+    float x, y, z, b[16];
+    sh_dir(centers, v, cam, x, y, z);
+    const int nb = sh_basis(deg, x, y, z, b);
+    const float* s = shs + (size_t)v * (M - 1) * 3;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+    {
+        float res = SH_C0 * sh0[3 * v + ch];
+        if (nb > 1) res = res - (SH_C1 * y) * s[ch] + (SH_C1 * z) * s[3 + ch] - (SH_C1 * x) * s[6 + ch];
+        for (int k = 4; k < nb; ++k) res = res + b[k] * s[(k - 1) * 3 + ch];
+        res = res + 0.5f;
+        rgbs[3 * v + ch] = res > 0.f ? res : 0.f;
+    }
+}
+
+// sh_compute.cu:110-176, 212-240: no clamp mask on the incoming gradient.
+__global__ __launch_bounds__(256) void sh_bwd_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,
+                                                     const float* __restrict__ centers, const float* __restrict__ cam,
+                                                     const float* __restrict__ dL_drgbs, int M,
+                                                     float* __restrict__ dsh0, float* __restrict__ dshs)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int64_t v = idx ? idx[t] : t;
+    float x, y, z, b[16];
+    sh_dir(centers, v, cam, x, y, z);
+    const int nb = sh_basis(deg, x, y, z, b);
+    float* d = dshs + (size_t)v * (M - 1) * 3;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+    {
+        const float g = dL_drgbs[3 * v + ch];
+        dsh0[3 * v + ch] = SH_C0 * g;
+        for (int k = 1; k < nb; ++k) d[(k - 1) * 3 + ch] = b[k] * g;
+    }
+}
+
+// adam_step.cu:21-64 (unbiased) and 67-102 (biased); lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0))
+template <bool SPARSE, bool BIASED>
+__global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, float step_size,
+                                                   float beta1, float beta2, float rb2, float eps)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const float gv = g[i];
+        if (SPARSE && gv == 0.0f) continue;
+        const float mv = __builtin_fmaf(beta1, m[i], __builtin_fmaf(-beta1, gv, gv));
+        const float gg = gv * gv;
+        const float vv = __builtin_fmaf(beta2, v[i], __builtin_fmaf(-beta2, gg, gg));
+        float denom;
+        if (BIASED)
+            denom = __builtin_sqrtf(vv) + eps;
+        else
+            denom = __builtin_fmaf(__builtin_sqrtf(vv), rb2, eps);
+        p[i] -= step_size * (mv / denom);
+        m[i] = mv;
+        v[i] = vv;
+    }
+}
+
+__global__ void order_rank_kernel(int64_t n, const int64_t* __restrict__ paths, int64_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t p = (uint64_t)paths[i];
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) out[q * n + i] = (int64_t)(p ^ order_table(q));
+}
+
+__global__ void ijk2path_kernel(int64_t n, const int64_t* __restrict__ ijk, const int8_t* __restrict__ lv,
+                                int64_t* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint64_t i = (uint64_t)ijk[3 * t], j = (uint64_t)ijk[3 * t + 1], k = (uint64_t)ijk[3 * t + 2];
+    uint64_t path = 0;
+    for (int l = 0; l < MAX_NUM_LEVELS; ++l)
+        path |= ((((i >> l) & 1ull) << 2) | (((j >> l) & 1ull) << 1) | ((k >> l) & 1ull)) << (3 * l);
+    out[t] = (int64_t)(path << (3 * (MAX_NUM_LEVELS - lv[t])));
+}
+
+__global__ void path2ijk_kernel(int64_t n, const int64_t* __restrict__ paths, const int8_t* __restrict__ lv,
+                                int64_t* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t path = (uint64_t)paths[t] >> (3 * (MAX_NUM_LEVELS - lv[t]));
+    int64_t i = 0, j = 0, k = 0;
+    for (int l = 0; l < MAX_NUM_LEVELS; ++l)
+    {
+        const uint64_t bits = (path >> (3 * l)) & 7ull;
+        i |= (int64_t)((bits >> 2) & 1ull) << l;
+        j |= (int64_t)((bits >> 1) & 1ull) << l;
+        k |= (int64_t)(bits & 1ull) << l;
+    }
+    out[3 * t] = i; out[3 * t + 1] = j; out[3 * t + 2] = k;
+}
+
+inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > cap) b = cap;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace svr
+
+using namespace svr;
+
+extern "C" {
+
+int svr_sh_compute(int32_t deg, int64_t n_idx, const int64_t* idx, int64_t N, const float* centers,
+                   const float* cam_pos, const float* sh0, const float* shs, int32_t M, float* rgbs,
+                   svr_stream_t stream)
+{
+    SVR_REQUIRE(deg >= 0 && deg <= 3 && M >= (deg + 1) * (deg + 1), SVR_ERR_INVALID,
+                "sh degree %d needs %d coefficients, got %d", deg, (deg + 1) * (deg + 1), M);
+    if (N == 0) return SVR_OK;
+    if (fill_u32(rgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+    const int64_t cnt = n_idx ? n_idx : N;
+    if (cnt == 0) return SVR_OK;
+    sh_fwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers, cam_pos,
+                                                                 sh0, shs, M, rgbs);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx, int64_t N, const float* centers,
+                      const float* cam_pos, const float* dL_drgbs, float* dsh0, float* dshs, svr_stream_t stream)
+{
+    SVR_REQUIRE(deg >= 0 && deg <= 3 && M >= (deg + 1) * (deg + 1), SVR_ERR_INVALID,
+                "sh degree %d needs %d coefficients, got %d", deg, (deg + 1) * (deg + 1), M);
+    if (N == 0) return SVR_OK;
+    if (fill_u32(dsh0, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+    if (M > 1 && fill_u32(dshs, 0u, 3 * N * (int64_t)(M - 1), stream) != SVR_OK) return SVR_ERR_HIP;
+    const int64_t cnt = n_idx ? n_idx : N;
+    if (cnt == 0) return SVR_OK;
+    sh_bwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers, cam_pos,
+                                                                 dL_drgbs, M, dsh0, dshs);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, float* m, float* v, double step,
+                           double lr, double beta1, double beta2, float eps, svr_stream_t stream)
+{
+    if (n == 0) return SVR_OK;
+    // adam_step.cu:119-124 (bias terms in double on the host)
+    const double bc1 = 1.0 - pow(beta1, step);
+    const double bc2 = 1.0 - pow(beta2, step);
+    const float step_size = (float)(lr / bc1);
+    const float rb2 = (float)(1.0 / sqrt(bc2));
+    const unsigned grid = grid_for(n, 8192);
+    if (sparse)
+        adam_kernel<true, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2, rb2,
+                                                           eps);
+    else
+        adam_kernel<false, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2,
+                                                            rb2, eps);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_biased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, float* m, float* v, float lr,
+                         float beta1, float beta2, float eps, svr_stream_t stream)
+{
+    if (n == 0) return SVR_OK;
+    const unsigned grid = grid_for(n, 8192);
+    if (sparse)
+        adam_kernel<true, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps);
+    else
+        adam_kernel<false, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_voxel_order_rank(int64_t n, const int64_t* paths, int64_t* out, svr_stream_t stream)
+{
+    if (n == 0) return SVR_OK;
+    order_rank_kernel<<<grid_for(n, 1ll << 30), 256, 0, stream>>>(n, paths, out);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_ijk_2_octpath(int64_t n, const int64_t* ijk, const int8_t* lv, int64_t* out, svr_stream_t stream)
+{
+    if (n == 0) return SVR_OK;
+    ijk2path_kernel<<<grid_for(n, 1ll << 30), 256, 0, stream>>>(n, ijk, lv, out);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_octpath_2_ijk(int64_t n, const int64_t* paths, const int8_t* lv, int64_t* out, svr_stream_t stream)
+{
+    if (n == 0) return SVR_OK;
+    path2ijk_kernel<<<grid_for(n, 1ll << 30), 256, 0, stream>>>(n, paths, lv, out);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+// Device-side constants and math shared by the rasterizer kernels (gfx950).
+//
+// Floating-point contract: every file is compiled with -ffp-contract=off and
+// IEEE-correct fp32 division/sqrt, so the only fused operations are the
+// explicit fmaf() calls written below and in the kernels.  exp() is
+// svr_expf(), a Cody-Waite reduction + degree-7 Taylor polynomial built only
+// from correctly rounded operations, so its result is the same on any IEEE
+// machine; this is what keeps the integer outputs (bboxes, n_contrib,
+// tile_last) reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svr {
+
+// backends/new_cuda/src/config.h:14-34
+constexpr int TILE_X = 16;
+constexpr int TILE_Y = 16;
+constexpr int TILE_PIX = TILE_X * TILE_Y;
+constexpr int MAX_NUM_LEVELS = 21;
+constexpr int ORDER_RANK_BITS = 3 * MAX_NUM_LEVELS;
+constexpr float MAX_ALPHA = 0.99999f;
+constexpr float MIN_ALPHA = 0.00001f;
+constexpr float EARLY_STOP_T = 0.0001f;
+constexpr float STEP_SZ_SCALE = 100.f;
+constexpr int MAX_N_SAMP = 3;
+constexpr int MAX_SORT_KEY_BITS = 128;
+constexpr int MAX_PACKED_IMAGE_DIM = (1 << 16) - 1;
+constexpr int MAX_TILE_GRID_X = (MAX_PACKED_IMAGE_DIM + TILE_X) / TILE_X;
+constexpr int MAX_TILE_GRID_Y = (MAX_PACKED_IMAGE_DIM + TILE_Y) / TILE_Y;
+constexpr int64_t MAX_RENDER_TILES = (int64_t)MAX_TILE_GRID_X * MAX_TILE_GRID_Y;
+constexpr uint32_t VOX_ID_MASK = (1u << 29) - 1u;
+
+__host__ __device__ inline uint64_t order_table(uint32_t q)
+{
+    // q repeated in all 21 three-bit groups (auxiliary.h:33-55)
+    return (uint64_t)q * 0x0249249249249249ull;
+}
+
+__host__ __device__ inline uint32_t required_bits(uint64_t n)
+{
+    uint32_t b = 0;
+    if (n <= 1) return 0;
+    n -= 1;
+    while (n) { n >>= 1; ++b; }
+    return b;
+}
+
+__device__ inline float svr_expf(float x)
+{
+    const float kf = __builtin_rintf(x * 1.44269502162933349609375f);
+    float r = __builtin_fmaf(-kf, 0.693145751953125f, x);
+    r = __builtin_fmaf(-kf, 1.428606765330187045e-06f, r);
+    float p = 1.98412701976485550403594970703125e-04f;
+    p = __builtin_fmaf(p, r, 1.388888922519981861114501953125e-03f);
+    p = __builtin_fmaf(p, r, 8.33333376795053482055664062500e-03f);
+    p = __builtin_fmaf(p, r, 4.16666679084300994873046875000e-02f);
+    p = __builtin_fmaf(p, r, 1.66666671633720397949218750000e-01f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const float xc = fminf(fmaxf(x, -104.f), 89.f);   // keeps k in [-150, 129]
+    const int k = (int)__builtin_rintf(xc * 1.44269502162933349609375f);
+    const int k1 = k / 2;
+    const int k2 = k - k1;
+    const float s1 = __uint_as_float((uint32_t)(k1 + 127) << 23);
+    const float s2 = __uint_as_float((uint32_t)(k2 + 127) << 23);
+    float res = (p * s1) * s2;
+    res = (x > 88.72283935546875f) ? __uint_as_float(0x7f800000u) : res;
+    res = (x < -103.97208404541015625f) ? 0.f : res;
+    res = (x != x) ? x : res;
+    return res;
+}
+
+// auxiliary.h:348-356
+__device__ inline float exp_linear_11(float x)
+{
+    return (x > 1.1f) ? x : svr_expf(__builtin_fmaf(0.909090909091f, x, -0.904689820196f));
+}
+
+__device__ inline float exp_linear_11_bw(float x)
+{
+    return (x > 1.1f) ? 1.0f : 0.909090909091f * svr_expf(__builtin_fmaf(0.909090909091f, x, -0.904689820196f));
+}
+
+__device__ inline float smin(float a, float b) { return (a < b) ? a : b; }
+__device__ inline float smax(float a, float b) { return (a > b) ? a : b; }
+
+__device__ inline float inv_nsamp(int n) { return n == 1 ? 1.f : (n == 2 ? 0.5f : 1.f / 3.f); }
+
+struct PixRay
+{
+    float ro[3], rd[3], rd_inv[3];
+    uint32_t pq;
+};
+
+// forward.cu:102-113; auxiliary.h:140-148, 294-302
+__device__ inline PixRay make_ray(int px, int py, int W, int H, float tanfovx, float tanfovy,
+                                  float cx, float cy, const float* __restrict__ c2w)
+{
+    PixRay r;
+    const float pfx = (float)px, pfy = (float)py;
+    const float dx = (pfx + 0.5f - cx) * 2.f * tanfovx / (float)W;
+    const float dy = (pfy + 0.5f - cy) * 2.f * tanfovy / (float)H;
+    const float dz = 1.f;
+    const float nrm = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    const float rx = c2w[0] * dx + c2w[1] * dy + c2w[2] * dz;
+    const float ry = c2w[4] * dx + c2w[5] * dy + c2w[6] * dz;
+    const float rz = c2w[8] * dx + c2w[9] * dy + c2w[10] * dz;
+    const float ninv = 1.f / nrm;
+    r.ro[0] = c2w[3]; r.ro[1] = c2w[7]; r.ro[2] = c2w[11];
+    r.rd[0] = rx * ninv; r.rd[1] = ry * ninv; r.rd[2] = rz * ninv;
+    r.rd_inv[0] = 1.f / r.rd[0]; r.rd_inv[1] = 1.f / r.rd[1]; r.rd_inv[2] = 1.f / r.rd[2];
+    r.pq = ((r.rd[0] < 0.f) << 2) | ((r.rd[1] < 0.f) << 1) | (r.rd[2] < 0.f);
+    return r;
+}
+
+// auxiliary.h:310-323
+__device__ inline void ray_aabb(float cx_, float cy_, float cz_, float l, const PixRay& ray, float& a, float& b)
+{
+    const float r = 0.5f * l;
+    const float d0 = cx_ - ray.ro[0], d1 = cy_ - ray.ro[1], d2 = cz_ - ray.ro[2];
+    const float a0 = (d0 - r) * ray.rd_inv[0], b0 = (d0 + r) * ray.rd_inv[0];
+    const float a1 = (d1 - r) * ray.rd_inv[1], b1 = (d1 + r) * ray.rd_inv[1];
+    const float a2 = (d2 - r) * ray.rd_inv[2], b2 = (d2 + r) * ray.rd_inv[2];
+    const float lo0 = smin(a0, b0), hi0 = smax(a0, b0);
+    const float lo1 = smin(a1, b1), hi1 = smax(a1, b1);
+    const float lo2 = smin(a2, b2), hi2 = smax(a2, b2);
+    a = smax(smax(lo0, lo1), lo2);
+    b = smin(smin(hi0, hi1), hi2);
+}
+
+// Volume integral of one voxel segment (forward.cu:224-253, backward.cu:283-328).
+// When DI is true, dI[k] receives sum_s 100*step*exp_linear_11'(d_s)*w_k(s).
+template <int NS, bool DI>
+__device__ inline float vox_vol(float c0, float c1, float c2, float l, const float* geo, float a, float b,
+                                const PixRay& ray, float* dI)
+{
+    const float l_inv = 1.f / l;
+    const float step_sz = (b - a) * inv_nsamp(NS);
+    const float t0 = __builtin_fmaf(0.5f, step_sz, a);
+    const float r = 0.5f * l;
+    float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - (c0 - r)) * l_inv;
+    float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - (c1 - r)) * l_inv;
+    float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - (c2 - r)) * l_inv;
+    const float qs0 = (step_sz * ray.rd[0]) * l_inv;
+    const float qs1 = (step_sz * ray.rd[1]) * l_inv;
+    const float qs2 = (step_sz * ray.rd[2]) * l_inv;
+    const float scale = STEP_SZ_SCALE * step_sz;
+    float vol = 0.f;
+    if (DI)
+    {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dI[i] = 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+    {
+        const float wx[2] = {1.f - qt0, qt0};
+        const float wy[2] = {1.f - qt1, qt1};
+        const float wz[2] = {1.f - qt2, qt2};
+        float w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = (wx[(i >> 2) & 1] * wy[(i >> 1) & 1]) * wz[i & 1];
+        float d = geo[0] * w[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i) d = __builtin_fmaf(geo[i], w[i], d);
+        vol = vol + scale * exp_linear_11(d);
+        if (DI)
+        {
+            const float dd = scale * exp_linear_11_bw(d);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dI[i] = __builtin_fmaf(dd, w[i], dI[i]);
+        }
+        qt0 = qt0 + qs0; qt1 = qt1 + qs1; qt2 = qt2 + qs2;
+    }
+    return vol;
+}
+
+// Transpose a 64x64 bit matrix held one 64-bit row per lane (row = lane):
+// afterwards lane c holds bit r set iff lane r held bit c.
+__device__ inline uint64_t wave_transpose64(uint64_t row, int lane)
+{
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        const int s = 32 >> i;
+        const uint64_t m = masks[i];
+        const uint32_t lo = __shfl_xor((uint32_t)row, s, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(row >> 32), s, 64);
+        const uint64_t other = ((uint64_t)hi << 32) | lo;
+        row = (lane & s) ? ((row & ~m) | ((other & ~m) >> s)) : ((row & m) | ((other & m) << s));
+    }
+    return row;
+}
+
+__device__ inline uint64_t lanemask_lt(int lane) { return (lane == 0) ? 0ull : (~0ull >> (64 - lane)); }
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block -> tile mapping that keeps runs of neighbouring tiles on one XCD
+// (workgroups are dispatched round-robin over the 8 XCDs).
+__device__ inline int xcd_tile(int b, int per_xcd) { return (b & 7) * per_xcd + (b >> 3); }
+
+}  // namespace svr

@@ -1,0 +1,172 @@
+// Host-side helpers for the C-ABI: status codes, the per-thread error string,
+// launch checks and the state-buffer layouts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/svraster_hip.h"
+
+namespace svr {
+
+void set_error(const char* fmt, ...);
+
+#define SVR_CHECK_HIP(expr)                                                       \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            ::svr::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                 \
+            return SVR_ERR_HIP;                                                   \
+        }                                                                         \
+    } while (0)
+
+#define SVR_CHECK_LAUNCH()                                                        \
+    do {                                                                          \
+        hipError_t _e = hipGetLastError();                                        \
+        if (_e != hipSuccess) {                                                   \
+            ::svr::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                 \
+            return SVR_ERR_HIP;                                                   \
+        }                                                                         \
+    } while (0)
+
+#define SVR_REQUIRE(cond, code, ...)                                              \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            ::svr::set_error(__VA_ARGS__);                                        \
+            return (code);                                                        \
+        }                                                                         \
+    } while (0)
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Carves typed sub-arrays out of one opaque byte buffer (256-byte aligned).
+struct Carver
+{
+    char* base;
+    size_t off;
+    explicit Carver(void* p) : base((char*)p), off(0) {}
+    template <typename T>
+    T* take(size_t count)
+    {
+        off = align_up(off, 256);
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += sizeof(T) * count;
+        return p;
+    }
+    size_t size() const { return align_up(off, 256); }
+};
+
+constexpr int SCAN_BLOCK = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+// Geometry state (one per preprocess call): per-voxel results + scan scratch.
+struct GeomState
+{
+    uint32_t* n_dup;       // [N]
+    uint32_t* scan;        // [N] inclusive prefix sum of n_dup
+    uint2* bbox;           // [N] packed pixel bbox (x<<16 | y) min / max
+    uint32_t* qbits;       // [N] camera-quadrant bitset
+    uint32_t* block_sums;  // [ceil(N / SCAN_TILE) + 1]
+    uint32_t* counters;    // [4]: 0 = R (total entries), 1 = large-voxel queue length
+    uint32_t* big_list;    // [N] voxels with many entries, expanded by whole workgroups
+    static GeomState carve(void* p, int64_t N, size_t* bytes)
+    {
+        Carver c(p);
+        GeomState g;
+        g.n_dup = c.take<uint32_t>(N);
+        g.scan = c.take<uint32_t>(N);
+        g.bbox = c.take<uint2>(N);
+        g.qbits = c.take<uint32_t>(N);
+        g.block_sums = c.take<uint32_t>((N + SCAN_TILE - 1) / SCAN_TILE + 1);
+        g.counters = c.take<uint32_t>(4);
+        g.big_list = c.take<uint32_t>(N);
+        if (bytes) *bytes = c.size();
+        return g;
+    }
+};
+
+// Per-octree order table: the rank-order position of every (voxel, quadrant) pair.
+struct OrderTable
+{
+    uint32_t* pos;    // [8N] pair (v*8+q) -> position in the global rank order
+    uint32_t* pair;   // [8N] position -> pair
+    uint32_t* meta;   // [4]: 0 = pos_bits
+    static OrderTable carve(void* p, int64_t N, size_t* bytes)
+    {
+        Carver c(p);
+        OrderTable t;
+        t.pos = c.take<uint32_t>(8 * N);
+        t.pair = c.take<uint32_t>(8 * N);
+        t.meta = c.take<uint32_t>(4);
+        if (bytes) *bytes = c.size();
+        return t;
+    }
+};
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+
+inline int64_t sort_blocks(int64_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+
+// Per-frame binning state: sort keys (double buffer), the sorted entry list.
+struct BinningState
+{
+    uint64_t* keys;     // [R] sorted (tile << pos_bits | pos)
+    uint64_t* keys_alt; // [R]
+    uint32_t* vox_list; // [R] (q << 29) | vox_id in sorted order
+    uint32_t* counts;   // [RADIX * blocks + RADIX] radix-sort digit counts
+    static BinningState carve(void* p, int64_t R, size_t* bytes)
+    {
+        Carver c(p);
+        BinningState b;
+        const int64_t nb = sort_blocks(R > 0 ? R : 1);
+        b.keys = c.take<uint64_t>(R);
+        b.keys_alt = c.take<uint64_t>(R);
+        b.vox_list = c.take<uint32_t>(R);
+        b.counts = c.take<uint32_t>(RADIX * nb + RADIX);
+        if (bytes) *bytes = c.size();
+        return b;
+    }
+};
+
+// Per-frame image state (backends/new_cuda/src/raster_state.h:61-68).
+struct ImageState
+{
+    uint2* ranges;       // [T] [start, end) of each tile's entries
+    uint32_t* tile_last; // [T] range.x + max n_contrib over the tile
+    uint32_t* n_contrib; // [H*W]
+    static ImageState carve(void* p, int64_t P, int64_t T, size_t* bytes)
+    {
+        Carver c(p);
+        ImageState s;
+        s.ranges = c.take<uint2>(T);
+        s.tile_last = c.take<uint32_t>(T);
+        s.n_contrib = c.take<uint32_t>(P);
+        if (bytes) *bytes = c.size();
+        return s;
+    }
+};
+
+inline int tiles_x(int W) { return (W + 15) / 16; }
+inline int tiles_y(int H) { return (H + 15) / 16; }
+
+// Stable LSD radix sort of 64-bit keys (optionally with 32-bit values) over
+// bits [0, nbits).  The sorted data ends in the buffers returned through
+// out_keys / out_vals; counts needs RADIX * sort_blocks(n) + RADIX words.
+int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
+                  uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals);
+
+// Key duplication, sort, tile ranges and the sorted entry list of one frame.
+int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
+                    BinningState& b, ImageState& img, hipStream_t stream);
+
+int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
+
+}  // namespace svr
