@@ -32,11 +32,13 @@ constexpr int MAX_TILE_GRID_Y = (MAX_PACKED_IMAGE_DIM + TILE_Y) / TILE_Y;
 constexpr int64_t MAX_RENDER_TILES = (int64_t)MAX_TILE_GRID_X * MAX_TILE_GRID_Y;
 constexpr uint32_t VOX_ID_MASK = (1u << 29) - 1u;
 
-__host__ __device__ inline uint64_t order_table(uint32_t q)
+__host__ __device__ constexpr uint64_t order_table(uint32_t q)
 {
-    // q repeated in all 21 three-bit groups (auxiliary.h:33-55)
-    return (uint64_t)q * 0x0249249249249249ull;
+    // q repeated in all 21 three-bit groups (auxiliary.h:33-55): q * sum_{i<21} 8^i
+    return (uint64_t)q * 0x1249249249249249ull;
 }
+static_assert(order_table(7) == 0x7FFFFFFFFFFFFFFFull, "order table must cover all 21 levels");
+static_assert(order_table(1) == ((1ull << 63) - 1ull) / 7ull, "order table must repeat q 21 times");
 
 __host__ __device__ inline uint32_t required_bits(uint64_t n)
 {

@@ -164,15 +164,17 @@ def make_params(n, seed, is_shell=None, sh_degree=3):
     return dict(geos=geos, sh0=sh0, shs=shs)
 
 
-def make_camera(W, H, seed, radius=3.0, fovx_deg=50.0, near=0.02):
-    """Pinhole camera on a sphere looking at the origin; matrices float32 row-major 4x4."""
+def make_camera(W, H, seed, radius=3.0, fovx_deg=50.0, near=0.02, direction=None):
+    """Pinhole camera on a sphere looking at the origin; matrices float32 row-major 4x4.
+    `direction` (unit vector from the origin to the camera) overrides the seeded draw."""
     rng = np.random.default_rng(seed + 104729)
-    while True:
+    while direction is None:
         d = rng.normal(size=3)
         d /= np.linalg.norm(d)
         if abs(d[1]) < 0.95:
-            break
-    pos = radius * d
+            direction = d
+    d = np.asarray(direction, np.float64)
+    pos = radius * d / np.linalg.norm(d)
     lookat = -pos / np.linalg.norm(pos)
     down = np.array([0.0, 1.0, 0.0])
     right = np.cross(down, lookat)

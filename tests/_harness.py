@@ -5,11 +5,11 @@ import numpy as np
 from sv_raster_amd import synthetic as S
 
 
-def make_case(name, seed=0, n=None, view=0, cfg=None):
+def make_case(name, seed=0, n=None, view=0, cfg=None, direction=None):
     scene = S.make_scene(cfg if cfg is not None else name, seed=seed, n=n)
     views = S.config_views(name, seed) if cfg is None else cfg["views"]
     W, H = views[view]
-    cam = S.make_camera(W, H, seed=seed + view)
+    cam = S.make_camera(W, H, seed=seed + view, direction=direction)
     rng = np.random.default_rng(seed + 99)
     dL_dcolor = (rng.normal(0, 1, (3, H, W)) / (H * W)).astype(np.float32)
     dL_dT = (rng.normal(0, 1, (1, H, W)) / (H * W)).astype(np.float32)
@@ -93,4 +93,8 @@ def rel_err(a, b, floor_frac=1e-3):
     scale = np.abs(b).max() if b.size else 0.0
     if scale == 0.0:
         return float(np.abs(a).max()) if a.size else 0.0
-    return float((np.abs(a - b) / (np.abs(b) + floor_frac * scale)).max())
+    diff = np.abs(a - b)
+    den = np.abs(b) + floor_frac * scale
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(diff == 0, 0.0, diff / den)
+    return float(r.max())

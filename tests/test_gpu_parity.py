@@ -51,6 +51,20 @@ def test_parity_small(name, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name,direction", [("C", (0.02, 0.03, 1.0)), ("C", (1.0, -0.01, 0.02)),
+                                            ("A", (0.01, -0.02, -1.0))])
+def test_parity_multi_quadrant(name, direction):
+    """Camera close to a coordinate axis: the view straddles axis planes, so pixels
+    of one tile have different ray quadrants and tile lists interleave quadrants."""
+    case = make_case(name, seed=3, direction=direction)
+    o = run_oracle(case)
+    assert np.unique(o["vox_list"] >> 29).size >= 4
+    g = run_gpu(case)
+    _check_forward(g, o)
+    _check_backward(g, o)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n_samp", [2, 3])
 def test_parity_nsamp(n_samp):
     case = make_case("C", seed=5)
