@@ -1,0 +1,242 @@
+"""Benchmark of the sparse-voxel rasterizer training step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+
+Workload (the 1M-voxel 1080p training configuration): an octree of 1,000,000
+voxels (inner levels 9-17 grown on 8 random sphere surfaces, plus the
+reference's 5-level outside shell), one 1920x1080 pinhole camera (fovx 50 deg)
+on a radius-3 sphere, per-voxel FP32 corner densities and degree-3 SH.  One
+step is the reference's training step through the drop-in API:
+rasterize_voxels (preprocess, SH colour, binning, compositor) -> MSE loss ->
+backward (compositor adjoint, SH backward) -> SparseAdam (sparse=True,
+lr 0.025 density / 0.01 SH, betas (0.1, 0.99)).
+
+Prints one JSON line: steps/s, forward Mpix/s, per-stage kernel times measured
+with HIP events recorded on the launch stream inside the timed region, the
+roofline of the dominant kernel and the oracle (CPU restatement) timed on the
+host cores on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+STAGES = ["preprocess", "scan", "binning", "render_fwd", "render_bwd", "sh_fwd", "sh_bwd", "adam"]
+METRIC = "train steps/s (fwd+bwd+Adam) & fwd Mpix/s, 1M voxels 1080p; batched vs plain"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="B")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--fwd-iters", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
+    return ap.parse_args()
+
+
+def collect_stages(L):
+    out = {}
+    for i, name in enumerate(STAGES):
+        tot, cnt = ctypes.c_double(0), ctypes.c_int64(0)
+        rc = L.svr_profile_collect(i, ctypes.byref(tot), ctypes.byref(cnt))
+        if rc != 0:
+            raise RuntimeError("svr_profile_collect failed")
+        out[name] = (float(tot.value), int(cnt.value))
+    return out
+
+
+def cpu_baseline(case, steps):
+    """The oracle's training step (preprocess, SH, binning, compositor fwd/bwd, SH bwd, Adam) on host cores."""
+    from oracle import oracle as O
+    sc, cam = case["scene"], case["cam"]
+    cam_pos = cam["c2w"][:3, 3]
+    geos, sh0, shs = sc["geos"].copy(), sc["sh0"].copy(), sc["shs"].copy()
+    state = {k: (np.zeros_like(v), np.zeros_like(v)) for k, v in (("g", geos), ("s0", sh0), ("s", shs))}
+    target = case["target"]
+    H, W = cam["H"], cam["W"]
+
+    def one(step):
+        n_dup, bbox, qbits = O.preprocess(cam, sc["vox_centers"], sc["vox_lengths"])
+        idx = np.nonzero(n_dup > 0)[0]
+        rgbs = O.sh_fwd(3, idx, sc["vox_centers"], cam_pos, sh0, shs)
+        _, _, vl, ranges = O.binning(cam, sc["octpath"], n_dup, bbox, qbits)
+        color, T, nc, tl = O.render_fwd(cam, ranges, vl, bbox, sc["vox_centers"], sc["vox_lengths"], geos, rgbs)
+        dcol = (2.0 / color.size) * (color - target)
+        dg, dr, _ = O.render_bwd(cam, ranges, vl, bbox, sc["vox_centers"], sc["vox_lengths"], geos, rgbs, T, nc, tl,
+                                 dcol.astype(np.float32), np.zeros((H, W), np.float32))
+        d0, ds = O.sh_bwd(3, 16, idx, sc["vox_centers"], cam_pos, dr.astype(np.float32))
+        for key, p, g, lr in (("g", geos, dg.astype(np.float32), 0.025), ("s0", sh0, d0, 0.01), ("s", shs, ds, 0.01)):
+            m, v = state[key]
+            O.adam_step(p, np.ascontiguousarray(g), m, v, step, lr, 0.1, 0.99, 1e-15, sparse=True)
+
+    one(1)  # warm-up (page faults, thread pool)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        one(s + 2)
+    dt = time.perf_counter() - t0
+    return dict(value=steps / dt, unit="steps/s", cores=O.num_threads(), kind="port",
+                sample=f"{steps} full training steps of the same 1M-voxel 1920x1080 workload "
+                       f"(oracle C++ restatement, OpenMP, {O.num_threads()} threads, after 1 warm-up step)",
+                sec_per_step=dt / steps)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sv_raster_amd import _C, renderer, sparse_adam, synthetic as S
+    from sv_raster_amd import _lib
+
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    cfg = S.CONFIGS[args.config]
+    sc = S.make_scene(args.config, seed=args.seed)
+    W, H = S.config_views(args.config, args.seed)[0]
+    cam = S.make_camera(W, H, seed=args.seed)
+    target_np = S.make_target(H, W, args.seed)
+    case = dict(scene=sc, cam=cam, target=target_np)
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
+    geos = t(sc["geos"]).requires_grad_()
+    sh0 = t(sc["sh0"]).requires_grad_()
+    shs = t(sc["shs"]).requires_grad_()
+    subdiv = torch.ones((centers.shape[0], 1), device=dev)
+    w2c, c2w = t(cam["w2c"]), t(cam["c2w"])
+    target = t(target_np)
+    N = centers.shape[0]
+
+    def vox_fn(idx, cam_pos, color_mode):
+        rgbs = renderer.SH_eval.apply(3, idx, centers, cam_pos, None, sh0, shs)
+        return {"geos": geos, "rgbs": rgbs, "subdiv_p": subdiv}
+
+    settings = renderer.RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=W, image_height=H,
+                                       tanfovx=cam["tanfovx"], tanfovy=cam["tanfovy"], cx=cam["cx"], cy=cam["cy"],
+                                       w2c_matrix=w2c, c2w_matrix=c2w)
+    opt = sparse_adam.SparseAdam([{"params": [geos], "lr": 0.025}, {"params": [sh0], "lr": 0.01},
+                                  {"params": [shs], "lr": 0.01}], betas=(0.1, 0.99), eps=1e-15, sparse=True)
+
+    def step():
+        color, _, _, _, _ = renderer.rasterize_voxels(settings, octpath, centers, lengths, vox_fn)
+        loss = torch.nn.functional.mse_loss(color, target)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return loss
+
+    # one-time: rank-order table of this octree (reused while the octree is unchanged)
+    torch.cuda.synchronize()
+    t_tab = time.perf_counter()
+    _C.order_table_cache.get(octpath)
+    torch.cuda.synchronize()
+    order_table_ms = (time.perf_counter() - t_tab) * 1e3
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    _lib.check(L.svr_profile_enable(1), "profile")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = collect_stages(L)
+    _lib.check(L.svr_profile_enable(0), "profile")
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # forward only (the FPS path): preprocess + SH + binning + compositor
+    with torch.no_grad():
+        for _ in range(3):
+            renderer.rasterize_voxels(settings, octpath, centers, lengths, vox_fn)
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        for _ in range(args.fwd_iters):
+            renderer.rasterize_voxels(settings, octpath, centers, lengths, vox_fn)
+        torch.cuda.synchronize()
+        fwd_ms = (time.perf_counter() - f0) * 1e3 / args.fwd_iters
+
+    # algorithmic traffic of the compositor kernels on this frame
+    n_dup, geom = _C.rasterize_preprocess(W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w,
+                                          cam["near"], octpath, centers, lengths)
+    with torch.no_grad():
+        rgbs = _C.sh_compute(3, torch.where(n_dup > 0)[0], centers, c2w[:3, 3], sh0, shs)
+        R, binning, img, color, _, _, T_img, _ = _C.rasterize_voxels(
+            1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w, 0.0, False, False, False, False,
+            octpath, centers, lengths, geos, rgbs, geom)
+        ranges, tile_last, _ = _C.unpack_ImageState(W, H, img)
+        E = int((tile_last.reshape(-1).long() - ranges.reshape(-1, 2)[:, 0].long()).sum().item())
+        dcol = 2.0 * (color - target) / color.numel()
+        dg, _, _ = _C.rasterize_voxels_backward(R, 1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c,
+                                               c2w, 0.0, octpath, centers, lengths, geos, rgbs, geom, binning, img,
+                                               T_img, dcol, None, None, torch.zeros_like(T_img), 0.0, None, 0.0, 0.0,
+                                               False, False, None, None)
+        N_h = int((dg != 0).any(1).sum().item())
+    P = W * H
+    T_tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    bytes_fwd = 72 * E + 20 * P + 12 * T_tiles
+    bytes_bwd = 72 * E + 24 * P + 12 * T_tiles + 48 * N_h
+
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items()}
+    dom = "render_bwd" if stage_ms["render_bwd"] >= stage_ms["render_fwd"] else "render_fwd"
+    dom_bytes = bytes_bwd if dom == "render_bwd" else bytes_fwd
+    achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(case, args.cpu_steps)
+
+    result = {
+        "metric": METRIC,
+        "value": 1e3 / ms_per_step,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded octree, camera and target image; random-init parameters)",
+        "config": {"workload": f"{args.config}: {N} voxels (levels {cfg['lv_min'] + cfg['outside']}-"
+                               f"{cfg['lv_max'] + cfg['outside']}, outside shell {cfg['outside']} levels), "
+                               f"{W}x{H}, plain single view, fwd+bwd+sparse Adam",
+                   "voxels": N, "width": W, "height": H, "entries_R": R, "entries_traversed_E": E,
+                   "voxels_hit": N_h, "views": 1, "path": "plain"},
+        "fwd_ms": fwd_ms,
+        "fwd_mpix_s": P / (fwd_ms * 1e-3) / 1e6,
+        "loss_last": float(loss.item()),
+        "order_table_ms_once_per_octree": order_table_ms,
+        "stages_ms_per_launch": stage_ms,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes,
+                     "avg_launch_ms": stage_ms[dom],
+                     "other_compositor_gbs": (bytes_fwd if dom == "render_bwd" else bytes_bwd) /
+                                             (stage_ms["render_fwd" if dom == "render_bwd" else "render_bwd"]
+                                              * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

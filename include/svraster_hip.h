@@ -150,6 +150,16 @@ int svr_ijk_2_octpath(int64_t n, const int64_t* ijk, const int8_t* octlevel, int
 int svr_octpath_2_ijk(int64_t n, const int64_t* octpath, const int8_t* octlevel, int64_t* ijk,
                       svr_stream_t stream);
 
+/* Live stage timing (no reference counterpart; measurement support).  While
+ * enabled, each entry point records a HIP event pair on its stream around the
+ * kernels of its stage: 0 preprocess, 1 scan, 2 binning (duplicate + sort +
+ * ranges), 3 forward compositor kernel, 4 backward compositor kernel, 5 SH
+ * forward, 6 SH backward, 7 Adam.  Enabling resets the records. */
+int svr_profile_enable(int32_t on);
+/* Synchronises on the recorded events of `stage`, returns their summed
+ * duration and count, and clears them. */
+int svr_profile_collect(int32_t stage, double* total_ms_host, int64_t* count_host);
+
 /* Waits for `stream` and reports any pending device error (the reference's
  * CHECK_CUDA(debug), auxiliary.h:394-401). */
 int svr_debug_sync(svr_stream_t stream);

@@ -6,7 +6,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsvraster_hip.so")
+LIB_PATH = os.environ.get("SVR_LIB_PATH") or os.path.join(_HERE, "libsvraster_hip.so")
 
 c_i32, c_i64, c_f32, c_f64, c_p, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double,
                                           ctypes.c_void_p, ctypes.c_size_t)
@@ -49,6 +49,8 @@ SIGNATURES = {
     "svr_ijk_2_octpath": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_octpath_2_ijk": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_debug_sync": (c_i32, [c_p]),
+    "svr_profile_enable": (c_i32, [c_i32]),
+    "svr_profile_collect": (c_i32, [c_i32, ctypes.POINTER(c_f64), ctypes.POINTER(c_i64)]),
 }
 
 _lib = None

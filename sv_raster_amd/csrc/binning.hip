@@ -390,8 +390,13 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
 {
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int64_t T = (int64_t)gx * gy;
+    prof_begin(PROF_BINNING, stream);
     SVR_REQUIRE(fill_u32(img.ranges, 0u, 2 * T, stream) == SVR_OK, SVR_ERR_HIP, "fill ranges failed");
-    if (R == 0) return SVR_OK;
+    if (R == 0)
+    {
+        prof_end(PROF_BINNING, stream);
+        return SVR_OK;
+    }
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
     SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
@@ -413,6 +418,7 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(b.keys, R, pos_bits, ot.pair, img.ranges,
                                                                    b.vox_list);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_BINNING, stream);
     return SVR_OK;
 }
 

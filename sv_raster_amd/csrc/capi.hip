@@ -1,6 +1,7 @@
 // C-ABI plumbing: error strings, constants, fills, debug sync, state sizes.
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include "svr_device.h"
 #include "svr_host.h"
@@ -33,6 +34,49 @@ int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream)
     fill_u32_kernel<<<(unsigned)blocks, 256, 0, stream>>>((uint32_t*)dst, value, count);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
+}
+
+struct ProfPair
+{
+    hipEvent_t start, stop;
+};
+
+static bool g_prof_on = false;
+static ProfPair* g_prof[PROF_NUM_STAGES] = {nullptr};
+static int g_prof_cap[PROF_NUM_STAGES] = {0};
+static int g_prof_n[PROF_NUM_STAGES] = {0};
+
+void prof_begin(int stage, hipStream_t stream)
+{
+    if (!g_prof_on) return;
+    if (g_prof_n[stage] == g_prof_cap[stage])
+    {
+        const int cap = g_prof_cap[stage] ? 2 * g_prof_cap[stage] : 64;
+        ProfPair* np = (ProfPair*)realloc(g_prof[stage], sizeof(ProfPair) * cap);
+        if (!np)
+        {
+            g_prof_on = false;
+            return;
+        }
+        g_prof[stage] = np;
+        for (int i = g_prof_cap[stage]; i < cap; ++i)
+        {
+            if (hipEventCreate(&np[i].start) != hipSuccess || hipEventCreate(&np[i].stop) != hipSuccess)
+            {
+                g_prof_on = false;
+                return;
+            }
+            g_prof_cap[stage] = i + 1;
+        }
+    }
+    (void)hipEventRecord(g_prof[stage][g_prof_n[stage]].start, stream);
+}
+
+void prof_end(int stage, hipStream_t stream)
+{
+    if (!g_prof_on || g_prof_n[stage] >= g_prof_cap[stage]) return;
+    (void)hipEventRecord(g_prof[stage][g_prof_n[stage]].stop, stream);
+    ++g_prof_n[stage];
 }
 
 }  // namespace svr
@@ -84,6 +128,31 @@ size_t svr_image_state_size(int32_t W, int32_t H)
     const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
     ImageState::carve(nullptr, (int64_t)W * H, T, &bytes);
     return bytes;
+}
+
+int svr_profile_enable(int32_t on)
+{
+    g_prof_on = on != 0;
+    for (int i = 0; i < PROF_NUM_STAGES; ++i) g_prof_n[i] = 0;
+    return SVR_OK;
+}
+
+int svr_profile_collect(int32_t stage, double* total_ms_host, int64_t* count_host)
+{
+    SVR_REQUIRE(stage >= 0 && stage < PROF_NUM_STAGES && total_ms_host && count_host, SVR_ERR_INVALID,
+                "svr_profile_collect: bad stage or null output");
+    double total = 0.0;
+    for (int i = 0; i < g_prof_n[stage]; ++i)
+    {
+        SVR_CHECK_HIP(hipEventSynchronize(g_prof[stage][i].stop));
+        float ms = 0.f;
+        SVR_CHECK_HIP(hipEventElapsedTime(&ms, g_prof[stage][i].start, g_prof[stage][i].stop));
+        total += ms;
+    }
+    *total_ms_host = total;
+    *count_host = g_prof_n[stage];
+    g_prof_n[stage] = 0;
+    return SVR_OK;
 }
 
 int svr_debug_sync(svr_stream_t stream)

@@ -205,10 +205,12 @@ int svr_rasterize_preprocess(int32_t W, int32_t H, float tanfovx, float tanfovy,
     const float focal_x = 0.5f * (float)W / tanfovx;
     const float focal_y = 0.5f * (float)H / tanfovy;
     const int64_t blocks = (N + 255) / 256;
+    prof_begin(PROF_PREPROCESS, stream);
     preprocess_kernel<<<(unsigned)blocks, 256, 0, stream>>>(N, W, H, tiles_x(W), tiles_y(H), focal_x, focal_y, cx,
                                                             cy, w2c, c2w, near_plane, centers, lengths, out_n_dup,
                                                             g.n_dup, g.bbox, g.qbits);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_PREPROCESS, stream);
     return SVR_OK;
 }
 
@@ -219,12 +221,14 @@ int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* 
     if (N == 0) return SVR_OK;
     GeomState g = GeomState::carve(geom_state, N, nullptr);
     const int64_t nb = (N + SCAN_TILE - 1) / SCAN_TILE;
+    prof_begin(PROF_SCAN, stream);
     scan_reduce_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums);
     SVR_CHECK_LAUNCH();
     scan_sums_kernel<<<1, SCAN_BLOCK, 0, stream>>>(g.block_sums, nb, g.counters);
     SVR_CHECK_LAUNCH();
     scan_apply_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums, g.scan);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_SCAN, stream);
     uint32_t cnt[2] = {0, 0};
     SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
     SVR_CHECK_HIP(hipStreamSynchronize(stream));

@@ -25,6 +25,9 @@
 namespace svr {
 
 constexpr int NWAVE = 4;
+#ifndef BWD_SYNC_MIN
+#define BWD_SYNC_MIN 24
+#endif
 
 struct Stage
 {
@@ -64,6 +67,48 @@ __device__ inline uint32_t wave_max_u32(uint32_t x)
     return x;
 }
 
+// One list entry as loaded by a lane: voxel id, pixel mask on the wave's
+// sub-tile and the render payload, kept in registers until staged in LDS.
+struct EntryRegs
+{
+    uint32_t v;
+    uint64_t pm;
+    float c0, c1, c2, l;
+    float4 g0, g1;
+    float r0, r1, r2;
+};
+
+__device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes,
+                                  const float* __restrict__ centers, const float* __restrict__ lengths,
+                                  const float* __restrict__ geos, const float* __restrict__ rgbs, int sx0, int sy0,
+                                  int sx1, int sy1, const uint64_t* qpix)
+{
+    e.pm = 0ull;
+    e.v = val & VOX_ID_MASK;
+    if (!valid) return;
+    e.pm = bbox_pixel_mask(bboxes[e.v], sx0, sy0, sx1, sy1, select_q(qpix, val >> 29));
+    if (!e.pm) return;
+    const size_t v = e.v;
+    e.c0 = centers[3 * v + 0];
+    e.c1 = centers[3 * v + 1];
+    e.c2 = centers[3 * v + 2];
+    e.l = lengths[v];
+    e.g0 = *reinterpret_cast<const float4*>(geos + 8 * v);
+    e.g1 = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
+    e.r0 = rgbs[3 * v + 0];
+    e.r1 = rgbs[3 * v + 1];
+    e.r2 = rgbs[3 * v + 2];
+}
+
+__device__ inline void stage_entry(Stage& st, int w, int lane, const EntryRegs& e)
+{
+    if (!e.pm) return;
+    st.c[0][w][lane] = e.c0; st.c[1][w][lane] = e.c1; st.c[2][w][lane] = e.c2; st.l[w][lane] = e.l;
+    st.g[0][w][lane] = e.g0.x; st.g[1][w][lane] = e.g0.y; st.g[2][w][lane] = e.g0.z; st.g[3][w][lane] = e.g0.w;
+    st.g[4][w][lane] = e.g1.x; st.g[5][w][lane] = e.g1.y; st.g[6][w][lane] = e.g1.z; st.g[7][w][lane] = e.g1.w;
+    st.rgb[0][w][lane] = e.r0; st.rgb[1][w][lane] = e.r1; st.rgb[2][w][lane] = e.r2;
+}
+
 template <int NS>
 __global__ __launch_bounds__(256) void render_fwd_kernel(
     int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
@@ -89,37 +134,39 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
     uint32_t last = 0;
     bool done = !inside;
 
-    for (uint32_t base = range.x; base < range.y; base += 64)
-    {
-        uint64_t qpix[8];
+    // The next chunk is loaded (into registers) while the current one is
+    // composited; its pixel masks use the quadrant masks of the lanes active
+    // at load time, and lanes that finish meanwhile drop their candidates.
+    uint64_t qpix[8];
 #pragma unroll
-        for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
-        if (__ballot(!done) == 0ull) break;
-
+    for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
+    EntryRegs cur;
+    uint32_t base = range.x;
+    {
         const uint32_t j = base + (uint32_t)lane;
-        uint64_t pm = 0ull;
-        if (j < range.y)
-        {
-            const uint32_t val = vox_list[j];
-            const uint32_t v = val & VOX_ID_MASK, q = val >> 29;
-            pm = bbox_pixel_mask(bboxes[v], sx0, sy0, sx1, sy1, select_q(qpix, q));
-            if (pm)
-            {
-                st.c[0][w][lane] = centers[3 * (size_t)v + 0];
-                st.c[1][w][lane] = centers[3 * (size_t)v + 1];
-                st.c[2][w][lane] = centers[3 * (size_t)v + 2];
-                st.l[w][lane] = lengths[v];
-                const float4 g0 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v);
-                const float4 g1 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v + 4);
-                st.g[0][w][lane] = g0.x; st.g[1][w][lane] = g0.y; st.g[2][w][lane] = g0.z; st.g[3][w][lane] = g0.w;
-                st.g[4][w][lane] = g1.x; st.g[5][w][lane] = g1.y; st.g[6][w][lane] = g1.z; st.g[7][w][lane] = g1.w;
-                st.rgb[0][w][lane] = rgbs[3 * (size_t)v + 0];
-                st.rgb[1][w][lane] = rgbs[3 * (size_t)v + 1];
-                st.rgb[2][w][lane] = rgbs[3 * (size_t)v + 2];
-            }
-        }
+        const bool valid = j < range.y;
+        load_entry(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
+                   qpix);
+    }
+    while (base < range.y)
+    {
         wave_sync();
-        uint64_t hm = wave_transpose64(pm, lane);
+        stage_entry(st, w, lane, cur);
+        wave_sync();
+        uint64_t hm = wave_transpose64(cur.pm, lane);
+        if (done) hm = 0ull;
+
+        // prefetch the next chunk
+        const uint32_t nbase = base + 64;
+        if (nbase < range.y)
+        {
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
+            const uint32_t j = nbase + (uint32_t)lane;
+            const bool valid = j < range.y;
+            load_entry(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
+                       sy1, qpix);
+        }
 
         while (hm)
         {
@@ -147,7 +194,8 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
                 break;
             }
         }
-        wave_sync();
+        base = nbase;
+        if (__ballot(!done) == 0ull) break;
     }
 
     if (inside)
@@ -167,7 +215,52 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
         tile_last[tile] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
 }
 
+// Adjoint of one (pixel, entry) hit (backward.cu:259-527 without the
+// regularizers): recovers T before the entry, updates the running dL/dT and
+// writes the 12 gradient terms (8 densities, 3 colours, subdivision priority).
 template <int NS>
+__device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray, float& T, float& last_dT, float dp0,
+                               float dp1, float dp2, float* gq)
+{
+    const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
+    float a, b;
+    ray_aabb(c0, c1, c2, len, ray, a, b);
+    if (a > b) return false;
+    float g[8], dI[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
+    const float vol = vox_vol<NS, true>(c0, c1, c2, len, g, a, b, ray, dI);
+    const float e = svr_expf(-vol);
+    const float alpha = fminf(MAX_ALPHA, 1.f - e);
+    if (alpha < MIN_ALPHA) return false;
+    T = T / (1.f - alpha);
+    const float pt_w = alpha * T;
+    const float r0 = st.rgb[0][w][l], r1 = st.rgb[1][w][l], r2 = st.rgb[2][w][l];
+    const float dpt = __builtin_fmaf(dp2, r2, __builtin_fmaf(dp1, r1, dp0 * r0));
+    const float diff = dpt - last_dT;
+    const float dalpha = T * diff;
+    last_dT = __builtin_fmaf(alpha, diff, last_dT);
+    const float dIs = dalpha * e;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) gq[i] = dIs * dI[i];
+    gq[8] = pt_w * dp0;
+    gq[9] = pt_w * dp1;
+    gq[10] = pt_w * dp2;
+    gq[11] = fabsf(dalpha * alpha);
+    return true;
+}
+
+// Entries whose bbox covers at least SYNC_MIN pixels of the wave (coarse
+// voxels, the outside shell) are "sync slots": every pixel processes them at
+// the same step, at their place in its own order, and their gradient is summed
+// across the wave with DPP instead of up to 64-way conflicting LDS atomics.
+// Between two sync slots each lane walks its own remaining hits in lock-step
+// and accumulates with ds_add.
+//
+// The chunk loop is software-pipelined: chunk c+1 is loaded into registers
+// before chunk c's gradients are flushed with global atomics, because on this
+// architecture a later load's wait also waits for all earlier stores/atomics.
+template <int NS, int SYNC_MIN>
 __global__ __launch_bounds__(256) void render_bwd_kernel(
     int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
@@ -207,94 +300,100 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     uint64_t qpix[8];
 #pragma unroll
     for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(inside && nc > 0 && ray.pq == q);
-    const int64_t wave_end = (int64_t)range.x + wave_max_u32(nc);
+    const int64_t rx = (int64_t)range.x;
+    int64_t end = rx + wave_max_u32(nc);
 
-    for (int64_t end = wave_end; end > (int64_t)range.x; end -= 64)
+    EntryRegs cur;
     {
         const int64_t jl = end - 1 - lane;
-        uint64_t pm = 0ull;
-        uint32_t v = 0;
-        if (jl >= (int64_t)range.x)
-        {
-            const uint32_t val = vox_list[jl];
-            v = val & VOX_ID_MASK;
-            const uint32_t q = val >> 29;
-            pm = bbox_pixel_mask(bboxes[v], sx0, sy0, sx1, sy1, select_q(qpix, q));
-            if (pm)
-            {
-                st.c[0][w][lane] = centers[3 * (size_t)v + 0];
-                st.c[1][w][lane] = centers[3 * (size_t)v + 1];
-                st.c[2][w][lane] = centers[3 * (size_t)v + 2];
-                st.l[w][lane] = lengths[v];
-                const float4 g0 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v);
-                const float4 g1 = *reinterpret_cast<const float4*>(geos + 8 * (size_t)v + 4);
-                st.g[0][w][lane] = g0.x; st.g[1][w][lane] = g0.y; st.g[2][w][lane] = g0.z; st.g[3][w][lane] = g0.w;
-                st.g[4][w][lane] = g1.x; st.g[5][w][lane] = g1.y; st.g[6][w][lane] = g1.z; st.g[7][w][lane] = g1.w;
-                st.rgb[0][w][lane] = rgbs[3 * (size_t)v + 0];
-                st.rgb[1][w][lane] = rgbs[3 * (size_t)v + 1];
-                st.rgb[2][w][lane] = rgbs[3 * (size_t)v + 2];
-            }
-        }
+        const bool valid = jl >= rx;
+        load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
+                   qpix);
+    }
+    while (end > rx)
+    {
         wave_sync();
-        uint64_t hm = wave_transpose64(pm, lane);
+        stage_entry(st, w, lane, cur);
+        uint64_t sync = __ballot(__popcll(cur.pm) >= SYNC_MIN);
+        wave_sync();
+        uint64_t hm = wave_transpose64(cur.pm, lane);
         // entry end-1-l is behind this pixel's last contributor when
         // end-1-l >= range.x + nc, i.e. l < end - range.x - nc
-        const int64_t cut = end - (int64_t)range.x - (int64_t)nc;
+        const int64_t cut = end - rx - (int64_t)nc;
         if (cut >= 64) hm = 0ull;
         else if (cut > 0) hm &= ~((1ull << cut) - 1ull);
 
-        while (hm)
+        while (true)
         {
-            const int l = __builtin_ctzll(hm);
-            hm &= hm - 1ull;
-            const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
-            float a, b;
-            ray_aabb(c0, c1, c2, len, ray, a, b);
-            if (a > b) continue;
-            float g[8], dI[8];
+            const int s = sync ? __builtin_ctzll(sync) : 64;
+            uint64_t seg = (s == 64) ? hm : (hm & ((1ull << s) - 1ull));
+            hm &= ~seg;
+            while (seg)
+            {
+                const int l = __builtin_ctzll(seg);
+                seg &= seg - 1ull;
+                float gq[12];
+                if (bwd_hit<NS>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
+                {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-            const float vol = vox_vol<NS, true>(c0, c1, c2, len, g, a, b, ray, dI);
-            const float e = svr_expf(-vol);
-            const float alpha = fminf(MAX_ALPHA, 1.f - e);
-            if (alpha < MIN_ALPHA) continue;
-            T = T / (1.f - alpha);
-            const float pt_w = alpha * T;
-            const float r0 = st.rgb[0][w][l], r1 = st.rgb[1][w][l], r2 = st.rgb[2][w][l];
-            const float dpt = __builtin_fmaf(dp2, r2, __builtin_fmaf(dp1, r1, dp0 * r0));
-            const float diff = dpt - last_dT;
-            const float dalpha = T * diff;
-            last_dT = __builtin_fmaf(alpha, diff, last_dT);
-            const float dIs = dalpha * e;
+                    for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
+                }
+            }
+            if (s == 64) break;
+            sync &= sync - 1ull;
+            const bool mine = (hm >> s) & 1ull;
+            hm &= ~(1ull << s);
+            float gq[12];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) atomicAdd(&acc[i][w][l], dIs * dI[i]);
-            atomicAdd(&acc[8][w][l], pt_w * dp0);
-            atomicAdd(&acc[9][w][l], pt_w * dp1);
-            atomicAdd(&acc[10][w][l], pt_w * dp2);
-            atomicAdd(&acc[11][w][l], fabsf(dalpha * alpha));
+            for (int i = 0; i < 12; ++i) gq[i] = 0.f;
+            bool hit = false;
+            if (mine) hit = bwd_hit<NS>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
+            if (__ballot(hit) == 0ull) continue;
+            if (!hit)
+            {
+#pragma unroll
+                for (int i = 0; i < 12; ++i) gq[i] = 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < 12; ++i)
+            {
+                const float r = wave_sum_f32(gq[i]);
+                if (lane == 0) acc[i][w][s] += r;
+            }
+        }
+
+        // prefetch the next chunk before this chunk's atomics are issued
+        const uint32_t fv = cur.v;
+        const bool fpm = cur.pm != 0ull;
+        const int64_t nend = end - 64;
+        {
+            const int64_t jl = nend - 1 - lane;
+            const bool valid = jl >= rx;
+            load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
+                       sy1, qpix);
         }
         wave_sync();
-        if (pm)
+        if (fpm)
         {
 #pragma unroll
             for (int i = 0; i < 8; ++i)
             {
                 const float a = acc[i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(dL_dgeos + 8 * (size_t)v + i, a);
+                if (a != 0.f) unsafeAtomicAdd(dL_dgeos + 8 * (size_t)fv + i, a);
                 acc[i][w][lane] = 0.f;
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
             {
                 const float a = acc[8 + i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(dL_drgbs + 3 * (size_t)v + i, a);
+                if (a != 0.f) unsafeAtomicAdd(dL_drgbs + 3 * (size_t)fv + i, a);
                 acc[8 + i][w][lane] = 0.f;
             }
             const float a = acc[11][w][lane];
-            if (a != 0.f) unsafeAtomicAdd(dL_dsub + v, a);
+            if (a != 0.f) unsafeAtomicAdd(dL_dsub + fv, a);
             acc[11][w][lane] = 0.f;
         }
-        wave_sync();
+        end = nend;
     }
 }
 
@@ -307,11 +406,13 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
     const int per = (T + 7) / 8;
+    prof_begin(PROF_RENDER_FWD, stream);
     render_fwd_kernel<NS><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
                                                                    img.ranges, b.vox_list, g.bbox, centers, lengths,
                                                                    geos, rgbs, img.tile_last, img.n_contrib,
                                                                    out_color, out_T);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;
 }
 
@@ -324,11 +425,13 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
     const int per = (T + 7) / 8;
-    render_bwd_kernel<NS><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
+    prof_begin(PROF_RENDER_BWD, stream);
+    render_bwd_kernel<NS, BWD_SYNC_MIN><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
                                                                    img.ranges, b.vox_list, g.bbox, centers, lengths,
                                                                    geos, rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT,
                                                                    dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_RENDER_BWD, stream);
     return SVR_OK;
 }
 

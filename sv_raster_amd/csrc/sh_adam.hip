@@ -96,6 +96,91 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int deg, int64_t cnt, const
     }
 }
 
+// Coalesced variants for the full degree-3 layout (M = 16): one wave stages
+// the 48 coefficients (sh0 + shs) of its 64 voxels through LDS with
+// contiguous loads (rows padded to 49 floats so that lane k reading row k is
+// bank-conflict free), then each lane evaluates its own voxel.  Same
+// arithmetic as sh_fwd_kernel / sh_bwd_kernel.
+constexpr int SH_F = 48;
+constexpr int SH_PAD = 49;
+
+__global__ __launch_bounds__(64) void sh_fwd16_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,
+                                                      const float* __restrict__ centers,
+                                                      const float* __restrict__ cam, const float* __restrict__ sh0,
+                                                      const float* __restrict__ shs, float* __restrict__ rgbs)
+{
+    __shared__ float buf[64 * SH_PAD];
+    __shared__ int64_t sv[64];
+    const int lane = threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.x * 64;
+    const int nv = (int)min((int64_t)64, cnt - t0);
+    int64_t myv = 0;
+    if (lane < nv) myv = idx ? idx[t0 + lane] : t0 + lane;
+    sv[lane] = myv;
+    wave_sync();
+    for (int e = lane; e < nv * SH_F; e += 64)
+    {
+        const int k = e / SH_F, f = e - k * SH_F;
+        const int64_t v = sv[k];
+        buf[k * SH_PAD + f] = (f < 3) ? sh0[3 * v + f] : shs[v * 45 + (f - 3)];
+    }
+    wave_sync();
+    if (lane >= nv) return;
+    const int64_t v = myv;
+    float x, y, z, b[16];
+    sh_dir(centers, v, cam, x, y, z);
+    const int nb = sh_basis(deg, x, y, z, b);
+    const float* row = buf + lane * SH_PAD;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+    {
+        float res = SH_C0 * row[ch];
+        if (nb > 1) res = res - (SH_C1 * y) * row[3 + ch] + (SH_C1 * z) * row[6 + ch] - (SH_C1 * x) * row[9 + ch];
+        for (int k = 4; k < nb; ++k) res = res + b[k] * row[3 * k + ch];
+        res = res + 0.5f;
+        rgbs[3 * v + ch] = res > 0.f ? res : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(64) void sh_bwd16_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,
+                                                      const float* __restrict__ centers,
+                                                      const float* __restrict__ cam,
+                                                      const float* __restrict__ dL_drgbs, float* __restrict__ dsh0,
+                                                      float* __restrict__ dshs)
+{
+    __shared__ float buf[64 * SH_PAD];
+    __shared__ int64_t sv[64];
+    const int lane = threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.x * 64;
+    const int nv = (int)min((int64_t)64, cnt - t0);
+    int64_t myv = 0;
+    if (lane < nv) myv = idx ? idx[t0 + lane] : t0 + lane;
+    sv[lane] = myv;
+    if (lane < nv)
+    {
+        float x, y, z, b[16];
+        sh_dir(centers, myv, cam, x, y, z);
+        const int nb = sh_basis(deg, x, y, z, b);
+        float* row = buf + lane * SH_PAD;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            const float g = dL_drgbs[3 * myv + ch];
+            row[ch] = SH_C0 * g;
+            for (int k = 1; k < 16; ++k) row[3 * k + ch] = (k < nb) ? b[k] * g : 0.f;
+        }
+    }
+    wave_sync();
+    for (int e = lane; e < nv * SH_F; e += 64)
+    {
+        const int k = e / SH_F, f = e - k * SH_F;
+        const int64_t v = sv[k];
+        const float val = buf[k * SH_PAD + f];
+        if (f < 3) dsh0[3 * v + f] = val;
+        else dshs[v * 45 + (f - 3)] = val;
+    }
+}
+
 // adam_step.cu:21-64 (unbiased) and 67-102 (biased); lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0))
 template <bool SPARSE, bool BIASED>
 __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
@@ -182,9 +267,15 @@ int svr_sh_compute(int32_t deg, int64_t n_idx, const int64_t* idx, int64_t N, co
     if (fill_u32(rgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
     const int64_t cnt = n_idx ? n_idx : N;
     if (cnt == 0) return SVR_OK;
-    sh_fwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers, cam_pos,
-                                                                 sh0, shs, M, rgbs);
+    prof_begin(PROF_SH_FWD, stream);
+    if (M == 16)
+        sh_fwd16_kernel<<<(unsigned)((cnt + 63) / 64), 64, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers,
+                                                                         cam_pos, sh0, shs, rgbs);
+    else
+        sh_fwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers,
+                                                                     cam_pos, sh0, shs, M, rgbs);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_SH_FWD, stream);
     return SVR_OK;
 }
 
@@ -198,9 +289,15 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
     if (M > 1 && fill_u32(dshs, 0u, 3 * N * (int64_t)(M - 1), stream) != SVR_OK) return SVR_ERR_HIP;
     const int64_t cnt = n_idx ? n_idx : N;
     if (cnt == 0) return SVR_OK;
-    sh_bwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers, cam_pos,
-                                                                 dL_drgbs, M, dsh0, dshs);
+    prof_begin(PROF_SH_BWD, stream);
+    if (M == 16)
+        sh_bwd16_kernel<<<(unsigned)((cnt + 63) / 64), 64, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers,
+                                                                         cam_pos, dL_drgbs, dsh0, dshs);
+    else
+        sh_bwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers,
+                                                                     cam_pos, dL_drgbs, M, dsh0, dshs);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_SH_BWD, stream);
     return SVR_OK;
 }
 
@@ -214,6 +311,7 @@ int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, 
     const float step_size = (float)(lr / bc1);
     const float rb2 = (float)(1.0 / sqrt(bc2));
     const unsigned grid = grid_for(n, 8192);
+    prof_begin(PROF_ADAM, stream);
     if (sparse)
         adam_kernel<true, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2, rb2,
                                                            eps);
@@ -221,6 +319,7 @@ int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, 
         adam_kernel<false, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2,
                                                             rb2, eps);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_ADAM, stream);
     return SVR_OK;
 }
 
@@ -229,11 +328,13 @@ int svr_biased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, fl
 {
     if (n == 0) return SVR_OK;
     const unsigned grid = grid_for(n, 8192);
+    prof_begin(PROF_ADAM, stream);
     if (sparse)
         adam_kernel<true, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps);
     else
         adam_kernel<false, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps);
     SVR_CHECK_LAUNCH();
+    prof_end(PROF_ADAM, stream);
     return SVR_OK;
 }
 

@@ -51,8 +51,12 @@ __host__ __device__ inline uint32_t required_bits(uint64_t n)
 
 __device__ inline float svr_expf(float x)
 {
-    const float kf = __builtin_rintf(x * 1.44269502162933349609375f);
-    float r = __builtin_fmaf(-kf, 0.693145751953125f, x);
+    // Clamping first keeps k in [-150, 129]; inside [-103.97, 88.72] xc == x, so
+    // the result equals the unclamped evaluation there, and outside that range
+    // it is replaced by 0 / +inf below.
+    const float xc = fminf(fmaxf(x, -104.f), 89.f);
+    const float kf = __builtin_rintf(xc * 1.44269502162933349609375f);
+    float r = __builtin_fmaf(-kf, 0.693145751953125f, xc);
     r = __builtin_fmaf(-kf, 1.428606765330187045e-06f, r);
     float p = 1.98412701976485550403594970703125e-04f;
     p = __builtin_fmaf(p, r, 1.388888922519981861114501953125e-03f);
@@ -62,8 +66,7 @@ __device__ inline float svr_expf(float x)
     p = __builtin_fmaf(p, r, 0.5f);
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
-    const float xc = fminf(fmaxf(x, -104.f), 89.f);   // keeps k in [-150, 129]
-    const int k = (int)__builtin_rintf(xc * 1.44269502162933349609375f);
+    const int k = (int)kf;
     const int k1 = k / 2;
     const int k2 = k - k1;
     const float s1 = __uint_as_float((uint32_t)(k1 + 127) << 23);
@@ -168,10 +171,13 @@ __device__ inline float vox_vol(float c0, float c1, float c2, float l, const flo
         float d = geo[0] * w[0];
 #pragma unroll
         for (int i = 1; i < 8; ++i) d = __builtin_fmaf(geo[i], w[i], d);
-        vol = vol + scale * exp_linear_11(d);
+        // exp_linear_11 and its derivative share one exponential
+        const float ex = svr_expf(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
+        const bool lin = d > 1.1f;
+        vol = vol + scale * (lin ? d : ex);
         if (DI)
         {
-            const float dd = scale * exp_linear_11_bw(d);
+            const float dd = scale * (lin ? 1.0f : 0.909090909091f * ex);
 #pragma unroll
             for (int i = 0; i < 8; ++i) dI[i] = __builtin_fmaf(dd, w[i], dI[i]);
         }
@@ -197,6 +203,27 @@ __device__ inline uint64_t wave_transpose64(uint64_t row, int lane)
         row = (lane & s) ? ((row & ~m) | ((other & ~m) >> s)) : ((row & m) | ((other & m) << s));
     }
     return row;
+}
+
+template <int CTRL>
+__device__ inline float dpp_f32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 64 lanes of a wave (all lanes must be active); every lane gets
+// the total.  DPP within each 16-lane row, then the four row sums by readlane.
+__device__ inline float wave_sum_f32(float v)
+{
+    v = v + dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp_f32<0x141>(v);  // row_half_mirror
+    v = v + dpp_f32<0x140>(v);  // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
 }
 
 __device__ inline uint64_t lanemask_lt(int lane) { return (lane == 0) ? 0ull : (~0ull >> (64 - lane)); }

@@ -169,4 +169,11 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
 
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
 
+// Live stage timing: when enabled, prof_begin/prof_end record a HIP event pair
+// on the launch stream around the kernels of one stage.
+enum ProfStage { PROF_PREPROCESS = 0, PROF_SCAN = 1, PROF_BINNING = 2, PROF_RENDER_FWD = 3, PROF_RENDER_BWD = 4,
+                 PROF_SH_FWD = 5, PROF_SH_BWD = 6, PROF_ADAM = 7, PROF_NUM_STAGES = 8 };
+void prof_begin(int stage, hipStream_t stream);
+void prof_end(int stage, hipStream_t stream);
+
 }  // namespace svr
