@@ -30,7 +30,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ["preprocess", "scan", "binning", "render_fwd", "render_bwd", "sh_fwd", "sh_bwd", "adam"]
+STAGES = ["preprocess", "scan", "binning", "render_fwd", "render_bwd", "sh_fwd", "sh_bwd", "adam", "grad_reduce"]
 METRIC = "train steps/s (fwd+bwd+Adam) & fwd Mpix/s, 1M voxels 1080p; batched vs plain"
 
 

@@ -93,15 +93,20 @@ int svr_rasterize_forward(int32_t n_samp, int32_t width, int32_t height, float t
                           const void* order_table, int64_t num_rendered, void* binning_state, void* image_state,
                           float* out_color, float* out_T, svr_stream_t stream);
 
+/* Workspace of svr_rasterize_backward for R entries: one 12-float partial
+ * gradient row per list entry. */
+size_t svr_backward_workspace_size(int64_t num_rendered);
 /* Replaces `_C.rasterize_voxels_backward` (backward.cu:654-772): reverse
- * traversal with T recovered from the stored final T and n_contrib.  Writes
- * (after zeroing) dL_dgeos [N,8], dL_drgbs [N,3], subdiv_p_bw [N,1]. */
+ * traversal with T recovered from the stored final T and n_contrib.  Gradients
+ * are summed per list entry in `workspace`, then per voxel over its entries.
+ * Writes every row of dL_dgeos [N,8], dL_drgbs [N,3], subdiv_p_bw [N,1]
+ * (zero for voxels with no entry).  dL_dgeos must be 16-byte aligned. */
 int svr_rasterize_backward(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy, float cx,
                            float cy, const float* c2w, float bg_color, int64_t n_voxels, const float* vox_centers,
                            const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,
                            int64_t num_rendered, const void* binning_state, const void* image_state,
-                           const float* out_T, const float* dL_dout_color, const float* dL_dout_T, float* dL_dgeos,
-                           float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
+                           const float* out_T, const float* dL_dout_color, const float* dL_dout_T, void* workspace,
+                           float* dL_dgeos, float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
 
 /* Replaces `_C.unpack_ImageState` (raster_state.cu:150-205): copies ranges
  * [T,2], tile_last [T] and n_contrib [H,W] (int32, device) out of the image state. */
@@ -154,7 +159,8 @@ int svr_octpath_2_ijk(int64_t n, const int64_t* octpath, const int8_t* octlevel,
  * enabled, each entry point records a HIP event pair on its stream around the
  * kernels of its stage: 0 preprocess, 1 scan, 2 binning (duplicate + sort +
  * ranges), 3 forward compositor kernel, 4 backward compositor kernel, 5 SH
- * forward, 6 SH backward, 7 Adam.  Enabling resets the records. */
+ * forward, 6 SH backward, 7 Adam, 8 per-voxel gradient reduction.  Enabling
+ * resets the records. */
 int svr_profile_enable(int32_t on);
 /* Synchronises on the recorded events of `stage`, returns their summed
  * duration and count, and clears them. */

@@ -194,11 +194,12 @@ def rasterize_voxels_backward(R, n_samp_per_vox, image_width, image_height, tan_
     centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
     dcol = _f32(dL_dout_color)
     dT = _f32(dL_dout_T) if dL_dout_T is not None else torch.zeros((1, H, W), device=dev)
+    work = torch.empty(int(_L.svr_backward_workspace_size(int(R))), dtype=torch.uint8, device=dev)
     _lib.check(_L.svr_rasterize_backward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
                                          float(cy), _ptr(_f32(c2w_matrix)), float(bg_color), N, _ptr(centers),
                                          _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), int(R),
                                          _ptr(binningBuffer), _ptr(imageBuffer), _ptr(_f32(out_T)), _ptr(dcol),
-                                         _ptr(dT), _ptr(dgeos), _ptr(drgbs), _ptr(dsub), _stream(dev)),
+                                         _ptr(dT), _ptr(work), _ptr(dgeos), _ptr(drgbs), _ptr(dsub), _stream(dev)),
                "rasterize_voxels_backward")
     _debug(debug, dev, "rasterize_voxels_backward")
     return dgeos, drgbs, dsub

@@ -34,9 +34,10 @@ SIGNATURES = {
     "svr_build_order_table": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_rasterize_forward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
                                       c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_backward_workspace_size": (c_sz, [c_i64]),
     "svr_rasterize_backward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
                                        c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
-                                       c_p]),
+                                       c_p, c_p]),
     "svr_unpack_image_state": (c_i32, [c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
     "svr_unpack_geom_state": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_export_sort_keys": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p]),

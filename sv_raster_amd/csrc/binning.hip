@@ -16,7 +16,6 @@
 
 namespace svr {
 
-constexpr uint32_t BIG_DUP = 32;  // voxels with more entries are expanded by a whole workgroup
 
 __device__ inline uint64_t match_digit(uint32_t d, uint64_t active)
 {
@@ -285,7 +284,8 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int g
                                                         const uint2* __restrict__ bboxes,
                                                         const uint32_t* __restrict__ qbits,
                                                         const uint32_t* __restrict__ order_pos,
-                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ big_list,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                        uint32_t* __restrict__ big_list,
                                                         uint32_t* __restrict__ big_count)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,7 +307,11 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int g
         const uint64_t pos = order_pos[8 * v + q];
         for (int y = tymin; y <= tymax; ++y)
             for (int x = txmin; x <= txmax; ++x)
-                keys[off++] = ((uint64_t)((uint32_t)y * (uint32_t)gx + (uint32_t)x) << pos_bits) | pos;
+            {
+                keys[off] = ((uint64_t)((uint32_t)y * (uint32_t)gx + (uint32_t)x) << pos_bits) | pos;
+                vals[off] = off;
+                ++off;
+            }
     }
 }
 
@@ -318,6 +322,7 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(int gx, int gy, int 
                                                             const uint32_t* __restrict__ qbits,
                                                             const uint32_t* __restrict__ order_pos,
                                                             uint64_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals,
                                                             const uint32_t* __restrict__ big_list,
                                                             const uint32_t* __restrict__ big_count)
 {
@@ -341,6 +346,7 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(int gx, int gy, int 
             const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
             const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
             keys[off + e] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | (uint64_t)order_pos[8 * v + q];
+            vals[off + e] = off + e;
         }
     }
 }
@@ -401,19 +407,21 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     const int tile_bits = (int)required_bits((uint64_t)T);
     SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
     duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, gx, gy, pos_bits, g.n_dup, g.scan, g.bbox,
-                                                                      g.qbits, ot.pos, b.keys, g.big_list,
+                                                                      g.qbits, ot.pos, b.keys, b.orig, g.big_list,
                                                                       g.counters + 2);
     SVR_CHECK_LAUNCH();
     duplicate_big_kernel<<<1024, 256, 0, stream>>>(gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos,
-                                                   b.keys, g.big_list, g.counters + 2);
+                                                   b.keys, b.orig, g.big_list, g.counters + 2);
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
-    int rc = radix_sort_ex(b.keys, b.keys_alt, nullptr, nullptr, R, pos_bits + tile_bits, b.counts, stream, &sorted,
-                           nullptr);
+    uint32_t* sorted_orig = nullptr;
+    int rc = radix_sort_ex(b.keys, b.keys_alt, b.orig, b.orig_alt, R, pos_bits + tile_bits, b.counts, stream,
+                           &sorted, &sorted_orig);
     if (rc != SVR_OK) return rc;
     if (sorted != b.keys)
     {
         SVR_CHECK_HIP(hipMemcpyAsync(b.keys, sorted, sizeof(uint64_t) * R, hipMemcpyDeviceToDevice, stream));
+        SVR_CHECK_HIP(hipMemcpyAsync(b.orig, sorted_orig, sizeof(uint32_t) * R, hipMemcpyDeviceToDevice, stream));
     }
     ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(b.keys, R, pos_bits, ot.pair, img.ranges,
                                                                    b.vox_list);

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-    float* __restrict__ dL_dgeos, float* __restrict__ dL_drgbs, float* __restrict__ dL_dsub)
+    const uint32_t* __restrict__ orig, float* __restrict__ partial)
 {
     __shared__ Stage st;
     __shared__ float acc[12][NWAVE][64];
@@ -304,11 +304,13 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     int64_t end = rx + wave_max_u32(nc);
 
     EntryRegs cur;
+    uint32_t cur_o = 0;
     {
         const int64_t jl = end - 1 - lane;
         const bool valid = jl >= rx;
         load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
                    qpix);
+        if (valid) cur_o = orig[jl];
     }
     while (end > rx)
     {
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
         }
 
         // prefetch the next chunk before this chunk's atomics are issued
-        const uint32_t fv = cur.v;
+        const uint32_t fo = cur_o;
         const bool fpm = cur.pm != 0ull;
         const int64_t nend = end - 64;
         {
@@ -371,29 +373,110 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
             const bool valid = jl >= rx;
             load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
                        sy1, qpix);
+            if (valid) cur_o = orig[jl];
         }
         wave_sync();
+        // per-entry partial sums: an entry belongs to one tile, so at most the
+        // four waves of this workgroup add into its row
         if (fpm)
         {
+            float* row = partial + 12 * (size_t)fo;
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < 12; ++i)
             {
                 const float a = acc[i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(dL_dgeos + 8 * (size_t)fv + i, a);
+                if (a != 0.f) unsafeAtomicAdd(row + i, a);
                 acc[i][w][lane] = 0.f;
             }
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-            {
-                const float a = acc[8 + i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(dL_drgbs + 3 * (size_t)fv + i, a);
-                acc[8 + i][w][lane] = 0.f;
-            }
-            const float a = acc[11][w][lane];
-            if (a != 0.f) unsafeAtomicAdd(dL_dsub + fv, a);
-            acc[11][w][lane] = 0.f;
         }
         end = nend;
+    }
+}
+
+// Per-voxel gradient = sum of the partial rows of its entries, which are
+// contiguous in emission order: [scan[v] - n_dup[v], scan[v]).  Voxels with
+// more than BIG_DUP entries (listed in big_list by the duplication pass) are
+// summed by a whole workgroup.
+
+__device__ inline void write_grad(size_t v, const float* a, float* __restrict__ dgeos, float* __restrict__ drgbs,
+                                  float* __restrict__ dsub)
+{
+    *reinterpret_cast<float4*>(dgeos + 8 * v) = make_float4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<float4*>(dgeos + 8 * v + 4) = make_float4(a[4], a[5], a[6], a[7]);
+    drgbs[3 * v + 0] = a[8];
+    drgbs[3 * v + 1] = a[9];
+    drgbs[3 * v + 2] = a[10];
+    dsub[v] = a[11];
+}
+
+__global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint32_t* __restrict__ n_dup,
+                                                          const uint32_t* __restrict__ scan,
+                                                          const float* __restrict__ partial,
+                                                          float* __restrict__ dgeos, float* __restrict__ drgbs,
+                                                          float* __restrict__ dsub)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const uint32_t nd = n_dup[v];
+    if (nd > BIG_DUP) return;
+    float a[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) a[i] = 0.f;
+    const uint32_t off = scan[v] - nd;
+    for (uint32_t e = 0; e < nd; ++e)
+    {
+        const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
+        const float4 p0 = r[0], p1 = r[1], p2 = r[2];
+        a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
+        a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
+        a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+    }
+    write_grad((size_t)v, a, dgeos, drgbs, dsub);
+}
+
+__global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __restrict__ big_list,
+                                                              const uint32_t* __restrict__ big_count,
+                                                              const uint32_t* __restrict__ n_dup,
+                                                              const uint32_t* __restrict__ scan,
+                                                              const float* __restrict__ partial,
+                                                              float* __restrict__ dgeos, float* __restrict__ drgbs,
+                                                              float* __restrict__ dsub)
+{
+    __shared__ float red[12][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // one workgroup per voxel with many entries; blocks stride over the queue
+    const uint32_t count = *big_count;
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    {
+        const uint32_t v = big_list[i];
+        const uint32_t nd = n_dup[v];
+        const uint32_t off = scan[v] - nd;
+        float a[12];
+#pragma unroll
+        for (int i = 0; i < 12; ++i) a[i] = 0.f;
+        for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x)
+        {
+            const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
+            const float4 p0 = r[0], p1 = r[1], p2 = r[2];
+            a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
+            a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
+            a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+        }
+#pragma unroll
+        for (int i = 0; i < 12; ++i)
+        {
+            const float sum = wave_sum_f32(a[i]);
+            if (lane == 0) red[i][w] = sum;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            float t[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) t[i] = (red[i][0] + red[i][1]) + (red[i][2] + red[i][3]);
+            write_grad((size_t)v, t, dgeos, drgbs, dsub);
+        }
+        __syncthreads();
     }
 }
 
@@ -418,20 +501,29 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
 
 template <int NS>
 int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, const float* c2w, float bg,
-               const GeomState& g, const BinningState& b, const ImageState& img, const float* centers,
-               const float* lengths, const float* geos, const float* rgbs, const float* out_T, const float* dL_dcolor,
-               const float* dL_dT, float* dgeos, float* drgbs, float* dsub, hipStream_t stream)
+               int64_t N, int64_t R, const GeomState& g, const BinningState& b, const ImageState& img,
+               const float* centers, const float* lengths, const float* geos, const float* rgbs, const float* out_T,
+               const float* dL_dcolor, const float* dL_dT, float* partial, float* dgeos, float* drgbs, float* dsub,
+               hipStream_t stream)
 {
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
     const int per = (T + 7) / 8;
+    if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
-    render_bwd_kernel<NS, BWD_SYNC_MIN><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
-                                                                   img.ranges, b.vox_list, g.bbox, centers, lengths,
-                                                                   geos, rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT,
-                                                                   dgeos, drgbs, dsub);
+    render_bwd_kernel<NS, BWD_SYNC_MIN><<<(unsigned)(8 * per), 256, 0, stream>>>(
+        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
+    prof_begin(PROF_GRAD_REDUCE, stream);
+    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, g.n_dup, g.scan, partial, dgeos, drgbs,
+                                                                       dsub);
+    SVR_CHECK_LAUNCH();
+    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(g.big_list, g.counters + 2, g.n_dup, g.scan, partial, dgeos,
+                                                     drgbs, dsub);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_GRAD_REDUCE, stream);
     return SVR_OK;
 }
 
@@ -475,36 +567,46 @@ int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, f
     }
 }
 
+size_t svr_backward_workspace_size(int64_t R)
+{
+    return align_up(sizeof(float) * 12 * (size_t)(R > 0 ? R : 0), 256);
+}
+
 int svr_rasterize_backward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,
                            const float* c2w, float bg, int64_t N, const float* centers, const float* lengths,
                            const float* geos, const float* rgbs, const void* geom_state, int64_t R,
                            const void* binning_state, const void* image_state, const float* out_T,
-                           const float* dL_dcolor, const float* dL_dT, float* dgeos, float* drgbs, float* dsub,
-                           svr_stream_t stream)
+                           const float* dL_dcolor, const float* dL_dT, void* workspace, float* dgeos, float* drgbs,
+                           float* dsub, svr_stream_t stream)
 {
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
     if (N == 0) return SVR_OK;
     SVR_REQUIRE(geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub && c2w,
                 SVR_ERR_INVALID, "svr_rasterize_backward: null pointer");
-    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
-    if (fill_u32(dgeos, 0u, 8 * N, stream) != SVR_OK) return SVR_ERR_HIP;
-    if (fill_u32(drgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
-    if (fill_u32(dsub, 0u, N, stream) != SVR_OK) return SVR_ERR_HIP;
-    if (R == 0) return SVR_OK;
-    SVR_REQUIRE(binning_state, SVR_ERR_INVALID, "svr_rasterize_backward: null binning state");
+    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0 && ((uintptr_t)dgeos & 15u) == 0, SVR_ERR_INVALID,
+                "geos and dL_dgeos must be 16-byte aligned");
+    if (R == 0)
+    {
+        if (fill_u32(dgeos, 0u, 8 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(drgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(dsub, 0u, N, stream) != SVR_OK) return SVR_ERR_HIP;
+        return SVR_OK;
+    }
+    SVR_REQUIRE(binning_state && workspace, SVR_ERR_INVALID, "svr_rasterize_backward: null binning state or workspace");
     const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
     GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
     BinningState b = BinningState::carve((void*)binning_state, R, nullptr);
     ImageState img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr);
+    float* partial = (float*)workspace;
     switch (n_samp)
     {
-    case 1: return launch_bwd<1>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
-                                 out_T, dL_dcolor, dL_dT, dgeos, drgbs, dsub, stream);
-    case 2: return launch_bwd<2>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
-                                 out_T, dL_dcolor, dL_dT, dgeos, drgbs, dsub, stream);
-    default: return launch_bwd<3>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
-                                  out_T, dL_dcolor, dL_dT, dgeos, drgbs, dsub, stream);
+    case 1: return launch_bwd<1>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, N, R, g, b, img, centers, lengths, geos,
+                                 rgbs, out_T, dL_dcolor, dL_dT, partial, dgeos, drgbs, dsub, stream);
+    case 2: return launch_bwd<2>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, N, R, g, b, img, centers, lengths, geos,
+                                 rgbs, out_T, dL_dcolor, dL_dT, partial, dgeos, drgbs, dsub, stream);
+    default: return launch_bwd<3>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, N, R, g, b, img, centers, lengths, geos,
+                                  rgbs, out_T, dL_dcolor, dL_dT, partial, dgeos, drgbs, dsub, stream);
     }
 }
 

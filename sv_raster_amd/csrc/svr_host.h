@@ -59,6 +59,10 @@ struct Carver
     size_t size() const { return align_up(off, 256); }
 };
 
+// Voxels with more list entries than this are expanded / reduced by a whole
+// workgroup (they are queued in GeomState::big_list by the duplication pass).
+constexpr uint32_t BIG_DUP = 32;
+
 constexpr int SCAN_BLOCK = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
@@ -121,6 +125,8 @@ struct BinningState
     uint64_t* keys;     // [R] sorted (tile << pos_bits | pos)
     uint64_t* keys_alt; // [R]
     uint32_t* vox_list; // [R] (q << 29) | vox_id in sorted order
+    uint32_t* orig;     // [R] emission-order position of each sorted entry
+    uint32_t* orig_alt; // [R]
     uint32_t* counts;   // [RADIX * blocks + RADIX] radix-sort digit counts
     static BinningState carve(void* p, int64_t R, size_t* bytes)
     {
@@ -130,6 +136,8 @@ struct BinningState
         b.keys = c.take<uint64_t>(R);
         b.keys_alt = c.take<uint64_t>(R);
         b.vox_list = c.take<uint32_t>(R);
+        b.orig = c.take<uint32_t>(R);
+        b.orig_alt = c.take<uint32_t>(R);
         b.counts = c.take<uint32_t>(RADIX * nb + RADIX);
         if (bytes) *bytes = c.size();
         return b;
@@ -172,7 +180,8 @@ int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
 // Live stage timing: when enabled, prof_begin/prof_end record a HIP event pair
 // on the launch stream around the kernels of one stage.
 enum ProfStage { PROF_PREPROCESS = 0, PROF_SCAN = 1, PROF_BINNING = 2, PROF_RENDER_FWD = 3, PROF_RENDER_BWD = 4,
-                 PROF_SH_FWD = 5, PROF_SH_BWD = 6, PROF_ADAM = 7, PROF_NUM_STAGES = 8 };
+                 PROF_SH_FWD = 5, PROF_SH_BWD = 6, PROF_ADAM = 7, PROF_GRAD_REDUCE = 8,
+                 PROF_NUM_STAGES = 9 };
 void prof_begin(int stage, hipStream_t stream);
 void prof_end(int stage, hipStream_t stream);
 
