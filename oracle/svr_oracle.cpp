@@ -42,7 +42,7 @@ constexpr float STEP_SZ_SCALE = 100.f;
 // exp(x) by Cody & Waite argument reduction (x = k*ln2 + r, ln2 split in a
 // high part with trailing zero bits and a low correction) followed by the
 // degree-7 Taylor polynomial of e^r on |r| <= ln2/2 in Horner form, and an
-// exact scaling by 2^k assembled from the exponent bits.  Error <= 2 ulp.
+// exact scaling by 2^k (ldexp, one rounding).  Error <= 2 ulp.
 inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 
 float svr_ref_expf(float x)
@@ -61,13 +61,8 @@ float svr_ref_expf(float x)
     p = std::fma(p, r, 0.5f);
     p = std::fma(p, r, 1.0f);
     p = std::fma(p, r, 1.0f);
-    int k = (int)kf;
-    // 2^k in two factors so that k in [-150, 128] never leaves the normal range
-    const int k1 = k / 2;
-    const int k2 = k - k1;
-    const float s1 = u2f((uint32_t)(k1 + 127) << 23);
-    const float s2 = u2f((uint32_t)(k2 + 127) << 23);
-    return (p * s1) * s2;
+    // exact scaling by 2^k with a single rounding (also into the denormal range)
+    return std::ldexp(p, (int)kf);
 }
 
 template <typename R> inline R rexp(R x);

@@ -33,6 +33,7 @@ struct Stage
 {
     float c[3][NWAVE][64];
     float l[NWAVE][64];
+    float linv[NWAVE][64];
     float g[8][NWAVE][64];
     float rgb[3][NWAVE][64];
 };
@@ -73,7 +74,7 @@ struct EntryRegs
 {
     uint32_t v;
     uint64_t pm;
-    float c0, c1, c2, l;
+    float c0, c1, c2, l, linv;
     float4 g0, g1;
     float r0, r1, r2;
 };
@@ -93,6 +94,7 @@ __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const 
     e.c1 = centers[3 * v + 1];
     e.c2 = centers[3 * v + 2];
     e.l = lengths[v];
+    e.linv = 1.f / e.l;
     e.g0 = *reinterpret_cast<const float4*>(geos + 8 * v);
     e.g1 = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
     e.r0 = rgbs[3 * v + 0];
@@ -104,6 +106,7 @@ __device__ inline void stage_entry(Stage& st, int w, int lane, const EntryRegs& 
 {
     if (!e.pm) return;
     st.c[0][w][lane] = e.c0; st.c[1][w][lane] = e.c1; st.c[2][w][lane] = e.c2; st.l[w][lane] = e.l;
+    st.linv[w][lane] = e.linv;
     st.g[0][w][lane] = e.g0.x; st.g[1][w][lane] = e.g0.y; st.g[2][w][lane] = e.g0.z; st.g[3][w][lane] = e.g0.w;
     st.g[4][w][lane] = e.g1.x; st.g[5][w][lane] = e.g1.y; st.g[6][w][lane] = e.g1.z; st.g[7][w][lane] = e.g1.w;
     st.rgb[0][w][lane] = e.r0; st.rgb[1][w][lane] = e.r1; st.rgb[2][w][lane] = e.r2;
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
             float g[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-            const float vol = vox_vol<NS, false>(c0, c1, c2, len, g, a, b, ray, nullptr);
+            const float vol = vox_vol<NS, false>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, nullptr);
             const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
             if (alpha < MIN_ALPHA) continue;
             const float pt_w = T * alpha;
@@ -229,7 +232,7 @@ __device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray,
     float g[8], dI[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-    const float vol = vox_vol<NS, true>(c0, c1, c2, len, g, a, b, ray, dI);
+    const float vol = vox_vol<NS, true>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, dI);
     const float e = svr_expf(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     if (alpha < MIN_ALPHA) return false;
