@@ -42,8 +42,31 @@ def parse():
     ap.add_argument("--config", default="B")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--fwd-iters", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the bench, run 3 plain copy kernels (torch.neg) of this many MiB: the known-size "
+                         "dispatch that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes")
     return ap.parse_args()
+
+
+def load_traffic(kernel, config, n_vox, width, height):
+    """HBM bytes per dispatch of `kernel` on this workload, from the committed rocprofv3 PMC summary
+    (profiles/<kernel>_<config>_traffic.txt, written from FETCH_SIZE and WRITE_SIZE passes over this same
+    bench command and scaled by a copy kernel of known size).  None when no summary matches the workload."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    fields = {}
+    for line in open(path):
+        if ":" in line and not line.startswith((" ", "#", "|")):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("workload") != f"{config} {n_vox} {width}x{height}":
+        return None
+    try:
+        return int(fields["traffic_bytes_per_dispatch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def collect_stages(L):
@@ -189,15 +212,26 @@ def main():
                                                T_img, dcol, None, None, torch.zeros_like(T_img), 0.0, None, 0.0, 0.0,
                                                False, False, None, None)
         N_h = int((dg != 0).any(1).sum().item())
+        N_v = int((n_dup > 0).sum().item())
     P = W * H
     T_tiles = ((W + 15) // 16) * ((H + 15) // 16)
     bytes_fwd = 72 * E + 20 * P + 12 * T_tiles
     bytes_bwd = 72 * E + 24 * P + 12 * T_tiles + 48 * N_h
 
     stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items()}
+    # algorithmic (compulsory) bytes per training step of each stage; Adam counts its 3 launches
+    # (8 density + 48 SH floats per voxel): a 4-byte gradient test per element plus 24 bytes per touched one
+    stage_bytes = {"preprocess": 36 * N, "scan": 8 * N, "sh_fwd": 216 * N_v,
+                   "binning": 28 * N + 36 * R + 8 * T_tiles, "render_fwd": bytes_fwd, "render_bwd": bytes_bwd,
+                   "grad_reduce": 48 * R + 48 * N_v, "sh_bwd": 228 * N_v, "adam": 4 * 56 * N + 24 * 56 * N_h}
+    stage_ms_step = {k: v[0] / args.steps for k, v in stages.items()}
+    stages_gbs = {k: stage_bytes[k] / (stage_ms_step[k] * 1e-3) / 1e9 if stage_ms_step[k] else None
+                  for k in STAGES}
+    step_bytes = sum(stage_bytes.values())
     dom = "render_bwd" if stage_ms["render_bwd"] >= stage_ms["render_fwd"] else "render_fwd"
     dom_bytes = bytes_bwd if dom == "render_bwd" else bytes_fwd
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = load_traffic(dom, args.config, N, W, H)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -227,15 +261,25 @@ def main():
         "order_table_ms_once_per_octree": order_table_ms,
         "stages_ms_per_launch": stage_ms,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "avg_launch_ms": stage_ms[dom],
                      "other_compositor_gbs": (bytes_fwd if dom == "render_bwd" else bytes_bwd) /
                                              (stage_ms["render_fwd" if dom == "render_bwd" else "render_bwd"]
                                               * 1e-3) / 1e9},
+        "stages_gbs_algorithmic": stages_gbs,
+        "step_gbs_algorithmic": step_bytes / (ms_per_step * 1e-3) / 1e9,
+        "tile_entry_pixel_pairs_per_s_fwd": E * 256 / (stage_ms["render_fwd"] * 1e-3),
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
+
+    if args.calib_copy_mib > 0:
+        a = torch.ones(args.calib_copy_mib * 262144, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -61,6 +61,20 @@ __device__ inline uint64_t select_q(const uint64_t* qpix, uint32_t q)
     return s;
 }
 
+// True when the packed bbox (inclusive) overlaps exactly one of the four 8x8
+// sub-tiles of the 16x16 tile at (tx0, ty0): then only that sub-tile's wave can
+// produce gradient terms for the entry.
+__device__ inline bool single_subtile(uint2 bb, int tx0, int ty0, int W, int H)
+{
+    const int bxmin = (int)(bb.x >> 16), bymin = (int)(bb.x & 0xFFFFu);
+    const int bxmax = (int)(bb.y >> 16), bymax = (int)(bb.y & 0xFFFFu);
+    const int cl = (bxmin <= tx0 + 7 && bxmax >= tx0) ? 1 : 0;
+    const int cr = (tx0 + 8 <= W - 1 && bxmax >= tx0 + 8 && bxmin <= tx0 + 15) ? 1 : 0;
+    const int rt = (bymin <= ty0 + 7 && bymax >= ty0) ? 1 : 0;
+    const int rb = (ty0 + 8 <= H - 1 && bymax >= ty0 + 8 && bymin <= ty0 + 15) ? 1 : 0;
+    return (cl + cr) * (rt + rb) == 1;
+}
+
 __device__ inline uint32_t wave_max_u32(uint32_t x)
 {
 #pragma unroll
@@ -74,6 +88,7 @@ struct EntryRegs
 {
     uint32_t v;
     uint64_t pm;
+    uint2 bb;
     float c0, c1, c2, l, linv;
     float4 g0, g1;
     float r0, r1, r2;
@@ -87,7 +102,8 @@ __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const 
     e.pm = 0ull;
     e.v = val & VOX_ID_MASK;
     if (!valid) return;
-    e.pm = bbox_pixel_mask(bboxes[e.v], sx0, sy0, sx1, sy1, select_q(qpix, val >> 29));
+    e.bb = bboxes[e.v];
+    e.pm = bbox_pixel_mask(e.bb, sx0, sy0, sx1, sy1, select_q(qpix, val >> 29));
     if (!e.pm) return;
     const size_t v = e.v;
     e.c0 = centers[3 * v + 0];
@@ -112,28 +128,18 @@ __device__ inline void stage_entry(Stage& st, int w, int lane, const EntryRegs& 
     st.rgb[0][w][lane] = e.r0; st.rgb[1][w][lane] = e.r1; st.rgb[2][w][lane] = e.r2;
 }
 
-template <int NS>
-__global__ __launch_bounds__(256) void render_fwd_kernel(
-    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
-    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
-    const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
-    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
+// Front-to-back compositing of one wave's 8x8 sub-tile over the tile's list
+// (forward.cu:150-328 for colour and T).  Returns the pixel's n_contrib and
+// leaves its transmittance and colour in T, C0..C2.  EXACT selects the
+// ray-box min/max form (see ray_aabb); it is uniform over the wave.
+template <int NS, bool EXACT>
+__device__ inline uint32_t composite_subtile(Stage& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
+                                             int sx0, int sy0, int sx1, int sy1,
+                                             const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                             const float* __restrict__ centers, const float* __restrict__ lengths,
+                                             const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                             float& T, float& C0, float& C1, float& C2)
 {
-    __shared__ Stage st;
-    __shared__ uint32_t s_last[NWAVE];
-    const int tile = xcd_tile(blockIdx.x, per_xcd);
-    if (tile >= T_tiles) return;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tx = tile % gx, ty = tile / gx;
-    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
-    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
-    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
-    const bool inside = px < W && py < H;
-    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
-    const uint2 range = ranges[tile];
-
-    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
     uint32_t last = 0;
     bool done = !inside;
 
@@ -177,7 +183,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
             hm &= hm - 1ull;
             const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb(c0, c1, c2, len, ray, a, b);
+            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
             if (a > b) continue;
             float g[8];
 #pragma unroll
@@ -200,6 +206,38 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
         base = nbase;
         if (__ballot(!done) == 0ull) break;
     }
+    return last;
+}
+
+// Launched twice per frame: EXACT = false handles the tiles whose 256 rays
+// all have finite 1/rd (hardware min/max in the ray-box test), EXACT = true
+// the remaining tiles (the reference's ternary min/max, whose NaN handling
+// differs); each launch returns at once from the other's tiles.
+template <int NS, bool EXACT>
+__global__ __launch_bounds__(256) void render_fwd_kernel(
+    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
+    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
+    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
+{
+    __shared__ Stage st;
+    __shared__ uint32_t s_last[NWAVE];
+    const int tile = xcd_tile(blockIdx.x, per_xcd);
+    if (tile >= T_tiles) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tx = tile % gx, ty = tile / gx;
+    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
+    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
+    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+    const bool inside = px < W && py < H;
+    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
+    const uint2 range = ranges[tile];
+
+    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+    const uint32_t last = composite_subtile<NS, EXACT>(st, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list,
+                                                       bboxes, centers, lengths, geos, rgbs, T, C0, C1, C2);
 
     if (inside)
     {
@@ -221,13 +259,13 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
 // Adjoint of one (pixel, entry) hit (backward.cu:259-527 without the
 // regularizers): recovers T before the entry, updates the running dL/dT and
 // writes the 12 gradient terms (8 densities, 3 colours, subdivision priority).
-template <int NS>
+template <int NS, bool EXACT>
 __device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray, float& T, float& last_dT, float dp0,
                                float dp1, float dp2, float* gq)
 {
     const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
     float a, b;
-    ray_aabb(c0, c1, c2, len, ray, a, b);
+    ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
     if (a > b) return false;
     float g[8], dI[8];
 #pragma unroll
@@ -253,6 +291,10 @@ __device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray,
     return true;
 }
 
+// Reverse traversal of one wave's 8x8 sub-tile (backward.cu:175-527 for the
+// colour and T losses).  EXACT selects the ray-box min/max form (see
+// ray_aabb); it is uniform over the wave.
+//
 // Entries whose bbox covers at least SYNC_MIN pixels of the wave (coarse
 // voxels, the outside shell) are "sync slots": every pixel processes them at
 // the same step, at their place in its own order, and their gradient is summed
@@ -263,30 +305,17 @@ __device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray,
 // The chunk loop is software-pipelined: chunk c+1 is loaded into registers
 // before chunk c's gradients are flushed with global atomics, because on this
 // architecture a later load's wait also waits for all earlier stores/atomics.
-template <int NS, int SYNC_MIN>
-__global__ __launch_bounds__(256) void render_bwd_kernel(
-    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
-    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
-    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
-    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-    const uint32_t* __restrict__ orig, float* __restrict__ partial)
+template <int NS, int SYNC_MIN, bool EXACT>
+__device__ inline void adjoint_subtile(Stage& st, float (&acc)[12][NWAVE][64], int w, int lane, int W, int H, int tx,
+                                       int ty, int sx0, int sy0, int sx1, int sy1, bool inside, size_t pid, size_t HW,
+                                       const PixRay& ray, uint2 range, float bg,
+                                       const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                       const float* __restrict__ centers, const float* __restrict__ lengths,
+                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                       const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+                                       const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+                                       const uint32_t* __restrict__ orig, float* __restrict__ partial)
 {
-    __shared__ Stage st;
-    __shared__ float acc[12][NWAVE][64];
-    const int tile = xcd_tile(blockIdx.x, per_xcd);
-    if (tile >= T_tiles) return;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tx = tile % gx, ty = tile / gx;
-    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
-    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
-    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
-    const bool inside = px < W && py < H;
-    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
-    const uint2 range = ranges[tile];
-    const size_t HW = (size_t)H * W;
-    const size_t pid = inside ? (size_t)py * W + px : 0;
-
     float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
     uint32_t nc = 0;
     if (inside)
@@ -338,7 +367,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
                 const int l = __builtin_ctzll(seg);
                 seg &= seg - 1ull;
                 float gq[12];
-                if (bwd_hit<NS>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
+                if (bwd_hit<NS, EXACT>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
                 {
 #pragma unroll
                     for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
@@ -352,7 +381,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
 #pragma unroll
             for (int i = 0; i < 12; ++i) gq[i] = 0.f;
             bool hit = false;
-            if (mine) hit = bwd_hit<NS>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
+            if (mine) hit = bwd_hit<NS, EXACT>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
             if (__ballot(hit) == 0ull) continue;
             if (!hit)
             {
@@ -370,6 +399,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
         // prefetch the next chunk before this chunk's atomics are issued
         const uint32_t fo = cur_o;
         const bool fpm = cur.pm != 0ull;
+        const bool fsole = fpm && single_subtile(cur.bb, tx * TILE_X, ty * TILE_Y, W, H);
         const int64_t nend = end - 64;
         {
             const int64_t jl = nend - 1 - lane;
@@ -380,8 +410,29 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
         }
         wave_sync();
         // per-entry partial sums: an entry belongs to one tile, so at most the
-        // four waves of this workgroup add into its row
-        if (fpm)
+        // four waves of this workgroup add into its row; when the entry's bbox
+        // lies in this wave's sub-tile alone, the wave owns the (zeroed) row
+        // and writes it with plain stores
+        if (fsole)
+        {
+            float a[12];
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < 12; ++i)
+            {
+                a[i] = acc[i][w][lane];
+                any = any || (a[i] != 0.f);
+                acc[i][w][lane] = 0.f;
+            }
+            if (any)
+            {
+                float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)fo);
+                row[0] = make_float4(a[0], a[1], a[2], a[3]);
+                row[1] = make_float4(a[4], a[5], a[6], a[7]);
+                row[2] = make_float4(a[8], a[9], a[10], a[11]);
+            }
+        }
+        else if (fpm)
         {
             float* row = partial + 12 * (size_t)fo;
 #pragma unroll
@@ -394,6 +445,36 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
         }
         end = nend;
     }
+}
+
+// Launched twice per frame, split by tiles like render_fwd_kernel.
+template <int NS, int SYNC_MIN, bool EXACT>
+__global__ __launch_bounds__(256) void render_bwd_kernel(
+    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
+    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
+    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+    const uint32_t* __restrict__ orig, float* __restrict__ partial)
+{
+    __shared__ Stage st;
+    __shared__ float acc[12][NWAVE][64];
+    const int tile = xcd_tile(blockIdx.x, per_xcd);
+    if (tile >= T_tiles) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tx = tile % gx, ty = tile / gx;
+    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
+    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
+    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+    const bool inside = px < W && py < H;
+    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    const uint2 range = ranges[tile];
+    const size_t HW = (size_t)H * W;
+    const size_t pid = inside ? (size_t)py * W + px : 0;
+    if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
+    adjoint_subtile<NS, SYNC_MIN, EXACT>(st, acc, w, lane, W, H, tx, ty, sx0, sy0, sx1, sy1, inside, pid, HW, ray,
+                                         range, bg, vox_list, bboxes, centers, lengths, geos, rgbs, out_T, n_contrib,
+                                         dL_dcolor, dL_dT, orig, partial);
 }
 
 // Per-voxel gradient = sum of the partial rows of its entries, which are
@@ -493,10 +574,13 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int T = gx * gy;
     const int per = (T + 7) / 8;
     prof_begin(PROF_RENDER_FWD, stream);
-    render_fwd_kernel<NS><<<(unsigned)(8 * per), 256, 0, stream>>>(W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg,
-                                                                   img.ranges, b.vox_list, g.bbox, centers, lengths,
-                                                                   geos, rgbs, img.tile_last, img.n_contrib,
-                                                                   out_color, out_T);
+    render_fwd_kernel<NS, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
+        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        rgbs, img.tile_last, img.n_contrib, out_color, out_T);
+    SVR_CHECK_LAUNCH();
+    render_fwd_kernel<NS, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
+        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        rgbs, img.tile_last, img.n_contrib, out_color, out_T);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;
@@ -514,7 +598,11 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int per = (T + 7) / 8;
     if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
-    render_bwd_kernel<NS, BWD_SYNC_MIN><<<(unsigned)(8 * per), 256, 0, stream>>>(
+    render_bwd_kernel<NS, BWD_SYNC_MIN, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
+        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
+    SVR_CHECK_LAUNCH();
+    render_bwd_kernel<NS, BWD_SYNC_MIN, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
     SVR_CHECK_LAUNCH();
@@ -597,6 +685,7 @@ int svr_rasterize_backward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, 
         return SVR_OK;
     }
     SVR_REQUIRE(binning_state && workspace, SVR_ERR_INVALID, "svr_rasterize_backward: null binning state or workspace");
+    SVR_REQUIRE(((uintptr_t)workspace & 15u) == 0, SVR_ERR_INVALID, "backward workspace must be 16-byte aligned");
     const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
     GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
     BinningState b = BinningState::carve((void*)binning_state, R, nullptr);

@@ -113,7 +113,14 @@ __device__ inline PixRay make_ray(int px, int py, int W, int H, float tanfovx, f
     return r;
 }
 
-// auxiliary.h:310-323
+// auxiliary.h:310-323.  The reference's min/max are ternaries ((a < b) ? a : b).
+// They differ from the hardware min/max only for NaN operands and in the sign
+// of a zero result.  The sign of a zero a or b cannot change any output (the
+// test a > b, the segment length b - a and the first sample point
+// a + step / 2 come out the same), and a slab bound is NaN only when 1/rd is
+// not finite (a zero or denormal direction component).  EXACT = false uses
+// v_min/v_max; the kernels use it for waves whose rays all have finite 1/rd.
+template <bool EXACT>
 __device__ inline void ray_aabb(float cx_, float cy_, float cz_, float l, const PixRay& ray, float& a, float& b)
 {
     const float r = 0.5f * l;
@@ -121,11 +128,33 @@ __device__ inline void ray_aabb(float cx_, float cy_, float cz_, float l, const 
     const float a0 = (d0 - r) * ray.rd_inv[0], b0 = (d0 + r) * ray.rd_inv[0];
     const float a1 = (d1 - r) * ray.rd_inv[1], b1 = (d1 + r) * ray.rd_inv[1];
     const float a2 = (d2 - r) * ray.rd_inv[2], b2 = (d2 + r) * ray.rd_inv[2];
-    const float lo0 = smin(a0, b0), hi0 = smax(a0, b0);
-    const float lo1 = smin(a1, b1), hi1 = smax(a1, b1);
-    const float lo2 = smin(a2, b2), hi2 = smax(a2, b2);
-    a = smax(smax(lo0, lo1), lo2);
-    b = smin(smin(hi0, hi1), hi2);
+    if constexpr (EXACT)
+    {
+        const float lo0 = smin(a0, b0), hi0 = smax(a0, b0);
+        const float lo1 = smin(a1, b1), hi1 = smax(a1, b1);
+        const float lo2 = smin(a2, b2), hi2 = smax(a2, b2);
+        a = smax(smax(lo0, lo1), lo2);
+        b = smin(smin(hi0, hi1), hi2);
+    }
+    else
+    {
+        // med3(x, y, -inf) = min(x, y) and med3(x, y, +inf) = max(x, y); written
+        // this way the compiler emits plain v_min/v_max (and v_min3/v_max3)
+        // without quieting the operands first
+        const float ninf = -__builtin_inff(), pinf = __builtin_inff();
+        const float lo0 = __builtin_amdgcn_fmed3f(a0, b0, ninf), hi0 = __builtin_amdgcn_fmed3f(a0, b0, pinf);
+        const float lo1 = __builtin_amdgcn_fmed3f(a1, b1, ninf), hi1 = __builtin_amdgcn_fmed3f(a1, b1, pinf);
+        const float lo2 = __builtin_amdgcn_fmed3f(a2, b2, ninf), hi2 = __builtin_amdgcn_fmed3f(a2, b2, pinf);
+        a = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(lo0, lo1, pinf), lo2, pinf);
+        b = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(hi0, hi1, ninf), hi2, ninf);
+    }
+}
+
+// True when every component of 1/rd is finite (the ray-box slabs cannot be NaN).
+__device__ inline bool ray_inv_finite(const PixRay& r)
+{
+    return __builtin_fabsf(r.rd_inv[0]) < __builtin_inff() && __builtin_fabsf(r.rd_inv[1]) < __builtin_inff() &&
+           __builtin_fabsf(r.rd_inv[2]) < __builtin_inff();
 }
 
 // Volume integral of one voxel segment (forward.cu:224-253, backward.cu:283-328).

@@ -92,3 +92,22 @@ def test_parity_config_b_full():
     g = run_gpu(case)
     _check_forward(g, o)
     _check_backward(g, o)
+
+
+@pytest.mark.gpu
+def test_parity_axis_aligned_rays():
+    """Camera on the z axis with an axis-aligned frame and an odd image size: the
+    centre row and column rays have a zero direction component (1/rd infinite)
+    and start on the octree's x = 0 and y = 0 face planes, so their ray-box
+    slabs produce NaN and the reference's ternary min/max decides the hits.
+    Tiles holding such rays take the exact ray-box form, the others the
+    hardware min/max; both must match the oracle bit for bit."""
+    from sv_raster_amd import synthetic as S
+    cfg = dict(S.CONFIGS["C"], views=[(63, 63)])
+    case = make_case("C", seed=4, cfg=cfg, direction=(0.0, 0.0, -1.0))
+    c2w = case["cam"]["c2w"]
+    assert c2w[0, 3] == 0.0 and c2w[1, 3] == 0.0 and c2w[0, 1] == 0.0 and c2w[0, 2] == 0.0
+    o = run_oracle(case)
+    g = run_gpu(case)
+    _check_forward(g, o)
+    _check_backward(g, o)
