@@ -156,6 +156,81 @@ int svr_ijk_2_octpath(int64_t n, const int64_t* ijk, const int8_t* octlevel, int
 int svr_octpath_2_ijk(int64_t n, const int64_t* octpath, const int8_t* octlevel, int64_t* ijk,
                       svr_stream_t stream);
 
+/* ---- multi-view batch ---------------------------------------------------
+ * No reference counterpart: the reference renders one camera per call
+ * (renderer.py:39-127).  These entry points rasterize V cameras of one octree
+ * in one pass per stage, for training steps whose loss sums over views.  Each
+ * view's results equal what the single-view entry points give for that
+ * camera; gradients of shared parameters are the sums over the views.
+ * Per (view, voxel) arrays are view-major: index u * N + v. */
+
+/* One camera of a batch.  The caller keeps an array of these in host memory
+ * (for validation and sizes) and the same bytes in device memory (read by the
+ * kernels).  tile_base = sum of the tile counts ((W+15)/16 * (H+15)/16) of the
+ * previous views; pix_base = sum of their pixel counts. */
+typedef struct svr_view {
+    int32_t width, height;
+    float tanfovx, tanfovy, cx, cy, near_plane, bg_color;
+    float w2c[12]; /* first 3 rows of the row-major world-to-camera matrix */
+    float c2w[12]; /* first 3 rows of the row-major camera-to-world matrix */
+    int32_t tile_base;
+    int32_t reserved;
+    int64_t pix_base;
+} svr_view;
+
+/* Sizes: geometry state of V views (its scan runs over V * N entries, so
+ * svr_num_rendered(V * N, ...) returns the batch's total entry count R), the
+ * batch image state for T_total tiles and P_total pixels, and the backward
+ * workspace (per-entry partial rows plus per-(view, voxel) gradient rows). */
+size_t svr_batch_geom_state_size(int32_t n_views, int64_t n_voxels);
+size_t svr_batch_image_state_size(int64_t total_tiles, int64_t total_pixels);
+size_t svr_batch_backward_workspace_size(int32_t n_views, int64_t n_voxels, int64_t num_rendered);
+
+/* Preprocess of every (view, voxel) pair: out_n_duplicates [V, N] (int32) and
+ * the geometry state, as svr_rasterize_preprocess per view. */
+int svr_batch_preprocess(int32_t n_views, const svr_view* views_host, const svr_view* views_dev, int64_t n_voxels,
+                         const float* vox_centers, const float* vox_lengths, int32_t* out_n_duplicates,
+                         void* geom_state, svr_stream_t stream);
+
+/* SH colours of every (view, voxel) pair with entries (rgbs [V, N, 3], zero
+ * rows elsewhere), reading each voxel's coefficients once for all views.
+ * Same arithmetic per pair as svr_sh_compute.  M = 16 (degree-3 layout). */
+int svr_batch_sh_compute(int32_t deg, int32_t n_views, const svr_view* views_dev, int64_t n_voxels,
+                         const float* vox_centers, const void* geom_state, const float* sh0, const float* shs,
+                         float* rgbs, svr_stream_t stream);
+/* dL_dsh0 [N, 3] and dL_dshs [N, 15, 3] summed over the views (in view
+ * order) from dL_drgbs [V, N, 3], as svr_sh_compute_bw per view. */
+int svr_batch_sh_compute_bw(int32_t deg, int32_t n_views, const svr_view* views_dev, int64_t n_voxels,
+                            const float* vox_centers, const void* geom_state, const float* dL_drgbs,
+                            float* dL_dsh0, float* dL_dshs, svr_stream_t stream);
+
+/* Key duplication, one sort over all views' entries, tile ranges, and the
+ * forward compositor as one persistent launch over a queue of all views'
+ * tiles (longest lists first).  out_color holds each view's planar [3, H, W]
+ * block at 3 * pix_base, out_T its [H, W] block at pix_base. */
+int svr_batch_rasterize_forward(int32_t n_samp, int32_t n_views, const svr_view* views_host,
+                                const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,
+                                const float* vox_lengths, const float* geos, const float* rgbs,
+                                const void* geom_state, const void* order_table, int64_t num_rendered,
+                                void* binning_state, void* image_state, float* out_color, float* out_T,
+                                svr_stream_t stream);
+
+/* Reverse-order adjoint over all views (persistent launch), then per voxel:
+ * dL_dgeos [N, 8] and subdiv_p_bw [N] summed over the views, dL_drgbs
+ * [V, N, 3] per view.  workspace: svr_batch_backward_workspace_size bytes. */
+int svr_batch_rasterize_backward(int32_t n_samp, int32_t n_views, const svr_view* views_host,
+                                 const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,
+                                 const float* vox_lengths, const float* geos, const float* rgbs,
+                                 const void* geom_state, int64_t num_rendered, const void* binning_state,
+                                 const void* image_state, const float* out_T, const float* dL_dout_color,
+                                 const float* dL_dout_T, void* workspace, float* dL_dgeos, float* dL_drgbs,
+                                 float* subdiv_p_bw, svr_stream_t stream);
+
+/* Copies ranges [T_total, 2], tile_last [T_total] and n_contrib [P_total] out
+ * of a batch image state. */
+int svr_batch_unpack_image_state(int64_t total_tiles, int64_t total_pixels, const void* image_state,
+                                 int32_t* ranges, int32_t* tile_last, int32_t* n_contrib, svr_stream_t stream);
+
 /* Live stage timing (no reference counterpart; measurement support).  While
  * enabled, each entry point records a HIP event pair on its stream around the
  * kernels of its stage: 0 preprocess, 1 scan, 2 binning (duplicate + sort +

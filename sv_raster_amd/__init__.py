@@ -11,3 +11,4 @@ from . import meta  # noqa: F401
 from . import utils  # noqa: F401
 from . import renderer  # noqa: F401
 from . import sparse_adam  # noqa: F401
+from . import batch  # noqa: F401

@@ -19,6 +19,17 @@ class SvrConstants(ctypes.Structure):
                 ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32)]
 
 
+class SvrView(ctypes.Structure):
+    """svr_view of include/svraster_hip.h (144 bytes)."""
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("tanfovx", ctypes.c_float),
+                ("tanfovy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("near_plane", ctypes.c_float), ("bg_color", ctypes.c_float), ("w2c", ctypes.c_float * 12),
+                ("c2w", ctypes.c_float * 12), ("tile_base", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("pix_base", ctypes.c_int64)]
+
+
+assert ctypes.sizeof(SvrView) == 144
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "svr_last_error": (ctypes.c_char_p, []),
@@ -49,6 +60,17 @@ SIGNATURES = {
     "svr_voxel_order_rank": (c_i32, [c_i64, c_p, c_p, c_p]),
     "svr_ijk_2_octpath": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_octpath_2_ijk": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
+    "svr_batch_geom_state_size": (c_sz, [c_i32, c_i64]),
+    "svr_batch_image_state_size": (c_sz, [c_i64, c_i64]),
+    "svr_batch_backward_workspace_size": (c_sz, [c_i32, c_i64, c_i64]),
+    "svr_batch_preprocess": (c_i32, [c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_sh_compute": (c_i32, [c_i32, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_sh_compute_bw": (c_i32, [c_i32, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_rasterize_forward": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_i64,
+                                            c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_rasterize_backward": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_i64, c_p,
+                                             c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_unpack_image_state": (c_i32, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_debug_sync": (c_i32, [c_p]),
     "svr_profile_enable": (c_i32, [c_i32]),
     "svr_profile_collect": (c_i32, [c_i32, ctypes.POINTER(c_f64), ctypes.POINTER(c_i64)]),

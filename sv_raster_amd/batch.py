@@ -1,0 +1,227 @@
+"""Multi-view batch: one pass per stage over V cameras of one octree.
+
+No reference counterpart as a whole.  A reference training loop calls
+`renderer.rasterize_voxels` once per camera (backends/new_cuda/
+new_svraster_cuda/renderer.py:39-127) and its `vox_fn` evaluates SH colours
+per camera (renderer.py:251-316).  `rasterize_voxels_batch` gives, for every
+view, the image and transmittance that `renderer.rasterize_voxels` gives for
+that camera with SH colours of `sh0`/`shs`, and back-propagates the sum over
+views of the per-view gradients into the shared `geos`, `sh0`, `shs` and
+`subdiv_p`, with all views handled by one launch per stage:
+
+* preprocess, key duplication and one sort over all (view, voxel) pairs;
+* SH colours of all views from one read of the coefficients;
+* the compositor forward and backward as persistent launches over one queue
+  of all views' tiles (longest lists first);
+* per-voxel gradient sums over views in the order a per-view loop would
+  accumulate them.
+
+The camera set is fixed in a `ViewBatch` (host and device copies of the
+`svr_view` table of include/svraster_hip.h).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _C, _lib
+
+_L = _lib.load()
+
+
+def _mat12(m):
+    a = m.detach().float().cpu().numpy() if torch.is_tensor(m) else np.asarray(m, np.float32)
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    if a.size < 12:
+        raise Exception("camera matrices need at least 12 values (3 rows of a 4x4)")
+    return a[:12]
+
+
+class ViewBatch:
+    """A fixed set of cameras rendered together.
+
+    `settings` is a sequence of `renderer.RasterSettings` (color_mode "sh";
+    the same n_samp_per_vox for all views; depth, normal and max_w tracking
+    off).  Reading the camera matrices synchronises once, here.
+    """
+
+    def __init__(self, settings, device=None):
+        settings = list(settings)
+        if not settings:
+            raise Exception("ViewBatch needs at least one view")
+        self.n_samp = int(settings[0].n_samp_per_vox)
+        if not 1 <= self.n_samp <= _C.MAX_N_SAMP:
+            raise Exception(f"n_samp_per_vox should be in range [1, {_C.MAX_N_SAMP}].")
+        V = len(settings)
+        table = (_lib.SvrView * V)()
+        self.sizes = []
+        tile_base, pix_base = 0, 0
+        for u, s in enumerate(settings):
+            if int(s.n_samp_per_vox) != self.n_samp:
+                raise Exception("all views of a batch need the same n_samp_per_vox")
+            _C._unsupported(bool(getattr(s, "need_depth", False)), False, bool(getattr(s, "need_normal", False)),
+                            bool(getattr(s, "track_max_w", False)))
+            W, H = int(s.image_width), int(s.image_height)
+            v = table[u]
+            v.width, v.height = W, H
+            v.tanfovx, v.tanfovy = float(s.tanfovx), float(s.tanfovy)
+            v.cx, v.cy = float(s.cx), float(s.cy)
+            v.near_plane = float(getattr(s, "near", 0.02))
+            v.bg_color = float(getattr(s, "bg_color", 0.0))
+            v.w2c[:] = _mat12(s.w2c_matrix).tolist()
+            v.c2w[:] = _mat12(s.c2w_matrix).tolist()
+            v.tile_base, v.reserved, v.pix_base = tile_base, 0, pix_base
+            self.sizes.append((W, H))
+            gx, gy = _C.tile_grid(W, H)
+            tile_base += gx * gy
+            pix_base += W * H
+        if device is None:
+            m = settings[0].c2w_matrix
+            device = m.device if torch.is_tensor(m) and m.is_cuda else torch.device("cuda")
+        self.device = torch.device(device)
+        self.host = table
+        raw = np.frombuffer(bytes(memoryview(table)), dtype=np.uint8).copy()
+        self.dev = torch.from_numpy(raw).to(self.device)
+        self.n_views = V
+        self.total_tiles = tile_base
+        self.total_pixels = pix_base
+        self.pix_base = [table[u].pix_base for u in range(V)]
+
+    def host_ptr(self):
+        return ctypes.cast(self.host, ctypes.c_void_p)
+
+    def split(self, color_flat, T_flat):
+        """Per-view [3, H, W] colour and [1, H, W] transmittance views of the flat buffers."""
+        colors, Ts = [], []
+        for (W, H), pb in zip(self.sizes, self.pix_base):
+            colors.append(color_flat.narrow(0, 3 * pb, 3 * W * H).view(3, H, W))
+            Ts.append(T_flat.narrow(0, pb, W * H).view(1, H, W))
+        return colors, Ts
+
+
+def _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree):
+    """Preprocess of all pairs, R, and the SH colours of all views."""
+    dev = vox_centers.device
+    V, N = views.n_views, vox_centers.size(0)
+    st = _C._stream(dev)
+    centers, lengths = _C._f32(vox_centers), _C._f32(vox_lengths)
+    n_dup = torch.zeros(V * N, dtype=torch.int32, device=dev)
+    geom = torch.empty(int(_L.svr_batch_geom_state_size(V, N)), dtype=torch.uint8, device=dev)
+    R = ctypes.c_int64(0)
+    if N > 0:
+        _lib.check(_L.svr_batch_preprocess(V, views.host_ptr(), _C._ptr(views.dev), N, _C._ptr(centers),
+                                           _C._ptr(lengths), _C._ptr(n_dup), _C._ptr(geom), st),
+                   "batch_preprocess")
+        _lib.check(_L.svr_num_rendered(V * N, _C._ptr(geom), st, ctypes.byref(R)), "num_rendered")
+    rgbs = torch.empty((V * N, 3), dtype=torch.float32, device=dev)
+    if N > 0:
+        _lib.check(_L.svr_batch_sh_compute(int(active_sh_degree), V, _C._ptr(views.dev), N, _C._ptr(centers),
+                                           _C._ptr(geom), _C._ptr(_C._f32(sh0)), _C._ptr(_C._f32(shs)),
+                                           _C._ptr(rgbs), st), "batch_sh_compute")
+    return n_dup, geom, int(R.value), rgbs
+
+
+def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs):
+    """Forward of all views; returns the state the backward needs (also used by the tests):
+    color [3 * P_total], T [P_total], R, rgbs [V * N, 3], n_dup [V * N] and the opaque buffers."""
+    dev = vox_centers.device
+    V, N = views.n_views, vox_centers.size(0)
+    n_dup, geom, R, rgbs = _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree)
+    binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
+    img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels)), dtype=torch.uint8,
+                      device=dev)
+    color = torch.empty(3 * views.total_pixels, dtype=torch.float32, device=dev)
+    T = torch.empty(views.total_pixels, dtype=torch.float32, device=dev)
+    table = _C.order_table_cache.get(octree_paths) if N > 0 else None
+    centers, lengths, geos_c = _C._f32(vox_centers), _C._f32(vox_lengths), _C._f32_a16(geos)
+    _lib.check(_L.svr_batch_rasterize_forward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
+                                              _C._ptr(centers), _C._ptr(lengths), _C._ptr(geos_c), _C._ptr(rgbs),
+                                              _C._ptr(geom), _C._ptr(table), R, _C._ptr(binning), _C._ptr(img),
+                                              _C._ptr(color), _C._ptr(T), _C._stream(dev)),
+               "batch_rasterize_forward")
+    return dict(color=color, T=T, R=R, rgbs=rgbs, n_dup=n_dup, geom=geom, binning=binning, img=img,
+                centers=centers, lengths=lengths, geos=geos_c, deg=int(active_sh_degree))
+
+
+def batch_backward(views, state, dL_dcolor, dL_dT):
+    """Gradients (dgeos [N, 8], dsh0 [N, 3], dshs [N, 15, 3], dsub [N, 1], drgbs [V * N, 3]) of a batch forward."""
+    centers, lengths, T = state["centers"], state["lengths"], state["T"]
+    R = state["R"]
+    dev = centers.device
+    V, N = views.n_views, centers.size(0)
+    st = _C._stream(dev)
+    dL_dcolor = torch.zeros(3 * T.numel(), dtype=torch.float32, device=dev) if dL_dcolor is None \
+        else _C._f32(dL_dcolor)
+    dL_dT = torch.zeros_like(T) if dL_dT is None else _C._f32(dL_dT)
+    ws = torch.empty(int(_L.svr_batch_backward_workspace_size(V, N, R)), dtype=torch.uint8, device=dev)
+    dgeos = torch.empty((N, 8), dtype=torch.float32, device=dev)
+    drgbs = torch.empty((V * N, 3), dtype=torch.float32, device=dev)
+    dsub = torch.empty((N, 1), dtype=torch.float32, device=dev)
+    dsh0 = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    dshs = torch.empty((N, 15, 3), dtype=torch.float32, device=dev)
+    if N > 0:
+        _lib.check(_L.svr_batch_rasterize_backward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
+                                                   _C._ptr(centers), _C._ptr(lengths), _C._ptr(state["geos"]),
+                                                   _C._ptr(state["rgbs"]), _C._ptr(state["geom"]), R,
+                                                   _C._ptr(state["binning"]), _C._ptr(state["img"]), _C._ptr(T),
+                                                   _C._ptr(dL_dcolor), _C._ptr(dL_dT), _C._ptr(ws), _C._ptr(dgeos),
+                                                   _C._ptr(drgbs), _C._ptr(dsub), st), "batch_rasterize_backward")
+        _lib.check(_L.svr_batch_sh_compute_bw(state["deg"], V, _C._ptr(views.dev), N, _C._ptr(centers),
+                                              _C._ptr(state["geom"]), _C._ptr(drgbs), _C._ptr(dsh0), _C._ptr(dshs),
+                                              st), "batch_sh_compute_bw")
+    return dgeos, dsh0, dshs, dsub, drgbs
+
+
+class _RasterizeBatch(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, subdiv_p):
+        state = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
+        ctx.views, ctx.state = views, state
+        return state["color"], state["T"]
+
+    @staticmethod
+    def backward(ctx, dL_dcolor, dL_dT):
+        dgeos, dsh0, dshs, dsub, _ = batch_backward(ctx.views, ctx.state, dL_dcolor, dL_dT)
+        ctx.state = None
+        return None, None, None, None, None, dgeos, dsh0, dshs, dsub
+
+
+def rasterize_voxels_batch(views, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, subdiv_p=None,
+                           active_sh_degree=3):
+    """Render every camera of `views` (a ViewBatch).
+
+    geos [N, 8], sh0 [N, 3] and shs [N, 15, 3] are shared by all views.
+    Returns (colors, Ts): lists with one [3, H, W] and one [1, H, W] tensor per
+    view.  Gradients flow to geos, sh0, shs and subdiv_p ([N, 1]; its gradient
+    is the summed subdivision priority of the backward).
+    """
+    N = octree_paths.shape[0]
+    if vox_centers.shape[0] != N or vox_lengths.shape[0] != N or geos.shape[0] != N or sh0.shape[0] != N \
+            or shs.shape[0] != N:
+        raise Exception("Size mismatched.")
+    if len(vox_centers.shape) != 2 or vox_centers.shape[1] != 3:
+        raise Exception("Expect vox_centers in shape [N, 3].")
+    if len(geos.shape) != 2 or geos.shape[1] != 8:
+        raise Exception("Expect geos in shape [N, 8].")
+    if tuple(shs.shape[1:]) != (15, 3) or tuple(sh0.shape[1:]) != (3,):
+        raise Exception("Expect sh0 in shape [N, 3] and shs in shape [N, 15, 3].")
+    if not 0 <= int(active_sh_degree) <= 3:
+        raise Exception("active_sh_degree should be in range [0, 3].")
+    _C._require_cuda(octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
+    if subdiv_p is None:
+        subdiv_p = torch.zeros((N, 1), dtype=torch.float32, device=vox_centers.device)
+    color, T = _RasterizeBatch.apply(views, int(active_sh_degree), octree_paths, vox_centers, vox_lengths, geos,
+                                     sh0, shs, subdiv_p)
+    return views.split(color, T)
+
+
+def unpack_image_state(views, imageBuffer):
+    """ranges [T_total, 2], tile_last [T_total], n_contrib [P_total] (int32) of a batch image state."""
+    dev = imageBuffer.device
+    T, P = views.total_tiles, views.total_pixels
+    ranges = torch.empty((T, 2), dtype=torch.int32, device=dev)
+    tile_last = torch.empty(T, dtype=torch.int32, device=dev)
+    n_contrib = torch.empty(P, dtype=torch.int32, device=dev)
+    _lib.check(_L.svr_batch_unpack_image_state(T, P, _C._ptr(imageBuffer), _C._ptr(ranges), _C._ptr(tile_last),
+                                               _C._ptr(n_contrib), _C._stream(dev)), "batch_unpack_image_state")
+    return ranges, tile_last, n_contrib

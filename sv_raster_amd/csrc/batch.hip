@@ -1,0 +1,849 @@
+// Multi-view batch: V cameras of one octree rendered, differentiated and
+// colour-evaluated together.
+//
+// No reference counterpart as a whole: the reference renders one camera per
+// call (backends/new_cuda/new_svraster_cuda/renderer.py:39-127) and a training
+// loop over views repeats every stage per view.  Per (view, voxel) pair the
+// arithmetic here is the single-view path's (preprocess.cu:24-132,
+// forward.cu:507-604 and 43-394, backward.cu:37-530, sh_compute.cu:55-240), so
+// each view's images, n_contrib and tile lists equal the single-view results.
+//
+// What the batch changes:
+//   * one launch per stage over all (view, voxel) pairs instead of one per view;
+//   * one sort of all views' entries: keys (global tile << pos_bits) | pos, the
+//     global tile id being the view's tile_base plus its tile index;
+//   * the compositor forward and backward each run as one launch over the
+//     tiles of all views, queued by list length (longest first; workgroup b
+//     takes queue slot b, and the hardware dispatches workgroups in index
+//     order), so views of different resolution share the chip with no tail
+//     per view; the few tiles with a non-finite 1/rd ray are deferred to a
+//     small persistent launch that loops over them;
+//   * SH colours of all views are computed from one read of each voxel's 48
+//     coefficients, and the SH backward sums the views' gradients in registers
+//     before one write.
+#include "compositor.h"
+#include "svr_host.h"
+#include "voxel_ops.h"
+
+namespace svr {
+
+// Batch image state: per global tile ranges / tile_last, per pixel n_contrib,
+// and the tile queue (order, deferred exact-ray tiles, counters).
+struct BatchImageState
+{
+    uint2* ranges;        // [T]
+    uint32_t* tile_last;  // [T]
+    uint32_t* n_contrib;  // [P]
+    uint64_t* qkeys;      // [T] queue sort keys
+    uint64_t* qkeys_alt;  // [T]
+    uint32_t* order;      // [T] tile order (longest list first)
+    uint32_t* order_alt;  // [T]
+    uint32_t* exact_list; // [T] tiles deferred to the exact ray-box launch
+    uint32_t* qcounts;    // radix counts for the queue sort
+    uint32_t* ctr;        // [8]: 0 fwd head, 1 exact count, 2 fwd exact head, 3 bwd head, 4 bwd exact head
+    static BatchImageState carve(void* p, int64_t T, int64_t P, size_t* bytes)
+    {
+        Carver c(p);
+        BatchImageState s;
+        s.ranges = c.take<uint2>(T);
+        s.tile_last = c.take<uint32_t>(T);
+        s.n_contrib = c.take<uint32_t>(P);
+        s.qkeys = c.take<uint64_t>(T);
+        s.qkeys_alt = c.take<uint64_t>(T);
+        s.order = c.take<uint32_t>(T);
+        s.order_alt = c.take<uint32_t>(T);
+        s.exact_list = c.take<uint32_t>(T);
+        s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
+        s.ctr = c.take<uint32_t>(8);
+        if (bytes) *bytes = c.size();
+        return s;
+    }
+};
+
+// Backward workspace: per-entry partial rows, then per-pair density and
+// subdivision-priority gradients before the sum over views.
+struct BatchBwdWork
+{
+    float* partial;   // [R][12]
+    float* pair_geo;  // [V*N][8]
+    float* pair_sub;  // [V*N]
+    static BatchBwdWork carve(void* p, int64_t VN, int64_t R, size_t* bytes)
+    {
+        Carver c(p);
+        BatchBwdWork w;
+        w.partial = c.take<float>(12 * R);
+        w.pair_geo = c.take<float>(8 * VN);
+        w.pair_sub = c.take<float>(VN);
+        if (bytes) *bytes = c.size();
+        return w;
+    }
+};
+
+__device__ inline int view_of_tile(const svr_view* __restrict__ views, int V, uint32_t g)
+{
+    int lo = 0, hi = V - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((uint32_t)views[mid].tile_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- preprocess and key duplication over (view, voxel) pairs -------------
+
+__global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* __restrict__ views, int64_t N,
+                                                               const float* __restrict__ centers,
+                                                               const float* __restrict__ lengths,
+                                                               int32_t* __restrict__ out_n_dup,
+                                                               uint32_t* __restrict__ n_dup,
+                                                               uint2* __restrict__ bboxes,
+                                                               uint32_t* __restrict__ qbits_out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const svr_view& vw = views[blockIdx.y];
+    const int W = vw.width, H = vw.height;
+    const float focal_x = 0.5f * (float)W / vw.tanfovx;
+    const float focal_y = 0.5f * (float)H / vw.tanfovy;
+    float m[12], cw[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+    {
+        m[i] = vw.w2c[i];
+        cw[i] = vw.c2w[i];
+    }
+    uint32_t nd, bits;
+    uint2 bb;
+    preprocess_voxel(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W, H,
+                     tiles_x(W), tiles_y(H), focal_x, focal_y, vw.cx, vw.cy, m, cw, vw.near_plane, nd, bb, bits);
+    const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
+    out_n_dup[p] = (int32_t)nd;
+    n_dup[p] = nd;
+    bboxes[p] = bb;
+    qbits_out[p] = bits;
+}
+
+__device__ inline void emit_entries(uint32_t pair_v, uint32_t off, uint2 bb, uint32_t bits, int gx, int gy,
+                                    uint32_t tile_base, int pos_bits, const uint32_t* __restrict__ order_pos,
+                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    int txmin, tymin, txmax, tymax;
+    rect_of(bb, gx, gy, txmin, tymin, txmax, tymax);
+    for (uint32_t q = 0; q < 8; ++q)
+    {
+        if (!(bits & (1u << q))) continue;
+        const uint64_t pos = order_pos[8 * (size_t)pair_v + q];
+        for (int y = tymin; y <= tymax; ++y)
+            for (int x = txmin; x <= txmax; ++x)
+            {
+                const uint32_t g = tile_base + (uint32_t)y * (uint32_t)gx + (uint32_t)x;
+                keys[off] = ((uint64_t)g << pos_bits) | pos;
+                vals[off] = off;
+                ++off;
+            }
+    }
+}
+
+// forward.cu:523-577 per (view, voxel) pair; pairs with many entries are
+// queued (as u * N + v) for batch_duplicate_big_kernel.
+__global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __restrict__ views, int64_t N,
+                                                              int pos_bits, const uint32_t* __restrict__ n_dup,
+                                                              const uint32_t* __restrict__ scan,
+                                                              const uint2* __restrict__ bboxes,
+                                                              const uint32_t* __restrict__ qbits,
+                                                              const uint32_t* __restrict__ order_pos,
+                                                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                              uint32_t* __restrict__ big_list,
+                                                              uint32_t* __restrict__ big_count)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
+    const uint32_t nd = n_dup[p];
+    if (nd == 0) return;
+    if (nd > BIG_DUP)
+    {
+        big_list[atomicAdd(big_count, 1u)] = (uint32_t)p;
+        return;
+    }
+    const svr_view& vw = views[blockIdx.y];
+    emit_entries((uint32_t)v, scan[p] - nd, bboxes[p], qbits[p], tiles_x(vw.width), tiles_y(vw.height),
+                 (uint32_t)vw.tile_base, pos_bits, order_pos, keys, vals);
+}
+
+__global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view* __restrict__ views, int64_t N,
+                                                                  int pos_bits, const uint32_t* __restrict__ n_dup,
+                                                                  const uint32_t* __restrict__ scan,
+                                                                  const uint2* __restrict__ bboxes,
+                                                                  const uint32_t* __restrict__ qbits,
+                                                                  const uint32_t* __restrict__ order_pos,
+                                                                  uint64_t* __restrict__ keys,
+                                                                  uint32_t* __restrict__ vals,
+                                                                  const uint32_t* __restrict__ big_list,
+                                                                  const uint32_t* __restrict__ big_count)
+{
+    const uint32_t count = *big_count;
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    {
+        const uint32_t p = big_list[i];
+        const uint32_t u = (uint32_t)(p / (uint64_t)N), v = (uint32_t)(p - (uint64_t)u * (uint64_t)N);
+        const svr_view& vw = views[u];
+        const int gx = tiles_x(vw.width), gy = tiles_y(vw.height);
+        const uint32_t nd = n_dup[p];
+        const uint32_t off = scan[p] - nd;
+        int txmin, tymin, txmax, tymax;
+        rect_of(bboxes[p], gx, gy, txmin, tymin, txmax, tymax);
+        const uint32_t w = (uint32_t)(txmax - txmin + 1);
+        const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
+        const uint32_t bits = qbits[p];
+        for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x)
+        {
+            const uint32_t qi = e / tiles, rem = e - qi * tiles;
+            uint32_t b = bits;
+            for (uint32_t s = 0; s < qi; ++s) b &= b - 1u;
+            const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
+            const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
+            const uint32_t g = (uint32_t)vw.tile_base + y * (uint32_t)gx + x;
+            keys[off + e] = ((uint64_t)g << pos_bits) | (uint64_t)order_pos[8 * (size_t)v + q];
+            vals[off + e] = off + e;
+        }
+    }
+}
+
+// Queue keys: tiles bucketed by list length, longest bucket first
+// (bucket = bit length of the length, key = 32 - bucket).
+__global__ __launch_bounds__(256) void queue_keys_kernel(int64_t T, const uint2* __restrict__ ranges,
+                                                         uint64_t* __restrict__ qkeys, uint32_t* __restrict__ order)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= T) return;
+    const uint2 r = ranges[g];
+    const uint32_t len = r.y - r.x;
+    const uint32_t bucket = len ? 32u - (uint32_t)__clz((int)len) : 0u;
+    qkeys[g] = 32u - bucket;
+    order[g] = (uint32_t)g;
+}
+
+// ---- compositor over the tile queue ---------------------------------------
+
+struct TileGeom
+{
+    int u, W, H, tx, ty, sx0, sy0, sx1, sy1, px, py;
+    bool inside;
+};
+
+__device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, uint32_t g, int w, int lane)
+{
+    TileGeom t;
+    t.u = view_of_tile(views, V, g);
+    const svr_view& vw = views[t.u];
+    t.W = vw.width;
+    t.H = vw.height;
+    const int gx = tiles_x(t.W);
+    const uint32_t tile = g - (uint32_t)vw.tile_base;
+    t.tx = (int)(tile % (uint32_t)gx);
+    t.ty = (int)(tile / (uint32_t)gx);
+    t.sx0 = t.tx * TILE_X + (w & 1) * 8;
+    t.sy0 = t.ty * TILE_Y + (w >> 1) * 8;
+    t.sx1 = min(t.sx0 + 7, t.W - 1);
+    t.sy1 = min(t.sy0 + 7, t.H - 1);
+    t.px = t.sx0 + (lane & 7);
+    t.py = t.sy0 + (lane >> 3);
+    t.inside = t.px < t.W && t.py < t.H;
+    return t;
+}
+
+// One tile of the batch forward.  The fast form (EXACT = false) defers a tile
+// holding a ray whose 1/rd is not finite to exact_list (count ctr[1]).
+template <int NS, bool EXACT>
+__device__ inline void batch_fwd_tile(Stage& st, uint32_t* s_last, uint32_t g, int V,
+                                      const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
+                                      uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges,
+                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                      const float* __restrict__ centers, const float* __restrict__ lengths,
+                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                      uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
+                                      float* __restrict__ out_color, float* __restrict__ out_T)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const TileGeom t = tile_geom(views, V, g, w, lane);
+    const svr_view& vw = views[t.u];
+    const PixRay ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
+    const bool need_exact = __syncthreads_or(!ray_inv_finite(ray)) != 0;
+    if (!EXACT && need_exact)
+    {
+        if (threadIdx.x == 0) exact_list[atomicAdd(&ctr[1], 1u)] = g;
+        return;
+    }
+    const uint2 range = ranges[g];
+    const size_t vo = (size_t)t.u * (size_t)N;
+    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+    const uint32_t last = composite_subtile<NS, EXACT>(st, w, lane, ray, t.inside, range, t.sx0, t.sy0, t.sx1, t.sy1,
+                                                       vox_list, bboxes + vo, centers, lengths, geos, rgbs + 3 * vo,
+                                                       T, C0, C1, C2);
+    if (t.inside)
+    {
+        const size_t HW = (size_t)t.H * t.W;
+        const size_t pid = (size_t)t.py * t.W + t.px;
+        const size_t pb = (size_t)vw.pix_base;
+        const float bg = vw.bg_color;
+        n_contrib[pb + pid] = last;
+        out_color[3 * pb + pid] = C0 + T * bg;
+        out_color[3 * pb + HW + pid] = C1 + T * bg;
+        out_color[3 * pb + 2 * HW + pid] = C2 + T * bg;
+        out_T[pb + pid] = T;
+    }
+    const uint32_t wm = wave_max_u32(t.inside ? last : 0u);
+    if (lane == 0) s_last[w] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_last[g] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+}
+
+// Fast form: workgroup b renders tile order[b] (the hardware dispatches
+// workgroups in index order, so the longest lists start first).
+template <int NS>
+__global__ __launch_bounds__(256) void batch_fwd_kernel(
+    int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order, uint32_t* __restrict__ ctr,
+    uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
+    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
+{
+    __shared__ Stage st;
+    __shared__ uint32_t s_last[NWAVE];
+    const uint32_t g = order[blockIdx.x];
+    batch_fwd_tile<NS, false>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers, lengths,
+                              geos, rgbs, tile_last, n_contrib, out_color, out_T);
+}
+
+// Exact form: a few workgroups loop over the deferred tiles (head ctr[2]).
+template <int NS>
+__global__ __launch_bounds__(256) void batch_fwd_exact_kernel(
+    int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr, uint32_t* __restrict__ exact_list,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
+    const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
+    float* __restrict__ out_color, float* __restrict__ out_T)
+{
+    __shared__ Stage st;
+    __shared__ uint32_t s_last[NWAVE];
+    __shared__ uint32_t s_i;
+    const uint32_t count = ctr[1];
+    while (true)
+    {
+        if (threadIdx.x == 0) s_i = atomicAdd(&ctr[2], 1u);
+        __syncthreads();
+        const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
+        if (i >= count) break;
+        const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
+        batch_fwd_tile<NS, true>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers,
+                                 lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T);
+        __syncthreads();
+    }
+}
+
+template <int NS, bool EXACT>
+__device__ inline void batch_bwd_tile(Stage& st, float (&acc)[12][NWAVE][64], uint32_t g, int V,
+                                      const svr_view* __restrict__ views, int64_t N, const uint2* __restrict__ ranges,
+                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                      const float* __restrict__ centers, const float* __restrict__ lengths,
+                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                      const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+                                      const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+                                      const uint32_t* __restrict__ orig, float* __restrict__ partial)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const TileGeom t = tile_geom(views, V, g, w, lane);
+    const svr_view& vw = views[t.u];
+    const PixRay ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
+    const bool need_exact = __syncthreads_or(!ray_inv_finite(ray)) != 0;
+    if (!EXACT && need_exact) return;  // rendered by the exact launch
+    const size_t vo = (size_t)t.u * (size_t)N;
+    const size_t pb = (size_t)vw.pix_base;
+    const size_t HW = (size_t)t.H * t.W;
+    const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
+    adjoint_subtile<NS, BWD_SYNC_MIN, EXACT>(st, acc, w, lane, t.W, t.H, t.tx, t.ty, t.sx0, t.sy0, t.sx1, t.sy1,
+                                             t.inside, pid, HW, ray, ranges[g], vw.bg_color, vox_list, bboxes + vo,
+                                             centers, lengths, geos, rgbs + 3 * vo, out_T + pb, n_contrib + pb,
+                                             dL_dcolor + 3 * pb, dL_dT + pb, orig, partial);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void batch_bwd_kernel(
+    int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
+    const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
+    float* __restrict__ partial)
+{
+    __shared__ Stage st;
+    __shared__ float acc[12][NWAVE][64];
+    const uint32_t g = order[blockIdx.x];
+    batch_bwd_tile<NS, false>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs, out_T,
+                              n_contrib, dL_dcolor, dL_dT, orig, partial);
+}
+
+// The forward's deferred tiles (exact_list, count ctr[1]; head ctr[4]).
+template <int NS>
+__global__ __launch_bounds__(256) void batch_bwd_exact_kernel(
+    int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
+    const uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
+    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+    const uint32_t* __restrict__ orig, float* __restrict__ partial)
+{
+    __shared__ Stage st;
+    __shared__ float acc[12][NWAVE][64];
+    __shared__ uint32_t s_i;
+    const uint32_t count = ctr[1];
+    while (true)
+    {
+        if (threadIdx.x == 0) s_i = atomicAdd(&ctr[4], 1u);
+        __syncthreads();
+        const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
+        if (i >= count) break;
+        const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
+        batch_bwd_tile<NS, true>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
+                                 out_T, n_contrib, dL_dcolor, dL_dT, orig, partial);
+        __syncthreads();
+    }
+}
+
+// Per voxel: gradients of the shared parameters summed over the views in
+// view order (the order in which a per-view training loop accumulates them).
+__global__ __launch_bounds__(256) void batch_combine_kernel(int V, int64_t N, const float* __restrict__ pair_geo,
+                                                            const float* __restrict__ pair_sub,
+                                                            float* __restrict__ dgeos, float* __restrict__ dsub)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    float4 a0 = reinterpret_cast<const float4*>(pair_geo + 8 * v)[0];
+    float4 a1 = reinterpret_cast<const float4*>(pair_geo + 8 * v)[1];
+    float s = pair_sub[v];
+    for (int u = 1; u < V; ++u)
+    {
+        const size_t p = (size_t)u * (size_t)N + (size_t)v;
+        const float4 b0 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[0];
+        const float4 b1 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[1];
+        a0.x += b0.x; a0.y += b0.y; a0.z += b0.z; a0.w += b0.w;
+        a1.x += b1.x; a1.y += b1.y; a1.z += b1.z; a1.w += b1.w;
+        s += pair_sub[p];
+    }
+    reinterpret_cast<float4*>(dgeos + 8 * v)[0] = a0;
+    reinterpret_cast<float4*>(dgeos + 8 * v)[1] = a1;
+    dsub[v] = s;
+}
+
+// ---- SH colour of all views ----------------------------------------------
+
+constexpr int BSH_PAD = 49;
+
+// One wave per 64 consecutive voxels: their 48 coefficients are staged in LDS
+// with contiguous loads, then each lane evaluates its voxel for every view
+// where it has entries (sh_compute.cu:55-106 arithmetic).
+__global__ __launch_bounds__(64) void batch_sh_fwd_kernel(int deg, int V, const svr_view* __restrict__ views,
+                                                          int64_t N, const float* __restrict__ centers,
+                                                          const uint32_t* __restrict__ n_dup,
+                                                          const float* __restrict__ sh0,
+                                                          const float* __restrict__ shs, float* __restrict__ rgbs)
+{
+    __shared__ float buf[64 * BSH_PAD];
+    const int lane = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int nv = (int)min((int64_t)64, N - v0);
+    for (int e = lane; e < nv * 3; e += 64)
+    {
+        const int k = e / 3, f = e - 3 * k;
+        buf[k * BSH_PAD + f] = sh0[3 * v0 + e];
+    }
+    for (int e = lane; e < nv * 45; e += 64)
+    {
+        const int k = e / 45, f = e - 45 * k;
+        buf[k * BSH_PAD + 3 + f] = shs[45 * v0 + e];
+    }
+    wave_sync();
+    if (lane >= nv) return;
+    const int64_t v = v0 + lane;
+    const float* row = buf + lane * BSH_PAD;
+    for (int u = 0; u < V; ++u)
+    {
+        const size_t p = (size_t)u * (size_t)N + (size_t)v;
+        float* out = rgbs + 3 * p;
+        if (n_dup[p] == 0)
+        {
+            out[0] = 0.f; out[1] = 0.f; out[2] = 0.f;
+            continue;
+        }
+        const svr_view& vw = views[u];
+        const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
+        float x, y, z, b[16];
+        sh_dir(centers, v, cam, x, y, z);
+        const int nb = sh_basis(deg, x, y, z, b);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            float res = SH_C0 * row[ch];
+            if (nb > 1) res = res - (SH_C1 * y) * row[3 + ch] + (SH_C1 * z) * row[6 + ch] - (SH_C1 * x) * row[9 + ch];
+            for (int k = 4; k < nb; ++k) res = res + b[k] * row[3 * k + ch];
+            res = res + 0.5f;
+            out[ch] = res > 0.f ? res : 0.f;
+        }
+    }
+}
+
+// sh_compute.cu:110-176 per view, summed over the views in registers; the
+// 48 sums of the wave's 64 voxels leave through LDS with contiguous stores.
+__global__ __launch_bounds__(64) void batch_sh_bwd_kernel(int deg, int V, const svr_view* __restrict__ views,
+                                                          int64_t N, const float* __restrict__ centers,
+                                                          const uint32_t* __restrict__ n_dup,
+                                                          const float* __restrict__ dL_drgbs,
+                                                          float* __restrict__ dsh0, float* __restrict__ dshs)
+{
+    __shared__ float buf[64 * BSH_PAD];
+    const int lane = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int nv = (int)min((int64_t)64, N - v0);
+    if (lane < nv)
+    {
+        const int64_t v = v0 + lane;
+        float a[48];
+#pragma unroll
+        for (int k = 0; k < 48; ++k) a[k] = 0.f;
+        for (int u = 0; u < V; ++u)
+        {
+            const size_t p = (size_t)u * (size_t)N + (size_t)v;
+            if (n_dup[p] == 0) continue;
+            const svr_view& vw = views[u];
+            const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
+            float x, y, z, b[16];
+            sh_dir(centers, v, cam, x, y, z);
+            const int nb = sh_basis(deg, x, y, z, b);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch)
+            {
+                const float g = dL_drgbs[3 * p + ch];
+                a[ch] += SH_C0 * g;
+#pragma unroll
+                for (int k = 1; k < 16; ++k)
+                    if (k < nb) a[3 * k + ch] += b[k] * g;
+            }
+        }
+        float* row = buf + lane * BSH_PAD;
+#pragma unroll
+        for (int k = 0; k < 48; ++k) row[k] = a[k];
+    }
+    wave_sync();
+    for (int e = lane; e < nv * 3; e += 64)
+    {
+        const int k = e / 3, f = e - 3 * k;
+        dsh0[3 * v0 + e] = buf[k * BSH_PAD + f];
+    }
+    for (int e = lane; e < nv * 45; e += 64)
+    {
+        const int k = e / 45, f = e - 45 * k;
+        dshs[45 * v0 + e] = buf[k * BSH_PAD + 3 + f];
+    }
+}
+
+// ---- host side --------------------------------------------------------------
+
+// Workgroups of the launches that loop over the deferred exact-ray tiles.
+constexpr int64_t EXACT_GRID = 256;
+
+int check_views(int V, const svr_view* views, int64_t* T_out, int64_t* P_out)
+{
+    SVR_REQUIRE(V >= 1 && V <= 65536, SVR_ERR_INVALID, "number of views %d out of range", V);
+    SVR_REQUIRE(views != nullptr, SVR_ERR_INVALID, "null view table");
+    int64_t T = 0, P = 0;
+    for (int u = 0; u < V; ++u)
+    {
+        const svr_view& vw = views[u];
+        SVR_REQUIRE(vw.width > 0 && vw.height > 0 && vw.width <= MAX_PACKED_IMAGE_DIM &&
+                        vw.height <= MAX_PACKED_IMAGE_DIM,
+                    SVR_ERR_INVALID, "view %d: image size %dx%d outside [1, %d]", u, vw.width, vw.height,
+                    MAX_PACKED_IMAGE_DIM);
+        SVR_REQUIRE(vw.tile_base == T && vw.pix_base == P, SVR_ERR_INVALID,
+                    "view %d: tile_base / pix_base must be the running sums of the previous views", u);
+        T += (int64_t)tiles_x(vw.width) * tiles_y(vw.height);
+        P += (int64_t)vw.width * vw.height;
+    }
+    SVR_REQUIRE(T < (1ll << 31), SVR_ERR_INVALID, "too many tiles in the batch");
+    *T_out = T;
+    *P_out = P;
+    return SVR_OK;
+}
+
+template <int NS>
+int batch_fwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
+                     const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
+                     const float* geos, const float* rgbs, float* out_color, float* out_T, hipStream_t stream)
+{
+    batch_fwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(V, views, N, img.order, img.ctr, img.exact_list, img.ranges,
+                                                          b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+                                                          img.tile_last, img.n_contrib, out_color, out_T);
+    SVR_CHECK_LAUNCH();
+    batch_fwd_exact_kernel<NS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), 256, 0, stream>>>(
+        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+        img.tile_last, img.n_contrib, out_color, out_T);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+template <int NS>
+int batch_bwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
+                     const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
+                     const float* geos, const float* rgbs, const float* out_T, const float* dL_dcolor,
+                     const float* dL_dT, float* partial, hipStream_t stream)
+{
+    batch_bwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(V, views, N, img.order, img.ranges, b.vox_list, g.bbox,
+                                                          centers, lengths, geos, rgbs, out_T, img.n_contrib,
+                                                          dL_dcolor, dL_dT, b.orig, partial);
+    SVR_CHECK_LAUNCH();
+    batch_bwd_exact_kernel<NS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), 256, 0, stream>>>(
+        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T,
+        img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+}  // namespace svr
+
+using namespace svr;
+
+extern "C" {
+
+size_t svr_batch_geom_state_size(int32_t V, int64_t N)
+{
+    size_t bytes = 0;
+    GeomState::carve(nullptr, (V > 0 ? V : 0) * (N > 0 ? N : 0), &bytes);
+    return bytes;
+}
+
+size_t svr_batch_image_state_size(int64_t T, int64_t P)
+{
+    size_t bytes = 0;
+    BatchImageState::carve(nullptr, T > 0 ? T : 0, P > 0 ? P : 0, &bytes);
+    return bytes;
+}
+
+size_t svr_batch_backward_workspace_size(int32_t V, int64_t N, int64_t R)
+{
+    size_t bytes = 0;
+    BatchBwdWork::carve(nullptr, (V > 0 ? V : 0) * (N > 0 ? N : 0), R > 0 ? R : 0, &bytes);
+    return bytes;
+}
+
+int svr_batch_preprocess(int32_t V, const svr_view* views_host, const svr_view* views_dev, int64_t N,
+                         const float* centers, const float* lengths, int32_t* out_n_dup, void* geom_state,
+                         svr_stream_t stream)
+{
+    int64_t T = 0, P = 0;
+    int rc = check_views(V, views_host, &T, &P);
+    if (rc != SVR_OK) return rc;
+    SVR_REQUIRE(N >= 0 && N <= (int64_t)VOX_ID_MASK + 1, SVR_ERR_INVALID, "voxel count %lld out of range",
+                (long long)N);
+    SVR_REQUIRE((int64_t)V * N < (1ll << 32), SVR_ERR_INVALID, "views x voxels must stay below 2^32");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(views_dev && centers && lengths && out_n_dup && geom_state, SVR_ERR_INVALID,
+                "svr_batch_preprocess: null pointer");
+    GeomState g = GeomState::carve(geom_state, (int64_t)V * N, nullptr);
+    prof_begin(PROF_PREPROCESS, stream);
+    batch_preprocess_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(
+        views_dev, N, centers, lengths, out_n_dup, g.n_dup, g.bbox, g.qbits);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_PREPROCESS, stream);
+    return SVR_OK;
+}
+
+int svr_batch_sh_compute(int32_t deg, int32_t V, const svr_view* views_dev, int64_t N, const float* centers,
+                         const void* geom_state, const float* sh0, const float* shs, float* rgbs, svr_stream_t stream)
+{
+    SVR_REQUIRE(deg >= 0 && deg <= 3, SVR_ERR_INVALID, "SH degree %d outside [0, 3]", deg);
+    SVR_REQUIRE(V >= 1, SVR_ERR_INVALID, "number of views must be positive");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(views_dev && centers && geom_state && sh0 && shs && rgbs, SVR_ERR_INVALID,
+                "svr_batch_sh_compute: null pointer");
+    GeomState g = GeomState::carve((void*)geom_state, (int64_t)V * N, nullptr);
+    prof_begin(PROF_SH_FWD, stream);
+    batch_sh_fwd_kernel<<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, V, views_dev, N, centers, g.n_dup, sh0,
+                                                                     shs, rgbs);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_SH_FWD, stream);
+    return SVR_OK;
+}
+
+int svr_batch_sh_compute_bw(int32_t deg, int32_t V, const svr_view* views_dev, int64_t N, const float* centers,
+                            const void* geom_state, const float* dL_drgbs, float* dsh0, float* dshs,
+                            svr_stream_t stream)
+{
+    SVR_REQUIRE(deg >= 0 && deg <= 3, SVR_ERR_INVALID, "SH degree %d outside [0, 3]", deg);
+    SVR_REQUIRE(V >= 1, SVR_ERR_INVALID, "number of views must be positive");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(views_dev && centers && geom_state && dL_drgbs && dsh0 && dshs, SVR_ERR_INVALID,
+                "svr_batch_sh_compute_bw: null pointer");
+    GeomState g = GeomState::carve((void*)geom_state, (int64_t)V * N, nullptr);
+    prof_begin(PROF_SH_BWD, stream);
+    batch_sh_bwd_kernel<<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, V, views_dev, N, centers, g.n_dup,
+                                                                     dL_drgbs, dsh0, dshs);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_SH_BWD, stream);
+    return SVR_OK;
+}
+
+int svr_batch_rasterize_forward(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                                int64_t N, const float* centers, const float* lengths, const float* geos,
+                                const float* rgbs, const void* geom_state, const void* order_table, int64_t R,
+                                void* binning_state, void* image_state, float* out_color, float* out_T,
+                                svr_stream_t stream)
+{
+    SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
+                MAX_N_SAMP);
+    int64_t T = 0, P = 0;
+    int rc = check_views(V, views_host, &T, &P);
+    if (rc != SVR_OK) return rc;
+    SVR_REQUIRE(R >= 0 && R < (1ll << 32), SVR_ERR_INVALID, "num_rendered out of range");
+    SVR_REQUIRE(views_dev && image_state && out_color && out_T, SVR_ERR_INVALID,
+                "svr_batch_rasterize_forward: null pointer");
+    SVR_REQUIRE(N == 0 || (geom_state && order_table && centers && lengths && geos && rgbs), SVR_ERR_INVALID,
+                "svr_batch_rasterize_forward: null pointer");
+    SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_batch_rasterize_forward: null binning state");
+    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
+    BatchImageState img = BatchImageState::carve(image_state, T, P, nullptr);
+    GeomState g = GeomState::carve((void*)geom_state, (int64_t)V * N, nullptr);
+    OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
+    BinningState b = BinningState::carve(binning_state, R, nullptr);
+
+    prof_begin(PROF_BINNING, stream);
+    SVR_REQUIRE(fill_u32(img.ranges, 0u, 2 * T, stream) == SVR_OK, SVR_ERR_HIP, "fill ranges failed");
+    SVR_REQUIRE(fill_u32(img.ctr, 0u, 8, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
+    const int pos_bits = (int)required_bits((uint64_t)8 * (N > 0 ? N : 1));
+    const int tile_bits = (int)required_bits((uint64_t)T);
+    if (R > 0)
+    {
+        SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
+        batch_duplicate_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(
+            views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, b.keys, b.orig, g.big_list,
+            g.counters + 2);
+        SVR_CHECK_LAUNCH();
+        batch_duplicate_big_kernel<<<1024, 256, 0, stream>>>(views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox,
+                                                             g.qbits, ot.pos, b.keys, b.orig, g.big_list,
+                                                             g.counters + 2);
+        SVR_CHECK_LAUNCH();
+        uint64_t* sorted = nullptr;
+        uint32_t* sorted_orig = nullptr;
+        rc = radix_sort_ex(b.keys, b.keys_alt, b.orig, b.orig_alt, R, pos_bits + tile_bits, b.counts, stream, &sorted,
+                           &sorted_orig);
+        if (rc != SVR_OK) return rc;
+        if (sorted != b.keys)
+        {
+            SVR_CHECK_HIP(hipMemcpyAsync(b.keys, sorted, sizeof(uint64_t) * R, hipMemcpyDeviceToDevice, stream));
+            SVR_CHECK_HIP(hipMemcpyAsync(b.orig, sorted_orig, sizeof(uint32_t) * R, hipMemcpyDeviceToDevice, stream));
+        }
+        rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
+        if (rc != SVR_OK) return rc;
+    }
+    // queue order: longest tile lists first (one 8-bit radix pass on the length bucket)
+    queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, img.ranges, img.qkeys, img.order);
+    SVR_CHECK_LAUNCH();
+    {
+        uint64_t* qk = nullptr;
+        uint32_t* qo = nullptr;
+        rc = radix_sort_ex(img.qkeys, img.qkeys_alt, img.order, img.order_alt, T, 8, img.qcounts, stream, &qk, &qo);
+        if (rc != SVR_OK) return rc;
+        if (qo != img.order)
+            SVR_CHECK_HIP(hipMemcpyAsync(img.order, qo, sizeof(uint32_t) * T, hipMemcpyDeviceToDevice, stream));
+    }
+    prof_end(PROF_BINNING, stream);
+
+    prof_begin(PROF_RENDER_FWD, stream);
+    switch (n_samp)
+    {
+    case 1: rc = batch_fwd_launch<1>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
+                                     stream); break;
+    case 2: rc = batch_fwd_launch<2>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
+                                     stream); break;
+    default: rc = batch_fwd_launch<3>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
+                                      stream); break;
+    }
+    prof_end(PROF_RENDER_FWD, stream);
+    return rc;
+}
+
+int svr_batch_rasterize_backward(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                                 int64_t N, const float* centers, const float* lengths, const float* geos,
+                                 const float* rgbs, const void* geom_state, int64_t R, const void* binning_state,
+                                 const void* image_state, const float* out_T, const float* dL_dcolor,
+                                 const float* dL_dT, void* workspace, float* dgeos, float* drgbs, float* dsub,
+                                 svr_stream_t stream)
+{
+    SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
+                MAX_N_SAMP);
+    int64_t T = 0, P = 0;
+    int rc = check_views(V, views_host, &T, &P);
+    if (rc != SVR_OK) return rc;
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(views_dev && geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub &&
+                    workspace,
+                SVR_ERR_INVALID, "svr_batch_rasterize_backward: null pointer");
+    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0 && ((uintptr_t)dgeos & 15u) == 0 && ((uintptr_t)workspace & 15u) == 0,
+                SVR_ERR_INVALID, "geos, dL_dgeos and workspace must be 16-byte aligned");
+    SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_batch_rasterize_backward: null binning state");
+    const int64_t VN = (int64_t)V * N;
+    GeomState g = GeomState::carve((void*)geom_state, VN, nullptr);
+    BinningState b = BinningState::carve((void*)binning_state, R, nullptr);
+    BatchImageState img = BatchImageState::carve((void*)image_state, T, P, nullptr);
+    BatchBwdWork wk = BatchBwdWork::carve(workspace, VN, R, nullptr);
+
+    if (R > 0)
+    {
+        if (fill_u32(wk.partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(img.ctr + 3, 0u, 2, stream) != SVR_OK) return SVR_ERR_HIP;
+        prof_begin(PROF_RENDER_BWD, stream);
+        switch (n_samp)
+        {
+        case 1: rc = batch_bwd_launch<1>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_T,
+                                         dL_dcolor, dL_dT, wk.partial, stream); break;
+        case 2: rc = batch_bwd_launch<2>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_T,
+                                         dL_dcolor, dL_dT, wk.partial, stream); break;
+        default: rc = batch_bwd_launch<3>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_T,
+                                          dL_dcolor, dL_dT, wk.partial, stream); break;
+        }
+        prof_end(PROF_RENDER_BWD, stream);
+        if (rc != SVR_OK) return rc;
+    }
+    prof_begin(PROF_GRAD_REDUCE, stream);
+    if (R > 0)
+    {
+        rc = grad_reduce_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, stream);
+        if (rc != SVR_OK) return rc;
+    }
+    else
+    {
+        if (fill_u32(wk.pair_geo, 0u, 8 * VN, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(drgbs, 0u, 3 * VN, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(wk.pair_sub, 0u, VN, stream) != SVR_OK) return SVR_ERR_HIP;
+    }
+    batch_combine_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(V, N, wk.pair_geo, wk.pair_sub, dgeos,
+                                                                          dsub);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_GRAD_REDUCE, stream);
+    return SVR_OK;
+}
+
+int svr_batch_unpack_image_state(int64_t T, int64_t P, const void* image_state, int32_t* ranges, int32_t* tile_last,
+                                 int32_t* n_contrib, svr_stream_t stream)
+{
+    SVR_REQUIRE(image_state && ranges && tile_last && n_contrib, SVR_ERR_INVALID,
+                "svr_batch_unpack_image_state: null pointer");
+    BatchImageState img = BatchImageState::carve((void*)image_state, T, P, nullptr);
+    SVR_CHECK_HIP(hipMemcpyAsync(ranges, img.ranges, sizeof(uint32_t) * 2 * T, hipMemcpyDeviceToDevice, stream));
+    SVR_CHECK_HIP(hipMemcpyAsync(tile_last, img.tile_last, sizeof(uint32_t) * T, hipMemcpyDeviceToDevice, stream));
+    SVR_CHECK_HIP(hipMemcpyAsync(n_contrib, img.n_contrib, sizeof(uint32_t) * P, hipMemcpyDeviceToDevice, stream));
+    return SVR_OK;
+}
+
+}  // extern "C"

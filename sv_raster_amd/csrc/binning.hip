@@ -11,7 +11,7 @@
 // pair, ties by (v, q) exactly as the reference's stable sort orders them).
 // Per frame the key is (tile << pos_bits) | pos, which sorts into exactly the
 // reference order with 30..40 significant bits instead of 76.
-#include "svr_device.h"
+#include "voxel_ops.h"
 #include "svr_host.h"
 
 namespace svr {
@@ -268,15 +268,6 @@ struct OrderScratch
 
 // ---- per-frame duplication, ranges and key export --------------------
 
-__device__ inline void rect_of(uint2 bb, int gx, int gy, int& txmin, int& tymin, int& txmax, int& tymax)
-{
-    const uint32_t xmin = bb.x >> 16, ymin = bb.x & 0xFFFFu, xmax = bb.y >> 16, ymax = bb.y & 0xFFFFu;
-    txmin = min((int)(xmin / TILE_X), gx - 1);
-    tymin = min((int)(ymin / TILE_Y), gy - 1);
-    txmax = min((int)(xmax / TILE_X), gx - 1);
-    tymax = min((int)(ymax / TILE_Y), gy - 1);
-}
-
 // forward.cu:523-577, emitting (tile << pos_bits | pos) keys.
 __global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int gy, int pos_bits,
                                                         const uint32_t* __restrict__ n_dup,
@@ -391,6 +382,14 @@ __global__ void export_keys_kernel(const uint64_t* __restrict__ keys, const uint
     vals[i] = val;
 }
 
+int ranges_launch(const uint64_t* keys, int64_t R, int pos_bits, const uint32_t* order_pair, uint2* ranges,
+                  uint32_t* vox_list, hipStream_t stream)
+{
+    ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(keys, R, pos_bits, order_pair, ranges, vox_list);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
                     BinningState& b, ImageState& img, hipStream_t stream)
 {
@@ -423,9 +422,8 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
         SVR_CHECK_HIP(hipMemcpyAsync(b.keys, sorted, sizeof(uint64_t) * R, hipMemcpyDeviceToDevice, stream));
         SVR_CHECK_HIP(hipMemcpyAsync(b.orig, sorted_orig, sizeof(uint32_t) * R, hipMemcpyDeviceToDevice, stream));
     }
-    ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(b.keys, R, pos_bits, ot.pair, img.ranges,
-                                                                   b.vox_list);
-    SVR_CHECK_LAUNCH();
+    rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
+    if (rc != SVR_OK) return rc;
     prof_end(PROF_BINNING, stream);
     return SVR_OK;
 }

@@ -3,13 +3,12 @@
 //
 // Reference: backends/new_cuda/src/preprocess.cu:24-199 (kernel + host),
 // forward.cu:649-663 (scan + reading the number of rendered entries).
-#include "svr_device.h"
+#include "voxel_ops.h"
 #include "svr_host.h"
 
 namespace svr {
 
-// preprocess.cu:24-132.  One thread per voxel; the arithmetic order matches
-// the oracle exactly (no contraction), so the packed bboxes are bit-identical.
+// One thread per voxel.
 __global__ __launch_bounds__(256) void preprocess_kernel(
     int64_t N, int W, int H, int gx, int gy, float focal_x, float focal_y, float cx, float cy,
     const float* __restrict__ w2c, const float* __restrict__ c2w, float near_plane,
@@ -19,64 +18,17 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
-
-    const float c0 = centers[3 * v + 0], c1 = centers[3 * v + 1], c2 = centers[3 * v + 2];
-    const float r = 0.5f * lengths[v];
-    const float ro0 = c2w[3], ro1 = c2w[7], ro2 = c2w[11];
-    float m[12];
+    float m[12], cw[12];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) m[i] = w2c[i];
-
-    uint32_t nd = 0, bits = 0;
-    uint2 bb = make_uint2(0u, 0u);
-    const float rel0 = c0 - ro0, rel1 = c1 - ro1, rel2 = c2 - ro2;
-    if (!(rel0 * rel0 + rel1 * rel1 + rel2 * rel2 < near_plane * near_plane))
+    for (int i = 0; i < 12; ++i)
     {
-        float umin = 1e9f, vmin = 1e9f, umax = -1e9f, vmax = -1e9f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-        {
-            const float sx = (float)(((i & 4) >> 2) * 2 - 1);
-            const float sy = (float)(((i & 2) >> 1) * 2 - 1);
-            const float sz = (float)((i & 1) * 2 - 1);
-            const float p0 = c0 + r * sx, p1 = c1 + r * sy, p2 = c2 + r * sz;
-            const float camx = m[0] * p0 + m[1] * p1 + m[2] * p2 + m[3];
-            const float camy = m[4] * p0 + m[5] * p1 + m[6] * p2 + m[7];
-            const float camz = m[8] * p0 + m[9] * p1 + m[10] * p2 + m[11];
-            if (camz < near_plane) continue;
-            const float iz = 1.0f / camz;
-            const float u = camx * iz, vv = camy * iz;
-            const uint32_t q = ((p0 < ro0) << 2) | ((p1 < ro1) << 1) | (p2 < ro2);
-            umin = smin(umin, u); vmin = smin(vmin, vv);
-            umax = smax(umax, u); vmax = smax(vmax, vv);
-            bits |= 1u << q;
-        }
-        const float cxh = cx - 0.5f, cyh = cy - 0.5f;
-        const float bminx = fmaxf(focal_x * umin + cxh, 0.0f);
-        const float bminy = fmaxf(focal_y * vmin + cyh, 0.0f);
-        const float bmaxx = fminf(focal_x * umax + cxh, (float)W);
-        const float bmaxy = fminf(focal_y * vmax + cyh, (float)H);
-        if (!(bminx > bmaxx || bminy > bmaxy))
-        {
-            const uint32_t xmin = (uint32_t)(int)__builtin_rintf(bminx);
-            const uint32_t ymin = (uint32_t)(int)__builtin_rintf(bminy);
-            const uint32_t xmax = (uint32_t)(int)__builtin_rintf(bmaxx);
-            const uint32_t ymax = (uint32_t)(int)__builtin_rintf(bmaxy);
-            bb = make_uint2((xmin << 16) | ymin, (xmax << 16) | ymax);
-            // auxiliary.h:150-170
-            const int txmin = min((int)(xmin / TILE_X), gx - 1), tymin = min((int)(ymin / TILE_Y), gy - 1);
-            const int txmax = min((int)(xmax / TILE_X), gx - 1), tymax = min((int)(ymax / TILE_Y), gy - 1);
-            nd = (uint32_t)((1 + tymax - tymin) * (1 + txmax - txmin)) * (uint32_t)__popc(bits);
-        }
-        else
-        {
-            bits = 0;
-        }
+        m[i] = w2c[i];
+        cw[i] = c2w[i];
     }
-    else
-    {
-        bits = 0;
-    }
+    uint32_t nd, bits;
+    uint2 bb;
+    preprocess_voxel(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W, H, gx, gy,
+                     focal_x, focal_y, cx, cy, m, cw, near_plane, nd, bb, bits);
     out_n_dup[v] = (int32_t)nd;
     n_dup[v] = nd;
     bboxes[v] = bb;

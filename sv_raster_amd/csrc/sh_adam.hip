@@ -5,50 +5,10 @@
 // utils.cu:106-189.
 #include <math.h>
 
-#include "svr_device.h"
+#include "voxel_ops.h"
 #include "svr_host.h"
 
 namespace svr {
-
-// sh_compute.cu:33-50 (real SH normalisations with their signs)
-__constant__ float SH_C2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
-                               -1.0925484305920792f, 0.5462742152960396f};
-__constant__ float SH_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
-                               0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
-                               -0.5900435899266435f};
-constexpr float SH_C0 = 0.28209479177387814f;
-constexpr float SH_C1 = 0.4886025119029199f;
-
-__device__ inline int sh_basis(int deg, float x, float y, float z, float* b)
-{
-    b[0] = SH_C0;
-    if (deg < 1) return 1;
-    b[1] = -SH_C1 * y; b[2] = SH_C1 * z; b[3] = -SH_C1 * x;
-    if (deg < 2) return 4;
-    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    b[4] = SH_C2[0] * xy;
-    b[5] = SH_C2[1] * yz;
-    b[6] = SH_C2[2] * (2.f * zz - xx - yy);
-    b[7] = SH_C2[3] * xz;
-    b[8] = SH_C2[4] * (xx - yy);
-    if (deg < 3) return 9;
-    b[9] = SH_C3[0] * y * (3.f * xx - yy);
-    b[10] = SH_C3[1] * xy * z;
-    b[11] = SH_C3[2] * y * (4.f * zz - xx - yy);
-    b[12] = SH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
-    b[13] = SH_C3[4] * x * (4.f * zz - xx - yy);
-    b[14] = SH_C3[5] * z * (xx - yy);
-    b[15] = SH_C3[6] * x * (xx - 3.f * yy);
-    return 16;
-}
-
-__device__ inline void sh_dir(const float* __restrict__ centers, int64_t v, const float* __restrict__ cam, float& x,
-                              float& y, float& z)
-{
-    const float dx = centers[3 * v] - cam[0], dy = centers[3 * v + 1] - cam[1], dz = centers[3 * v + 2] - cam[2];
-    const float rn = 1.f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
-    x = dx * rn; y = dy * rn; z = dz * rn;
-}
 
 // sh_compute.cu:55-106, 180-208
 __global__ __launch_bounds__(256) void sh_fwd_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,

@@ -162,8 +162,8 @@ struct ImageState
     }
 };
 
-inline int tiles_x(int W) { return (W + 15) / 16; }
-inline int tiles_y(int H) { return (H + 15) / 16; }
+__host__ __device__ inline int tiles_x(int W) { return (W + 15) / 16; }
+__host__ __device__ inline int tiles_y(int H) { return (H + 15) / 16; }
 
 // Stable LSD radix sort of 64-bit keys (optionally with 32-bit values) over
 // bits [0, nbits).  The sorted data ends in the buffers returned through
@@ -176,6 +176,16 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
                     BinningState& b, ImageState& img, hipStream_t stream);
 
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
+
+// Tile ranges and the sorted entry list from sorted (tile << pos_bits | pos)
+// keys; ranges must be zeroed first.
+int ranges_launch(const uint64_t* keys, int64_t R, int pos_bits, const uint32_t* order_pair, uint2* ranges,
+                  uint32_t* vox_list, hipStream_t stream);
+
+// Per-voxel (or per (view, voxel) pair) sums of the per-entry partial
+// gradient rows over each pair's contiguous emission-order segment.
+int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
+                       float* dsub, hipStream_t stream);
 
 // Live stage timing: when enabled, prof_begin/prof_end record a HIP event pair
 // on the launch stream around the kernels of one stage.

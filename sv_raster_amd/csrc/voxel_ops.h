@@ -1,0 +1,130 @@
+// Per-voxel device helpers shared by the single-view and multi-view paths:
+// the preprocess of one voxel for one camera, the tile rectangle of a packed
+// bbox and the real spherical-harmonics basis.
+//
+// Reference: backends/new_cuda/src/preprocess.cu:24-132, auxiliary.h:150-170,
+// sh_compute.cu:33-106.
+#pragma once
+
+#include "svr_device.h"
+
+namespace svr {
+
+// preprocess.cu:24-132 for one voxel (centre c0..c2, half size r) seen by one
+// camera: near-sphere cull, 8-corner projection, packed pixel bbox, quadrant
+// bitset and the number of list entries (tiles x quadrants).  The arithmetic
+// order matches the oracle exactly (no contraction), so the packed bboxes are
+// bit-identical.
+__device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, int W, int H, int gx, int gy,
+                                        float focal_x, float focal_y, float cx, float cy, const float* w2c,
+                                        const float* c2w, float near_plane, uint32_t& nd_out, uint2& bb_out,
+                                        uint32_t& bits_out)
+{
+    const float ro0 = c2w[3], ro1 = c2w[7], ro2 = c2w[11];
+    const float* m = w2c;
+    uint32_t nd = 0, bits = 0;
+    uint2 bb = make_uint2(0u, 0u);
+    const float rel0 = c0 - ro0, rel1 = c1 - ro1, rel2 = c2 - ro2;
+    if (!(rel0 * rel0 + rel1 * rel1 + rel2 * rel2 < near_plane * near_plane))
+    {
+        float umin = 1e9f, vmin = 1e9f, umax = -1e9f, vmax = -1e9f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+        {
+            const float sx = (float)(((i & 4) >> 2) * 2 - 1);
+            const float sy = (float)(((i & 2) >> 1) * 2 - 1);
+            const float sz = (float)((i & 1) * 2 - 1);
+            const float p0 = c0 + r * sx, p1 = c1 + r * sy, p2 = c2 + r * sz;
+            const float camx = m[0] * p0 + m[1] * p1 + m[2] * p2 + m[3];
+            const float camy = m[4] * p0 + m[5] * p1 + m[6] * p2 + m[7];
+            const float camz = m[8] * p0 + m[9] * p1 + m[10] * p2 + m[11];
+            if (camz < near_plane) continue;
+            const float iz = 1.0f / camz;
+            const float u = camx * iz, vv = camy * iz;
+            const uint32_t q = ((p0 < ro0) << 2) | ((p1 < ro1) << 1) | (p2 < ro2);
+            umin = smin(umin, u); vmin = smin(vmin, vv);
+            umax = smax(umax, u); vmax = smax(vmax, vv);
+            bits |= 1u << q;
+        }
+        const float cxh = cx - 0.5f, cyh = cy - 0.5f;
+        const float bminx = fmaxf(focal_x * umin + cxh, 0.0f);
+        const float bminy = fmaxf(focal_y * vmin + cyh, 0.0f);
+        const float bmaxx = fminf(focal_x * umax + cxh, (float)W);
+        const float bmaxy = fminf(focal_y * vmax + cyh, (float)H);
+        if (!(bminx > bmaxx || bminy > bmaxy))
+        {
+            const uint32_t xmin = (uint32_t)(int)__builtin_rintf(bminx);
+            const uint32_t ymin = (uint32_t)(int)__builtin_rintf(bminy);
+            const uint32_t xmax = (uint32_t)(int)__builtin_rintf(bmaxx);
+            const uint32_t ymax = (uint32_t)(int)__builtin_rintf(bmaxy);
+            bb = make_uint2((xmin << 16) | ymin, (xmax << 16) | ymax);
+            // auxiliary.h:150-170
+            const int txmin = min((int)(xmin / TILE_X), gx - 1), tymin = min((int)(ymin / TILE_Y), gy - 1);
+            const int txmax = min((int)(xmax / TILE_X), gx - 1), tymax = min((int)(ymax / TILE_Y), gy - 1);
+            nd = (uint32_t)((1 + tymax - tymin) * (1 + txmax - txmin)) * (uint32_t)__popc(bits);
+        }
+        else
+        {
+            bits = 0;
+        }
+    }
+    else
+    {
+        bits = 0;
+    }
+    nd_out = nd;
+    bb_out = bb;
+    bits_out = bits;
+}
+
+// Tile rectangle of a packed bbox (auxiliary.h:150-170).
+__device__ inline void rect_of(uint2 bb, int gx, int gy, int& txmin, int& tymin, int& txmax, int& tymax)
+{
+    const uint32_t xmin = bb.x >> 16, ymin = bb.x & 0xFFFFu, xmax = bb.y >> 16, ymax = bb.y & 0xFFFFu;
+    txmin = min((int)(xmin / TILE_X), gx - 1);
+    tymin = min((int)(ymin / TILE_Y), gy - 1);
+    txmax = min((int)(xmax / TILE_X), gx - 1);
+    tymax = min((int)(ymax / TILE_Y), gy - 1);
+}
+
+// sh_compute.cu:33-50 (real SH normalisations with their signs)
+constexpr float SH_C2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                            -1.0925484305920792f, 0.5462742152960396f};
+constexpr float SH_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                            0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                            -0.5900435899266435f};
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+
+__device__ inline int sh_basis(int deg, float x, float y, float z, float* b)
+{
+    b[0] = SH_C0;
+    if (deg < 1) return 1;
+    b[1] = -SH_C1 * y; b[2] = SH_C1 * z; b[3] = -SH_C1 * x;
+    if (deg < 2) return 4;
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    b[4] = SH_C2[0] * xy;
+    b[5] = SH_C2[1] * yz;
+    b[6] = SH_C2[2] * (2.f * zz - xx - yy);
+    b[7] = SH_C2[3] * xz;
+    b[8] = SH_C2[4] * (xx - yy);
+    if (deg < 3) return 9;
+    b[9] = SH_C3[0] * y * (3.f * xx - yy);
+    b[10] = SH_C3[1] * xy * z;
+    b[11] = SH_C3[2] * y * (4.f * zz - xx - yy);
+    b[12] = SH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
+    b[13] = SH_C3[4] * x * (4.f * zz - xx - yy);
+    b[14] = SH_C3[5] * z * (xx - yy);
+    b[15] = SH_C3[6] * x * (xx - 3.f * yy);
+    return 16;
+}
+
+__device__ inline void sh_dir(const float* __restrict__ centers, int64_t v, const float* __restrict__ cam, float& x,
+                              float& y, float& z)
+{
+    const float dx = centers[3 * v] - cam[0], dy = centers[3 * v + 1] - cam[1], dz = centers[3 * v + 2] - cam[2];
+    const float rn = 1.f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    x = dx * rn; y = dy * rn; z = dz * rn;
+}
+
+}  // namespace svr
