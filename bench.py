@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--fwd-iters", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=8, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--batch-config", default="D", help="multi-view workload for the batched-vs-plain figure "
+                                                          "('none' to skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the bench, run 3 plain copy kernels (torch.neg) of this many MiB: the known-size "
                          "dispatch that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -113,6 +116,86 @@ def cpu_baseline(case, steps):
                 sample=f"{steps} full training steps of the same 1M-voxel 1920x1080 workload "
                        f"(oracle C++ restatement, OpenMP, {O.num_threads()} threads, after 1 warm-up step)",
                 sec_per_step=dt / steps)
+
+
+def batch_vs_plain(args, dev):
+    """Multi-view training on one octree: the batched path (all views of a step in one launch per stage, loss
+    summed over views, one SparseAdam step) against the plain path (one drop-in training step per view), both
+    in views per second."""
+    import torch
+    from sv_raster_amd import _C, batch, renderer, sparse_adam, synthetic as S
+    name = args.batch_config
+    sc = S.make_scene(name, seed=args.seed)
+    sizes = S.config_views(name, args.seed)
+    cams = [S.make_camera(W, H, seed=args.seed + 1000 + u) for u, (W, H) in enumerate(sizes)]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
+    targets = [t(S.make_target(c["H"], c["W"], args.seed + u)) for u, c in enumerate(cams)]
+    settings = [renderer.RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=c["W"], image_height=c["H"],
+                                        tanfovx=c["tanfovx"], tanfovy=c["tanfovy"], cx=c["cx"], cy=c["cy"],
+                                        w2c_matrix=t(c["w2c"]), c2w_matrix=t(c["c2w"])) for c in cams]
+    N = centers.shape[0]
+    V = len(cams)
+
+    def fresh():
+        geos = t(sc["geos"]).requires_grad_()
+        sh0 = t(sc["sh0"]).requires_grad_()
+        shs = t(sc["shs"]).requires_grad_()
+        opt = sparse_adam.SparseAdam([{"params": [geos], "lr": 0.025}, {"params": [sh0], "lr": 0.01},
+                                      {"params": [shs], "lr": 0.01}], betas=(0.1, 0.99), eps=1e-15, sparse=True)
+        return geos, sh0, shs, opt
+
+    _C.order_table_cache.get(octpath)
+    # plain: one training step per view
+    geos, sh0, shs, opt = fresh()
+    subdiv = torch.ones((N, 1), device=dev)
+
+    def plain_round():
+        for u in range(V):
+            def vox_fn(idx, cam_pos, color_mode):
+                rgbs = renderer.SH_eval.apply(3, idx, centers, cam_pos, None, sh0, shs)
+                return {"geos": geos, "rgbs": rgbs, "subdiv_p": subdiv}
+            color, _, _, _, _ = renderer.rasterize_voxels(settings[u], octpath, centers, lengths, vox_fn)
+            loss = torch.nn.functional.mse_loss(color, targets[u])
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+
+    plain_round()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        plain_round()
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / args.batch_steps
+
+    # batched: all views in one step
+    geos, sh0, shs, opt = fresh()
+    views = batch.ViewBatch(settings)
+
+    def batch_step():
+        colors, _ = batch.rasterize_voxels_batch(views, octpath, centers, lengths, geos, sh0, shs)
+        loss = sum(torch.nn.functional.mse_loss(c, tg) for c, tg in zip(colors, targets))
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    batch_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        batch_step()
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / args.batch_steps
+    pix = sum(W * H for W, H in sizes)
+    return {"workload": f"{name}: {N} voxels, {V} views of uneven resolution ({pix / 1e6:.2f} Mpix per round), "
+                        "fwd+bwd+sparse Adam",
+            "views": V, "rounds_timed": args.batch_steps,
+            "plain_views_per_s": V / plain_s, "batched_views_per_s": V / batch_s,
+            "batched_over_plain": plain_s / batch_s,
+            "plain_ms_per_round": plain_s * 1e3, "batched_ms_per_step": batch_s * 1e3,
+            "note": "plain = one drop-in training step (rasterize_voxels + SH_eval, MSE, SparseAdam) per view; "
+                    "batched = rasterize_voxels_batch over all views, summed MSE, one SparseAdam step"}
 
 
 def main():
@@ -233,6 +316,10 @@ def main():
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
     traffic = load_traffic(dom, args.config, N, W, H)
 
+    batched = None
+    if args.batch_config.lower() != "none":
+        batched = batch_vs_plain(args, dev)
+
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(case, args.cpu_steps)
@@ -271,6 +358,7 @@ def main():
         "step_gbs_algorithmic": step_bytes / (ms_per_step * 1e-3) / 1e9,
         "tile_entry_pixel_pairs_per_s_fwd": E * 256 / (stage_ms["render_fwd"] * 1e-3),
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(result), flush=True)
 
