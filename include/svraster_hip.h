@@ -144,6 +144,13 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
 int svr_unbiased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
                            float* exp_avg_sq, double step, double lr, double beta1, double beta2, float eps,
                            svr_stream_t stream);
+/* Capture-safe form of svr_unbiased_adam_step for hipGraph replay (no
+ * reference counterpart): increments the step count *step_dev (int32, device)
+ * and computes the bias terms from it on the device, in double as the host
+ * form does; coef_dev is 2 floats of device scratch. */
+int svr_unbiased_adam_step_device(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
+                                  float* exp_avg_sq, int32_t* step_dev, float* coef_dev, double lr, double beta1,
+                                  double beta2, float eps, svr_stream_t stream);
 /* Replaces `_C.biased_adam_step` (adam_step.cu:139-162). */
 int svr_biased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
                          float* exp_avg_sq, float lr, float beta1, float beta2, float eps, svr_stream_t stream);
@@ -225,6 +232,30 @@ int svr_batch_rasterize_backward(int32_t n_samp, int32_t n_views, const svr_view
                                  const void* image_state, const float* out_T, const float* dL_dout_color,
                                  const float* dL_dout_T, void* workspace, float* dL_dgeos, float* dL_drgbs,
                                  float* subdiv_p_bw, svr_stream_t stream);
+
+/* Capture-safe forms for hipGraph replay of a batch training step: no host
+ * synchronisation.  svr_scan_entries runs the scan of svr_num_rendered and
+ * leaves the entry count on the device.  The capacity forms take buffers
+ * sized for `capacity` entries (binning state, backward workspace) and read
+ * the count on the device; when it exceeds the capacity, *overflow_dev
+ * (uint32, device) is set to 1 and the frame renders no entries (background
+ * colour, T = 1, zero gradients).  The flag is only ever set, never cleared:
+ * the caller zeroes it, checks it after any number of replays and
+ * re-sizes. */
+int svr_scan_entries(int64_t n_counts, void* geom_state, svr_stream_t stream);
+int svr_batch_rasterize_forward_capacity(int32_t n_samp, int32_t n_views, const svr_view* views_host,
+                                         const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,
+                                         const float* vox_lengths, const float* geos, const float* rgbs,
+                                         const void* geom_state, const void* order_table, int64_t capacity,
+                                         void* binning_state, void* image_state, float* out_color, float* out_T,
+                                         uint32_t* overflow_dev, svr_stream_t stream);
+int svr_batch_rasterize_backward_capacity(int32_t n_samp, int32_t n_views, const svr_view* views_host,
+                                          const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,
+                                          const float* vox_lengths, const float* geos, const float* rgbs,
+                                          const void* geom_state, int64_t capacity, const void* binning_state,
+                                          const void* image_state, const float* out_T, const float* dL_dout_color,
+                                          const float* dL_dout_T, void* workspace, float* dL_dgeos,
+                                          float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
 
 /* Copies ranges [T_total, 2], tile_last [T_total] and n_contrib [P_total] out
  * of a batch image state. */

@@ -286,6 +286,16 @@ def unbiased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, step, lr, beta1
                                          float(eps), _stream(param.device)), "unbiased_adam_step")
 
 
+def unbiased_adam_step_device(sparse, param, grad, exp_avg, exp_avg_sq, step_dev, coef_dev, lr, beta1, beta2, eps):
+    """Capture-safe form (no reference counterpart): the step count lives in step_dev (int32 [1]) and is
+    incremented on the device; coef_dev (float32 [2]) is scratch for the bias terms."""
+    _check_adam(param, grad, exp_avg, exp_avg_sq)
+    _lib.check(_L.svr_unbiased_adam_step_device(int(bool(sparse)), param.numel(), _ptr(param), _ptr(grad),
+                                                _ptr(exp_avg), _ptr(exp_avg_sq), _ptr(step_dev), _ptr(coef_dev),
+                                                float(lr), float(beta1), float(beta2), float(eps),
+                                                _stream(param.device)), "unbiased_adam_step_device")
+
+
 def biased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, lr, beta1, beta2, eps):
     _check_adam(param, grad, exp_avg, exp_avg_sq)
     _lib.check(_L.svr_biased_adam_step(int(bool(sparse)), param.numel(), _ptr(param), _ptr(grad), _ptr(exp_avg),

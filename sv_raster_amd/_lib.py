@@ -71,6 +71,13 @@ SIGNATURES = {
     "svr_batch_rasterize_backward": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_i64, c_p,
                                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_batch_unpack_image_state": (c_i32, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_scan_entries": (c_i32, [c_i64, c_p, c_p]),
+    "svr_batch_rasterize_forward_capacity": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p,
+                                                     c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_rasterize_backward_capacity": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
+                                                      c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_unbiased_adam_step_device": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_f64, c_f64, c_f64,
+                                              c_f32, c_p]),
     "svr_debug_sync": (c_i32, [c_p]),
     "svr_profile_enable": (c_i32, [c_i32]),
     "svr_profile_collect": (c_i32, [c_i32, ctypes.POINTER(c_f64), ctypes.POINTER(c_i64)]),

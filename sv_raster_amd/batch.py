@@ -99,20 +99,24 @@ class ViewBatch:
         return colors, Ts
 
 
-def _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree):
-    """Preprocess of all pairs, R, and the SH colours of all views."""
+def _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree, capacity=None):
+    """Preprocess of all pairs, R, and the SH colours of all views.  With a capacity the entry count stays
+    on the device (no synchronisation) and R is the capacity."""
     dev = vox_centers.device
     V, N = views.n_views, vox_centers.size(0)
     st = _C._stream(dev)
     centers, lengths = _C._f32(vox_centers), _C._f32(vox_lengths)
-    n_dup = torch.zeros(V * N, dtype=torch.int32, device=dev)
+    n_dup = torch.empty(V * N, dtype=torch.int32, device=dev)
     geom = torch.empty(int(_L.svr_batch_geom_state_size(V, N)), dtype=torch.uint8, device=dev)
-    R = ctypes.c_int64(0)
+    R = ctypes.c_int64(0 if capacity is None else int(capacity))
     if N > 0:
         _lib.check(_L.svr_batch_preprocess(V, views.host_ptr(), _C._ptr(views.dev), N, _C._ptr(centers),
                                            _C._ptr(lengths), _C._ptr(n_dup), _C._ptr(geom), st),
                    "batch_preprocess")
-        _lib.check(_L.svr_num_rendered(V * N, _C._ptr(geom), st, ctypes.byref(R)), "num_rendered")
+        if capacity is None:
+            _lib.check(_L.svr_num_rendered(V * N, _C._ptr(geom), st, ctypes.byref(R)), "num_rendered")
+        else:
+            _lib.check(_L.svr_scan_entries(V * N, _C._ptr(geom), st), "scan_entries")
     rgbs = torch.empty((V * N, 3), dtype=torch.float32, device=dev)
     if N > 0:
         _lib.check(_L.svr_batch_sh_compute(int(active_sh_degree), V, _C._ptr(views.dev), N, _C._ptr(centers),
@@ -121,12 +125,19 @@ def _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree):
     return n_dup, geom, int(R.value), rgbs
 
 
-def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs):
+def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, capacity=None,
+                  overflow=None):
     """Forward of all views; returns the state the backward needs (also used by the tests):
-    color [3 * P_total], T [P_total], R, rgbs [V * N, 3], n_dup [V * N] and the opaque buffers."""
+    color [3 * P_total], T [P_total], R, rgbs [V * N, 3], n_dup [V * N] and the opaque buffers.
+
+    capacity / overflow (an int32 [1] device tensor): the capture-safe form, with buffers sized for
+    `capacity` entries and no host synchronisation; a frame over the capacity sets overflow to 1 and renders
+    no entries."""
     dev = vox_centers.device
     V, N = views.n_views, vox_centers.size(0)
-    n_dup, geom, R, rgbs = _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree)
+    if (capacity is None) != (overflow is None):
+        raise Exception("capacity and overflow go together")
+    n_dup, geom, R, rgbs = _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree, capacity)
     binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
     img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels)), dtype=torch.uint8,
                       device=dev)
@@ -134,17 +145,27 @@ def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_length
     T = torch.empty(views.total_pixels, dtype=torch.float32, device=dev)
     table = _C.order_table_cache.get(octree_paths) if N > 0 else None
     centers, lengths, geos_c = _C._f32(vox_centers), _C._f32(vox_lengths), _C._f32_a16(geos)
-    _lib.check(_L.svr_batch_rasterize_forward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
-                                              _C._ptr(centers), _C._ptr(lengths), _C._ptr(geos_c), _C._ptr(rgbs),
-                                              _C._ptr(geom), _C._ptr(table), R, _C._ptr(binning), _C._ptr(img),
-                                              _C._ptr(color), _C._ptr(T), _C._stream(dev)),
-               "batch_rasterize_forward")
+    if capacity is None:
+        _lib.check(_L.svr_batch_rasterize_forward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
+                                                  _C._ptr(centers), _C._ptr(lengths), _C._ptr(geos_c),
+                                                  _C._ptr(rgbs), _C._ptr(geom), _C._ptr(table), R,
+                                                  _C._ptr(binning), _C._ptr(img), _C._ptr(color), _C._ptr(T),
+                                                  _C._stream(dev)), "batch_rasterize_forward")
+    else:
+        _lib.check(_L.svr_batch_rasterize_forward_capacity(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev),
+                                                           N, _C._ptr(centers), _C._ptr(lengths), _C._ptr(geos_c),
+                                                           _C._ptr(rgbs), _C._ptr(geom), _C._ptr(table), R,
+                                                           _C._ptr(binning), _C._ptr(img), _C._ptr(color),
+                                                           _C._ptr(T), _C._ptr(overflow), _C._stream(dev)),
+                   "batch_rasterize_forward_capacity")
     return dict(color=color, T=T, R=R, rgbs=rgbs, n_dup=n_dup, geom=geom, binning=binning, img=img,
-                centers=centers, lengths=lengths, geos=geos_c, deg=int(active_sh_degree))
+                centers=centers, lengths=lengths, geos=geos_c, deg=int(active_sh_degree),
+                capacity_mode=capacity is not None)
 
 
-def batch_backward(views, state, dL_dcolor, dL_dT):
-    """Gradients (dgeos [N, 8], dsh0 [N, 3], dshs [N, 15, 3], dsub [N, 1], drgbs [V * N, 3]) of a batch forward."""
+def batch_backward(views, state, dL_dcolor, dL_dT, out=None):
+    """Gradients (dgeos [N, 8], dsh0 [N, 3], dshs [N, 15, 3], dsub [N, 1], drgbs [V * N, 3]) of a batch forward.
+    `out` may hold preallocated (dgeos, dsh0, dshs, dsub) tensors to write into."""
     centers, lengths, T = state["centers"], state["lengths"], state["T"]
     R = state["R"]
     dev = centers.device
@@ -154,18 +175,22 @@ def batch_backward(views, state, dL_dcolor, dL_dT):
         else _C._f32(dL_dcolor)
     dL_dT = torch.zeros_like(T) if dL_dT is None else _C._f32(dL_dT)
     ws = torch.empty(int(_L.svr_batch_backward_workspace_size(V, N, R)), dtype=torch.uint8, device=dev)
-    dgeos = torch.empty((N, 8), dtype=torch.float32, device=dev)
+    if out is None:
+        dgeos = torch.empty((N, 8), dtype=torch.float32, device=dev)
+        dsh0 = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        dshs = torch.empty((N, 15, 3), dtype=torch.float32, device=dev)
+        dsub = torch.empty((N, 1), dtype=torch.float32, device=dev)
+    else:
+        dgeos, dsh0, dshs, dsub = out
     drgbs = torch.empty((V * N, 3), dtype=torch.float32, device=dev)
-    dsub = torch.empty((N, 1), dtype=torch.float32, device=dev)
-    dsh0 = torch.empty((N, 3), dtype=torch.float32, device=dev)
-    dshs = torch.empty((N, 15, 3), dtype=torch.float32, device=dev)
+    backward = _L.svr_batch_rasterize_backward_capacity if state.get("capacity_mode") else \
+        _L.svr_batch_rasterize_backward
     if N > 0:
-        _lib.check(_L.svr_batch_rasterize_backward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
-                                                   _C._ptr(centers), _C._ptr(lengths), _C._ptr(state["geos"]),
-                                                   _C._ptr(state["rgbs"]), _C._ptr(state["geom"]), R,
-                                                   _C._ptr(state["binning"]), _C._ptr(state["img"]), _C._ptr(T),
-                                                   _C._ptr(dL_dcolor), _C._ptr(dL_dT), _C._ptr(ws), _C._ptr(dgeos),
-                                                   _C._ptr(drgbs), _C._ptr(dsub), st), "batch_rasterize_backward")
+        _lib.check(backward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N, _C._ptr(centers),
+                            _C._ptr(lengths), _C._ptr(state["geos"]), _C._ptr(state["rgbs"]), _C._ptr(state["geom"]),
+                            R, _C._ptr(state["binning"]), _C._ptr(state["img"]), _C._ptr(T), _C._ptr(dL_dcolor),
+                            _C._ptr(dL_dT), _C._ptr(ws), _C._ptr(dgeos), _C._ptr(drgbs), _C._ptr(dsub), st),
+                   "batch_rasterize_backward")
         _lib.check(_L.svr_batch_sh_compute_bw(state["deg"], V, _C._ptr(views.dev), N, _C._ptr(centers),
                                               _C._ptr(state["geom"]), _C._ptr(drgbs), _C._ptr(dsh0), _C._ptr(dshs),
                                               st), "batch_sh_compute_bw")
@@ -225,3 +250,76 @@ def unpack_image_state(views, imageBuffer):
     _lib.check(_L.svr_batch_unpack_image_state(T, P, _C._ptr(imageBuffer), _C._ptr(ranges), _C._ptr(tile_last),
                                                _C._ptr(n_contrib), _C._stream(dev)), "batch_unpack_image_state")
     return ranges, tile_last, n_contrib
+
+
+class GraphTrainStep:
+    """A batch training step replayed from a captured HIP graph (no reference counterpart).
+
+    One replay = forward of all views of `views`, the gradient of the summed per-view MSE to `targets`
+    (one [3, H, W] tensor per view; dL/dcolor = 2 (color - target) / (3 H W) per view, the backward of
+    torch's mse_loss), the batch backward and `optimizer.step()` (a SparseAdam switched to device step
+    counts).  Nothing in a replay synchronises with the host: the list buffers are sized for `capacity`
+    entries (by default 1.25 x the count of the current parameters plus a margin), and a frame over it sets
+    `overflow` and leaves the parameters unchanged (zero gradients); `overflowed()` reads the flag and
+    `recapture()` re-sizes.  Parameters are updated in place; `color` holds the last replay's images
+    (flat, per-view blocks as in ViewBatch.split).
+    """
+
+    def __init__(self, views, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, optimizer, targets,
+                 active_sh_degree=3, capacity=None, headroom=1.25):
+        self.views, self.octree_paths = views, octree_paths
+        self.centers, self.lengths = _C._f32(vox_centers), _C._f32(vox_lengths)
+        self.geos, self.sh0, self.shs = geos, sh0, shs
+        self.optimizer, self.deg, self.headroom = optimizer, int(active_sh_degree), headroom
+        dev = self.centers.device
+        if len(targets) != views.n_views:
+            raise Exception("one target image per view")
+        self.target = torch.cat([_C._f32(t).reshape(-1) for t in targets])
+        self.weight = torch.cat([torch.full((3 * W * H,), 2.0 / (3 * W * H), dtype=torch.float32, device=dev)
+                                 for W, H in views.sizes])
+        if self.target.numel() != 3 * views.total_pixels:
+            raise Exception("target sizes do not match the views")
+        self.zero_dT = torch.zeros(views.total_pixels, dtype=torch.float32, device=dev)
+        self.overflow = torch.zeros(1, dtype=torch.int32, device=dev)
+        for p in (geos, sh0, shs):
+            p.grad = torch.zeros_like(p)
+        self.dsub = torch.zeros((self.centers.size(0), 1), dtype=torch.float32, device=dev)
+        optimizer.use_device_steps()
+        _C.order_table_cache.get(octree_paths)
+        self.recapture(capacity)
+
+    def _measure(self):
+        with torch.no_grad():
+            st = batch_forward(self.views, self.deg, self.octree_paths, self.centers, self.lengths, self.geos,
+                               self.sh0, self.shs)
+        return st["R"]
+
+    def _fwd_bwd(self):
+        st = batch_forward(self.views, self.deg, self.octree_paths, self.centers, self.lengths, self.geos, self.sh0,
+                           self.shs, capacity=self.capacity, overflow=self.overflow)
+        dcol = (st["color"] - self.target) * self.weight
+        batch_backward(self.views, st, dcol, self.zero_dT,
+                       out=(self.geos.grad, self.sh0.grad, self.shs.grad, self.dsub))
+        self.color = st["color"]
+
+    def recapture(self, capacity=None):
+        """(Re)size the entry buffers (measuring the current count when capacity is None) and capture."""
+        if capacity is None:
+            capacity = int(self._measure() * self.headroom) + 65536
+        self.capacity = int(capacity)
+        with torch.no_grad():
+            self._fwd_bwd()  # outside the graph: allocator and first-use warm-up
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._fwd_bwd()
+                self.optimizer.step()
+        torch.cuda.synchronize()
+
+    def step(self):
+        self.graph.replay()
+
+    def overflowed(self):
+        """True if a replay since the last capture exceeded the capacity (synchronises)."""
+        return bool(int(self.overflow.item()))
+

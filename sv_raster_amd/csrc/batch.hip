@@ -147,7 +147,8 @@ __device__ inline void emit_entries(uint32_t pair_v, uint32_t off, uint2 bb, uin
 }
 
 // forward.cu:523-577 per (view, voxel) pair; pairs with many entries are
-// queued (as u * N + v) for batch_duplicate_big_kernel.
+// queued (as u * N + v) for batch_duplicate_big_kernel.  Pairs whose entries
+// would end beyond `cap` (a frame over a fixed capacity) emit nothing.
 __global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __restrict__ views, int64_t N,
                                                               int pos_bits, const uint32_t* __restrict__ n_dup,
                                                               const uint32_t* __restrict__ scan,
@@ -156,13 +157,13 @@ __global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __
                                                               const uint32_t* __restrict__ order_pos,
                                                               uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                               uint32_t* __restrict__ big_list,
-                                                              uint32_t* __restrict__ big_count)
+                                                              uint32_t* __restrict__ big_count, uint32_t cap)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
     const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
     const uint32_t nd = n_dup[p];
-    if (nd == 0) return;
+    if (nd == 0 || scan[p] > cap) return;
     if (nd > BIG_DUP)
     {
         big_list[atomicAdd(big_count, 1u)] = (uint32_t)p;
@@ -694,11 +695,18 @@ int svr_batch_sh_compute_bw(int32_t deg, int32_t V, const svr_view* views_dev, i
     return SVR_OK;
 }
 
-int svr_batch_rasterize_forward(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
-                                int64_t N, const float* centers, const float* lengths, const float* geos,
-                                const float* rgbs, const void* geom_state, const void* order_table, int64_t R,
-                                void* binning_state, void* image_state, float* out_color, float* out_T,
-                                svr_stream_t stream)
+}  // extern "C"
+
+namespace svr {
+
+// R: the exact entry count (eager form) or the capacity the buffers were
+// sized for (capacity form, overflow != nullptr: the count is read on the
+// device and a frame over the capacity renders no entries).
+static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                              int64_t N, const float* centers, const float* lengths, const float* geos,
+                              const float* rgbs, const void* geom_state, const void* order_table, int64_t R,
+                              void* binning_state, void* image_state, float* out_color, float* out_T,
+                              uint32_t* overflow, hipStream_t stream)
 {
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
@@ -722,40 +730,48 @@ int svr_batch_rasterize_forward(int32_t n_samp, int32_t V, const svr_view* views
     SVR_REQUIRE(fill_u32(img.ctr, 0u, 8, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
     const int pos_bits = (int)required_bits((uint64_t)8 * (N > 0 ? N : 1));
     const int tile_bits = (int)required_bits((uint64_t)T);
+    const bool capacity_mode = overflow != nullptr;
+    const uint32_t cap = capacity_mode ? (uint32_t)R : 0xFFFFFFFFu;
+    const uint32_t* R_dev = capacity_mode ? g.counters + 4 : nullptr;
+    if (capacity_mode && N > 0)
+    {
+        rc = clamp_count_launch(g.counters, (uint64_t)R, overflow, stream);
+        if (rc != SVR_OK) return rc;
+    }
     if (R > 0)
     {
         SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
+        // emit into the buffer that makes the sorted result land in b.keys / b.orig
+        const int nbits = pos_bits + tile_bits;
+        const bool odd = ((nbits + RADIX_BITS - 1) / RADIX_BITS) & 1;
+        uint64_t* k0 = odd ? b.keys_alt : b.keys;
+        uint32_t* o0 = odd ? b.orig_alt : b.orig;
         batch_duplicate_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(
-            views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, b.keys, b.orig, g.big_list,
-            g.counters + 2);
+            views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0, g.big_list, g.counters + 2,
+            cap);
         SVR_CHECK_LAUNCH();
         batch_duplicate_big_kernel<<<1024, 256, 0, stream>>>(views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox,
-                                                             g.qbits, ot.pos, b.keys, b.orig, g.big_list,
-                                                             g.counters + 2);
+                                                             g.qbits, ot.pos, k0, o0, g.big_list, g.counters + 2);
         SVR_CHECK_LAUNCH();
         uint64_t* sorted = nullptr;
         uint32_t* sorted_orig = nullptr;
-        rc = radix_sort_ex(b.keys, b.keys_alt, b.orig, b.orig_alt, R, pos_bits + tile_bits, b.counts, stream, &sorted,
-                           &sorted_orig);
+        rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, nbits, b.counts, stream,
+                           &sorted, &sorted_orig, R_dev);
         if (rc != SVR_OK) return rc;
-        if (sorted != b.keys)
-        {
-            SVR_CHECK_HIP(hipMemcpyAsync(b.keys, sorted, sizeof(uint64_t) * R, hipMemcpyDeviceToDevice, stream));
-            SVR_CHECK_HIP(hipMemcpyAsync(b.orig, sorted_orig, sizeof(uint32_t) * R, hipMemcpyDeviceToDevice, stream));
-        }
-        rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
+        SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
+        rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream, R_dev);
         if (rc != SVR_OK) return rc;
     }
-    // queue order: longest tile lists first (one 8-bit radix pass on the length bucket)
-    queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, img.ranges, img.qkeys, img.order);
+    // queue order: longest tile lists first (one 8-bit radix pass on the
+    // length bucket, from the alternate buffers into img.order)
+    queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, img.ranges, img.qkeys_alt, img.order_alt);
     SVR_CHECK_LAUNCH();
     {
         uint64_t* qk = nullptr;
         uint32_t* qo = nullptr;
-        rc = radix_sort_ex(img.qkeys, img.qkeys_alt, img.order, img.order_alt, T, 8, img.qcounts, stream, &qk, &qo);
+        rc = radix_sort_ex(img.qkeys_alt, img.qkeys, img.order_alt, img.order, T, 8, img.qcounts, stream, &qk, &qo);
         if (rc != SVR_OK) return rc;
-        if (qo != img.order)
-            SVR_CHECK_HIP(hipMemcpyAsync(img.order, qo, sizeof(uint32_t) * T, hipMemcpyDeviceToDevice, stream));
+        SVR_REQUIRE(qo == img.order, SVR_ERR_HIP, "queue sort ended in the wrong buffer");
     }
     prof_end(PROF_BINNING, stream);
 
@@ -773,12 +789,12 @@ int svr_batch_rasterize_forward(int32_t n_samp, int32_t V, const svr_view* views
     return rc;
 }
 
-int svr_batch_rasterize_backward(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
-                                 int64_t N, const float* centers, const float* lengths, const float* geos,
-                                 const float* rgbs, const void* geom_state, int64_t R, const void* binning_state,
-                                 const void* image_state, const float* out_T, const float* dL_dcolor,
-                                 const float* dL_dT, void* workspace, float* dgeos, float* drgbs, float* dsub,
-                                 svr_stream_t stream)
+static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                               int64_t N, const float* centers, const float* lengths, const float* geos,
+                               const float* rgbs, const void* geom_state, int64_t R, const void* binning_state,
+                               const void* image_state, const float* out_T, const float* dL_dcolor, const float* dL_dT,
+                               void* workspace, float* dgeos, float* drgbs, float* dsub, bool capacity_mode,
+                               hipStream_t stream)
 {
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
@@ -818,7 +834,8 @@ int svr_batch_rasterize_backward(int32_t n_samp, int32_t V, const svr_view* view
     prof_begin(PROF_GRAD_REDUCE, stream);
     if (R > 0)
     {
-        rc = grad_reduce_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, stream);
+        rc = grad_reduce_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, stream,
+                                capacity_mode ? (uint32_t)R : 0xFFFFFFFFu);
         if (rc != SVR_OK) return rc;
     }
     else
@@ -832,6 +849,58 @@ int svr_batch_rasterize_backward(int32_t n_samp, int32_t V, const svr_view* view
     SVR_CHECK_LAUNCH();
     prof_end(PROF_GRAD_REDUCE, stream);
     return SVR_OK;
+}
+
+}  // namespace svr
+
+extern "C" {
+
+int svr_batch_rasterize_forward(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                                int64_t N, const float* centers, const float* lengths, const float* geos,
+                                const float* rgbs, const void* geom_state, const void* order_table, int64_t R,
+                                void* binning_state, void* image_state, float* out_color, float* out_T,
+                                svr_stream_t stream)
+{
+    return batch_forward_impl(n_samp, V, views_host, views_dev, N, centers, lengths, geos, rgbs, geom_state,
+                              order_table, R, binning_state, image_state, out_color, out_T, nullptr, stream);
+}
+
+int svr_batch_rasterize_forward_capacity(int32_t n_samp, int32_t V, const svr_view* views_host,
+                                         const svr_view* views_dev, int64_t N, const float* centers,
+                                         const float* lengths, const float* geos, const float* rgbs,
+                                         const void* geom_state, const void* order_table, int64_t capacity,
+                                         void* binning_state, void* image_state, float* out_color, float* out_T,
+                                         uint32_t* overflow_dev, svr_stream_t stream)
+{
+    SVR_REQUIRE(overflow_dev != nullptr, SVR_ERR_INVALID, "svr_batch_rasterize_forward_capacity: null overflow flag");
+    return batch_forward_impl(n_samp, V, views_host, views_dev, N, centers, lengths, geos, rgbs, geom_state,
+                              order_table, capacity, binning_state, image_state, out_color, out_T, overflow_dev,
+                              stream);
+}
+
+int svr_batch_rasterize_backward(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                                 int64_t N, const float* centers, const float* lengths, const float* geos,
+                                 const float* rgbs, const void* geom_state, int64_t R, const void* binning_state,
+                                 const void* image_state, const float* out_T, const float* dL_dcolor,
+                                 const float* dL_dT, void* workspace, float* dgeos, float* drgbs, float* dsub,
+                                 svr_stream_t stream)
+{
+    return batch_backward_impl(n_samp, V, views_host, views_dev, N, centers, lengths, geos, rgbs, geom_state, R,
+                               binning_state, image_state, out_T, dL_dcolor, dL_dT, workspace, dgeos, drgbs, dsub,
+                               false, stream);
+}
+
+int svr_batch_rasterize_backward_capacity(int32_t n_samp, int32_t V, const svr_view* views_host,
+                                          const svr_view* views_dev, int64_t N, const float* centers,
+                                          const float* lengths, const float* geos, const float* rgbs,
+                                          const void* geom_state, int64_t capacity, const void* binning_state,
+                                          const void* image_state, const float* out_T, const float* dL_dcolor,
+                                          const float* dL_dT, void* workspace, float* dgeos, float* drgbs,
+                                          float* dsub, svr_stream_t stream)
+{
+    return batch_backward_impl(n_samp, V, views_host, views_dev, N, centers, lengths, geos, rgbs, geom_state,
+                               capacity, binning_state, image_state, out_T, dL_dcolor, dL_dT, workspace, dgeos, drgbs,
+                               dsub, true, stream);
 }
 
 int svr_batch_unpack_image_state(int64_t T, int64_t P, const void* image_state, int32_t* ranges, int32_t* tile_last,

@@ -30,10 +30,21 @@ __device__ inline uint64_t match_digit(uint32_t d, uint64_t active)
     return m;
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void radix_upsweep(const uint64_t* __restrict__ keys, int64_t n,
-                                                              int shift, uint32_t dmask, uint32_t* __restrict__ counts,
+// Number of keys to sort: n, or the smaller count a previous kernel left at
+// n_dev (a 64-bit count as two 32-bit words) when n_dev is given.
+__device__ inline int64_t sort_count(int64_t n, const uint32_t* __restrict__ n_dev)
+{
+    if (!n_dev) return n;
+    const int64_t d = (int64_t)(((uint64_t)n_dev[1] << 32) | n_dev[0]);
+    return d < n ? d : n;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void radix_upsweep(const uint64_t* __restrict__ keys, int64_t n_cap,
+                                                              const uint32_t* __restrict__ n_dev, int shift,
+                                                              uint32_t dmask, uint32_t* __restrict__ counts,
                                                               int64_t nb)
 {
+    const int64_t n = sort_count(n_cap, n_dev);
     __shared__ uint32_t hist[SORT_THREADS / 64][RADIX];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
 #pragma unroll
@@ -94,9 +105,10 @@ __global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* __restrict__ co
 template <bool HAS_VALS>
 __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
     const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ vals_out, int64_t n, int shift, uint32_t dmask, const uint32_t* __restrict__ counts,
-    int64_t nb)
+    uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift, uint32_t dmask,
+    const uint32_t* __restrict__ counts, int64_t nb)
 {
+    const int64_t n = sort_count(n_cap, n_dev);
     constexpr int NW = SORT_THREADS / 64;
     __shared__ uint32_t wcnt[NW][RADIX];
     __shared__ uint32_t gbase[RADIX];
@@ -172,11 +184,12 @@ __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
 // Stable LSD radix sort over bits [0, nbits) in 8-bit digits.  The sorted
 // data ends in the buffers returned through out_keys / out_vals.
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
-                  uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals)
+                  uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
+                  const uint32_t* n_dev)
 {
     *out_keys = keys;
     if (out_vals) *out_vals = vals;
-    if (n <= 1 || nbits <= 0) return SVR_OK;
+    if (n <= 0 || nbits <= 0) return SVR_OK;
     const int64_t nb = sort_blocks(n);
     SVR_REQUIRE(nb < (1ll << 31), SVR_ERR_INVALID, "radix sort of %lld keys too large", (long long)n);
     uint64_t* kin = keys; uint64_t* kout = keys_alt;
@@ -185,16 +198,16 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
     {
         const int bits = (nbits - shift) < RADIX_BITS ? (nbits - shift) : RADIX_BITS;
         const uint32_t dmask = (1u << bits) - 1u;
-        radix_upsweep<<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, n, shift, dmask, counts, nb);
+        radix_upsweep<<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, n, n_dev, shift, dmask, counts, nb);
         SVR_CHECK_LAUNCH();
         radix_scan_rows<<<RADIX, 256, 0, stream>>>(counts, nb);
         SVR_CHECK_LAUNCH();
         if (vals)
-            radix_downsweep<true><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, vin, vout, n, shift, dmask,
-                                                                             counts, nb);
+            radix_downsweep<true><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, vin, vout, n, n_dev, shift,
+                                                                             dmask, counts, nb);
         else
-            radix_downsweep<false><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, nullptr, nullptr, n, shift,
-                                                                              dmask, counts, nb);
+            radix_downsweep<false><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, nullptr, nullptr, n, n_dev,
+                                                                              shift, dmask, counts, nb);
         SVR_CHECK_LAUNCH();
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
@@ -343,10 +356,12 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(int gx, int gy, int 
 }
 
 // forward.cu:582-604 plus the value decode: vox_list[i] = (q << 29) | vox_id.
-__global__ __launch_bounds__(256) void ranges_kernel(const uint64_t* __restrict__ keys, int64_t R, int pos_bits,
+__global__ __launch_bounds__(256) void ranges_kernel(const uint64_t* __restrict__ keys, int64_t R_cap,
+                                                     const uint32_t* __restrict__ R_dev, int pos_bits,
                                                      const uint32_t* __restrict__ order_pair,
                                                      uint2* __restrict__ ranges, uint32_t* __restrict__ vox_list)
 {
+    const int64_t R = sort_count(R_cap, R_dev);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= R) return;
     const uint64_t k = keys[i];
@@ -383,9 +398,27 @@ __global__ void export_keys_kernel(const uint64_t* __restrict__ keys, const uint
 }
 
 int ranges_launch(const uint64_t* keys, int64_t R, int pos_bits, const uint32_t* order_pair, uint2* ranges,
-                  uint32_t* vox_list, hipStream_t stream)
+                  uint32_t* vox_list, hipStream_t stream, const uint32_t* R_dev)
 {
-    ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(keys, R, pos_bits, order_pair, ranges, vox_list);
+    ranges_kernel<<<(unsigned)((R + 255) / 256), 256, 0, stream>>>(keys, R, R_dev, pos_bits, order_pair, ranges,
+                                                                   vox_list);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+__global__ void clamp_count_kernel(uint32_t* __restrict__ counters, uint64_t cap, uint32_t* __restrict__ overflow)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint64_t R = ((uint64_t)counters[1] << 32) | counters[0];
+    const uint64_t eff = R <= cap ? R : 0ull;
+    counters[4] = (uint32_t)eff;
+    counters[5] = (uint32_t)(eff >> 32);
+    if (R > cap) overflow[0] = 1u;
+}
+
+int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hipStream_t stream)
+{
+    clamp_count_kernel<<<1, 64, 0, stream>>>(counters, cap, overflow);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }
@@ -405,23 +438,25 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
     SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
+    // the sort ping-pongs between the two buffers, one 8-bit digit per pass:
+    // emit into the buffer that makes the sorted result land in b.keys / b.orig
+    const int nbits = pos_bits + tile_bits;
+    const bool odd = ((nbits + RADIX_BITS - 1) / RADIX_BITS) & 1;
+    uint64_t* k0 = odd ? b.keys_alt : b.keys;
+    uint32_t* o0 = odd ? b.orig_alt : b.orig;
     duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, gx, gy, pos_bits, g.n_dup, g.scan, g.bbox,
-                                                                      g.qbits, ot.pos, b.keys, b.orig, g.big_list,
+                                                                      g.qbits, ot.pos, k0, o0, g.big_list,
                                                                       g.counters + 2);
     SVR_CHECK_LAUNCH();
-    duplicate_big_kernel<<<1024, 256, 0, stream>>>(gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos,
-                                                   b.keys, b.orig, g.big_list, g.counters + 2);
+    duplicate_big_kernel<<<1024, 256, 0, stream>>>(gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0,
+                                                   g.big_list, g.counters + 2);
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
     uint32_t* sorted_orig = nullptr;
-    int rc = radix_sort_ex(b.keys, b.keys_alt, b.orig, b.orig_alt, R, pos_bits + tile_bits, b.counts, stream,
+    int rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, nbits, b.counts, stream,
                            &sorted, &sorted_orig);
     if (rc != SVR_OK) return rc;
-    if (sorted != b.keys)
-    {
-        SVR_CHECK_HIP(hipMemcpyAsync(b.keys, sorted, sizeof(uint64_t) * R, hipMemcpyDeviceToDevice, stream));
-        SVR_CHECK_HIP(hipMemcpyAsync(b.orig, sorted_orig, sizeof(uint32_t) * R, hipMemcpyDeviceToDevice, stream));
-    }
+    SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
     rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
     if (rc != SVR_OK) return rc;
     prof_end(PROF_BINNING, stream);

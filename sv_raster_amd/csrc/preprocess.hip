@@ -166,11 +166,11 @@ int svr_rasterize_preprocess(int32_t W, int32_t H, float tanfovx, float tanfovy,
     return SVR_OK;
 }
 
-int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* R_host)
+int svr_scan_entries(int64_t N, void* geom_state, svr_stream_t stream)
 {
-    SVR_REQUIRE(R_host != nullptr, SVR_ERR_INVALID, "svr_num_rendered: null output");
-    *R_host = 0;
+    SVR_REQUIRE(N >= 0 && N < (1ll << 32), SVR_ERR_INVALID, "entry-count array length out of range");
     if (N == 0) return SVR_OK;
+    SVR_REQUIRE(geom_state != nullptr, SVR_ERR_INVALID, "svr_scan_entries: null geometry state");
     GeomState g = GeomState::carve(geom_state, N, nullptr);
     const int64_t nb = (N + SCAN_TILE - 1) / SCAN_TILE;
     prof_begin(PROF_SCAN, stream);
@@ -181,6 +181,17 @@ int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* 
     scan_apply_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums, g.scan);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_SCAN, stream);
+    return SVR_OK;
+}
+
+int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* R_host)
+{
+    SVR_REQUIRE(R_host != nullptr, SVR_ERR_INVALID, "svr_num_rendered: null output");
+    *R_host = 0;
+    if (N == 0) return SVR_OK;
+    const int rc = svr_scan_entries(N, geom_state, stream);
+    if (rc != SVR_OK) return rc;
+    GeomState g = GeomState::carve(geom_state, N, nullptr);
     uint32_t cnt[2] = {0, 0};
     SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
     SVR_CHECK_HIP(hipStreamSynchronize(stream));

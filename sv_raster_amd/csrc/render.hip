@@ -118,16 +118,19 @@ __device__ inline void write_grad(size_t v, const float* a, float* __restrict__ 
     dsub[v] = a[11];
 }
 
+// Entries at or beyond `cap` were never written (a frame over a fixed
+// capacity): such voxels get zero gradients.
 __global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint32_t* __restrict__ n_dup,
-                                                          const uint32_t* __restrict__ scan,
+                                                          const uint32_t* __restrict__ scan, uint32_t cap,
                                                           const float* __restrict__ partial,
                                                           float* __restrict__ dgeos, float* __restrict__ drgbs,
                                                           float* __restrict__ dsub)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
-    const uint32_t nd = n_dup[v];
+    uint32_t nd = n_dup[v];
     if (nd > BIG_DUP) return;
+    if (scan[v] > cap) nd = 0;
     float a[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) a[i] = 0.f;
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint3
 __global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __restrict__ big_list,
                                                               const uint32_t* __restrict__ big_count,
                                                               const uint32_t* __restrict__ n_dup,
-                                                              const uint32_t* __restrict__ scan,
+                                                              const uint32_t* __restrict__ scan, uint32_t cap,
                                                               const float* __restrict__ partial,
                                                               float* __restrict__ dgeos, float* __restrict__ drgbs,
                                                               float* __restrict__ dsub)
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __
     for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
     {
         const uint32_t v = big_list[i];
-        const uint32_t nd = n_dup[v];
+        const uint32_t nd = scan[v] > cap ? 0u : n_dup[v];
         const uint32_t off = scan[v] - nd;
         float a[12];
 #pragma unroll
@@ -190,12 +193,12 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __
 }
 
 int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs, float* dsub,
-                       hipStream_t stream)
+                       hipStream_t stream, uint32_t cap)
 {
-    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, g.n_dup, g.scan, partial, dgeos, drgbs,
-                                                                       dsub);
+    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, g.n_dup, g.scan, cap, partial, dgeos,
+                                                                       drgbs, dsub);
     SVR_CHECK_LAUNCH();
-    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(g.big_list, g.counters + 2, g.n_dup, g.scan, partial, dgeos,
+    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(g.big_list, g.counters + 2, g.n_dup, g.scan, cap, partial, dgeos,
                                                      drgbs, dsub);
     SVR_CHECK_LAUNCH();
     return SVR_OK;

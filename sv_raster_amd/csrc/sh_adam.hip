@@ -142,11 +142,18 @@ __global__ __launch_bounds__(64) void sh_bwd16_kernel(int deg, int64_t cnt, cons
 }
 
 // adam_step.cu:21-64 (unbiased) and 67-102 (biased); lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0))
+// With coef (the device-step form), step_size and rb2 come from coef[0..1].
 template <bool SPARSE, bool BIASED>
 __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
                                                    float* __restrict__ m, float* __restrict__ v, float step_size,
-                                                   float beta1, float beta2, float rb2, float eps)
+                                                   float beta1, float beta2, float rb2, float eps,
+                                                   const float* __restrict__ coef)
 {
+    if (coef)
+    {
+        step_size = coef[0];
+        rb2 = coef[1];
+    }
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     {
@@ -164,6 +171,20 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict_
         m[i] = mv;
         v[i] = vv;
     }
+}
+
+// Device-step form of the host bias terms (adam_step.cu:119-124): t = ++step,
+// coef = {lr / (1 - beta1^t), 1 / sqrt(1 - beta2^t)}, computed in double.
+__global__ void adam_coef_kernel(int32_t* __restrict__ step, double lr, double beta1, double beta2,
+                                 float* __restrict__ coef)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t t = step[0] + 1;
+    step[0] = t;
+    const double bc1 = 1.0 - pow(beta1, (double)t);
+    const double bc2 = 1.0 - pow(beta2, (double)t);
+    coef[0] = (float)(lr / bc1);
+    coef[1] = (float)(1.0 / sqrt(bc2));
 }
 
 __global__ void order_rank_kernel(int64_t n, const int64_t* __restrict__ paths, int64_t* __restrict__ out)
@@ -274,10 +295,31 @@ int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, 
     prof_begin(PROF_ADAM, stream);
     if (sparse)
         adam_kernel<true, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2, rb2,
-                                                           eps);
+                                                           eps, nullptr);
     else
         adam_kernel<false, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2,
-                                                            rb2, eps);
+                                                            rb2, eps, nullptr);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_ADAM, stream);
+    return SVR_OK;
+}
+
+int svr_unbiased_adam_step_device(int32_t sparse, int64_t n, float* p, const float* g, float* m, float* v,
+                                  int32_t* step_dev, float* coef_dev, double lr, double beta1, double beta2, float eps,
+                                  svr_stream_t stream)
+{
+    if (n == 0) return SVR_OK;
+    SVR_REQUIRE(step_dev && coef_dev, SVR_ERR_INVALID, "svr_unbiased_adam_step_device: null step or coefficients");
+    const unsigned grid = grid_for(n, 8192);
+    prof_begin(PROF_ADAM, stream);
+    adam_coef_kernel<<<1, 64, 0, stream>>>(step_dev, lr, beta1, beta2, coef_dev);
+    SVR_CHECK_LAUNCH();
+    if (sparse)
+        adam_kernel<true, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps,
+                                                           coef_dev);
+    else
+        adam_kernel<false, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps,
+                                                            coef_dev);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_ADAM, stream);
     return SVR_OK;
@@ -290,9 +332,9 @@ int svr_biased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, fl
     const unsigned grid = grid_for(n, 8192);
     prof_begin(PROF_ADAM, stream);
     if (sparse)
-        adam_kernel<true, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps);
+        adam_kernel<true, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps, nullptr);
     else
-        adam_kernel<false, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps);
+        adam_kernel<false, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps, nullptr);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_ADAM, stream);
     return SVR_OK;

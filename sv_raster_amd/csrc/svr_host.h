@@ -75,7 +75,8 @@ struct GeomState
     uint2* bbox;           // [N] packed pixel bbox (x<<16 | y) min / max
     uint32_t* qbits;       // [N] camera-quadrant bitset
     uint32_t* block_sums;  // [ceil(N / SCAN_TILE) + 1]
-    uint32_t* counters;    // [4]: 0 = R (total entries), 1 = large-voxel queue length
+    uint32_t* counters;    // [8]: 0-1 R (total entries, 64-bit), 2 large-voxel queue length,
+                           // 4-5 entries to sort (R, or 0 when R exceeds a fixed capacity)
     uint32_t* big_list;    // [N] voxels with many entries, expanded by whole workgroups
     static GeomState carve(void* p, int64_t N, size_t* bytes)
     {
@@ -86,7 +87,7 @@ struct GeomState
         g.bbox = c.take<uint2>(N);
         g.qbits = c.take<uint32_t>(N);
         g.block_sums = c.take<uint32_t>((N + SCAN_TILE - 1) / SCAN_TILE + 1);
-        g.counters = c.take<uint32_t>(4);
+        g.counters = c.take<uint32_t>(8);
         g.big_list = c.take<uint32_t>(N);
         if (bytes) *bytes = c.size();
         return g;
@@ -168,8 +169,11 @@ __host__ __device__ inline int tiles_y(int H) { return (H + 15) / 16; }
 // Stable LSD radix sort of 64-bit keys (optionally with 32-bit values) over
 // bits [0, nbits).  The sorted data ends in the buffers returned through
 // out_keys / out_vals; counts needs RADIX * sort_blocks(n) + RADIX words.
+// With n_dev, the kernels sort min(n, *n_dev) keys (a 64-bit count stored as
+// two 32-bit words on the device); n then only sizes the launch.
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
-                  uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals);
+                  uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
+                  const uint32_t* n_dev = nullptr);
 
 // Key duplication, sort, tile ranges and the sorted entry list of one frame.
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
@@ -180,12 +184,16 @@ int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
 // Tile ranges and the sorted entry list from sorted (tile << pos_bits | pos)
 // keys; ranges must be zeroed first.
 int ranges_launch(const uint64_t* keys, int64_t R, int pos_bits, const uint32_t* order_pair, uint2* ranges,
-                  uint32_t* vox_list, hipStream_t stream);
+                  uint32_t* vox_list, hipStream_t stream, const uint32_t* R_dev = nullptr);
+
+// counters[4..5] = counters[0..1] (the entry count) when it is at most cap,
+// else 0, and then *overflow = 1 (the flag is never cleared here).
+int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hipStream_t stream);
 
 // Per-voxel (or per (view, voxel) pair) sums of the per-entry partial
 // gradient rows over each pair's contiguous emission-order segment.
 int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
-                       float* dsub, hipStream_t stream);
+                       float* dsub, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
 
 // Live stage timing: when enabled, prof_begin/prof_end record a HIP event pair
 // on the launch stream around the kernels of one stage.

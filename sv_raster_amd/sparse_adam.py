@@ -27,7 +27,38 @@ class SparseAdam(torch.optim.Optimizer):
         super(SparseAdam, self).__setstate__(state)
 
     @torch.no_grad()
+    def use_device_steps(self):
+        """Switch to the capture-safe update (no reference counterpart): each parameter's step count moves to
+        an int32 device tensor that the update itself increments, so a captured step can be replayed.  The
+        optimizer state (moments, step count) is created here, outside any capture.  Unbiased form only."""
+        if self.biased:
+            raise NotImplementedError("device steps are implemented for the unbiased update")
+        for group in self.param_groups:
+            for param in group['params']:
+                state = self.state[param]
+                if len(state) == 0:
+                    state['step'] = 0
+                    state['exp_avg'] = torch.zeros_like(param, memory_format=torch.preserve_format)
+                    state['exp_avg_sq'] = torch.zeros_like(param, memory_format=torch.preserve_format)
+                if 'step_dev' not in state:
+                    state['step_dev'] = torch.full((1,), int(state['step']), dtype=torch.int32, device=param.device)
+                    state['coef_dev'] = torch.zeros(2, dtype=torch.float32, device=param.device)
+        self.device_steps = True
+
+    @torch.no_grad()
     def step(self):
+        if getattr(self, 'device_steps', False):
+            for group in self.param_groups:
+                lr = group['lr']
+                beta1, beta2 = group['betas']
+                for param in group['params']:
+                    if param.grad is None:
+                        continue
+                    state = self.state[param]
+                    _C.unbiased_adam_step_device(self.sparse, param, param.grad, state['exp_avg'],
+                                                 state['exp_avg_sq'], state['step_dev'], state['coef_dev'], lr,
+                                                 beta1, beta2, group['eps'])
+            return
         for group in self.param_groups:
             lr = group['lr']
             beta1, beta2 = group['betas']
