@@ -187,6 +187,17 @@ def batch_vs_plain(args, dev):
         batch_step()
     torch.cuda.synchronize()
     batch_s = (time.perf_counter() - t0) / args.batch_steps
+    # batched, captured in a HIP graph: one replay per step (device step counts, fixed entry capacity)
+    geos, sh0, shs, opt = fresh()
+    trainer = batch.GraphTrainStep(views, octpath, centers, lengths, geos, sh0, shs, opt, targets)
+    trainer.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        trainer.step()
+    torch.cuda.synchronize()
+    graph_s = (time.perf_counter() - t0) / args.batch_steps
+    overflow = trainer.overflowed()
     pix = sum(W * H for W, H in sizes)
     return {"workload": f"{name}: {N} voxels, {V} views of uneven resolution ({pix / 1e6:.2f} Mpix per round), "
                         "fwd+bwd+sparse Adam",
@@ -194,8 +205,12 @@ def batch_vs_plain(args, dev):
             "plain_views_per_s": V / plain_s, "batched_views_per_s": V / batch_s,
             "batched_over_plain": plain_s / batch_s,
             "plain_ms_per_round": plain_s * 1e3, "batched_ms_per_step": batch_s * 1e3,
+            "graph_views_per_s": V / graph_s, "graph_ms_per_step": graph_s * 1e3,
+            "graph_over_plain": plain_s / graph_s, "graph_capacity": trainer.capacity,
+            "graph_overflow": overflow,
             "note": "plain = one drop-in training step (rasterize_voxels + SH_eval, MSE, SparseAdam) per view; "
-                    "batched = rasterize_voxels_batch over all views, summed MSE, one SparseAdam step"}
+                    "batched = rasterize_voxels_batch over all views, summed MSE, one SparseAdam step; "
+                    "graph = the batched step replayed from a captured HIP graph (GraphTrainStep)"}
 
 
 def main():
