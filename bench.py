@@ -169,9 +169,9 @@ def batch_vs_plain(args, dev):
     torch.cuda.synchronize()
     plain_s = (time.perf_counter() - t0) / args.batch_steps
 
-    # batched: all views in one step
+    # batched: all views in one step, binning tile picked by the autotuner
     geos, sh0, shs, opt = fresh()
-    views = batch.ViewBatch(settings)
+    views, tune_ms = batch.autotune(settings, octpath, centers, lengths, geos, sh0, shs)
 
     def batch_step():
         colors, _ = batch.rasterize_voxels_batch(views, octpath, centers, lengths, geos, sh0, shs)
@@ -208,6 +208,7 @@ def batch_vs_plain(args, dev):
             "graph_views_per_s": V / graph_s, "graph_ms_per_step": graph_s * 1e3,
             "graph_over_plain": plain_s / graph_s, "graph_capacity": trainer.capacity,
             "graph_overflow": overflow,
+            "tile_shape": views.tile_shape, "autotune_fwd_bwd_ms": tune_ms,
             "note": "plain = one drop-in training step (rasterize_voxels + SH_eval, MSE, SparseAdam) per view; "
                     "batched = rasterize_voxels_batch over all views, summed MSE, one SparseAdam step; "
                     "graph = the batched step replayed from a captured HIP graph (GraphTrainStep)"}

@@ -173,17 +173,23 @@ int svr_octpath_2_ijk(int64_t n, const int64_t* octpath, const int8_t* octlevel,
 
 /* One camera of a batch.  The caller keeps an array of these in host memory
  * (for validation and sizes) and the same bytes in device memory (read by the
- * kernels).  tile_base = sum of the tile counts ((W+15)/16 * (H+15)/16) of the
- * previous views; pix_base = sum of their pixel counts. */
+ * kernels).  tile_base = sum of the tile counts (ceil(W / tile width) *
+ * ceil(H / tile height)) of the previous views; pix_base = sum of their pixel
+ * counts. */
 typedef struct svr_view {
     int32_t width, height;
     float tanfovx, tanfovy, cx, cy, near_plane, bg_color;
     float w2c[12]; /* first 3 rows of the row-major world-to-camera matrix */
     float c2w[12]; /* first 3 rows of the row-major camera-to-world matrix */
     int32_t tile_base;
-    int32_t reserved;
+    int32_t tile_shape; /* SVR_TILE_*: binning tile of the batch, the same in every view */
     int64_t pix_base;
 } svr_view;
+
+/* Binning tiles of a batch (the single-view entry points use the reference's
+ * 16x16).  Images and gradients do not depend on the tile; n_contrib and the
+ * tile ranges are relative to the batch's tiles. */
+enum { SVR_TILE_16X16 = 0, SVR_TILE_8X8 = 1, SVR_TILE_32X8 = 2 };
 
 /* Sizes: geometry state of V views (its scan runs over V * N entries, so
  * svr_num_rendered(V * N, ...) returns the batch's total entry count R), the

@@ -24,7 +24,7 @@ class SvrView(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("tanfovx", ctypes.c_float),
                 ("tanfovy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
                 ("near_plane", ctypes.c_float), ("bg_color", ctypes.c_float), ("w2c", ctypes.c_float * 12),
-                ("c2w", ctypes.c_float * 12), ("tile_base", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("c2w", ctypes.c_float * 12), ("tile_base", ctypes.c_int32), ("tile_shape", ctypes.c_int32),
                 ("pix_base", ctypes.c_int64)]
 
 

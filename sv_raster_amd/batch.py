@@ -29,6 +29,10 @@ from . import _C, _lib
 _L = _lib.load()
 
 
+# binning tiles of a batch (svr_view.tile_shape): name -> (code, width, height)
+TILE_SHAPES = {"16x16": (0, 16, 16), "8x8": (1, 8, 8), "32x8": (2, 32, 8)}
+
+
 def _mat12(m):
     a = m.detach().float().cpu().numpy() if torch.is_tensor(m) else np.asarray(m, np.float32)
     a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
@@ -42,10 +46,17 @@ class ViewBatch:
 
     `settings` is a sequence of `renderer.RasterSettings` (color_mode "sh";
     the same n_samp_per_vox for all views; depth, normal and max_w tracking
-    off).  Reading the camera matrices synchronises once, here.
+    off).  `tile_shape` is the binning tile ("16x16", the reference's, or
+    "8x8" / "32x8"; images and gradients do not depend on it, see `autotune`).
+    Reading the camera matrices synchronises once, here.
     """
 
-    def __init__(self, settings, device=None):
+    def __init__(self, settings, device=None, tile_shape="16x16"):
+        if tile_shape not in TILE_SHAPES:
+            raise Exception(f"tile_shape must be one of {sorted(TILE_SHAPES)}")
+        code, tw, th = TILE_SHAPES[tile_shape]
+        self.tile_shape = tile_shape
+        self.settings = list(settings)
         settings = list(settings)
         if not settings:
             raise Exception("ViewBatch needs at least one view")
@@ -70,10 +81,9 @@ class ViewBatch:
             v.bg_color = float(getattr(s, "bg_color", 0.0))
             v.w2c[:] = _mat12(s.w2c_matrix).tolist()
             v.c2w[:] = _mat12(s.c2w_matrix).tolist()
-            v.tile_base, v.reserved, v.pix_base = tile_base, 0, pix_base
+            v.tile_base, v.tile_shape, v.pix_base = tile_base, code, pix_base
             self.sizes.append((W, H))
-            gx, gy = _C.tile_grid(W, H)
-            tile_base += gx * gy
+            tile_base += ((W + tw - 1) // tw) * ((H + th - 1) // th)
             pix_base += W * H
         if device is None:
             m = settings[0].c2w_matrix
@@ -322,4 +332,40 @@ class GraphTrainStep:
     def overflowed(self):
         """True if a replay since the last capture exceeded the capacity (synchronises)."""
         return bool(int(self.overflow.item()))
+
+
+_TUNED = {}
+
+
+def autotune(settings, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, candidates=tuple(TILE_SHAPES),
+             repeats=3, active_sh_degree=3):
+    """Picks the binning tile of a multi-view workload by timing a forward + backward of every candidate on
+    the actual scene and cameras (the first call for a workload; later calls with the same voxel count, image
+    sizes and n_samp return the cached choice).  Returns (ViewBatch with the fastest tile shape, {shape: ms}).
+
+    The tile shape trades sort size against list relevance: small tiles give more list entries (more keys to
+    sort) but each wave walks only the entries of its own 8x8 pixels; large tiles sort fewer entries but every
+    wave tests the entries of its neighbours' pixels too."""
+    settings = list(settings)
+    key = (int(vox_centers.shape[0]), tuple((int(s.image_width), int(s.image_height)) for s in settings),
+           int(settings[0].n_samp_per_vox))
+    if key in _TUNED:
+        best, times = _TUNED[key]
+        return ViewBatch(settings, tile_shape=best), dict(times)
+    import time
+    times = {}
+    with torch.no_grad():
+        for name in candidates:
+            views = ViewBatch(settings, tile_shape=name)
+            for r in range(repeats + 1):
+                if r == 1:
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                st = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
+                batch_backward(views, st, st["color"] - 0.5, None)
+            torch.cuda.synchronize()
+            times[name] = (time.perf_counter() - t0) * 1e3 / repeats
+    best = min(times, key=times.get)
+    _TUNED[key] = (best, dict(times))
+    return ViewBatch(settings, tile_shape=best), times
 

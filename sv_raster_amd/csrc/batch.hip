@@ -93,6 +93,7 @@ __device__ inline int view_of_tile(const svr_view* __restrict__ views, int V, ui
 
 // ---- preprocess and key duplication over (view, voxel) pairs -------------
 
+template <typename TS>
 __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* __restrict__ views, int64_t N,
                                                                const float* __restrict__ centers,
                                                                const float* __restrict__ lengths,
@@ -116,8 +117,9 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
     }
     uint32_t nd, bits;
     uint2 bb;
-    preprocess_voxel(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W, H,
-                     tiles_x(W), tiles_y(H), focal_x, focal_y, vw.cx, vw.cy, m, cw, vw.near_plane, nd, bb, bits);
+    preprocess_voxel<TS::TX, TS::TY>(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W,
+                                     H, tiles_along(W, TS::TX), tiles_along(H, TS::TY), focal_x, focal_y, vw.cx,
+                                     vw.cy, m, cw, vw.near_plane, nd, bb, bits);
     const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
     out_n_dup[p] = (int32_t)nd;
     n_dup[p] = nd;
@@ -125,12 +127,13 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
     qbits_out[p] = bits;
 }
 
+template <typename TS>
 __device__ inline void emit_entries(uint32_t pair_v, uint32_t off, uint2 bb, uint32_t bits, int gx, int gy,
                                     uint32_t tile_base, int pos_bits, const uint32_t* __restrict__ order_pos,
                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     int txmin, tymin, txmax, tymax;
-    rect_of(bb, gx, gy, txmin, tymin, txmax, tymax);
+    rect_of<TS::TX, TS::TY>(bb, gx, gy, txmin, tymin, txmax, tymax);
     for (uint32_t q = 0; q < 8; ++q)
     {
         if (!(bits & (1u << q))) continue;
@@ -149,6 +152,7 @@ __device__ inline void emit_entries(uint32_t pair_v, uint32_t off, uint2 bb, uin
 // forward.cu:523-577 per (view, voxel) pair; pairs with many entries are
 // queued (as u * N + v) for batch_duplicate_big_kernel.  Pairs whose entries
 // would end beyond `cap` (a frame over a fixed capacity) emit nothing.
+template <typename TS>
 __global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __restrict__ views, int64_t N,
                                                               int pos_bits, const uint32_t* __restrict__ n_dup,
                                                               const uint32_t* __restrict__ scan,
@@ -170,10 +174,12 @@ __global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __
         return;
     }
     const svr_view& vw = views[blockIdx.y];
-    emit_entries((uint32_t)v, scan[p] - nd, bboxes[p], qbits[p], tiles_x(vw.width), tiles_y(vw.height),
+    emit_entries<TS>((uint32_t)v, scan[p] - nd, bboxes[p], qbits[p], tiles_along(vw.width, TS::TX),
+                     tiles_along(vw.height, TS::TY),
                  (uint32_t)vw.tile_base, pos_bits, order_pos, keys, vals);
 }
 
+template <typename TS>
 __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view* __restrict__ views, int64_t N,
                                                                   int pos_bits, const uint32_t* __restrict__ n_dup,
                                                                   const uint32_t* __restrict__ scan,
@@ -191,11 +197,11 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view
         const uint32_t p = big_list[i];
         const uint32_t u = (uint32_t)(p / (uint64_t)N), v = (uint32_t)(p - (uint64_t)u * (uint64_t)N);
         const svr_view& vw = views[u];
-        const int gx = tiles_x(vw.width), gy = tiles_y(vw.height);
+        const int gx = tiles_along(vw.width, TS::TX), gy = tiles_along(vw.height, TS::TY);
         const uint32_t nd = n_dup[p];
         const uint32_t off = scan[p] - nd;
         int txmin, tymin, txmax, tymax;
-        rect_of(bboxes[p], gx, gy, txmin, tymin, txmax, tymax);
+        rect_of<TS::TX, TS::TY>(bboxes[p], gx, gy, txmin, tymin, txmax, tymax);
         const uint32_t w = (uint32_t)(txmax - txmin + 1);
         const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
         const uint32_t bits = qbits[p];
@@ -235,6 +241,7 @@ struct TileGeom
     bool inside;
 };
 
+template <typename TS>
 __device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, uint32_t g, int w, int lane)
 {
     TileGeom t;
@@ -242,12 +249,12 @@ __device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, 
     const svr_view& vw = views[t.u];
     t.W = vw.width;
     t.H = vw.height;
-    const int gx = tiles_x(t.W);
+    const int gx = tiles_along(t.W, TS::TX);
     const uint32_t tile = g - (uint32_t)vw.tile_base;
     t.tx = (int)(tile % (uint32_t)gx);
     t.ty = (int)(tile / (uint32_t)gx);
-    t.sx0 = t.tx * TILE_X + (w & 1) * 8;
-    t.sy0 = t.ty * TILE_Y + (w >> 1) * 8;
+    t.sx0 = t.tx * TS::TX + (w % TS::WX) * 8;
+    t.sy0 = t.ty * TS::TY + (w / TS::WX) * 8;
     t.sx1 = min(t.sx0 + 7, t.W - 1);
     t.sy1 = min(t.sy0 + 7, t.H - 1);
     t.px = t.sx0 + (lane & 7);
@@ -258,8 +265,8 @@ __device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, 
 
 // One tile of the batch forward.  The fast form (EXACT = false) defers a tile
 // holding a ray whose 1/rd is not finite to exact_list (count ctr[1]).
-template <int NS, bool EXACT>
-__device__ inline void batch_fwd_tile(Stage& st, uint32_t* s_last, uint32_t g, int V,
+template <int NS, bool EXACT, typename TS>
+__device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t* s_last, uint32_t g, int V,
                                       const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
                                       uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges,
                                       const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
@@ -269,7 +276,7 @@ __device__ inline void batch_fwd_tile(Stage& st, uint32_t* s_last, uint32_t g, i
                                       float* __restrict__ out_color, float* __restrict__ out_T)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const TileGeom t = tile_geom(views, V, g, w, lane);
+    const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
     const svr_view& vw = views[t.u];
     const PixRay ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
     const bool need_exact = __syncthreads_or(!ray_inv_finite(ray)) != 0;
@@ -299,37 +306,43 @@ __device__ inline void batch_fwd_tile(Stage& st, uint32_t* s_last, uint32_t g, i
     const uint32_t wm = wave_max_u32(t.inside ? last : 0u);
     if (lane == 0) s_last[w] = wm;
     __syncthreads();
-    if (threadIdx.x == 0) tile_last[g] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+    if (threadIdx.x == 0)
+    {
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < TS::NW; ++i) m = max(m, s_last[i]);
+        tile_last[g] = range.x + m;
+    }
 }
 
 // Fast form: workgroup b renders tile order[b] (the hardware dispatches
 // workgroups in index order, so the longest lists start first).
-template <int NS>
-__global__ __launch_bounds__(256) void batch_fwd_kernel(
+template <int NS, typename TS>
+__global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order, uint32_t* __restrict__ ctr,
     uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
 {
-    __shared__ Stage st;
-    __shared__ uint32_t s_last[NWAVE];
+    __shared__ StageT<TS::NW> st;
+    __shared__ uint32_t s_last[TS::NW];
     const uint32_t g = order[blockIdx.x];
-    batch_fwd_tile<NS, false>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers, lengths,
+    batch_fwd_tile<NS, false, TS>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers, lengths,
                               geos, rgbs, tile_last, n_contrib, out_color, out_T);
 }
 
 // Exact form: a few workgroups loop over the deferred tiles (head ctr[2]).
-template <int NS>
-__global__ __launch_bounds__(256) void batch_fwd_exact_kernel(
+template <int NS, typename TS>
+__global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr, uint32_t* __restrict__ exact_list,
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
     float* __restrict__ out_color, float* __restrict__ out_T)
 {
-    __shared__ Stage st;
-    __shared__ uint32_t s_last[NWAVE];
+    __shared__ StageT<TS::NW> st;
+    __shared__ uint32_t s_last[TS::NW];
     __shared__ uint32_t s_i;
     const uint32_t count = ctr[1];
     while (true)
@@ -339,14 +352,14 @@ __global__ __launch_bounds__(256) void batch_fwd_exact_kernel(
         const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_fwd_tile<NS, true>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers,
+        batch_fwd_tile<NS, true, TS>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers,
                                  lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T);
         __syncthreads();
     }
 }
 
-template <int NS, bool EXACT>
-__device__ inline void batch_bwd_tile(Stage& st, float (&acc)[12][NWAVE][64], uint32_t g, int V,
+template <int NS, bool EXACT, typename TS>
+__device__ inline void batch_bwd_tile(StageT<TS::NW>& st, float (&acc)[12][TS::NW][64], uint32_t g, int V,
                                       const svr_view* __restrict__ views, int64_t N, const uint2* __restrict__ ranges,
                                       const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
                                       const float* __restrict__ centers, const float* __restrict__ lengths,
@@ -356,7 +369,7 @@ __device__ inline void batch_bwd_tile(Stage& st, float (&acc)[12][NWAVE][64], ui
                                       const uint32_t* __restrict__ orig, float* __restrict__ partial)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const TileGeom t = tile_geom(views, V, g, w, lane);
+    const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
     const svr_view& vw = views[t.u];
     const PixRay ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
     const bool need_exact = __syncthreads_or(!ray_inv_finite(ray)) != 0;
@@ -365,14 +378,14 @@ __device__ inline void batch_bwd_tile(Stage& st, float (&acc)[12][NWAVE][64], ui
     const size_t pb = (size_t)vw.pix_base;
     const size_t HW = (size_t)t.H * t.W;
     const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
-    adjoint_subtile<NS, BWD_SYNC_MIN, EXACT>(st, acc, w, lane, t.W, t.H, t.tx, t.ty, t.sx0, t.sy0, t.sx1, t.sy1,
+    adjoint_subtile<NS, BWD_SYNC_MIN, EXACT, TS>(st, acc, w, lane, t.W, t.H, t.tx, t.ty, t.sx0, t.sy0, t.sx1, t.sy1,
                                              t.inside, pid, HW, ray, ranges[g], vw.bg_color, vox_list, bboxes + vo,
                                              centers, lengths, geos, rgbs + 3 * vo, out_T + pb, n_contrib + pb,
                                              dL_dcolor + 3 * pb, dL_dT + pb, orig, partial);
 }
 
-template <int NS>
-__global__ __launch_bounds__(256) void batch_bwd_kernel(
+template <int NS, typename TS>
+__global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order,
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
@@ -380,16 +393,16 @@ __global__ __launch_bounds__(256) void batch_bwd_kernel(
     const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
     float* __restrict__ partial)
 {
-    __shared__ Stage st;
-    __shared__ float acc[12][NWAVE][64];
+    __shared__ StageT<TS::NW> st;
+    __shared__ float acc[12][TS::NW][64];
     const uint32_t g = order[blockIdx.x];
-    batch_bwd_tile<NS, false>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs, out_T,
+    batch_bwd_tile<NS, false, TS>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs, out_T,
                               n_contrib, dL_dcolor, dL_dT, orig, partial);
 }
 
 // The forward's deferred tiles (exact_list, count ctr[1]; head ctr[4]).
-template <int NS>
-__global__ __launch_bounds__(256) void batch_bwd_exact_kernel(
+template <int NS, typename TS>
+__global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
     const uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
@@ -397,8 +410,8 @@ __global__ __launch_bounds__(256) void batch_bwd_exact_kernel(
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
     const uint32_t* __restrict__ orig, float* __restrict__ partial)
 {
-    __shared__ Stage st;
-    __shared__ float acc[12][NWAVE][64];
+    __shared__ StageT<TS::NW> st;
+    __shared__ float acc[12][TS::NW][64];
     __shared__ uint32_t s_i;
     const uint32_t count = ctr[1];
     while (true)
@@ -408,7 +421,7 @@ __global__ __launch_bounds__(256) void batch_bwd_exact_kernel(
         const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_bwd_tile<NS, true>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
+        batch_bwd_tile<NS, true, TS>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
                                  out_T, n_contrib, dL_dcolor, dL_dT, orig, partial);
         __syncthreads();
     }
@@ -555,10 +568,24 @@ __global__ __launch_bounds__(64) void batch_sh_bwd_kernel(int deg, int V, const 
 // Workgroups of the launches that loop over the deferred exact-ray tiles.
 constexpr int64_t EXACT_GRID = 256;
 
-int check_views(int V, const svr_view* views, int64_t* T_out, int64_t* P_out)
+// Binning tile of a tile_shape code (SVR_TILE_*).
+inline void tile_dims(int shape, int* tx, int* ty)
+{
+    *tx = shape == SVR_TILE_8X8 ? 8 : (shape == SVR_TILE_32X8 ? 32 : 16);
+    *ty = shape == SVR_TILE_16X16 ? 16 : 8;
+}
+
+// Validates the view table (sizes, running sums, one tile shape) and
+// returns the batch's tiles, pixels and tile shape.
+int check_views(int V, const svr_view* views, int64_t* T_out, int64_t* P_out, int* shape_out = nullptr)
 {
     SVR_REQUIRE(V >= 1 && V <= 65536, SVR_ERR_INVALID, "number of views %d out of range", V);
     SVR_REQUIRE(views != nullptr, SVR_ERR_INVALID, "null view table");
+    const int shape = views[0].tile_shape;
+    SVR_REQUIRE(shape == SVR_TILE_16X16 || shape == SVR_TILE_8X8 || shape == SVR_TILE_32X8, SVR_ERR_INVALID,
+                "unknown tile shape %d", shape);
+    int tw, th;
+    tile_dims(shape, &tw, &th);
     int64_t T = 0, P = 0;
     for (int u = 0; u < V; ++u)
     {
@@ -567,44 +594,55 @@ int check_views(int V, const svr_view* views, int64_t* T_out, int64_t* P_out)
                         vw.height <= MAX_PACKED_IMAGE_DIM,
                     SVR_ERR_INVALID, "view %d: image size %dx%d outside [1, %d]", u, vw.width, vw.height,
                     MAX_PACKED_IMAGE_DIM);
+        SVR_REQUIRE(vw.tile_shape == shape, SVR_ERR_INVALID, "view %d: all views need the same tile shape", u);
         SVR_REQUIRE(vw.tile_base == T && vw.pix_base == P, SVR_ERR_INVALID,
                     "view %d: tile_base / pix_base must be the running sums of the previous views", u);
-        T += (int64_t)tiles_x(vw.width) * tiles_y(vw.height);
+        T += (int64_t)tiles_along(vw.width, tw) * tiles_along(vw.height, th);
         P += (int64_t)vw.width * vw.height;
     }
     SVR_REQUIRE(T < (1ll << 31), SVR_ERR_INVALID, "too many tiles in the batch");
     *T_out = T;
     *P_out = P;
+    if (shape_out) *shape_out = shape;
     return SVR_OK;
 }
 
-template <int NS>
+// Runs BODY with TS bound to the tile shape of a SVR_TILE_* code.
+#define SVR_WITH_TILE(shape, BODY)                                   \
+    switch (shape)                                                   \
+    {                                                                \
+    case SVR_TILE_8X8: { using TS = TileShape<8, 8>; BODY; } break;   \
+    case SVR_TILE_32X8: { using TS = TileShape<32, 8>; BODY; } break; \
+    default: { using TS = Tile16; BODY; } break;                     \
+    }
+
+template <int NS, typename TS>
 int batch_fwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
                      const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
                      const float* geos, const float* rgbs, float* out_color, float* out_T, hipStream_t stream)
 {
-    batch_fwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(V, views, N, img.order, img.ctr, img.exact_list, img.ranges,
-                                                          b.vox_list, g.bbox, centers, lengths, geos, rgbs,
-                                                          img.tile_last, img.n_contrib, out_color, out_T);
+    batch_fwd_kernel<NS, TS><<<(unsigned)T, TS::THREADS, 0, stream>>>(
+        V, views, N, img.order, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+        img.tile_last, img.n_contrib, out_color, out_T);
     SVR_CHECK_LAUNCH();
-    batch_fwd_exact_kernel<NS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), 256, 0, stream>>>(
+    batch_fwd_exact_kernel<NS, TS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
         V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
         img.tile_last, img.n_contrib, out_color, out_T);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }
 
-template <int NS>
+template <int NS, typename TS>
 int batch_bwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
                      const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
                      const float* geos, const float* rgbs, const float* out_T, const float* dL_dcolor,
                      const float* dL_dT, float* partial, hipStream_t stream)
 {
-    batch_bwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(V, views, N, img.order, img.ranges, b.vox_list, g.bbox,
-                                                          centers, lengths, geos, rgbs, out_T, img.n_contrib,
-                                                          dL_dcolor, dL_dT, b.orig, partial);
+    batch_bwd_kernel<NS, TS><<<(unsigned)T, TS::THREADS, 0, stream>>>(
+        V, views, N, img.order, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T, img.n_contrib,
+        dL_dcolor, dL_dT, b.orig, partial);
     SVR_CHECK_LAUNCH();
-    batch_bwd_exact_kernel<NS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), 256, 0, stream>>>(
+    batch_bwd_exact_kernel<NS, TS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
         V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T,
         img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
     SVR_CHECK_LAUNCH();
@@ -643,7 +681,8 @@ int svr_batch_preprocess(int32_t V, const svr_view* views_host, const svr_view* 
                          svr_stream_t stream)
 {
     int64_t T = 0, P = 0;
-    int rc = check_views(V, views_host, &T, &P);
+    int shape = 0;
+    int rc = check_views(V, views_host, &T, &P, &shape);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(N >= 0 && N <= (int64_t)VOX_ID_MASK + 1, SVR_ERR_INVALID, "voxel count %lld out of range",
                 (long long)N);
@@ -653,8 +692,9 @@ int svr_batch_preprocess(int32_t V, const svr_view* views_host, const svr_view* 
                 "svr_batch_preprocess: null pointer");
     GeomState g = GeomState::carve(geom_state, (int64_t)V * N, nullptr);
     prof_begin(PROF_PREPROCESS, stream);
-    batch_preprocess_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(
-        views_dev, N, centers, lengths, out_n_dup, g.n_dup, g.bbox, g.qbits);
+    SVR_WITH_TILE(shape, (batch_preprocess_kernel<TS><<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0,
+                                                        stream>>>(views_dev, N, centers, lengths, out_n_dup, g.n_dup,
+                                                                  g.bbox, g.qbits)));
     SVR_CHECK_LAUNCH();
     prof_end(PROF_PREPROCESS, stream);
     return SVR_OK;
@@ -711,7 +751,8 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
     int64_t T = 0, P = 0;
-    int rc = check_views(V, views_host, &T, &P);
+    int shape = 0;
+    int rc = check_views(V, views_host, &T, &P, &shape);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(R >= 0 && R < (1ll << 32), SVR_ERR_INVALID, "num_rendered out of range");
     SVR_REQUIRE(views_dev && image_state && out_color && out_T, SVR_ERR_INVALID,
@@ -746,12 +787,14 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
         const bool odd = ((nbits + RADIX_BITS - 1) / RADIX_BITS) & 1;
         uint64_t* k0 = odd ? b.keys_alt : b.keys;
         uint32_t* o0 = odd ? b.orig_alt : b.orig;
-        batch_duplicate_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(
-            views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0, g.big_list, g.counters + 2,
-            cap);
+        SVR_WITH_TILE(shape, (batch_duplicate_kernel<TS><<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0,
+                                                           stream>>>(views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox,
+                                                                     g.qbits, ot.pos, k0, o0, g.big_list,
+                                                                     g.counters + 2, cap)));
         SVR_CHECK_LAUNCH();
-        batch_duplicate_big_kernel<<<1024, 256, 0, stream>>>(views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox,
-                                                             g.qbits, ot.pos, k0, o0, g.big_list, g.counters + 2);
+        SVR_WITH_TILE(shape, (batch_duplicate_big_kernel<TS><<<1024, 256, 0, stream>>>(
+                                 views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0, g.big_list,
+                                 g.counters + 2)));
         SVR_CHECK_LAUNCH();
         uint64_t* sorted = nullptr;
         uint32_t* sorted_orig = nullptr;
@@ -776,15 +819,15 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     prof_end(PROF_BINNING, stream);
 
     prof_begin(PROF_RENDER_FWD, stream);
-    switch (n_samp)
-    {
-    case 1: rc = batch_fwd_launch<1>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
-                                     stream); break;
-    case 2: rc = batch_fwd_launch<2>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
-                                     stream); break;
-    default: rc = batch_fwd_launch<3>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
-                                      stream); break;
-    }
+    SVR_WITH_TILE(shape, (rc = n_samp == 1   ? batch_fwd_launch<1, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                          lengths, geos, rgbs, out_color, out_T,
+                                                                          stream)
+                               : n_samp == 2 ? batch_fwd_launch<2, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                          lengths, geos, rgbs, out_color, out_T,
+                                                                          stream)
+                                             : batch_fwd_launch<3, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                          lengths, geos, rgbs, out_color, out_T,
+                                                                          stream)));
     prof_end(PROF_RENDER_FWD, stream);
     return rc;
 }
@@ -799,7 +842,8 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
     int64_t T = 0, P = 0;
-    int rc = check_views(V, views_host, &T, &P);
+    int shape = 0;
+    int rc = check_views(V, views_host, &T, &P, &shape);
     if (rc != SVR_OK) return rc;
     if (N == 0) return SVR_OK;
     SVR_REQUIRE(views_dev && geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub &&
@@ -819,15 +863,15 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
         if (fill_u32(wk.partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
         if (fill_u32(img.ctr + 3, 0u, 2, stream) != SVR_OK) return SVR_ERR_HIP;
         prof_begin(PROF_RENDER_BWD, stream);
-        switch (n_samp)
-        {
-        case 1: rc = batch_bwd_launch<1>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_T,
-                                         dL_dcolor, dL_dT, wk.partial, stream); break;
-        case 2: rc = batch_bwd_launch<2>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_T,
-                                         dL_dcolor, dL_dT, wk.partial, stream); break;
-        default: rc = batch_bwd_launch<3>(V, views_dev, N, T, b, img, g, centers, lengths, geos, rgbs, out_T,
-                                          dL_dcolor, dL_dT, wk.partial, stream); break;
-        }
+        SVR_WITH_TILE(shape, (rc = n_samp == 1   ? batch_bwd_launch<1, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                              lengths, geos, rgbs, out_T, dL_dcolor,
+                                                                              dL_dT, wk.partial, stream)
+                                   : n_samp == 2 ? batch_bwd_launch<2, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                              lengths, geos, rgbs, out_T, dL_dcolor,
+                                                                              dL_dT, wk.partial, stream)
+                                                 : batch_bwd_launch<3, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                              lengths, geos, rgbs, out_T, dL_dcolor,
+                                                                              dL_dT, wk.partial, stream)));
         prof_end(PROF_RENDER_BWD, stream);
         if (rc != SVR_OK) return rc;
     }

@@ -10,19 +10,37 @@
 
 namespace svr {
 
-constexpr int NWAVE = 4;
 #ifndef BWD_SYNC_MIN
 #define BWD_SYNC_MIN 24
 #endif
 
-struct Stage
+// Binning tile of TX x TY pixels; each wave of its workgroup owns one 8x8
+// sub-tile (wave w at column w % WX, row w / WX).  The single-view path uses
+// the reference's 16x16 tiles (forward.cu BLOCK_X / BLOCK_Y); the batch path
+// can also bin at 8x8 or 32x8 (per-pixel results do not depend on it).
+template <int TX_, int TY_>
+struct TileShape
 {
-    float c[3][NWAVE][64];
-    float l[NWAVE][64];
-    float linv[NWAVE][64];
-    float g[8][NWAVE][64];
-    float rgb[3][NWAVE][64];
+    static constexpr int TX = TX_, TY = TY_;
+    static constexpr int WX = TX_ / 8, WY = TY_ / 8;
+    static constexpr int NW = WX * WY;
+    static constexpr int THREADS = 64 * NW;
+    static_assert(TX_ % 8 == 0 && TY_ % 8 == 0 && NW >= 1 && NW <= 16, "tile must be whole 8x8 sub-tiles");
 };
+using Tile16 = TileShape<16, 16>;
+constexpr int NWAVE = Tile16::NW;
+
+// Staged payload of the current chunk, one 64-entry slot row per wave.
+template <int NW>
+struct StageT
+{
+    float c[3][NW][64];
+    float l[NW][64];
+    float linv[NW][64];
+    float g[8][NW][64];
+    float rgb[3][NW][64];
+};
+using Stage = StageT<NWAVE>;
 
 // Pixel mask (bit = 8 * row + column of the wave's 8x8 sub-tile) covered by a
 // packed bbox (inclusive), intersected with qsel.
@@ -47,18 +65,25 @@ __device__ inline uint64_t select_q(const uint64_t* qpix, uint32_t q)
     return s;
 }
 
-// True when the packed bbox (inclusive) overlaps exactly one of the four 8x8
-// sub-tiles of the 16x16 tile at (tx0, ty0): then only that sub-tile's wave can
+// Number of 8-pixel cells k (cell k = [base + 8k, base + 8k + 7] clipped to
+// [0, limit]) among the first `cells` that the inclusive range [lo, hi] meets.
+__device__ inline int cells_met(int lo, int hi, int base, int cells, int limit)
+{
+    const int first = max(lo, base);
+    const int last = min(min(hi, base + 8 * cells - 1), limit);
+    if (first > last) return 0;
+    return (last - base) / 8 - (first - base) / 8 + 1;
+}
+
+// True when the packed bbox (inclusive) overlaps exactly one of the 8x8
+// sub-tiles of the TS tile at (tx0, ty0): then only that sub-tile's wave can
 // produce gradient terms for the entry.
+template <typename TS>
 __device__ inline bool single_subtile(uint2 bb, int tx0, int ty0, int W, int H)
 {
     const int bxmin = (int)(bb.x >> 16), bymin = (int)(bb.x & 0xFFFFu);
     const int bxmax = (int)(bb.y >> 16), bymax = (int)(bb.y & 0xFFFFu);
-    const int cl = (bxmin <= tx0 + 7 && bxmax >= tx0) ? 1 : 0;
-    const int cr = (tx0 + 8 <= W - 1 && bxmax >= tx0 + 8 && bxmin <= tx0 + 15) ? 1 : 0;
-    const int rt = (bymin <= ty0 + 7 && bymax >= ty0) ? 1 : 0;
-    const int rb = (ty0 + 8 <= H - 1 && bymax >= ty0 + 8 && bymin <= ty0 + 15) ? 1 : 0;
-    return (cl + cr) * (rt + rb) == 1;
+    return cells_met(bxmin, bxmax, tx0, TS::WX, W - 1) * cells_met(bymin, bymax, ty0, TS::WY, H - 1) == 1;
 }
 
 __device__ inline uint32_t wave_max_u32(uint32_t x)
@@ -104,7 +129,8 @@ __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const 
     e.r2 = rgbs[3 * v + 2];
 }
 
-__device__ inline void stage_entry(Stage& st, int w, int lane, const EntryRegs& e)
+template <int NW>
+__device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryRegs& e)
 {
     if (!e.pm) return;
     st.c[0][w][lane] = e.c0; st.c[1][w][lane] = e.c1; st.c[2][w][lane] = e.c2; st.l[w][lane] = e.l;
@@ -118,8 +144,8 @@ __device__ inline void stage_entry(Stage& st, int w, int lane, const EntryRegs& 
 // (forward.cu:150-328 for colour and T).  Returns the pixel's n_contrib and
 // leaves its transmittance and colour in T, C0..C2.  EXACT selects the
 // ray-box min/max form (see ray_aabb); it is uniform over the wave.
-template <int NS, bool EXACT>
-__device__ inline uint32_t composite_subtile(Stage& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
+template <int NS, bool EXACT, int NW>
+__device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
                                              int sx0, int sy0, int sx1, int sy1,
                                              const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
                                              const float* __restrict__ centers, const float* __restrict__ lengths,
@@ -198,8 +224,8 @@ __device__ inline uint32_t composite_subtile(Stage& st, int w, int lane, const P
 // Adjoint of one (pixel, entry) hit (backward.cu:259-527 without the
 // regularizers): recovers T before the entry, updates the running dL/dT and
 // writes the 12 gradient terms (8 densities, 3 colours, subdivision priority).
-template <int NS, bool EXACT>
-__device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray, float& T, float& last_dT, float dp0,
+template <int NS, bool EXACT, int NW>
+__device__ inline bool bwd_hit(const StageT<NW>& st, int w, int l, const PixRay& ray, float& T, float& last_dT, float dp0,
                                float dp1, float dp2, float* gq)
 {
     const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
@@ -244,8 +270,8 @@ __device__ inline bool bwd_hit(const Stage& st, int w, int l, const PixRay& ray,
 // The chunk loop is software-pipelined: chunk c+1 is loaded into registers
 // before chunk c's gradients are flushed with global atomics, because on this
 // architecture a later load's wait also waits for all earlier stores/atomics.
-template <int NS, int SYNC_MIN, bool EXACT>
-__device__ inline void adjoint_subtile(Stage& st, float (&acc)[12][NWAVE][64], int w, int lane, int W, int H, int tx,
+template <int NS, int SYNC_MIN, bool EXACT, typename TS = Tile16>
+__device__ inline void adjoint_subtile(StageT<TS::NW>& st, float (&acc)[12][TS::NW][64], int w, int lane, int W, int H, int tx,
                                        int ty, int sx0, int sy0, int sx1, int sy1, bool inside, size_t pid, size_t HW,
                                        const PixRay& ray, uint2 range, float bg,
                                        const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
@@ -306,7 +332,7 @@ __device__ inline void adjoint_subtile(Stage& st, float (&acc)[12][NWAVE][64], i
                 const int l = __builtin_ctzll(seg);
                 seg &= seg - 1ull;
                 float gq[12];
-                if (bwd_hit<NS, EXACT>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
+                if (bwd_hit<NS, EXACT, TS::NW>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
                 {
 #pragma unroll
                     for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
@@ -320,7 +346,7 @@ __device__ inline void adjoint_subtile(Stage& st, float (&acc)[12][NWAVE][64], i
 #pragma unroll
             for (int i = 0; i < 12; ++i) gq[i] = 0.f;
             bool hit = false;
-            if (mine) hit = bwd_hit<NS, EXACT>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
+            if (mine) hit = bwd_hit<NS, EXACT, TS::NW>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
             if (__ballot(hit) == 0ull) continue;
             if (!hit)
             {
@@ -338,7 +364,7 @@ __device__ inline void adjoint_subtile(Stage& st, float (&acc)[12][NWAVE][64], i
         // prefetch the next chunk before this chunk's atomics are issued
         const uint32_t fo = cur_o;
         const bool fpm = cur.pm != 0ull;
-        const bool fsole = fpm && single_subtile(cur.bb, tx * TILE_X, ty * TILE_Y, W, H);
+        const bool fsole = fpm && single_subtile<TS>(cur.bb, tx * TS::TX, ty * TS::TY, W, H);
         const int64_t nend = end - 64;
         {
             const int64_t jl = nend - 1 - lane;

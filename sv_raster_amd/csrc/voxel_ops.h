@@ -15,6 +15,8 @@ namespace svr {
 // bitset and the number of list entries (tiles x quadrants).  The arithmetic
 // order matches the oracle exactly (no contraction), so the packed bboxes are
 // bit-identical.
+// TX x TY: the binning tile (16 x 16 for the reference's lists).
+template <int TX = TILE_X, int TY = TILE_Y>
 __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, int W, int H, int gx, int gy,
                                         float focal_x, float focal_y, float cx, float cy, const float* w2c,
                                         const float* c2w, float near_plane, uint32_t& nd_out, uint2& bb_out,
@@ -59,8 +61,8 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
             const uint32_t ymax = (uint32_t)(int)__builtin_rintf(bmaxy);
             bb = make_uint2((xmin << 16) | ymin, (xmax << 16) | ymax);
             // auxiliary.h:150-170
-            const int txmin = min((int)(xmin / TILE_X), gx - 1), tymin = min((int)(ymin / TILE_Y), gy - 1);
-            const int txmax = min((int)(xmax / TILE_X), gx - 1), tymax = min((int)(ymax / TILE_Y), gy - 1);
+            const int txmin = min((int)(xmin / TX), gx - 1), tymin = min((int)(ymin / TY), gy - 1);
+            const int txmax = min((int)(xmax / TX), gx - 1), tymax = min((int)(ymax / TY), gy - 1);
             nd = (uint32_t)((1 + tymax - tymin) * (1 + txmax - txmin)) * (uint32_t)__popc(bits);
         }
         else
@@ -77,15 +79,19 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
     bits_out = bits;
 }
 
-// Tile rectangle of a packed bbox (auxiliary.h:150-170).
+// Tile rectangle of a packed bbox (auxiliary.h:150-170) for TX x TY tiles.
+template <int TX = TILE_X, int TY = TILE_Y>
 __device__ inline void rect_of(uint2 bb, int gx, int gy, int& txmin, int& tymin, int& txmax, int& tymax)
 {
     const uint32_t xmin = bb.x >> 16, ymin = bb.x & 0xFFFFu, xmax = bb.y >> 16, ymax = bb.y & 0xFFFFu;
-    txmin = min((int)(xmin / TILE_X), gx - 1);
-    tymin = min((int)(ymin / TILE_Y), gy - 1);
-    txmax = min((int)(xmax / TILE_X), gx - 1);
-    tymax = min((int)(ymax / TILE_Y), gy - 1);
+    txmin = min((int)(xmin / TX), gx - 1);
+    tymin = min((int)(ymin / TY), gy - 1);
+    txmax = min((int)(xmax / TX), gx - 1);
+    tymax = min((int)(ymax / TY), gy - 1);
 }
+
+// Tiles along an image side of n pixels for tiles of t pixels.
+__host__ __device__ inline int tiles_along(int n, int t) { return (n + t - 1) / t; }
 
 // sh_compute.cu:33-50 (real SH normalisations with their signs)
 constexpr float SH_C2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
