@@ -155,6 +155,50 @@ int svr_unbiased_adam_step_device(int32_t sparse, int64_t numel, float* param, c
 int svr_biased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
                          float* exp_avg_sq, float lr, float beta1, float beta2, float eps, svr_stream_t stream);
 
+/* ---- grid-point parameters ------------------------------------------ */
+/* Replaces `_C.gather_triinterp_geo_params` (geo_params_gather.cu:22-45,
+ * 134-153): geo_params[idx, k] = grid_pts[vox_key[idx, k]] for the n_care voxels
+ * idx of care_idx (other rows untouched).  vox_key [N,8] int64. */
+int svr_gather_geo_params(int64_t n_care, const int64_t* vox_key, const int64_t* care_idx, const float* grid_pts,
+                          float* geo_params, svr_stream_t stream);
+/* Replaces `_C.gather_triinterp_geo_params_bw` (geo_params_gather.cu:47-72,
+ * 155-173): adds dL_dgeo_params[idx, k] into dL_dgrid_pts[vox_key[idx, k]]
+ * (float atomics; the caller zeroes dL_dgrid_pts). */
+int svr_gather_geo_params_bw(int64_t n_care, const int64_t* vox_key, const int64_t* care_idx,
+                             const float* dL_dgeo_params, float* dL_dgrid_pts, svr_stream_t stream);
+/* Replaces `_C.gather_triinterp_feat_params(_bw)` (geo_params_gather.cu:
+ * 74-132, 175-224) for n_dim = 3 channels per grid point: feat [N,8,3]. */
+int svr_gather_feat_params(int64_t n_care, int32_t n_dim, const int64_t* vox_key, const int64_t* care_idx,
+                           const float* grid_pts, float* feat_params, svr_stream_t stream);
+int svr_gather_feat_params_bw(int64_t n_care, int32_t n_dim, const int64_t* vox_key, const int64_t* care_idx,
+                              const float* dL_dfeat_params, float* dL_dgrid_pts, svr_stream_t stream);
+/* Replaces `_C.total_variation_bw` (tv_compute.cu:23-111): adds the gradient
+ * of the weighted L1 differences along each voxel's 12 corner-cube edges into
+ * grid_pts_grad [M, C] in place (weight x 0.01 / voxel size unless no_tv_s).
+ * tv_sparse: only (voxel, channel) items with a non-zero corner gradient take
+ * part, decided on grid_pts_grad as it is before this call (workspace:
+ * svr_total_variation_workspace_size bytes). */
+size_t svr_total_variation_workspace_size(int64_t n_voxels, int32_t n_channels);
+int svr_total_variation_bw(int64_t n_voxels, int32_t n_channels, const float* grid_pts, const int64_t* vox_key,
+                           float weight, const float* vox_size_inv, int32_t no_tv_s, int32_t tv_sparse,
+                           void* workspace, float* grid_pts_grad, svr_stream_t stream);
+
+/* ---- camera utilities (utils.cu:23-102, 188-245) ---------------------- */
+/* Replaces `_C.is_in_cone`: is_in [n] (bool bytes) = the point is in front of
+ * `near` and inside the camera's field-of-view cone. */
+int svr_is_in_cone(int64_t n, float tanfovx, float tanfovy, float near_plane, const float* w2c, const float* pts,
+                   uint8_t* is_in, svr_stream_t stream);
+/* Replaces `_C.compute_rd`: per-pixel world ray directions (unnormalised), [3,H,W]. */
+int svr_compute_rd(int32_t width, int32_t height, float cx, float cy, float tanfovx, float tanfovy,
+                   const float* c2w, float* rd, svr_stream_t stream);
+/* Replaces `_C.depth2pts`: camera centre + depth x ray direction, [3,H,W]. */
+int svr_depth2pts(int32_t width, int32_t height, float cx, float cy, float tanfovx, float tanfovy,
+                  const float* c2w, const float* depth, float* pts, svr_stream_t stream);
+/* Replaces `_C.filter_geomState` (raster_state.cu:207-260): a geometry state
+ * of the n_keep voxels `indices` (n_duplicates, bboxes, quadrant bits). */
+int svr_filter_geom_state(int64_t n_orig, int64_t n_keep, const int64_t* indices, const void* geom_state,
+                          void* new_geom_state, svr_stream_t stream);
+
 /* ---- octree utilities (utils.cu:106-189) ---------------------------- */
 int svr_voxel_order_rank(int64_t n_voxels, const int64_t* octree_paths, int64_t* order_ranks /* [8,N] */,
                          svr_stream_t stream);

@@ -198,3 +198,58 @@ def full_forward(cam, octpaths, centers, lengths, geos, rgbs, bg=0.0, n_samp=1):
     color, T, nc, tl = render_fwd(cam, ranges, vl, bbox, centers, lengths, geos, rgbs, bg=bg, n_samp=n_samp)
     return dict(n_dup=n_dup, bbox=bbox, qbits=qbits, keys_hi=hi, keys_lo=lo, vox_list=vl, ranges=ranges,
                 color=color, T=T, n_contrib=nc, tile_last=tl)
+
+
+def gather(vox_key, care_idx, grid, n_dim=1):
+    """NC/src/geo_params_gather.cu:22-108 -> float32 [N, 8] (n_dim 1) or [N, 8, n_dim]; rows outside care are 0."""
+    vox_key = np.ascontiguousarray(vox_key, np.int64)
+    care_idx = np.ascontiguousarray(care_idx, np.int64).reshape(-1)
+    grid = _f32(grid)
+    N = vox_key.shape[0]
+    out = np.zeros((N, 8, n_dim), np.float32)
+    lib().oracle_gather(ctypes.c_int64(care_idx.size), ctypes.c_int(n_dim), _p(vox_key), _p(care_idx), _p(grid),
+                        _p(out))
+    return out.reshape(N, 8) if n_dim == 1 else out
+
+
+def gather_bw(vox_key, care_idx, n_grid, dL, n_dim=1):
+    """NC/src/geo_params_gather.cu:47-132 -> float64 [n_grid, n_dim]."""
+    vox_key = np.ascontiguousarray(vox_key, np.int64)
+    care_idx = np.ascontiguousarray(care_idx, np.int64).reshape(-1)
+    out = np.zeros((n_grid, n_dim), np.float64)
+    lib().oracle_gather_bw(ctypes.c_int64(care_idx.size), ctypes.c_int(n_dim), _p(vox_key), _p(care_idx),
+                           _p(_f32(dL)), _p(out))
+    return out
+
+
+def tv_bw(grid, vox_key, weight, vox_size_inv, no_tv_s, tv_sparse, grad):
+    """NC/src/tv_compute.cu:23-80 -> float64 [M, C]: grad plus the TV gradient."""
+    grid = _f32(grid)
+    M, C = grid.shape
+    vox_key = np.ascontiguousarray(vox_key, np.int64)
+    out = np.ascontiguousarray(grad, np.float64).copy()
+    vsi = _f32(vox_size_inv).reshape(-1)
+    lib().oracle_tv_bw(ctypes.c_int64(vox_key.shape[0]), ctypes.c_int(C), _p(grid), _p(vox_key),
+                       ctypes.c_float(weight), _p(vsi), ctypes.c_int(int(no_tv_s)), ctypes.c_int(int(tv_sparse)),
+                       _p(out))
+    return out
+
+
+def is_in_cone(tanfovx, tanfovy, near, w2c, pts):
+    """NC/src/utils.cu:23-43 -> bool [n]."""
+    pts = _f32(pts)
+    out = np.zeros(pts.shape[0], np.uint8)
+    lib().oracle_is_in_cone(ctypes.c_int64(pts.shape[0]), ctypes.c_float(tanfovx), ctypes.c_float(tanfovy),
+                            ctypes.c_float(near), _p(_f32(w2c)), _p(pts), _p(out))
+    return out.astype(bool)
+
+
+def pixel_rays(W, H, cx, cy, tanfovx, tanfovy, c2w, depth=None):
+    """NC/src/utils.cu:45-102 -> float32 [3, H, W] (compute_rd, or depth2pts with depth)."""
+    out = np.zeros((3, H, W), np.float32)
+    d = _f32(depth).reshape(-1) if depth is not None else None
+    lib().oracle_pixel_rays(ctypes.c_int(W), ctypes.c_int(H), ctypes.c_float(cx), ctypes.c_float(cy),
+                            ctypes.c_float(tanfovx), ctypes.c_float(tanfovy), _p(_f32(c2w)),
+                            _p(d) if d is not None else ctypes.c_void_p(0), _p(out))
+    return out
+

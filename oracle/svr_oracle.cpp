@@ -630,4 +630,109 @@ void oracle_voxel_order_rank(int N, const int64_t* octpaths, int64_t* ranks /* 8
         for (int v = 0; v < N; ++v) ranks[(size_t)q * N + v] = (int64_t)((uint64_t)octpaths[v] ^ order_table(q));
 }
 
+// geo_params_gather.cu:22-45 / 74-108: out[idx, k, d] = grid[key[idx, k], d]
+// for idx in care (n_dim = 1 for the density form).
+void oracle_gather(int64_t n_care, int n_dim, const int64_t* vox_key, const int64_t* care, const float* grid,
+                   float* out)
+{
+    for (int64_t i = 0; i < n_care; ++i)
+    {
+        const int64_t idx = care[i];
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < n_dim; ++d)
+                out[(idx * 8 + k) * n_dim + d] = grid[vox_key[idx * 8 + k] * n_dim + d];
+    }
+}
+
+// geo_params_gather.cu:47-72 / 110-132: grad_grid[key[idx, k], d] += g[idx, k, d]
+// (summed in double here).
+void oracle_gather_bw(int64_t n_care, int n_dim, const int64_t* vox_key, const int64_t* care, const float* g,
+                      double* grad_grid)
+{
+    for (int64_t i = 0; i < n_care; ++i)
+    {
+        const int64_t idx = care[i];
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < n_dim; ++d)
+                grad_grid[vox_key[idx * 8 + k] * n_dim + d] += (double)g[(idx * 8 + k) * n_dim + d];
+    }
+}
+
+// tv_compute.cu:23-80.  grad (double, in/out) holds the gradient before the
+// call; with tv_sparse an item takes part when any corner gradient is
+// non-zero in that incoming gradient.  Each corner adds w * sum over its 3
+// cube neighbours of (v > v_neighbour ? 1 : -1).
+void oracle_tv_bw(int64_t N, int C, const float* grid, const int64_t* vox_key, float weight,
+                  const float* vox_size_inv, int no_tv_s, int tv_sparse, double* grad)
+{
+    std::vector<unsigned char> take((size_t)N * C, 1);
+    if (tv_sparse)
+        for (int64_t n = 0; n < N; ++n)
+            for (int c = 0; c < C; ++c)
+            {
+                bool any = false;
+                for (int k = 0; k < 8; ++k) any = any || (grad[vox_key[n * 8 + k] * C + c] != 0.0);
+                take[(size_t)n * C + c] = any;
+            }
+    for (int64_t n = 0; n < N; ++n)
+        for (int c = 0; c < C; ++c)
+        {
+            if (!take[(size_t)n * C + c]) continue;
+            float v[8];
+            for (int k = 0; k < 8; ++k) v[k] = grid[vox_key[n * 8 + k] * C + c];
+            float w = weight;
+            if (!no_tv_s) w *= 0.01f * vox_size_inv[n];
+            for (int k = 0; k < 8; ++k)
+            {
+                int sgn = 0;
+                sgn += (v[k] > v[k ^ 1]) ? 1 : -1;
+                sgn += (v[k] > v[k ^ 2]) ? 1 : -1;
+                sgn += (v[k] > v[k ^ 4]) ? 1 : -1;
+                grad[vox_key[n * 8 + k] * C + c] += (double)(w * (float)sgn);
+            }
+        }
+}
+
+// utils.cu:23-43 (no contraction)
+void oracle_is_in_cone(int64_t n, float tanfovx, float tanfovy, float near_plane, const float* w2c,
+                       const float* pts, unsigned char* is_in)
+{
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+        const float cx = w2c[0] * x + w2c[1] * y + w2c[2] * z + w2c[3];
+        const float cy = w2c[4] * x + w2c[5] * y + w2c[6] * z + w2c[7];
+        const float cz = w2c[8] * x + w2c[9] * y + w2c[10] * z + w2c[11];
+        is_in[i] = (cz > near_plane) && (std::fabs(cx) <= tanfovx * cz) && (std::fabs(cy) <= tanfovy * cz);
+    }
+}
+
+// utils.cu:45-102: rays (depth == nullptr) or points at depth, planar [3, H, W]
+void oracle_pixel_rays(int W, int H, float cx, float cy, float tanfovx, float tanfovy, const float* c2w,
+                       const float* depth, float* out)
+{
+    const int64_t n = (int64_t)W * H;
+    for (int64_t pid = 0; pid < n; ++pid)
+    {
+        const float yid = (float)(pid / W), xid = (float)(pid % W);
+        const float dx = (xid + 0.5f - cx) * 2.f * tanfovx / (float)W;
+        const float dy = (yid + 0.5f - cy) * 2.f * tanfovy / (float)H;
+        const float rx = c2w[0] * dx + c2w[1] * dy + c2w[2] * 1.f;
+        const float ry = c2w[4] * dx + c2w[5] * dy + c2w[6] * 1.f;
+        const float rz = c2w[8] * dx + c2w[9] * dy + c2w[10] * 1.f;
+        if (depth)
+        {
+            out[pid] = c2w[3] + depth[pid] * rx;
+            out[n + pid] = c2w[7] + depth[pid] * ry;
+            out[2 * n + pid] = c2w[11] + depth[pid] * rz;
+        }
+        else
+        {
+            out[pid] = rx;
+            out[n + pid] = ry;
+            out[2 * n + pid] = rz;
+        }
+    }
+}
+
 }  // extern "C"

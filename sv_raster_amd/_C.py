@@ -328,3 +328,110 @@ def octpath_2_ijk(octpath, octlevel):
     _lib.check(_L.svr_octpath_2_ijk(lv.numel(), _ptr(paths), _ptr(lv), _ptr(out), _stream(paths.device)),
                "octpath_2_ijk")
     return out
+
+
+# ---------------------------------------------------------------------------
+# binding.cpp:31-34 -> geo_params_gather.cu
+
+def _i64(t):
+    return t.contiguous() if t.dtype == torch.int64 else t.long().contiguous()
+
+
+def gather_triinterp_geo_params(vox_key, care_idx, grid_pts):
+    _require_cuda(vox_key, care_idx, grid_pts)
+    N = vox_key.size(0)
+    geo_params = torch.empty((N, 8), dtype=grid_pts.dtype, device=grid_pts.device)
+    key, idx, pts = _i64(vox_key), _i64(care_idx), _f32(grid_pts)
+    _lib.check(_L.svr_gather_geo_params(idx.numel(), _ptr(key), _ptr(idx), _ptr(pts), _ptr(geo_params),
+                                        _stream(grid_pts.device)), "gather_triinterp_geo_params")
+    return geo_params
+
+
+def gather_triinterp_geo_params_bw(vox_key, care_idx, num_grid_pts, dL_dgeo_params):
+    _require_cuda(vox_key, care_idx, dL_dgeo_params)
+    dL_dgrid_pts = torch.zeros((int(num_grid_pts), 1), dtype=dL_dgeo_params.dtype, device=dL_dgeo_params.device)
+    key, idx, g = _i64(vox_key), _i64(care_idx), _f32(dL_dgeo_params)
+    _lib.check(_L.svr_gather_geo_params_bw(idx.numel(), _ptr(key), _ptr(idx), _ptr(g), _ptr(dL_dgrid_pts),
+                                           _stream(g.device)), "gather_triinterp_geo_params_bw")
+    return dL_dgrid_pts
+
+
+def gather_triinterp_feat_params(vox_key, care_idx, grid_pts):
+    _require_cuda(vox_key, care_idx, grid_pts)
+    N, n_dim = vox_key.size(0), grid_pts.size(1)
+    if n_dim != 3:
+        raise RuntimeError("Only support n_dim=3 now.")
+    feat = torch.empty((N, 8, n_dim), dtype=grid_pts.dtype, device=grid_pts.device)
+    key, idx, pts = _i64(vox_key), _i64(care_idx), _f32(grid_pts)
+    _lib.check(_L.svr_gather_feat_params(idx.numel(), n_dim, _ptr(key), _ptr(idx), _ptr(pts), _ptr(feat),
+                                         _stream(grid_pts.device)), "gather_triinterp_feat_params")
+    return feat
+
+
+def gather_triinterp_feat_params_bw(vox_key, care_idx, num_grid_pts, dL_dfeat_params):
+    _require_cuda(vox_key, care_idx, dL_dfeat_params)
+    n_dim = dL_dfeat_params.size(2)
+    if n_dim != 3:
+        raise RuntimeError("Only support n_dim=3")
+    dL_dgrid_pts = torch.zeros((int(num_grid_pts), n_dim), dtype=dL_dfeat_params.dtype,
+                               device=dL_dfeat_params.device)
+    key, idx, g = _i64(vox_key), _i64(care_idx), _f32(dL_dfeat_params)
+    _lib.check(_L.svr_gather_feat_params_bw(idx.numel(), n_dim, _ptr(key), _ptr(idx), _ptr(g), _ptr(dL_dgrid_pts),
+                                            _stream(g.device)), "gather_triinterp_feat_params_bw")
+    return dL_dgrid_pts
+
+
+# binding.cpp:39 -> tv_compute.cu:83-111 (in place on grid_pts_grad)
+def total_variation_bw(grid_pts, vox_key, weight, vox_size_inv, no_tv_s, tv_sparse, grid_pts_grad):
+    _require_cuda(grid_pts, vox_key, vox_size_inv, grid_pts_grad)
+    if not grid_pts_grad.is_contiguous() or grid_pts_grad.dtype != torch.float32:
+        raise RuntimeError("grid_pts_grad must be a contiguous float32 tensor (updated in place)")
+    N, C = vox_key.size(0), grid_pts.size(1)
+    pts, key = _f32(grid_pts), _i64(vox_key)
+    vsi = _f32(vox_size_inv) if vox_size_inv is not None else None
+    ws = torch.empty(int(_L.svr_total_variation_workspace_size(N, C)) if tv_sparse else 0, dtype=torch.uint8,
+                     device=grid_pts.device)
+    _lib.check(_L.svr_total_variation_bw(N, C, _ptr(pts), _ptr(key), float(weight), _ptr(vsi), int(bool(no_tv_s)),
+                                         int(bool(tv_sparse)), _ptr(ws), _ptr(grid_pts_grad),
+                                         _stream(grid_pts.device)), "total_variation_bw")
+
+
+# binding.cpp:28, 41-43 -> raster_state.cu:227-260, utils.cu:188-245
+def filter_geomState(ori_P, indices, geomState):
+    _require_cuda(indices, geomState)
+    P = indices.numel()
+    new = torch.empty(int(_L.svr_geom_state_size(P)) if P > 0 else 0, dtype=torch.uint8, device=geomState.device)
+    if P > 0:
+        idx = _i64(indices)
+        _lib.check(_L.svr_filter_geom_state(int(ori_P), P, _ptr(idx), _ptr(geomState), _ptr(new),
+                                            _stream(geomState.device)), "filter_geomState")
+    return new
+
+
+def is_in_cone(tanfovx, tanfovy, near, w2c_matrix, pts):
+    _require_cuda(w2c_matrix, pts)
+    N = pts.size(0)
+    is_in = torch.zeros(N, dtype=torch.bool, device=pts.device)
+    if N > 0:
+        p, w2c = _f32(pts), _f32(w2c_matrix)
+        _lib.check(_L.svr_is_in_cone(N, float(tanfovx), float(tanfovy), float(near), _ptr(w2c), _ptr(p),
+                                     _ptr(is_in), _stream(pts.device)), "is_in_cone")
+    return is_in
+
+
+def compute_rd(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix):
+    _require_cuda(c2w_matrix)
+    rd = torch.empty((3, int(height), int(width)), dtype=torch.float32, device=c2w_matrix.device)
+    _lib.check(_L.svr_compute_rd(int(width), int(height), float(cx), float(cy), float(tanfovx), float(tanfovy),
+                                 _ptr(_f32(c2w_matrix)), _ptr(rd), _stream(c2w_matrix.device)), "compute_rd")
+    return rd
+
+
+def depth2pts(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix, depth):
+    _require_cuda(c2w_matrix, depth)
+    pts = torch.empty((3, int(height), int(width)), dtype=torch.float32, device=c2w_matrix.device)
+    _lib.check(_L.svr_depth2pts(int(width), int(height), float(cx), float(cy), float(tanfovx), float(tanfovy),
+                                _ptr(_f32(c2w_matrix)), _ptr(_f32(depth)), _ptr(pts), _stream(c2w_matrix.device)),
+               "depth2pts")
+    return pts
+

@@ -78,6 +78,16 @@ SIGNATURES = {
                                                       c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_unbiased_adam_step_device": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_f64, c_f64, c_f64,
                                               c_f32, c_p]),
+    "svr_gather_geo_params": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_gather_geo_params_bw": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
+    "svr_gather_feat_params": (c_i32, [c_i64, c_i32, c_p, c_p, c_p, c_p, c_p]),
+    "svr_gather_feat_params_bw": (c_i32, [c_i64, c_i32, c_p, c_p, c_p, c_p, c_p]),
+    "svr_total_variation_workspace_size": (c_sz, [c_i64, c_i32]),
+    "svr_total_variation_bw": (c_i32, [c_i64, c_i32, c_p, c_p, c_f32, c_p, c_i32, c_i32, c_p, c_p, c_p]),
+    "svr_is_in_cone": (c_i32, [c_i64, c_f32, c_f32, c_f32, c_p, c_p, c_p, c_p]),
+    "svr_compute_rd": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_p, c_p]),
+    "svr_depth2pts": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_p, c_p, c_p]),
+    "svr_filter_geom_state": (c_i32, [c_i64, c_i64, c_p, c_p, c_p, c_p]),
     "svr_debug_sync": (c_i32, [c_p]),
     "svr_profile_enable": (c_i32, [c_i32]),
     "svr_profile_collect": (c_i32, [c_i32, ctypes.POINTER(c_f64), ctypes.POINTER(c_i64)]),

@@ -238,6 +238,48 @@ class SH_eval(torch.autograd.Function):
         return None, None, None, None, None, dL_dsh0, dL_dshs
 
 
+class GatherGeoParams(torch.autograd.Function):
+    """renderer.py:319-345: per-voxel corner values [N, 8] gathered from grid points (rows of care_idx only)."""
+
+    @staticmethod
+    def forward(ctx, vox_key, care_idx, grid_pts):
+        assert len(vox_key.shape) == 2 and vox_key.shape[1] == 8
+        assert len(care_idx.shape) == 1
+        assert grid_pts.shape[0] == grid_pts.numel()
+        geo_params = _C.gather_triinterp_geo_params(vox_key, care_idx, grid_pts)
+        ctx.num_grid_pts = grid_pts.numel()
+        ctx.grid_shape = grid_pts.shape
+        ctx.save_for_backward(vox_key, care_idx)
+        return geo_params
+
+    @staticmethod
+    def backward(ctx, dL_dgeo_params):
+        vox_key, care_idx = ctx.saved_tensors
+        dL_dgrid_pts = _C.gather_triinterp_geo_params_bw(vox_key, care_idx, ctx.num_grid_pts, dL_dgeo_params)
+        # [M, 1] as the reference returns; also accepted for a flat [M] grid
+        return None, None, dL_dgrid_pts.view(ctx.grid_shape)
+
+
+class GatherFeatParams(torch.autograd.Function):
+    """renderer.py:348-374: per-voxel corner features [N, 8, 3] gathered from grid points."""
+
+    @staticmethod
+    def forward(ctx, vox_key, care_idx, grid_pts):
+        assert len(vox_key.shape) == 2 and vox_key.shape[1] == 8
+        assert len(care_idx.shape) == 1
+        assert len(grid_pts.shape) == 2
+        feat_params = _C.gather_triinterp_feat_params(vox_key, care_idx, grid_pts)
+        ctx.num_grid_pts = len(grid_pts)
+        ctx.save_for_backward(vox_key, care_idx)
+        return feat_params
+
+    @staticmethod
+    def backward(ctx, dL_dfeat_params):
+        vox_key, care_idx = ctx.saved_tensors
+        dL_dgrid_pts = _C.gather_triinterp_feat_params_bw(vox_key, care_idx, ctx.num_grid_pts, dL_dfeat_params)
+        return None, None, dL_dgrid_pts
+
+
 def mark_n_duplicates(
         image_width, image_height,
         tanfovx, tanfovy,

@@ -29,12 +29,23 @@ def octpath_2_ijk(octpath, octlevel):
 
 
 def is_in_cone(pts, cam):
-    raise NotImplementedError("is_in_cone is outside the rasterize path implemented by sv_raster_amd")
+    """utils.py:19-29: bool [N], point in front of cam.near and inside the field-of-view cone."""
+    assert torch.is_tensor(pts)
+    assert pts.device == cam.w2c.device
+    assert len(pts.shape) == 2
+    assert pts.shape[1] == 3
+    return _C.is_in_cone(cam.tanfovx, cam.tanfovy, cam.near, cam.w2c, pts)
 
 
 def compute_rd(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix):
-    raise NotImplementedError("compute_rd is outside the rasterize path implemented by sv_raster_amd")
+    """utils.py:32-34: per-pixel world ray directions [3, H, W] (unnormalised)."""
+    assert torch.is_tensor(c2w_matrix)
+    return _C.compute_rd(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix)
 
 
 def depth2pts(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix, depth):
-    raise NotImplementedError("depth2pts is outside the rasterize path implemented by sv_raster_amd")
+    """utils.py:37-41: world points [3, H, W] at the given per-pixel depth along the rays."""
+    assert torch.is_tensor(c2w_matrix)
+    assert depth.device == c2w_matrix.device
+    assert depth.numel() == width * height
+    return _C.depth2pts(width, height, cx, cy, tanfovx, tanfovy, c2w_matrix, depth)
