@@ -148,6 +148,65 @@ def render_bwd(cam, ranges, vox_list, bbox, centers, lengths, geos, rgbs, out_T,
     return acc[:, :8].copy(), acc[:, 8:11].copy(), acc[:, 11:12].copy()
 
 
+def render_fwd_ext(cam, ranges, vox_list, bbox, centers, lengths, geos, rgbs, bg=0.0, n_samp=1, need_depth=False,
+                   need_dist=False, need_normal=False, track_max_w=False, dist_loss=False, dtype=np.float32):
+    """NC/src/forward.cu:43-394 with the optional outputs -> dict(color [3,H,W], T [H,W], n_contrib, tile_last,
+    depth [3,H,W] (mean, distortion sum, median), normal [3,H,W], max_w [N], dist_loss [H,W] (the per-pixel
+    distortion loss value, oracle only: sum_ij w_i w_j |m_i - m_j| + 1/3 sum_i w_i^2 span_i))."""
+    W, H = cam["W"], cam["H"]
+    gx, gy = tile_grid(W, H)
+    cv = _f64 if dtype == np.float64 else _f32
+    ct = ctypes.c_double if dtype == np.float64 else ctypes.c_float
+    centers, lengths, geos, rgbs = cv(centers), cv(lengths).reshape(-1), cv(geos), cv(rgbs)
+    N = centers.shape[0]
+    c2w = cv(cam["c2w"]).ravel()
+    color = np.zeros((3, H, W), dtype)
+    T = np.zeros((H, W), dtype)
+    nc = np.zeros((H, W), np.uint32)
+    tl = np.zeros(gx * gy, np.uint32)
+    depth = np.zeros((3, H, W), dtype)
+    normal = np.zeros((3, H, W), dtype)
+    max_w = np.zeros(N, dtype)
+    dl = np.zeros((H, W), dtype)
+    fn = lib().oracle_render_fwd_ext_f64 if dtype == np.float64 else lib().oracle_render_fwd_ext_f32
+    fn(ctypes.c_int(W), ctypes.c_int(H), ct(cam["tanfovx"]), ct(cam["tanfovy"]), ct(cam["cx"]), ct(cam["cy"]),
+       _p(c2w), ct(bg), ctypes.c_int(n_samp),
+       _p(np.ascontiguousarray(ranges, np.uint32)), _p(np.ascontiguousarray(vox_list, np.uint32)),
+       _p(np.ascontiguousarray(bbox, np.uint32)), _p(centers), _p(lengths), _p(geos), _p(rgbs),
+       ctypes.c_int(int(need_depth)), ctypes.c_int(int(need_dist)), ctypes.c_int(int(need_normal)),
+       ctypes.c_int(int(track_max_w)), _p(color), _p(T), _p(nc), _p(tl), _p(depth), _p(normal), _p(max_w),
+       _p(dl) if dist_loss else ctypes.c_void_p(0))
+    return dict(color=color, T=T, n_contrib=nc, tile_last=tl, depth=depth, normal=normal, max_w=max_w,
+                dist_loss=dl)
+
+
+def render_bwd_ext(cam, ranges, vox_list, bbox, centers, lengths, geos, rgbs, out_T, n_contrib, tile_last,
+                   dL_dcolor, dL_dT, bg=0.0, n_samp=1, need_depth=False, need_normal=False, dL_ddepth=None,
+                   dL_dnormal=None, lambda_R=0.0, gt=None, lambda_asc=0.0, lambda_dist=0.0, out_depth=None,
+                   dtype=np.float32):
+    """NC/src/backward.cu:37-772 with the depth / normal / distortion gradients and the R_concen / ascending
+    regularizers -> (dL_dgeos [N,8], dL_drgbs [N,3], subdiv_p_bw [N,1]) in float64."""
+    W, H = cam["W"], cam["H"]
+    cv = _f64 if dtype == np.float64 else _f32
+    ct = ctypes.c_double if dtype == np.float64 else ctypes.c_float
+    centers, lengths, geos, rgbs = cv(centers), cv(lengths).reshape(-1), cv(geos), cv(rgbs)
+    N = centers.shape[0]
+    c2w = cv(cam["c2w"]).ravel()
+    acc = np.zeros((N, 12), np.float64)
+    z3 = np.zeros((3, H, W), dtype)
+    opt = lambda a: cv(a) if a is not None else z3  # noqa: E731
+    fn = lib().oracle_render_bwd_ext_f64 if dtype == np.float64 else lib().oracle_render_bwd_ext_f32
+    fn(ctypes.c_int(W), ctypes.c_int(H), ct(cam["tanfovx"]), ct(cam["tanfovy"]), ct(cam["cx"]), ct(cam["cy"]),
+       _p(c2w), ct(bg), ctypes.c_int(n_samp),
+       _p(np.ascontiguousarray(ranges, np.uint32)), _p(np.ascontiguousarray(vox_list, np.uint32)),
+       _p(np.ascontiguousarray(bbox, np.uint32)), _p(centers), _p(lengths), _p(geos), _p(rgbs), _p(cv(out_T)),
+       _p(np.ascontiguousarray(n_contrib, np.uint32)), _p(np.ascontiguousarray(tile_last, np.uint32)),
+       _p(cv(dL_dcolor)), _p(cv(dL_dT)), ctypes.c_int(int(need_depth)), ctypes.c_int(int(need_normal)),
+       _p(opt(dL_ddepth)), _p(opt(dL_dnormal)), ct(lambda_R), _p(opt(gt)), ct(lambda_asc), ct(lambda_dist),
+       _p(opt(out_depth)), ctypes.c_int(N), _p(acc))
+    return acc[:, :8].copy(), acc[:, 8:11].copy(), acc[:, 11:12].copy()
+
+
 def sh_fwd(deg, idx, centers, cam_pos, sh0, shs):
     """NC/src/sh_compute.cu:180-270 -> rgbs [N,3] (zero outside idx)."""
     centers, sh0, shs = _f32(centers), _f32(sh0), _f32(shs)

@@ -257,6 +257,242 @@ void render_fwd(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R 
     }
 }
 
+// ---- compositor with depth / distortion / normal / max_w and the
+// regularizer gradients (NC/src/forward.cu:259-366, backward.cu:346-510) ----
+
+// NC/src/auxiliary.h:305-308, 368-371
+template <typename R> inline R depth_contracted(R x) { return (x < R(1)) ? x : R(2) - R(1) / x; }
+template <typename R> inline R safe_rnorm(R x, R y, R z) { return R(1) / std::sqrt(x * x + y * y + z * z + R(1e-15f)); }
+
+// Per-sample form of vox_vol: same vol and dI, plus each sample's volume term
+// lv[k] and its dI contribution edI[k][i] = dd_k * w_k[i].
+template <typename R>
+inline R vox_samples(const R* c, R l, const R* geo, R a, R b, const Ray<R>& ray, int n_samp, R* lv, R (*edI)[8],
+                     R* dI)
+{
+    const R l_inv = R(1) / l;
+    const R step_sz = (b - a) * inv_nsamp<R>(n_samp);
+    const R t0 = rfma<R>(R(0.5f), step_sz, a);
+    const R r = R(0.5f) * l;
+    R qt[3], qs[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        const R pt = rfma<R>(t0, ray.rd[i], ray.ro[i]);
+        qt[i] = (pt - (c[i] - r)) * l_inv;
+        qs[i] = (step_sz * ray.rd[i]) * l_inv;
+    }
+    if (dI) for (int i = 0; i < 8; ++i) dI[i] = R(0);
+    R vol = R(0);
+    const R scale = R(STEP_SZ_SCALE) * step_sz;
+    for (int k = 0; k < n_samp; ++k)
+    {
+        R w[8];
+        tri_w<R>(qt, w);
+        R d = geo[0] * w[0];
+        for (int i = 1; i < 8; ++i) d = rfma<R>(geo[i], w[i], d);
+        const R term = scale * exp_linear_11<R>(d);
+        lv[k] = term;
+        vol = vol + term;
+        if (dI)
+        {
+            const R dd = scale * exp_linear_11_bw<R>(d);
+            for (int i = 0; i < 8; ++i)
+            {
+                edI[k][i] = dd * w[i];
+                dI[i] = rfma<R>(dd, w[i], dI[i]);
+            }
+        }
+        for (int i = 0; i < 3; ++i) qt[i] = qt[i] + qs[i];
+    }
+    return vol;
+}
+
+// NC/src/forward.cu:273-300: expected depth of one entry from its samples'
+// alphas (n_samp 2, 3) or its midpoint (n_samp 1); dD[k] = d dval / d lv[k]
+// scaled as in backward.cu:436-458 (without the dL_dD * T factor).
+template <typename R>
+inline R entry_depth(int n_samp, R a, R b, R alpha, const R* lv, R* dD)
+{
+    if (n_samp == 3)
+    {
+        const R st = R(0.3333333f) * (b - a);
+        const R a0 = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-lv[0]));
+        const R a1 = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-lv[1]));
+        const R a2 = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-lv[2]));
+        const R t0 = a + R(0.5f) * st, t1 = a + R(1.5f) * st, t2 = a + R(2.5f) * st;
+        if (dD)
+        {
+            dD[0] = (t0 + a1 * a2 * t2 - a1 * t1 - a2 * t2) * (R(1) - a0);
+            dD[1] = (t1 + a0 * a2 * t2 - a0 * t1 - a2 * t2) * (R(1) - a1);
+            dD[2] = (t2 + a0 * a1 * t2 - a0 * t2 - a1 * t2) * (R(1) - a2);
+        }
+        return a0 * t0 + (R(1) - a0) * a1 * t1 + (R(1) - a0) * (R(1) - a1) * a2 * t2;
+    }
+    if (n_samp == 2)
+    {
+        const R st = R(0.5f) * (b - a);
+        const R a0 = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-lv[0]));
+        const R a1 = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-lv[1]));
+        const R t0 = a + R(0.5f) * st, t1 = a + R(1.5f) * st;
+        if (dD)
+        {
+            dD[0] = (t0 - a1 * t1) * (R(1) - a0);
+            dD[1] = (t1 - a0 * t1) * (R(1) - a1);
+        }
+        return a0 * t0 + (R(1) - a0) * a1 * t1;
+    }
+    const R t0 = R(0.5f) * (a + b);
+    if (dD) dD[0] = t0 * (R(1) - alpha);
+    return alpha * t0;
+}
+
+// NC/src/forward.cu:318-324 (sums in the reference's order)
+template <typename R>
+inline void lin_normal(const R* g, R* n)
+{
+    n[0] = (g[4] + g[5] + g[6] + g[7]) - (g[0] + g[1] + g[2] + g[3]);
+    n[1] = (g[2] + g[3] + g[6] + g[7]) - (g[0] + g[1] + g[4] + g[5]);
+    n[2] = (g[1] + g[3] + g[5] + g[7]) - (g[0] + g[2] + g[4] + g[6]);
+}
+
+template <typename R>
+void render_fwd_ext(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R bg, int n_samp,
+                    const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox, const R* centers,
+                    const R* lengths, const R* geos, const R* rgbs, int need_depth, int need_dist, int need_normal,
+                    int track_max_w, R* out_color, R* out_T, uint32_t* n_contrib, uint32_t* tile_last,
+                    R* out_depth, R* out_normal, R* max_w, R* dist_loss)
+{
+    const int gx = (W + BLOCK_X - 1) / BLOCK_X, gy = (H + BLOCK_Y - 1) / BLOCK_Y;
+    const int T_tiles = gx * gy;
+    const size_t HW = (size_t)H * W;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < T_tiles; ++t)
+    {
+        const int tx = t % gx, ty = t / gx;
+        const uint32_t rx = ranges[2 * t], ry = ranges[2 * t + 1];
+        uint32_t tl = rx;
+        for (int py = ty * BLOCK_Y; py < std::min((ty + 1) * BLOCK_Y, H); ++py)
+            for (int px = tx * BLOCK_X; px < std::min((tx + 1) * BLOCK_X, W); ++px)
+            {
+                const Ray<R> ray = make_ray<R>(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+                const R dx = ((R)px + R(0.5f) - cx) * R(2) * tanfovx / (R)W;
+                const R dy = ((R)py + R(0.5f) - cy) * R(2) * tanfovy / (R)H;
+                const R rd_norm_inv = R(1) / std::sqrt(dx * dx + dy * dy + R(1) * R(1));
+                R T = R(1), C[3] = {R(0), R(0), R(0)}, D = R(0), Dd = R(0), Nn[3] = {R(0), R(0), R(0)};
+                int med_v = -1;
+                R med_T = R(0);
+                std::vector<R> ws, ms, spans;  // for the distortion loss value
+                uint32_t last = 0;
+                for (uint32_t j = rx; j < ry; ++j)
+                {
+                    const uint32_t val = vox_list[j];
+                    const uint32_t v = val & 0x1FFFFFFFu, q = val >> 29;
+                    if (!pix_in_bbox(px, py, bbox[2 * v], bbox[2 * v + 1]) || q != ray.pq) continue;
+                    const R* c = centers + 3 * (size_t)v;
+                    R a, b;
+                    ray_aabb<R>(c, lengths[v], ray, a, b);
+                    if (a > b) continue;
+                    const R* g = geos + 8 * (size_t)v;
+                    R lv[3];
+                    const R vol = vox_samples<R>(c, lengths[v], g, a, b, ray, n_samp, lv, nullptr, nullptr);
+                    const R alpha = std::min(R(MAX_ALPHA), R(1) - rexp<R>(-vol));
+                    if (alpha < R(MIN_ALPHA)) continue;
+                    const R pt_w = T * alpha;
+                    for (int i = 0; i < 3; ++i) C[i] = rfma<R>(pt_w, rgbs[3 * (size_t)v + i], C[i]);
+                    if (need_depth)
+                    {
+                        D = D + T * entry_depth<R>(n_samp, a, b, alpha, lv, nullptr);
+                        if (T > R(0.5f))
+                        {
+                            med_v = (int)v;
+                            med_T = T;
+                        }
+                    }
+                    const R m = R(0.5f) * (depth_contracted<R>(a) + depth_contracted<R>(b));
+                    if (need_dist) Dd = Dd + pt_w * m;
+                    if (dist_loss)
+                    {
+                        ws.push_back(pt_w);
+                        ms.push_back(m);
+                        spans.push_back(depth_contracted<R>(b) - depth_contracted<R>(a));
+                    }
+                    if (need_normal)
+                    {
+                        R n[3];
+                        lin_normal<R>(g, n);
+                        const R rl = safe_rnorm<R>(n[0], n[1], n[2]);
+                        for (int i = 0; i < 3; ++i) Nn[i] = Nn[i] + pt_w * rl * n[i];
+                    }
+                    T = T * (R(1) - alpha);
+                    last = j - rx + 1;
+                    if (track_max_w)
+                    {
+#pragma omp critical(oracle_max_w)
+                        max_w[v] = std::max(max_w[v], pt_w);
+                    }
+                    if (T < R(EARLY_STOP_T)) break;
+                }
+                const size_t pid = (size_t)py * W + px;
+                R Dmed = R(0);
+                if (need_depth && med_v >= 0)
+                {
+                    // NC/src/forward.cu:330-366: 16-sample refinement inside the median voxel
+                    const R* c = centers + 3 * (size_t)med_v;
+                    const R l = lengths[med_v];
+                    const R* g = geos + 8 * (size_t)med_v;
+                    R a, b;
+                    ray_aabb<R>(c, l, ray, a, b);
+                    const R l_inv = R(1) / l;
+                    const R step_sz = (b - a) * (R(1) / R(16));
+                    const R t0 = rfma<R>(R(0.5f), step_sz, a);
+                    const R r = R(0.5f) * l;
+                    R qt[3], qs[3];
+                    for (int i = 0; i < 3; ++i)
+                    {
+                        qt[i] = (rfma<R>(t0, ray.rd[i], ray.ro[i]) - (c[i] - r)) * l_inv;
+                        qs[i] = (step_sz * ray.rd[i]) * l_inv;
+                    }
+                    Dmed = a - R(0.5f) * step_sz;
+                    const R scale = R(STEP_SZ_SCALE) * step_sz;
+                    for (int k = 0; k < 16 && med_T > R(0.5f); ++k)
+                    {
+                        Dmed = Dmed + step_sz;
+                        R w[8];
+                        tri_w<R>(qt, w);
+                        R d = g[0] * w[0];
+                        for (int i = 1; i < 8; ++i) d = rfma<R>(g[i], w[i], d);
+                        med_T = med_T * rexp<R>(-(scale * exp_linear_11<R>(d)));
+                        for (int i = 0; i < 3; ++i) qt[i] = qt[i] + qs[i];
+                    }
+                }
+                n_contrib[pid] = last;
+                for (int i = 0; i < 3; ++i) out_color[i * HW + pid] = C[i] + T * bg;
+                out_T[pid] = T;
+                if (need_depth)
+                {
+                    out_depth[pid] = D * rd_norm_inv;
+                    out_depth[2 * HW + pid] = Dmed * rd_norm_inv;
+                }
+                if (need_dist) out_depth[HW + pid] = Dd;
+                if (need_normal)
+                    for (int i = 0; i < 3; ++i) out_normal[i * HW + pid] = Nn[i];
+                if (dist_loss)
+                {
+                    // sum_ij w_i w_j |m_i - m_j| + 1/3 sum_i w_i^2 (contracted b_i - contracted a_i)
+                    R L = R(0);
+                    for (size_t i = 0; i < ws.size(); ++i)
+                    {
+                        for (size_t k = 0; k < ws.size(); ++k) L += ws[i] * ws[k] * std::fabs(ms[i] - ms[k]);
+                        L += ws[i] * ws[i] * spans[i] / R(3);
+                    }
+                    dist_loss[pid] = L;
+                }
+                tl = std::max(tl, rx + last);
+            }
+        tile_last[t] = tl;
+    }
+}
+
 template <typename R>
 inline void atomic_add(double* p, double v)
 {
@@ -317,6 +553,164 @@ void render_bwd(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R 
                     for (int i = 0; i < 8; ++i) atomic_add<R>(g + i, (double)(dI_scale * dI[i]));
                     for (int i = 0; i < 3; ++i) atomic_add<R>(g + 8 + i, (double)(pt_w * dpix[i]));
                     atomic_add<R>(g + 11, (double)std::fabs(dalpha * alpha));
+                }
+            }
+    }
+}
+
+// NC/src/backward.cu:105-527 with depth (mean), distortion, normal and the
+// R_concen / ascending regularizers.  dL_ddepth: [3, H, W] (channel 0 used),
+// dL_dnormal [3, H, W], gt [3, H, W], out_depth [3, H, W] of the forward
+// (channel 1 = distortion sum) -- each may be null when its feature is off.
+template <typename R>
+void render_bwd_ext(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R bg, int n_samp,
+                    const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox, const R* centers,
+                    const R* lengths, const R* geos, const R* rgbs, const R* out_T, const uint32_t* n_contrib,
+                    const uint32_t* tile_last, const R* dL_dcolor, const R* dL_dT, int need_depth, int need_normal,
+                    const R* dL_ddepth, const R* dL_dnormal, R lambda_R, const R* gt, R lambda_asc, R lambda_dist,
+                    const R* out_depth, int N, double* acc)
+{
+    const int gx = (W + BLOCK_X - 1) / BLOCK_X, gy = (H + BLOCK_Y - 1) / BLOCK_Y;
+    const int T_tiles = gx * gy;
+    const size_t HW = (size_t)H * W;
+    const R WH_inv = R(1) / (R)(W * H);
+    const R w_R = lambda_R * WH_inv, w_asc = lambda_asc * WH_inv, w_dist = lambda_dist * WH_inv;
+    const bool need_dist = lambda_dist > R(0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < T_tiles; ++t)
+    {
+        const int tx = t % gx, ty = t / gx;
+        const uint32_t rx = ranges[2 * t], rend = tile_last[t];
+        for (int py = ty * BLOCK_Y; py < std::min((ty + 1) * BLOCK_Y, H); ++py)
+            for (int px = tx * BLOCK_X; px < std::min((tx + 1) * BLOCK_X, W); ++px)
+            {
+                const Ray<R> ray = make_ray<R>(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+                const R ddx = ((R)px + R(0.5f) - cx) * R(2) * tanfovx / (R)W;
+                const R ddy = ((R)py + R(0.5f) - cy) * R(2) * tanfovy / (R)H;
+                const R rd_norm_inv = R(1) / std::sqrt(ddx * ddx + ddy * ddy + R(1) * R(1));
+                const size_t pid = (size_t)py * W + px;
+                const R T_final = out_T[pid];
+                R T = T_final;
+                const uint32_t nc = n_contrib[pid];
+                const R dpix[3] = {dL_dcolor[pid], dL_dcolor[HW + pid], dL_dcolor[2 * HW + pid]};
+                R last = dL_dT[pid] + bg * (dpix[0] + dpix[1] + dpix[2]);
+                const R dL_dD = need_depth ? dL_ddepth[pid] * rd_norm_inv : R(0);
+                R dN[3] = {R(0), R(0), R(0)};
+                if (need_normal)
+                    for (int i = 0; i < 3; ++i) dN[i] = dL_dnormal[i * HW + pid];
+                R gtp[3] = {R(0), R(0), R(0)};
+                if (lambda_R > R(0))
+                    for (int i = 0; i < 3; ++i) gtp[i] = gt[i * HW + pid];
+                R prefix_wm = R(0), suffix_wm = R(0), prefix_w = R(0), suffix_w = R(0);
+                if (need_dist)
+                {
+                    prefix_wm = out_depth[HW + pid];
+                    prefix_w = R(1) - T_final;
+                }
+                for (uint32_t jj = rend; jj > rx; --jj)
+                {
+                    const uint32_t j = jj - 1;
+                    if (j - rx >= nc) continue;
+                    const uint32_t val = vox_list[j];
+                    const uint32_t v = val & 0x1FFFFFFFu, q = val >> 29;
+                    if (!pix_in_bbox(px, py, bbox[2 * v], bbox[2 * v + 1]) || q != ray.pq) continue;
+                    const R* c = centers + 3 * (size_t)v;
+                    const R l = lengths[v];
+                    R a, b;
+                    ray_aabb<R>(c, l, ray, a, b);
+                    if (a > b) continue;
+                    const R* g = geos + 8 * (size_t)v;
+                    R dI[8], lv[3], edI[3][8];
+                    const R vol = vox_samples<R>(c, l, g, a, b, ray, n_samp, lv, edI, dI);
+                    const R e = rexp<R>(-vol);
+                    const R alpha = std::min(R(MAX_ALPHA), R(1) - e);
+                    if (alpha < R(MIN_ALPHA)) continue;
+                    T = T / (R(1) - alpha);
+                    const R pt_w = alpha * T;
+                    const R* col = rgbs + 3 * (size_t)v;
+                    R dpt = rfma<R>(dpix[2], col[2], rfma<R>(dpix[1], col[1], dpix[0] * col[0]));
+                    R dgeo[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+                    if (need_dist)
+                    {
+                        const R ad = depth_contracted<R>(a), bd = depth_contracted<R>(b);
+                        const R now_m = R(0.5f) * (ad + bd);
+                        const R now_wm = now_m * pt_w;
+                        prefix_wm = prefix_wm - now_wm;
+                        prefix_w = prefix_w - pt_w;
+                        const R uni = R(0.6666666f) * pt_w * (bd - ad);
+                        const R bi = R(2) * (now_m * (prefix_w - suffix_w) - (prefix_wm - suffix_wm));
+                        dpt = dpt + w_dist * (uni + bi);
+                        suffix_wm = suffix_wm + now_wm;
+                        suffix_w = suffix_w + pt_w;
+                    }
+                    if (need_normal)
+                    {
+                        R n[3];
+                        lin_normal<R>(g, n);
+                        const R rl = safe_rnorm<R>(n[0], n[1], n[2]);
+                        const R sn[3] = {rl * n[0], rl * n[1], rl * n[2]};
+                        dpt = dpt + (dN[0] * sn[0] + dN[1] * sn[1] + dN[2] * sn[2]);
+                        const R ds[3] = {pt_w * dN[0], pt_w * dN[1], pt_w * dN[2]};
+                        const R dsn = ds[0] * n[0] + ds[1] * n[1] + ds[2] * n[2];
+                        R dl[3];
+                        for (int i = 0; i < 3; ++i) dl[i] = rl * (ds[i] - dsn * rl * sn[i]);
+                        for (int k = 0; k < 8; ++k)
+                            dgeo[k] = ((k & 4) ? dl[0] : -dl[0]) + ((k & 2) ? dl[1] : -dl[1]) +
+                                      ((k & 1) ? dl[2] : -dl[2]);
+                    }
+                    const R diff = dpt - last;
+                    const R dalpha = T * diff;
+                    last = rfma<R>(alpha, diff, last);
+                    const R dIs = dalpha * e;
+                    for (int k = 0; k < 8; ++k) dgeo[k] = dgeo[k] + dIs * dI[k];
+                    if (need_depth)
+                    {
+                        R dD[3];
+                        const R dval = entry_depth<R>(n_samp, a, b, alpha, lv, dD);
+                        last = last + dL_dD * dval;
+                        const R f = dL_dD * T;
+                        if (n_samp == 1)
+                        {
+                            for (int k = 0; k < 8; ++k) dgeo[k] = dgeo[k] + (f * dD[0]) * dI[k];
+                        }
+                        else
+                        {
+                            for (int k = 0; k < 8; ++k)
+                            {
+                                R s2 = (f * dD[0]) * edI[0][k];
+                                for (int m = 1; m < n_samp; ++m) s2 = s2 + (f * dD[m]) * edI[m][k];
+                                dgeo[k] = dgeo[k] + s2;
+                            }
+                        }
+                    }
+                    if (lambda_asc > R(0))
+                    {
+                        const R r = R(0.5f) * l, l_inv = R(1) / l;
+                        R qa[3], qb[3];
+                        for (int i = 0; i < 3; ++i)
+                        {
+                            qa[i] = (rfma<R>(a, ray.rd[i], ray.ro[i]) - (c[i] - r)) * l_inv;
+                            qb[i] = (rfma<R>(b, ray.rd[i], ray.ro[i]) - (c[i] - r)) * l_inv;
+                        }
+                        R wa[8], wb[8];
+                        tri_w<R>(qa, wa);
+                        tri_w<R>(qb, wb);
+                        R da = g[0] * wa[0], db = g[0] * wb[0];
+                        for (int k = 1; k < 8; ++k)
+                        {
+                            da = rfma<R>(g[k], wa[k], da);
+                            db = rfma<R>(g[k], wb[k], db);
+                        }
+                        const R rw = w_asc * pt_w * (R)(da > db ? 1 : 0);
+                        for (int k = 0; k < 8; ++k) dgeo[k] = dgeo[k] + rw * (wa[k] - wb[k]);
+                    }
+                    R drgb[3] = {pt_w * dpix[0], pt_w * dpix[1], pt_w * dpix[2]};
+                    if (lambda_R > R(0))
+                        for (int i = 0; i < 3; ++i) drgb[i] = drgb[i] + w_R * pt_w * R(2) * (col[i] - gtp[i]);
+                    double* gacc = acc + 12 * (size_t)v;
+                    for (int k = 0; k < 8; ++k) atomic_add<R>(gacc + k, (double)dgeo[k]);
+                    for (int i = 0; i < 3; ++i) atomic_add<R>(gacc + 8 + i, (double)drgb[i]);
+                    atomic_add<R>(gacc + 11, (double)std::fabs(dalpha * alpha));
                 }
             }
     }
@@ -734,5 +1128,35 @@ void oracle_pixel_rays(int W, int H, float cx, float cy, float tanfovx, float ta
         }
     }
 }
+
+#define ORACLE_EXT_FWD(NAME, R)                                                                              \
+    void NAME(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R bg, int n_samp,                 \
+              const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox, const R* centers,        \
+              const R* lengths, const R* geos, const R* rgbs, int need_depth, int need_dist, int need_normal,  \
+              int track_max_w, R* out_color, R* out_T, uint32_t* n_contrib, uint32_t* tile_last, R* out_depth, \
+              R* out_normal, R* max_w, R* dist_loss)                                                           \
+    {                                                                                                          \
+        render_fwd_ext<R>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, n_samp, ranges, vox_list, bbox, centers,   \
+                          lengths, geos, rgbs, need_depth, need_dist, need_normal, track_max_w, out_color,     \
+                          out_T, n_contrib, tile_last, out_depth, out_normal, max_w, dist_loss);               \
+    }
+ORACLE_EXT_FWD(oracle_render_fwd_ext_f32, float)
+ORACLE_EXT_FWD(oracle_render_fwd_ext_f64, double)
+
+#define ORACLE_EXT_BWD(NAME, R)                                                                              \
+    void NAME(int W, int H, R tanfovx, R tanfovy, R cx, R cy, const R* c2w, R bg, int n_samp,                 \
+              const uint32_t* ranges, const uint32_t* vox_list, const uint32_t* bbox, const R* centers,        \
+              const R* lengths, const R* geos, const R* rgbs, const R* out_T, const uint32_t* n_contrib,       \
+              const uint32_t* tile_last, const R* dL_dcolor, const R* dL_dT, int need_depth, int need_normal,  \
+              const R* dL_ddepth, const R* dL_dnormal, R lambda_R, const R* gt, R lambda_asc, R lambda_dist,   \
+              const R* out_depth, int N, double* acc)                                                          \
+    {                                                                                                          \
+        render_bwd_ext<R>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, n_samp, ranges, vox_list, bbox, centers,   \
+                          lengths, geos, rgbs, out_T, n_contrib, tile_last, dL_dcolor, dL_dT, need_depth,      \
+                          need_normal, dL_ddepth, dL_dnormal, lambda_R, gt, lambda_asc, lambda_dist,           \
+                          out_depth, N, acc);                                                                  \
+    }
+ORACLE_EXT_BWD(oracle_render_bwd_ext_f32, float)
+ORACLE_EXT_BWD(oracle_render_bwd_ext_f64, double)
 
 }  // extern "C"
