@@ -109,6 +109,42 @@ int svr_rasterize_backward(int32_t n_samp, int32_t width, int32_t height, float 
                            const float* out_T, const float* dL_dout_color, const float* dL_dout_T, void* workspace,
                            float* dL_dgeos, float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
 
+/* Replaces `_C.rasterize_voxels` (forward.cu:792-879) with its optional
+ * outputs (forward.cu:262-394): need_depth writes the mean depth (channel 0)
+ * and the median depth (channel 2) of out_depth [3,H,W], need_distortion the
+ * distortion sum sum_i w_i (contracted(a_i) + contracted(b_i)) / 2 (channel 1);
+ * channels not asked for are written as zero (out_depth may be null when
+ * neither is set).  need_normal writes out_normal [3,H,W]; track_max_w zeroes
+ * max_w [N] and stores each voxel's largest blending weight.  Colour, T and
+ * the states are those of svr_rasterize_forward. */
+int svr_rasterize_forward_ext(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy, float cx,
+                              float cy, const float* c2w, float bg_color, int32_t need_depth, int32_t need_distortion,
+                              int32_t need_normal, int32_t track_max_w, int64_t n_voxels, const float* vox_centers,
+                              const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,
+                              const void* order_table, int64_t num_rendered, void* binning_state, void* image_state,
+                              float* out_color, float* out_T, float* out_depth, float* out_normal, float* max_w,
+                              svr_stream_t stream);
+
+/* Replaces `_C.rasterize_voxels_backward` (backward.cu:654-772) with the
+ * optional terms (backward.cu:160-510): need_depth back-propagates
+ * dL_dout_depth channel 0 (mean depth), need_normal dL_dout_normal [3,H,W];
+ * lambda_dist > 0 adds the distortion gradient (out_depth: the forward's
+ * [3,H,W] output with need_distortion), lambda_R_concen > 0 the colour
+ * concentration toward gt_color [3,H,W], lambda_ascending > 0 the
+ * density-ascending regularizer; each regularizer is weighted by
+ * lambda / (width * height).  Same workspace, alignment and outputs as
+ * svr_rasterize_backward. */
+int svr_rasterize_backward_ext(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy, float cx,
+                               float cy, const float* c2w, float bg_color, int64_t n_voxels, const float* vox_centers,
+                               const float* vox_lengths, const float* geos, const float* rgbs,
+                               const void* geom_state, int64_t num_rendered, const void* binning_state,
+                               const void* image_state, const float* out_T, const float* dL_dout_color,
+                               const float* dL_dout_depth, const float* dL_dout_normal, const float* dL_dout_T,
+                               float lambda_R_concen, const float* gt_color, float lambda_ascending,
+                               float lambda_dist, int32_t need_depth, int32_t need_normal, const float* out_depth,
+                               void* workspace, float* dL_dgeos, float* dL_drgbs, float* subdiv_p_bw,
+                               svr_stream_t stream);
+
 /* Replaces `_C.unpack_ImageState` (raster_state.cu:150-205): copies ranges
  * [T,2], tile_last [T] and n_contrib [H,W] (int32, device) out of the image state. */
 int svr_unpack_image_state(int32_t width, int32_t height, const void* image_state, int32_t* ranges,

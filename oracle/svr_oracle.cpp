@@ -307,9 +307,9 @@ inline R vox_samples(const R* c, R l, const R* geo, R a, R b, const Ray<R>& ray,
     return vol;
 }
 
-// NC/src/forward.cu:273-300: expected depth of one entry from its samples'
+// NC/src/forward.cu:262-299: expected depth of one entry from its samples'
 // alphas (n_samp 2, 3) or its midpoint (n_samp 1); dD[k] = d dval / d lv[k]
-// scaled as in backward.cu:436-458 (without the dL_dD * T factor).
+// scaled as in backward.cu:421-454 (without the dL_dD * T factor).
 template <typename R>
 inline R entry_depth(int n_samp, R a, R b, R alpha, const R* lv, R* dD)
 {
@@ -346,7 +346,7 @@ inline R entry_depth(int n_samp, R a, R b, R alpha, const R* lv, R* dD)
     return alpha * t0;
 }
 
-// NC/src/forward.cu:318-324 (sums in the reference's order)
+// NC/src/forward.cu:301-314 (sums in the reference's order)
 template <typename R>
 inline void lin_normal(const R* g, R* n)
 {

@@ -127,13 +127,6 @@ def rasterize_preprocess(image_width, image_height, tan_fovx, tan_fovy, cx, cy, 
     return out_n_dup, geom
 
 
-def _unsupported(need_depth, need_distortion, need_normal, track_max_w):
-    flags = [n for n, f in (("need_depth", need_depth), ("need_distortion", need_distortion),
-                            ("need_normal", need_normal), ("track_max_w", track_max_w)) if f]
-    if flags:
-        raise NotImplementedError(f"sv_raster_amd does not implement {', '.join(flags)} yet")
-
-
 # binding.cpp:26 -> forward.cu:792-879
 def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix, c2w_matrix,
                      bg_color, need_depth, need_distortion, need_normal, track_max_w, octree_paths, vox_centers,
@@ -144,7 +137,6 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
         raise RuntimeError("rgbs should be either (num_points, 3)")
     if vox_centers.size(0) != rgbs.size(0):
         raise RuntimeError("size mismatch")
-    _unsupported(need_depth, need_distortion, need_normal, track_max_w)
     _require_cuda(octree_paths, vox_centers, vox_lengths, geos, rgbs, c2w_matrix)
     dev = vox_centers.device
     N = vox_centers.size(0)
@@ -152,6 +144,10 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
     out_color = torch.empty((3, H, W), dtype=torch.float32, device=dev)
     out_T = torch.empty((1, H, W), dtype=torch.float32, device=dev)
     empty = torch.empty(0, dtype=torch.float32, device=dev)
+    ext = bool(need_depth or need_distortion or need_normal or track_max_w)
+    out_depth = torch.empty((3, H, W), dtype=torch.float32, device=dev) if need_depth or need_distortion else empty
+    out_normal = torch.empty((3, H, W), dtype=torch.float32, device=dev) if need_normal else empty
+    max_w = torch.empty((N, 1), dtype=torch.float32, device=dev) if track_max_w else empty
     img = torch.empty(int(_L.svr_image_state_size(W, H)), dtype=torch.uint8, device=dev)
     R = ctypes.c_int64(0)
     if N > 0:
@@ -161,13 +157,22 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
     table = order_table_cache.get(octree_paths) if R > 0 else None
     centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
     c2w = _f32(c2w_matrix)
-    _lib.check(_L.svr_rasterize_forward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
-                                        float(cy), _ptr(c2w), float(bg_color), N, _ptr(centers), _ptr(lengths),
-                                        _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), _ptr(table), R,
-                                        _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), _stream(dev)),
-               "rasterize_voxels")
+    if ext:
+        # forward.cu:262-394: depth / distortion / normal / max_w outputs
+        _lib.check(_L.svr_rasterize_forward_ext(
+            int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx), float(cy), _ptr(c2w),
+            float(bg_color), int(bool(need_depth)), int(bool(need_distortion)), int(bool(need_normal)),
+            int(bool(track_max_w)), N, _ptr(centers), _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer),
+            _ptr(table), R, _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), _ptr(out_depth),
+            _ptr(out_normal), _ptr(max_w), _stream(dev)), "rasterize_voxels")
+    else:
+        _lib.check(_L.svr_rasterize_forward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
+                                            float(cy), _ptr(c2w), float(bg_color), N, _ptr(centers), _ptr(lengths),
+                                            _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), _ptr(table), R,
+                                            _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), _stream(dev)),
+                   "rasterize_voxels")
     _debug(debug, dev, "rasterize_voxels")
-    return R, binning, img, out_color, empty, empty, out_T, empty
+    return R, binning, img, out_color, out_depth, out_normal, out_T, max_w
 
 
 # binding.cpp:27 -> backward.cu:654-772
@@ -178,10 +183,6 @@ def rasterize_voxels_backward(R, n_samp_per_vox, image_width, image_height, tan_
                               need_normal, out_D, out_N, debug=False):
     if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
         raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
-    if lambda_R_concen > 0 or lambda_ascending > 0 or lambda_dist > 0:
-        raise NotImplementedError("sv_raster_amd does not implement the R_concen / ascending / distortion "
-                                  "regularizer gradients yet")
-    _unsupported(need_depth, False, need_normal, False)
     dev = vox_centers.device
     N = vox_centers.size(0)
     if N == 0:
@@ -195,6 +196,25 @@ def rasterize_voxels_backward(R, n_samp_per_vox, image_width, image_height, tan_
     dcol = _f32(dL_dout_color)
     dT = _f32(dL_dout_T) if dL_dout_T is not None else torch.zeros((1, H, W), device=dev)
     work = torch.empty(int(_L.svr_backward_workspace_size(int(R))), dtype=torch.uint8, device=dev)
+    if need_depth or need_normal or lambda_R_concen > 0 or lambda_ascending > 0 or lambda_dist > 0:
+        # backward.cu:160-510: depth / normal gradients and the regularizers
+        def req(t, shape, what):
+            if t is None or tuple(t.shape) != shape:
+                raise RuntimeError(f"{what} must have shape {shape}")
+            return _f32(t)
+        dD = req(dL_dout_depth, (3, H, W), "dL_dout_depth") if need_depth else None
+        dN = req(dL_dout_normal, (3, H, W), "dL_dout_normal") if need_normal else None
+        gt = req(gt_color, (3, H, W), "gt_color") if lambda_R_concen > 0 else None
+        oD = req(out_D, (3, H, W), "out_depth (forward run with lambda_dist > 0)") if lambda_dist > 0 else None
+        _lib.check(_L.svr_rasterize_backward_ext(
+            int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx), float(cy),
+            _ptr(_f32(c2w_matrix)), float(bg_color), N, _ptr(centers), _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c),
+            _ptr(geomBuffer), int(R), _ptr(binningBuffer), _ptr(imageBuffer), _ptr(_f32(out_T)), _ptr(dcol),
+            _ptr(dD), _ptr(dN), _ptr(dT), float(lambda_R_concen), _ptr(gt), float(lambda_ascending),
+            float(lambda_dist), int(bool(need_depth)), int(bool(need_normal)), _ptr(oD), _ptr(work), _ptr(dgeos),
+            _ptr(drgbs), _ptr(dsub), _stream(dev)), "rasterize_voxels_backward")
+        _debug(debug, dev, "rasterize_voxels_backward")
+        return dgeos, drgbs, dsub
     _lib.check(_L.svr_rasterize_backward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
                                          float(cy), _ptr(_f32(c2w_matrix)), float(bg_color), N, _ptr(centers),
                                          _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), int(R),

@@ -49,6 +49,12 @@ SIGNATURES = {
     "svr_rasterize_backward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
                                        c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                        c_p, c_p]),
+    "svr_rasterize_forward_ext": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i32,
+                                          c_i32, c_i32, c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_p,
+                                          c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_rasterize_backward_ext": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
+                                           c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                           c_f32, c_p, c_f32, c_f32, c_i32, c_i32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_unpack_image_state": (c_i32, [c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
     "svr_unpack_geom_state": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_export_sort_keys": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p]),

@@ -70,8 +70,11 @@ class ViewBatch:
         for u, s in enumerate(settings):
             if int(s.n_samp_per_vox) != self.n_samp:
                 raise Exception("all views of a batch need the same n_samp_per_vox")
-            _C._unsupported(bool(getattr(s, "need_depth", False)), False, bool(getattr(s, "need_normal", False)),
-                            bool(getattr(s, "track_max_w", False)))
+            opts = [n for n in ("need_depth", "need_normal", "track_max_w", "lambda_R_concen", "lambda_ascending",
+                                "lambda_dist") if getattr(s, n, 0)]
+            if opts:
+                raise NotImplementedError(f"the multi-view batch renders colour and T only (got {', '.join(opts)}); "
+                                          "use renderer.rasterize_voxels per view for these outputs")
             W, H = int(s.image_width), int(s.image_height)
             v = table[u]
             v.width, v.height = W, H

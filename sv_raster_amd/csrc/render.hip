@@ -257,13 +257,12 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
 
 using namespace svr;
 
-extern "C" {
+namespace svr {
 
-int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,
-                          const float* c2w, float bg, int64_t N, const float* centers, const float* lengths,
-                          const float* geos, const float* rgbs, const void* geom_state, const void* order_table,
-                          int64_t R, void* binning_state, void* image_state, float* out_color, float* out_T,
-                          svr_stream_t stream)
+int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* centers,
+                  const float* lengths, const float* geos, const float* rgbs, const void* geom_state,
+                  const void* order_table, int64_t R, void* binning_state, void* image_state, float* out_color,
+                  float* out_T, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream)
 {
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
@@ -276,11 +275,57 @@ int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, f
     SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_rasterize_forward: null binning state");
     SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
     const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
-    GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    g = GeomState::carve((void*)geom_state, N, nullptr);
     OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
-    BinningState b = BinningState::carve(binning_state, R, nullptr);
-    ImageState img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr);
-    int rc = binning_forward(W, H, N, g, ot, R, b, img, stream);
+    b = BinningState::carve(binning_state, R, nullptr);
+    img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr);
+    return binning_forward(W, H, N, g, ot, R, b, img, stream);
+}
+
+// Returns 1 (with the outputs zeroed) when there is nothing to composite.
+int backward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* geos,
+                   const void* geom_state, int64_t R, const void* binning_state, const void* image_state,
+                   const float* out_T, const float* dL_dcolor, const float* dL_dT, void* workspace, float* dgeos,
+                   float* drgbs, float* dsub, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream)
+{
+    SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
+                MAX_N_SAMP);
+    if (N == 0) return 1;
+    SVR_REQUIRE(geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub && c2w,
+                SVR_ERR_INVALID, "svr_rasterize_backward: null pointer");
+    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0 && ((uintptr_t)dgeos & 15u) == 0, SVR_ERR_INVALID,
+                "geos and dL_dgeos must be 16-byte aligned");
+    if (R == 0)
+    {
+        if (fill_u32(dgeos, 0u, 8 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(drgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(dsub, 0u, N, stream) != SVR_OK) return SVR_ERR_HIP;
+        return 1;
+    }
+    SVR_REQUIRE(binning_state && workspace, SVR_ERR_INVALID, "svr_rasterize_backward: null binning state or workspace");
+    SVR_REQUIRE(((uintptr_t)workspace & 15u) == 0, SVR_ERR_INVALID, "backward workspace must be 16-byte aligned");
+    const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
+    g = GeomState::carve((void*)geom_state, N, nullptr);
+    b = BinningState::carve((void*)binning_state, R, nullptr);
+    img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr);
+    return SVR_OK;
+}
+
+}  // namespace svr
+
+extern "C" {
+
+int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,
+                          const float* c2w, float bg, int64_t N, const float* centers, const float* lengths,
+                          const float* geos, const float* rgbs, const void* geom_state, const void* order_table,
+                          int64_t R, void* binning_state, void* image_state, float* out_color, float* out_T,
+                          svr_stream_t stream)
+{
+    GeomState g;
+    BinningState b;
+    ImageState img;
+    const int rc = forward_setup(n_samp, W, H, c2w, N, centers, lengths, geos, rgbs, geom_state, order_table, R,
+                                 binning_state, image_state, out_color, out_T, g, b, img, stream);
     if (rc != SVR_OK) return rc;
     switch (n_samp)
     {
@@ -305,26 +350,12 @@ int svr_rasterize_backward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, 
                            const float* dL_dcolor, const float* dL_dT, void* workspace, float* dgeos, float* drgbs,
                            float* dsub, svr_stream_t stream)
 {
-    SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
-                MAX_N_SAMP);
-    if (N == 0) return SVR_OK;
-    SVR_REQUIRE(geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub && c2w,
-                SVR_ERR_INVALID, "svr_rasterize_backward: null pointer");
-    SVR_REQUIRE(((uintptr_t)geos & 15u) == 0 && ((uintptr_t)dgeos & 15u) == 0, SVR_ERR_INVALID,
-                "geos and dL_dgeos must be 16-byte aligned");
-    if (R == 0)
-    {
-        if (fill_u32(dgeos, 0u, 8 * N, stream) != SVR_OK) return SVR_ERR_HIP;
-        if (fill_u32(drgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
-        if (fill_u32(dsub, 0u, N, stream) != SVR_OK) return SVR_ERR_HIP;
-        return SVR_OK;
-    }
-    SVR_REQUIRE(binning_state && workspace, SVR_ERR_INVALID, "svr_rasterize_backward: null binning state or workspace");
-    SVR_REQUIRE(((uintptr_t)workspace & 15u) == 0, SVR_ERR_INVALID, "backward workspace must be 16-byte aligned");
-    const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
-    GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
-    BinningState b = BinningState::carve((void*)binning_state, R, nullptr);
-    ImageState img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr);
+    GeomState g;
+    BinningState b;
+    ImageState img;
+    const int rc = backward_setup(n_samp, W, H, c2w, N, geos, geom_state, R, binning_state, image_state, out_T,
+                                  dL_dcolor, dL_dT, workspace, dgeos, drgbs, dsub, g, b, img, stream);
+    if (rc != SVR_OK) return rc == 1 ? SVR_OK : rc;
     float* partial = (float*)workspace;
     switch (n_samp)
     {

@@ -89,6 +89,7 @@ __device__ inline float inv_nsamp(int n) { return n == 1 ? 1.f : (n == 2 ? 0.5f 
 struct PixRay
 {
     float ro[3], rd[3], rd_inv[3];
+    float rd_norm_inv;  // 1 / |camera-space direction| (depth is scaled by it)
     uint32_t pq;
 };
 
@@ -106,6 +107,7 @@ __device__ inline PixRay make_ray(int px, int py, int W, int H, float tanfovx, f
     const float ry = c2w[4] * dx + c2w[5] * dy + c2w[6] * dz;
     const float rz = c2w[8] * dx + c2w[9] * dy + c2w[10] * dz;
     const float ninv = 1.f / nrm;
+    r.rd_norm_inv = ninv;
     r.ro[0] = c2w[3]; r.ro[1] = c2w[7]; r.ro[2] = c2w[11];
     r.rd[0] = rx * ninv; r.rd[1] = ry * ninv; r.rd[2] = rz * ninv;
     r.rd_inv[0] = 1.f / r.rd[0]; r.rd_inv[1] = 1.f / r.rd[1]; r.rd_inv[2] = 1.f / r.rd[2];
@@ -200,6 +202,61 @@ __device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_i
             const float dd = scale * (lin ? 1.0f : 0.909090909091f * ex);
 #pragma unroll
             for (int i = 0; i < 8; ++i) dI[i] = __builtin_fmaf(dd, w[i], dI[i]);
+        }
+        qt0 = qt0 + qs0; qt1 = qt1 + qs1; qt2 = qt2 + qs2;
+    }
+    return vol;
+}
+
+// Per-sample form of vox_vol (same vol and dI): lv[k] is sample k's volume
+// term and, with DI, edI[k][i] = dd_k * w_k[i] its share of dI[i]
+// (forward.cu:235-253 / backward.cu:300-320 keep them for the depth output).
+template <int NS, bool DI>
+__device__ inline float vox_samples(float c0, float c1, float c2, float l, float l_inv, const float* geo, float a,
+                                    float b, const PixRay& ray, float* lv, float (*edI)[8], float* dI)
+{
+    const float step_sz = (b - a) * inv_nsamp(NS);
+    const float t0 = __builtin_fmaf(0.5f, step_sz, a);
+    const float r = 0.5f * l;
+    float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - (c0 - r)) * l_inv;
+    float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - (c1 - r)) * l_inv;
+    float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - (c2 - r)) * l_inv;
+    const float qs0 = (step_sz * ray.rd[0]) * l_inv;
+    const float qs1 = (step_sz * ray.rd[1]) * l_inv;
+    const float qs2 = (step_sz * ray.rd[2]) * l_inv;
+    const float scale = STEP_SZ_SCALE * step_sz;
+    float vol = 0.f;
+    if (DI)
+    {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dI[i] = 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+    {
+        const float wx[2] = {1.f - qt0, qt0};
+        const float wy[2] = {1.f - qt1, qt1};
+        const float wz[2] = {1.f - qt2, qt2};
+        float w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = (wx[(i >> 2) & 1] * wy[(i >> 1) & 1]) * wz[i & 1];
+        float d = geo[0] * w[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i) d = __builtin_fmaf(geo[i], w[i], d);
+        const float ex = svr_expf(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
+        const bool lin = d > 1.1f;
+        const float term = scale * (lin ? d : ex);
+        lv[k] = term;
+        vol = vol + term;
+        if (DI)
+        {
+            const float dd = scale * (lin ? 1.0f : 0.909090909091f * ex);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+            {
+                edI[k][i] = dd * w[i];
+                dI[i] = __builtin_fmaf(dd, w[i], dI[i]);
+            }
         }
         qt0 = qt0 + qs0; qt1 = qt1 + qs1; qt2 = qt2 + qs2;
     }

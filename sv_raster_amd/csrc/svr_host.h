@@ -195,6 +195,18 @@ int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hip
 int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
                        float* dsub, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
 
+// Argument checks, state carving and (forward) binning shared by the plain and
+// the extended compositor entry points.  backward_setup returns 1 after
+// zeroing the gradients when there is nothing to composite (N == 0 or R == 0).
+int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* centers,
+                  const float* lengths, const float* geos, const float* rgbs, const void* geom_state,
+                  const void* order_table, int64_t R, void* binning_state, void* image_state, float* out_color,
+                  float* out_T, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream);
+int backward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* geos,
+                   const void* geom_state, int64_t R, const void* binning_state, const void* image_state,
+                   const float* out_T, const float* dL_dcolor, const float* dL_dT, void* workspace, float* dgeos,
+                   float* drgbs, float* dsub, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream);
+
 // Live stage timing: when enabled, prof_begin/prof_end record a HIP event pair
 // on the launch stream around the kernels of one stage.
 enum ProfStage { PROF_PREPROCESS = 0, PROF_SCAN = 1, PROF_BINNING = 2, PROF_RENDER_FWD = 3, PROF_RENDER_BWD = 4,

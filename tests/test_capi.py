@@ -103,7 +103,30 @@ def test_sparse_adam_constructor_checks():
     assert opt.sparse and not opt.biased
 
 
-def test_unsupported_options_fail_loudly():
+def test_ext_backward_checks_shapes_before_launch():
+    """The optional-term backward rejects missing or mis-shaped inputs on the host, before any launch."""
     from sv_raster_amd import _C
+    z, e = torch.zeros, torch.empty(0)
+    args = dict(R=10, n_samp_per_vox=1, image_width=16, image_height=16, tan_fovx=0.5, tan_fovy=0.5, cx=8.0,
+                cy=8.0, w2c_matrix=torch.eye(4), c2w_matrix=torch.eye(4), bg_color=0.0,
+                octree_paths=z((4, 1), dtype=torch.int64), vox_centers=z((4, 3)), vox_lengths=torch.ones((4, 1)),
+                geos=z((4, 8)), rgbs=z((4, 3)), geomBuffer=e, binningBuffer=e, imageBuffer=e,
+                out_T=torch.ones((1, 16, 16)), dL_dout_color=z((3, 16, 16)), dL_dout_depth=e, dL_dout_normal=e,
+                dL_dout_T=z((1, 16, 16)), lambda_R_concen=0.0, gt_color=e, lambda_ascending=0.0, lambda_dist=0.0,
+                need_depth=False, need_normal=False, out_D=e, out_N=e)
+    cases = [(dict(need_depth=True, dL_dout_depth=z((3, 8, 8))), "dL_dout_depth"),
+             (dict(need_normal=True), "dL_dout_normal"),
+             (dict(lambda_R_concen=0.1), "gt_color"),
+             (dict(lambda_dist=0.1), "out_depth")]
+    for over, what in cases:
+        with pytest.raises(RuntimeError, match=what):
+            _C.rasterize_voxels_backward(**dict(args, **over))
+
+
+def test_batch_rejects_optional_outputs():
+    from sv_raster_amd import batch, renderer
+    s = renderer.RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=16, image_height=16, tanfovx=0.5,
+                                tanfovy=0.5, cx=8.0, cy=8.0, w2c_matrix=torch.eye(4), c2w_matrix=torch.eye(4),
+                                need_depth=True)
     with pytest.raises(NotImplementedError, match="need_depth"):
-        _C._unsupported(True, False, False, False)
+        batch.ViewBatch([s])
