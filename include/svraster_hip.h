@@ -348,6 +348,38 @@ int svr_batch_rasterize_backward_capacity(int32_t n_samp, int32_t n_views, const
 int svr_batch_unpack_image_state(int64_t total_tiles, int64_t total_pixels, const void* image_state,
                                  int32_t* ranges, int32_t* tile_last, int32_t* n_contrib, svr_stream_t stream);
 
+/* Per-voxel statistics over many cameras, replacing the per-camera loops of
+ * SRC/sparse_voxel_gears/adaptive.py:190-221 (compute_training_stat) and
+ * NC/new_svraster_cuda/renderer.py:408-438 (mark_max_samp_rate, mark_near).
+ *
+ * svr_batch_rasterize_max_w: the batch forward as a weights-only pass (no
+ * colour; rgbs of svr_batch_rasterize_forward not needed), writing out_T and
+ * pair_max_w [V, N], each (view, voxel) pair's largest blending weight (the
+ * per-camera max_w of rasterize_voxels with track_max_w; zeroed here).
+ *
+ * svr_batch_training_stat: per voxel over the views in order, from
+ * pair_max_w: max_w = max_u w_u; min_samp_interval = min over the views with
+ * w_u > 0 of ((c - position_u) . lookat_u) * pix_size[u] (1e30 if none);
+ * view_cnt = number of such views (as float).  pix_size [V] (device):
+ * 2 * tanfovx / width per view, rounded to float once.
+ *
+ * svr_batch_mark_samp: per voxel over the views with pair_n_dup > 0 (the
+ * batch preprocess, whose near plane is each view's near_plane):
+ * max_samp_rate = max of length / (zdist * pix_size[u]) over views with
+ * zdist > near_plane (0 if none); is_near = 1 when some such view has
+ * zdist <= near_plane + length.  Either output may be null. */
+int svr_batch_rasterize_max_w(int32_t n_samp, int32_t n_views, const svr_view* views_host, const svr_view* views_dev,
+                              int64_t n_voxels, const float* vox_centers, const float* vox_lengths,
+                              const float* geos, const void* geom_state, const void* order_table,
+                              int64_t num_rendered, void* binning_state, void* image_state, float* out_T,
+                              float* pair_max_w, svr_stream_t stream);
+int svr_batch_training_stat(int32_t n_views, const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,
+                            const float* pix_size, const float* pair_max_w, float* max_w, float* min_samp_interval,
+                            float* view_cnt, svr_stream_t stream);
+int svr_batch_mark_samp(int32_t n_views, const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,
+                        const float* vox_lengths, const float* pix_size, const int32_t* pair_n_dup,
+                        float* max_samp_rate, uint8_t* is_near, svr_stream_t stream);
+
 /* Live stage timing (no reference counterpart; measurement support).  While
  * enabled, each entry point records a HIP event pair on its stream around the
  * kernels of its stage: 0 preprocess, 1 scan, 2 binning (duplicate + sort +

@@ -56,6 +56,10 @@ SIGNATURES = {
                                            c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                            c_f32, c_p, c_f32, c_f32, c_i32, c_i32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_unpack_image_state": (c_i32, [c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_rasterize_max_w": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p,
+                                          c_p, c_p, c_p]),
+    "svr_batch_training_stat": (c_i32, [c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_batch_mark_samp": (c_i32, [c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_unpack_geom_state": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_export_sort_keys": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p]),
     "svr_sh_compute": (c_i32, [c_i32, c_i64, c_p, c_i64, c_p, c_p, c_p, c_p, c_i32, c_p, c_p]),

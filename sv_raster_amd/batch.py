@@ -372,3 +372,99 @@ def autotune(settings, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, c
     _TUNED[key] = (best, dict(times))
     return ViewBatch(settings, tile_shape=best), times
 
+
+
+# ---------------------------------------------------------------------------
+# Per-voxel statistics over many cameras (the loops of SRC/sparse_voxel_gears/
+# adaptive.py:190-221 and NC renderer.py:408-438, as batch passes)
+
+def _pix_size(views):
+    """2 * tanfovx / width per view (cameras.py:54-55), in float64 on the host, then float32 on the device."""
+    ps = np.array([2.0 * float(s.tanfovx) / int(s.image_width) for s in views.settings], np.float64)
+    return torch.from_numpy(ps.astype(np.float32)).to(views.device)
+
+
+def _pair_preprocess(views, vox_centers, vox_lengths):
+    dev = vox_centers.device
+    V, N = views.n_views, vox_centers.size(0)
+    centers, lengths = _C._f32(vox_centers), _C._f32(vox_lengths)
+    n_dup = torch.empty(V * N, dtype=torch.int32, device=dev)
+    geom = torch.empty(int(_L.svr_batch_geom_state_size(V, N)), dtype=torch.uint8, device=dev)
+    _lib.check(_L.svr_batch_preprocess(V, views.host_ptr(), _C._ptr(views.dev), N, _C._ptr(centers),
+                                       _C._ptr(lengths), _C._ptr(n_dup), _C._ptr(geom), _C._stream(dev)),
+               "batch_preprocess")
+    return centers, lengths, n_dup, geom
+
+
+@torch.no_grad()
+def compute_training_stat(views, octree_paths, vox_centers, vox_lengths, geos):
+    """`SparseVoxelModel.compute_training_stat` (adaptive.py:190-221) over the cameras of a ViewBatch in one
+    pass: per voxel, max_w (largest blending weight over the views), min_samp_interval (minimum over the
+    views where the voxel has a positive weight of its camera-axis distance times the pixel size; 1e30 if
+    none) and view_cnt (number of such views).  Each [N, 1] float32; the same values as rendering every camera
+    with track_max_w and folding the results in camera order."""
+    dev = vox_centers.device
+    V, N = views.n_views, vox_centers.size(0)
+    max_w = torch.zeros((N, 1), dtype=torch.float32, device=dev)
+    msi = torch.full((N, 1), 1e30, dtype=torch.float32, device=dev)
+    cnt = torch.zeros((N, 1), dtype=torch.float32, device=dev)
+    if N == 0:
+        return {"max_w": max_w, "min_samp_interval": msi, "view_cnt": cnt}
+    st = _C._stream(dev)
+    centers, lengths, _, geom = _pair_preprocess(views, vox_centers, vox_lengths)
+    R = ctypes.c_int64(0)
+    _lib.check(_L.svr_num_rendered(V * N, _C._ptr(geom), st, ctypes.byref(R)), "num_rendered")
+    R = int(R.value)
+    binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
+    img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels)), dtype=torch.uint8,
+                      device=dev)
+    T = torch.empty(views.total_pixels, dtype=torch.float32, device=dev)
+    pair_w = torch.empty(V * N, dtype=torch.float32, device=dev)
+    table = _C.order_table_cache.get(octree_paths)
+    geos_c = _C._f32_a16(geos)
+    _lib.check(_L.svr_batch_rasterize_max_w(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
+                                            _C._ptr(centers), _C._ptr(lengths), _C._ptr(geos_c), _C._ptr(geom),
+                                            _C._ptr(table), R, _C._ptr(binning), _C._ptr(img), _C._ptr(T),
+                                            _C._ptr(pair_w), st), "batch_rasterize_max_w")
+    pix = _pix_size(views)
+    _lib.check(_L.svr_batch_training_stat(V, _C._ptr(views.dev), N, _C._ptr(centers), _C._ptr(pix), _C._ptr(pair_w),
+                                          _C._ptr(max_w), _C._ptr(msi), _C._ptr(cnt), st), "batch_training_stat")
+    return {"max_w": max_w, "min_samp_interval": msi, "view_cnt": cnt}
+
+
+def _camera_views(cameras, near):
+    from .renderer import RasterSettings
+    settings = [RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=cam.image_width,
+                               image_height=cam.image_height, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy, cx=cam.cx,
+                               cy=cam.cy, w2c_matrix=cam.w2c, c2w_matrix=cam.c2w, near=near) for cam in cameras]
+    return ViewBatch(settings)
+
+
+def _mark(cameras, octree_paths, vox_centers, vox_lengths, near, rate):
+    dev = vox_centers.device
+    N = vox_centers.size(0)
+    out = torch.zeros(N, dtype=torch.float32 if rate else torch.bool, device=dev)
+    cameras = list(cameras)
+    if N == 0 or not cameras:
+        return out
+    views = _camera_views(cameras, near)
+    centers, lengths, n_dup, _ = _pair_preprocess(views, vox_centers, vox_lengths)
+    pix = _pix_size(views)
+    _lib.check(_L.svr_batch_mark_samp(views.n_views, _C._ptr(views.dev), N, _C._ptr(centers), _C._ptr(lengths),
+                                      _C._ptr(pix), _C._ptr(n_dup), _C._ptr(out) if rate else None,
+                                      None if rate else _C._ptr(out), _C._stream(dev)), "batch_mark_samp")
+    return out
+
+
+@torch.no_grad()
+def mark_max_samp_rate(cameras, octree_paths, vox_centers, vox_lengths, near=0.02):
+    """`renderer.mark_max_samp_rate` (NC renderer.py:408-423) for all cameras in one pass: per voxel the
+    largest length / (camera-axis distance x pixel size) over the cameras that see it beyond `near` ([N])."""
+    return _mark(cameras, octree_paths, vox_centers, vox_lengths, near, True)
+
+
+@torch.no_grad()
+def mark_near(cameras, octree_paths, vox_centers, vox_lengths, near=0.2):
+    """`renderer.mark_near` (NC renderer.py:426-438) for all cameras in one pass: True for voxels some camera
+    sees within `near` + the voxel's length along its axis ([N] bool)."""
+    return _mark(cameras, octree_paths, vox_centers, vox_lengths, near, False)

@@ -265,15 +265,18 @@ __device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, 
 
 // One tile of the batch forward.  The fast form (EXACT = false) defers a tile
 // holding a ray whose 1/rd is not finite to exact_list (count ctr[1]).
-template <int NS, bool EXACT, typename TS>
-__device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t* s_last, uint32_t g, int V,
+// MAX_W: weights-only pass (no colour read or written) that raises
+// max_w[view * N + voxel] to each hit's blending weight.
+template <int NS, bool EXACT, typename TS, bool MAX_W>
+__device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t (*sv)[64], uint32_t* s_last, uint32_t g, int V,
                                       const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
                                       uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges,
                                       const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
                                       const float* __restrict__ centers, const float* __restrict__ lengths,
                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
                                       uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
-                                      float* __restrict__ out_color, float* __restrict__ out_T)
+                                      float* __restrict__ out_color, float* __restrict__ out_T,
+                                      float* __restrict__ max_w)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
@@ -288,9 +291,9 @@ __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t* s_last, uint
     const uint2 range = ranges[g];
     const size_t vo = (size_t)t.u * (size_t)N;
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    const uint32_t last = composite_subtile<NS, EXACT>(st, w, lane, ray, t.inside, range, t.sx0, t.sy0, t.sx1, t.sy1,
-                                                       vox_list, bboxes + vo, centers, lengths, geos, rgbs + 3 * vo,
-                                                       T, C0, C1, C2);
+    const uint32_t last = composite_subtile<NS, EXACT, TS::NW, MAX_W>(
+        st, w, lane, ray, t.inside, range, t.sx0, t.sy0, t.sx1, t.sy1, vox_list, bboxes + vo, centers, lengths, geos,
+        MAX_W ? rgbs : rgbs + 3 * vo, T, C0, C1, C2, sv, MAX_W ? reinterpret_cast<uint32_t*>(max_w + vo) : nullptr);
     if (t.inside)
     {
         const size_t HW = (size_t)t.H * t.W;
@@ -298,9 +301,12 @@ __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t* s_last, uint
         const size_t pb = (size_t)vw.pix_base;
         const float bg = vw.bg_color;
         n_contrib[pb + pid] = last;
-        out_color[3 * pb + pid] = C0 + T * bg;
-        out_color[3 * pb + HW + pid] = C1 + T * bg;
-        out_color[3 * pb + 2 * HW + pid] = C2 + T * bg;
+        if constexpr (!MAX_W)
+        {
+            out_color[3 * pb + pid] = C0 + T * bg;
+            out_color[3 * pb + HW + pid] = C1 + T * bg;
+            out_color[3 * pb + 2 * HW + pid] = C2 + T * bg;
+        }
         out_T[pb + pid] = T;
     }
     const uint32_t wm = wave_max_u32(t.inside ? last : 0u);
@@ -317,31 +323,34 @@ __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t* s_last, uint
 
 // Fast form: workgroup b renders tile order[b] (the hardware dispatches
 // workgroups in index order, so the longest lists start first).
-template <int NS, typename TS>
+template <int NS, typename TS, bool MAX_W>
 __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order, uint32_t* __restrict__ ctr,
     uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
-    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
+    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
+    float* __restrict__ max_w)
 {
     __shared__ StageT<TS::NW> st;
+    __shared__ uint32_t sv[MAX_W ? TS::NW : 1][64];
     __shared__ uint32_t s_last[TS::NW];
     const uint32_t g = order[blockIdx.x];
-    batch_fwd_tile<NS, false, TS>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers, lengths,
-                              geos, rgbs, tile_last, n_contrib, out_color, out_T);
+    batch_fwd_tile<NS, false, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes,
+                                         centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T, max_w);
 }
 
 // Exact form: a few workgroups loop over the deferred tiles (head ctr[2]).
-template <int NS, typename TS>
+template <int NS, typename TS, bool MAX_W>
 __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr, uint32_t* __restrict__ exact_list,
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
-    float* __restrict__ out_color, float* __restrict__ out_T)
+    float* __restrict__ out_color, float* __restrict__ out_T, float* __restrict__ max_w)
 {
     __shared__ StageT<TS::NW> st;
+    __shared__ uint32_t sv[MAX_W ? TS::NW : 1][64];
     __shared__ uint32_t s_last[TS::NW];
     __shared__ uint32_t s_i;
     const uint32_t count = ctr[1];
@@ -352,8 +361,9 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
         const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_fwd_tile<NS, true, TS>(st, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, centers,
-                                 lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T);
+        batch_fwd_tile<NS, true, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes,
+                                            centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T,
+                                            max_w);
         __syncthreads();
     }
 }
@@ -616,20 +626,34 @@ int check_views(int V, const svr_view* views, int64_t* T_out, int64_t* P_out, in
     default: { using TS = Tile16; BODY; } break;                     \
     }
 
-template <int NS, typename TS>
+template <int NS, typename TS, bool MAX_W>
 int batch_fwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
                      const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
-                     const float* geos, const float* rgbs, float* out_color, float* out_T, hipStream_t stream)
+                     const float* geos, const float* rgbs, float* out_color, float* out_T, float* max_w,
+                     hipStream_t stream)
 {
-    batch_fwd_kernel<NS, TS><<<(unsigned)T, TS::THREADS, 0, stream>>>(
+    batch_fwd_kernel<NS, TS, MAX_W><<<(unsigned)T, TS::THREADS, 0, stream>>>(
         V, views, N, img.order, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
-        img.tile_last, img.n_contrib, out_color, out_T);
+        img.tile_last, img.n_contrib, out_color, out_T, max_w);
     SVR_CHECK_LAUNCH();
-    batch_fwd_exact_kernel<NS, TS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
+    batch_fwd_exact_kernel<NS, TS, MAX_W><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
         V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
-        img.tile_last, img.n_contrib, out_color, out_T);
+        img.tile_last, img.n_contrib, out_color, out_T, max_w);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
+}
+
+template <int NS, typename TS>
+int batch_fwd_dispatch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
+                       const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
+                       const float* geos, const float* rgbs, float* out_color, float* out_T, float* max_w,
+                       hipStream_t stream)
+{
+    if (max_w)
+        return batch_fwd_launch<NS, TS, true>(V, views, N, T, b, img, g, centers, lengths, geos, rgbs, out_color,
+                                              out_T, max_w, stream);
+    return batch_fwd_launch<NS, TS, false>(V, views, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
+                                           nullptr, stream);
 }
 
 template <int NS, typename TS>
@@ -742,11 +766,12 @@ namespace svr {
 // R: the exact entry count (eager form) or the capacity the buffers were
 // sized for (capacity form, overflow != nullptr: the count is read on the
 // device and a frame over the capacity renders no entries).
+// max_w != nullptr: the weights-only pass (rgbs and out_color unused).
 static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
                               int64_t N, const float* centers, const float* lengths, const float* geos,
                               const float* rgbs, const void* geom_state, const void* order_table, int64_t R,
                               void* binning_state, void* image_state, float* out_color, float* out_T,
-                              uint32_t* overflow, hipStream_t stream)
+                              uint32_t* overflow, hipStream_t stream, float* max_w = nullptr)
 {
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
@@ -755,10 +780,12 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     int rc = check_views(V, views_host, &T, &P, &shape);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(R >= 0 && R < (1ll << 32), SVR_ERR_INVALID, "num_rendered out of range");
-    SVR_REQUIRE(views_dev && image_state && out_color && out_T, SVR_ERR_INVALID,
+    SVR_REQUIRE(views_dev && image_state && (out_color || max_w) && out_T, SVR_ERR_INVALID,
                 "svr_batch_rasterize_forward: null pointer");
-    SVR_REQUIRE(N == 0 || (geom_state && order_table && centers && lengths && geos && rgbs), SVR_ERR_INVALID,
-                "svr_batch_rasterize_forward: null pointer");
+    SVR_REQUIRE(N == 0 || (geom_state && order_table && centers && lengths && geos && (rgbs || max_w)),
+                SVR_ERR_INVALID, "svr_batch_rasterize_forward: null pointer");
+    if (max_w && N > 0)
+        SVR_REQUIRE(fill_u32(max_w, 0u, (int64_t)V * N, stream) == SVR_OK, SVR_ERR_HIP, "fill max_w failed");
     SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_batch_rasterize_forward: null binning state");
     SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
     BatchImageState img = BatchImageState::carve(image_state, T, P, nullptr);
@@ -819,15 +846,15 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     prof_end(PROF_BINNING, stream);
 
     prof_begin(PROF_RENDER_FWD, stream);
-    SVR_WITH_TILE(shape, (rc = n_samp == 1   ? batch_fwd_launch<1, TS>(V, views_dev, N, T, b, img, g, centers,
-                                                                          lengths, geos, rgbs, out_color, out_T,
-                                                                          stream)
-                               : n_samp == 2 ? batch_fwd_launch<2, TS>(V, views_dev, N, T, b, img, g, centers,
-                                                                          lengths, geos, rgbs, out_color, out_T,
-                                                                          stream)
-                                             : batch_fwd_launch<3, TS>(V, views_dev, N, T, b, img, g, centers,
-                                                                          lengths, geos, rgbs, out_color, out_T,
-                                                                          stream)));
+    SVR_WITH_TILE(shape, (rc = n_samp == 1   ? batch_fwd_dispatch<1, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                            lengths, geos, rgbs, out_color, out_T,
+                                                                            max_w, stream)
+                               : n_samp == 2 ? batch_fwd_dispatch<2, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                            lengths, geos, rgbs, out_color, out_T,
+                                                                            max_w, stream)
+                                             : batch_fwd_dispatch<3, TS>(V, views_dev, N, T, b, img, g, centers,
+                                                                            lengths, geos, rgbs, out_color, out_T,
+                                                                            max_w, stream)));
     prof_end(PROF_RENDER_FWD, stream);
     return rc;
 }
@@ -956,6 +983,122 @@ int svr_batch_unpack_image_state(int64_t T, int64_t P, const void* image_state, 
     SVR_CHECK_HIP(hipMemcpyAsync(ranges, img.ranges, sizeof(uint32_t) * 2 * T, hipMemcpyDeviceToDevice, stream));
     SVR_CHECK_HIP(hipMemcpyAsync(tile_last, img.tile_last, sizeof(uint32_t) * T, hipMemcpyDeviceToDevice, stream));
     SVR_CHECK_HIP(hipMemcpyAsync(n_contrib, img.n_contrib, sizeof(uint32_t) * P, hipMemcpyDeviceToDevice, stream));
+    return SVR_OK;
+}
+
+}  // extern "C"
+
+// ---- per-voxel statistics over a batch of cameras ---------------------------
+//
+// SRC/sparse_voxel_gears/adaptive.py:190-221 (compute_training_stat) and
+// NC/new_svraster_cuda/renderer.py:408-438 (mark_max_samp_rate, mark_near)
+// loop over cameras on the host, one render or preprocess per camera.  Here
+// one batch pass gives every (view, voxel) pair's max blending weight or
+// tile count, and one thread per voxel folds the views in view order.
+
+namespace svr {
+
+// Camera-axis distance of a voxel centre: ((c - position) * lookat).sum(-1)
+// with position = c2w[:3, 3] and lookat = c2w[:3, 2] (cameras.py:30-36).
+__device__ inline float axis_dist(const svr_view& vw, float c0, float c1, float c2)
+{
+    return (c0 - vw.c2w[3]) * vw.c2w[2] + (c1 - vw.c2w[7]) * vw.c2w[6] + (c2 - vw.c2w[11]) * vw.c2w[10];
+}
+
+__global__ __launch_bounds__(256) void training_stat_kernel(int V, const svr_view* __restrict__ views, int64_t N,
+                                                            const float* __restrict__ centers,
+                                                            const float* __restrict__ pix_size,
+                                                            const float* __restrict__ pair_max_w,
+                                                            float* __restrict__ max_w,
+                                                            float* __restrict__ min_samp_interval,
+                                                            float* __restrict__ view_cnt)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const float c0 = centers[3 * v], c1 = centers[3 * v + 1], c2 = centers[3 * v + 2];
+    float mw = 0.f, msi = 1e30f, cnt = 0.f;
+    for (int u = 0; u < V; ++u)
+    {
+        const float w = pair_max_w[(size_t)u * N + v];
+        mw = fmaxf(mw, w);
+        if (w > 0.f)
+        {
+            const float si = axis_dist(views[u], c0, c1, c2) * pix_size[u];
+            msi = fminf(msi, si);
+            cnt = cnt + 1.f;
+        }
+    }
+    max_w[v] = mw;
+    min_samp_interval[v] = msi;
+    view_cnt[v] = cnt;
+}
+
+__global__ __launch_bounds__(256) void mark_samp_kernel(int V, const svr_view* __restrict__ views, int64_t N,
+                                                        const float* __restrict__ centers,
+                                                        const float* __restrict__ lengths,
+                                                        const float* __restrict__ pix_size,
+                                                        const int32_t* __restrict__ pair_n_dup,
+                                                        float* __restrict__ max_samp_rate, uint8_t* __restrict__ is_near)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const float c0 = centers[3 * v], c1 = centers[3 * v + 1], c2 = centers[3 * v + 2], len = lengths[v];
+    float rate = 0.f;
+    bool near_hit = false;
+    for (int u = 0; u < V; ++u)
+    {
+        if (pair_n_dup[(size_t)u * N + v] <= 0) continue;
+        const svr_view& vw = views[u];
+        const float z = axis_dist(vw, c0, c1, c2);
+        if (z > vw.near_plane) rate = fmaxf(rate, len / (z * pix_size[u]));
+        near_hit = near_hit || (z <= vw.near_plane + len);
+    }
+    if (max_samp_rate) max_samp_rate[v] = rate;
+    if (is_near) is_near[v] = near_hit ? 1 : 0;
+}
+
+}  // namespace svr
+
+extern "C" {
+
+int svr_batch_rasterize_max_w(int32_t n_samp, int32_t V, const svr_view* views_host, const svr_view* views_dev,
+                              int64_t N, const float* centers, const float* lengths, const float* geos,
+                              const void* geom_state, const void* order_table, int64_t R, void* binning_state,
+                              void* image_state, float* out_T, float* pair_max_w, svr_stream_t stream)
+{
+    SVR_REQUIRE(pair_max_w != nullptr || N == 0, SVR_ERR_INVALID, "svr_batch_rasterize_max_w: null max_w");
+    if (N == 0) return SVR_OK;
+    return batch_forward_impl(n_samp, V, views_host, views_dev, N, centers, lengths, geos, nullptr, geom_state,
+                              order_table, R, binning_state, image_state, nullptr, out_T, nullptr, stream,
+                              pair_max_w);
+}
+
+int svr_batch_training_stat(int32_t V, const svr_view* views_dev, int64_t N, const float* centers,
+                            const float* pix_size, const float* pair_max_w, float* max_w, float* min_samp_interval,
+                            float* view_cnt, svr_stream_t stream)
+{
+    SVR_REQUIRE(V >= 1, SVR_ERR_INVALID, "number of views must be positive");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(views_dev && centers && pix_size && pair_max_w && max_w && min_samp_interval && view_cnt,
+                SVR_ERR_INVALID, "svr_batch_training_stat: null pointer");
+    training_stat_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(V, views_dev, N, centers, pix_size,
+                                                                          pair_max_w, max_w, min_samp_interval,
+                                                                          view_cnt);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+int svr_batch_mark_samp(int32_t V, const svr_view* views_dev, int64_t N, const float* centers, const float* lengths,
+                        const float* pix_size, const int32_t* pair_n_dup, float* max_samp_rate, uint8_t* is_near,
+                        svr_stream_t stream)
+{
+    SVR_REQUIRE(V >= 1, SVR_ERR_INVALID, "number of views must be positive");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(views_dev && centers && lengths && pix_size && pair_n_dup && (max_samp_rate || is_near),
+                SVR_ERR_INVALID, "svr_batch_mark_samp: null pointer");
+    mark_samp_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(V, views_dev, N, centers, lengths, pix_size,
+                                                                      pair_n_dup, max_samp_rate, is_near);
+    SVR_CHECK_LAUNCH();
     return SVR_OK;
 }
 

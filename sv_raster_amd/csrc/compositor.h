@@ -105,6 +105,8 @@ struct EntryRegs
     float r0, r1, r2;
 };
 
+// RGB = false: the colour is not read (weights-only passes) and staged as 0.
+template <bool RGB = true>
 __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes,
                                   const float* __restrict__ centers, const float* __restrict__ lengths,
                                   const float* __restrict__ geos, const float* __restrict__ rgbs, int sx0, int sy0,
@@ -124,9 +126,16 @@ __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const 
     e.linv = 1.f / e.l;
     e.g0 = *reinterpret_cast<const float4*>(geos + 8 * v);
     e.g1 = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
-    e.r0 = rgbs[3 * v + 0];
-    e.r1 = rgbs[3 * v + 1];
-    e.r2 = rgbs[3 * v + 2];
+    if constexpr (RGB)
+    {
+        e.r0 = rgbs[3 * v + 0];
+        e.r1 = rgbs[3 * v + 1];
+        e.r2 = rgbs[3 * v + 2];
+    }
+    else
+    {
+        e.r0 = e.r1 = e.r2 = 0.f;
+    }
 }
 
 template <int NW>
@@ -144,13 +153,18 @@ __device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryR
 // (forward.cu:150-328 for colour and T).  Returns the pixel's n_contrib and
 // leaves its transmittance and colour in T, C0..C2.  EXACT selects the
 // ray-box min/max form (see ray_aabb); it is uniform over the wave.
-template <int NS, bool EXACT, int NW>
+// MAX_W: weights-only pass (forward.cu:322-325 with track_max_w): colours are
+// not read, the voxel id of each staged entry goes to sv, and every hit raises
+// max_w_bits[voxel] to its blending weight (a float stored as its bits:
+// weights are positive, so their order is that of the bits).
+template <int NS, bool EXACT, int NW, bool MAX_W = false>
 __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
                                              int sx0, int sy0, int sx1, int sy1,
                                              const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
                                              const float* __restrict__ centers, const float* __restrict__ lengths,
                                              const float* __restrict__ geos, const float* __restrict__ rgbs,
-                                             float& T, float& C0, float& C1, float& C2)
+                                             float& T, float& C0, float& C1, float& C2,
+                                             uint32_t (*sv)[64] = nullptr, uint32_t* __restrict__ max_w_bits = nullptr)
 {
     uint32_t last = 0;
     bool done = !inside;
@@ -166,13 +180,17 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
     {
         const uint32_t j = base + (uint32_t)lane;
         const bool valid = j < range.y;
-        load_entry(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
-                   qpix);
+        load_entry<!MAX_W>(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
+                           sy1, qpix);
     }
     while (base < range.y)
     {
         wave_sync();
         stage_entry(st, w, lane, cur);
+        if constexpr (MAX_W)
+        {
+            if (cur.pm) sv[w][lane] = cur.v;
+        }
         wave_sync();
         uint64_t hm = wave_transpose64(cur.pm, lane);
         if (done) hm = 0ull;
@@ -185,8 +203,8 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
             const uint32_t j = nbase + (uint32_t)lane;
             const bool valid = j < range.y;
-            load_entry(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                       sy1, qpix);
+            load_entry<!MAX_W>(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0,
+                               sx1, sy1, qpix);
         }
 
         while (hm)
@@ -204,9 +222,16 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
             if (alpha < MIN_ALPHA) continue;
             const float pt_w = T * alpha;
-            C0 = __builtin_fmaf(pt_w, st.rgb[0][w][l], C0);
-            C1 = __builtin_fmaf(pt_w, st.rgb[1][w][l], C1);
-            C2 = __builtin_fmaf(pt_w, st.rgb[2][w][l], C2);
+            if constexpr (MAX_W)
+            {
+                atomicMax(max_w_bits + sv[w][l], __float_as_uint(pt_w));
+            }
+            else
+            {
+                C0 = __builtin_fmaf(pt_w, st.rgb[0][w][l], C0);
+                C1 = __builtin_fmaf(pt_w, st.rgb[1][w][l], C1);
+                C2 = __builtin_fmaf(pt_w, st.rgb[2][w][l], C2);
+            }
             T = T * (1.f - alpha);
             last = base + (uint32_t)l - range.x + 1u;
             if (T < EARLY_STOP_T)
