@@ -287,8 +287,9 @@ __device__ inline bool bwd_hit(const StageT<NW>& st, int w, int l, const PixRay&
 //
 // Entries whose bbox covers at least SYNC_MIN pixels of the wave (coarse
 // voxels, the outside shell) are "sync slots": every pixel processes them at
-// the same step, at their place in its own order, and their gradient is summed
-// across the wave with DPP instead of up to 64-way conflicting LDS atomics.
+// the same step, at their place in its own order, and their 12 gradient terms
+// are summed across the wave by one butterfly (wave_sum12) instead of up to
+// 64-way conflicting LDS atomics.
 // Between two sync slots each lane walks its own remaining hits in lock-step
 // and accumulates with ds_add.
 //
@@ -378,12 +379,9 @@ __device__ inline void adjoint_subtile(StageT<TS::NW>& st, float (&acc)[12][TS::
 #pragma unroll
                 for (int i = 0; i < 12; ++i) gq[i] = 0.f;
             }
-#pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                const float r = wave_sum_f32(gq[i]);
-                if (lane == 0) acc[i][w][s] += r;
-            }
+            const float r = wave_sum12(gq, lane);
+            const int ri = (lane >> 2) & 15;
+            if ((lane & 3) == 0 && ri < 12) acc[ri][w][s] += r;
         }
 
         // prefetch the next chunk before this chunk's atomics are issued
