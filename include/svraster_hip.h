@@ -57,8 +57,10 @@ size_t svr_order_table_size(int64_t n_voxels);
 size_t svr_order_table_scratch_size(int64_t n_voxels);
 /* Binning state of `rasterize_voxels` (raster_state.h:70-82) for R entries. */
 size_t svr_binning_state_size(int64_t num_rendered);
-/* Image state (ranges, tile_last, n_contrib) of an W x H image (raster_state.h:61-68). */
-size_t svr_image_state_size(int32_t width, int32_t height);
+/* Image state of a W x H frame with R entries: ranges, tile_last, n_contrib
+ * (raster_state.h:61-68) and the forward's per-pixel hit masks that the
+ * backward walks (64 bits per pixel and 64-entry chunk of its tile). */
+size_t svr_image_state_size(int32_t width, int32_t height, int64_t num_rendered);
 
 /* ---- forward path ------------------------------------------------------- */
 /* Replaces `_C.rasterize_preprocess` (preprocess.cu:136-199): per voxel the

@@ -148,11 +148,11 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
     out_depth = torch.empty((3, H, W), dtype=torch.float32, device=dev) if need_depth or need_distortion else empty
     out_normal = torch.empty((3, H, W), dtype=torch.float32, device=dev) if need_normal else empty
     max_w = torch.empty((N, 1), dtype=torch.float32, device=dev) if track_max_w else empty
-    img = torch.empty(int(_L.svr_image_state_size(W, H)), dtype=torch.uint8, device=dev)
     R = ctypes.c_int64(0)
     if N > 0:
         _lib.check(_L.svr_num_rendered(N, _ptr(geomBuffer), _stream(dev), ctypes.byref(R)), "num_rendered")
     R = int(R.value)
+    img = torch.empty(int(_L.svr_image_state_size(W, H, R)), dtype=torch.uint8, device=dev)
     binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
     table = order_table_cache.get(octree_paths) if R > 0 else None
     centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)

@@ -15,3 +15,4 @@ from . import renderer  # noqa: F401
 from . import sparse_adam  # noqa: F401
 from . import grid_loss_bw  # noqa: F401
 from . import batch  # noqa: F401
+from . import checkpoint  # noqa: F401

@@ -38,7 +38,7 @@ SIGNATURES = {
     "svr_order_table_size": (c_sz, [c_i64]),
     "svr_order_table_scratch_size": (c_sz, [c_i64]),
     "svr_binning_state_size": (c_sz, [c_i64]),
-    "svr_image_state_size": (c_sz, [c_i32, c_i32]),
+    "svr_image_state_size": (c_sz, [c_i32, c_i32, c_i64]),
     "svr_rasterize_preprocess": (c_i32, [c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_p, c_f32, c_i64,
                                          c_p, c_p, c_p, c_p, c_p]),
     "svr_num_rendered": (c_i32, [c_i64, c_p, c_p, ctypes.POINTER(c_i64)]),

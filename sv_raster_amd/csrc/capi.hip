@@ -122,11 +122,11 @@ size_t svr_binning_state_size(int64_t R)
     return bytes;
 }
 
-size_t svr_image_state_size(int32_t W, int32_t H)
+size_t svr_image_state_size(int32_t W, int32_t H, int64_t R)
 {
     size_t bytes = 0;
     const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
-    ImageState::carve(nullptr, (int64_t)W * H, T, &bytes);
+    ImageState::carve(nullptr, (int64_t)W * H, T, &bytes, R > 0 ? R : 0);
     return bytes;
 }
 

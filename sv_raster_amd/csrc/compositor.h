@@ -10,8 +10,15 @@
 
 namespace svr {
 
+// Sync-slot thresholds (pixels of a wave that use an entry): bbox coverage
+// in the traversal that tests candidates, real hits in the one driven by the
+// forward's hit masks (measured on the 1080p scene: 8 is best there, the
+// remaining per-lane LDS atomics conflict less than wave-wide sums cost).
 #ifndef BWD_SYNC_MIN
 #define BWD_SYNC_MIN 24
+#endif
+#ifndef BWD_SYNC_MIN_MASKED
+#define BWD_SYNC_MIN_MASKED 8
 #endif
 
 // Binning tile of TX x TY pixels; each wave of its workgroup owns one 8x8
@@ -157,14 +164,17 @@ __device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryR
 // not read, the voxel id of each staged entry goes to sv, and every hit raises
 // max_w_bits[voxel] to its blending weight (a float stored as its bits:
 // weights are positive, so their order is that of the bits).
-template <int NS, bool EXACT, int NW, bool MAX_W = false>
+// MASK: after each chunk c the lane stores its hit mask (bit l: entry l of the
+// chunk contributed) at mask_out[c * 64 * NW].
+template <int NS, bool EXACT, int NW, bool MAX_W = false, bool MASK = false>
 __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
                                              int sx0, int sy0, int sx1, int sy1,
                                              const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
                                              const float* __restrict__ centers, const float* __restrict__ lengths,
                                              const float* __restrict__ geos, const float* __restrict__ rgbs,
                                              float& T, float& C0, float& C1, float& C2,
-                                             uint32_t (*sv)[64] = nullptr, uint32_t* __restrict__ max_w_bits = nullptr)
+                                             uint32_t (*sv)[64] = nullptr, uint32_t* __restrict__ max_w_bits = nullptr,
+                                             uint64_t* __restrict__ mask_out = nullptr)
 {
     uint32_t last = 0;
     bool done = !inside;
@@ -207,6 +217,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
                                sx1, sy1, qpix);
         }
 
+        uint64_t hits = 0ull;
         while (hm)
         {
             const int l = __builtin_ctzll(hm);
@@ -221,6 +232,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             const float vol = vox_vol<NS, false>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, nullptr);
             const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
             if (alpha < MIN_ALPHA) continue;
+            if constexpr (MASK) hits |= 1ull << l;
             const float pt_w = T * alpha;
             if constexpr (MAX_W)
             {
@@ -240,6 +252,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
                 break;
             }
         }
+        if constexpr (MASK) mask_out[(size_t)((base - range.x) >> 6) * (64 * NW)] = hits;
         base = nbase;
         if (__ballot(!done) == 0ull) break;
     }
@@ -432,6 +445,160 @@ __device__ inline void adjoint_subtile(StageT<TS::NW>& st, float (&acc)[12][TS::
             }
         }
         end = nend;
+    }
+}
+
+// The reverse traversal driven by the forward's hit masks (see
+// composite_subtile's MASK): chunk c of the tile is read as the 64-bit mask of
+// entries that contributed to this lane's pixel, so a lane walks only its real
+// hits (no bbox, quadrant, ray-box or alpha rejections, no n_contrib cut) and
+// an entry's payload is loaded only when some pixel of the wave used it.  The
+// chunks are the forward's (aligned at range.x), walked from the wave's last
+// contributor down; within a chunk each lane takes its hits from the highest
+// bit down.  Sync slots, LDS accumulation and the per-entry row flush are as in
+// adjoint_subtile, with the slot test on real-hit counts.
+template <int NS, int SYNC_MIN, bool EXACT, typename TS = Tile16>
+__device__ inline void adjoint_masked(StageT<TS::NW>& st, float (&acc)[12][TS::NW][64], int w, int lane, int W, int H,
+                                      int tx, int ty, bool inside, size_t pid, size_t HW, const PixRay& ray,
+                                      uint2 range, float bg, const uint64_t* __restrict__ masks,
+                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                      const float* __restrict__ centers, const float* __restrict__ lengths,
+                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                      const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+                                      const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+                                      const uint32_t* __restrict__ orig, float* __restrict__ partial)
+{
+    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
+    uint32_t nc = 0;
+    if (inside)
+    {
+        T = out_T[pid];
+        nc = n_contrib[pid];
+        dp0 = dL_dcolor[pid];
+        dp1 = dL_dcolor[HW + pid];
+        dp2 = dL_dcolor[2 * HW + pid];
+        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k][w][lane] = 0.f;
+    const uint32_t rx = range.x;
+    int c = (int)((wave_max_u32(nc) + 63u) >> 6) - 1;
+    constexpr size_t STRIDE = 64 * TS::NW;
+
+    // chunk c: this lane's hit mask, the entry-major transpose (bit p of lane
+    // l: pixel p used entry l) and entry l's payload if any pixel used it
+    EntryRegs cur;
+    uint64_t m = 0ull, er = 0ull;
+    uint32_t cur_o = 0;
+    auto fetch = [&](int cc) {
+        m = masks[(size_t)cc * STRIDE];
+        er = wave_transpose64(m, lane);
+        cur.pm = er;
+        if (er)
+        {
+            const uint32_t j = rx + ((uint32_t)cc << 6) + (uint32_t)lane;
+            const uint32_t val = vox_list[j];
+            const size_t v = val & VOX_ID_MASK;
+            cur.v = (uint32_t)v;
+            cur.bb = bboxes[v];
+            cur.c0 = centers[3 * v + 0];
+            cur.c1 = centers[3 * v + 1];
+            cur.c2 = centers[3 * v + 2];
+            cur.l = lengths[v];
+            cur.linv = 1.f / cur.l;
+            cur.g0 = *reinterpret_cast<const float4*>(geos + 8 * v);
+            cur.g1 = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
+            cur.r0 = rgbs[3 * v + 0];
+            cur.r1 = rgbs[3 * v + 1];
+            cur.r2 = rgbs[3 * v + 2];
+            cur_o = orig[j];
+        }
+    };
+    if (c >= 0) fetch(c);
+    while (c >= 0)
+    {
+        wave_sync();
+        stage_entry(st, w, lane, cur);
+        uint64_t sync = __ballot(__popcll(er) >= SYNC_MIN);
+        wave_sync();
+        uint64_t hm = m;
+        // this chunk's row-flush data, then the next chunk's loads, which
+        // complete while this chunk is traversed
+        const uint32_t fo = cur_o;
+        const bool fpm = er != 0ull;
+        const bool fsole = fpm && single_subtile<TS>(cur.bb, tx * TS::TX, ty * TS::TY, W, H);
+        if (c > 0) fetch(c - 1);
+
+        while (true)
+        {
+            const int s = sync ? 63 - __builtin_clzll(sync) : -1;
+            // this lane's hits above the next sync slot, highest first
+            uint64_t seg = (s < 0) ? hm : (hm & ~((2ull << s) - 1ull));
+            if (s == 63) seg = 0ull;
+            hm &= ~seg;
+            while (seg)
+            {
+                const int l = 63 - __builtin_clzll(seg);
+                seg &= ~(1ull << l);
+                float gq[12];
+                if (bwd_hit<NS, EXACT, TS::NW>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
+                {
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
+                }
+            }
+            if (s < 0) break;
+            sync &= ~(1ull << s);
+            const bool mine = (hm >> s) & 1ull;
+            hm &= ~(1ull << s);
+            float gq[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) gq[i] = 0.f;
+            bool hit = false;
+            if (mine) hit = bwd_hit<NS, EXACT, TS::NW>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
+            if (__ballot(hit) == 0ull) continue;
+            if (!hit)
+            {
+#pragma unroll
+                for (int i = 0; i < 12; ++i) gq[i] = 0.f;
+            }
+            const float r = wave_sum12(gq, lane);
+            const int ri = (lane >> 2) & 15;
+            if ((lane & 3) == 0 && ri < 12) acc[ri][w][s] += r;
+        }
+
+        wave_sync();
+        if (fsole)
+        {
+            float a[12];
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < 12; ++i)
+            {
+                a[i] = acc[i][w][lane];
+                any = any || (a[i] != 0.f);
+                acc[i][w][lane] = 0.f;
+            }
+            if (any)
+            {
+                float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)fo);
+                row[0] = make_float4(a[0], a[1], a[2], a[3]);
+                row[1] = make_float4(a[4], a[5], a[6], a[7]);
+                row[2] = make_float4(a[8], a[9], a[10], a[11]);
+            }
+        }
+        else if (fpm)
+        {
+            float* row = partial + 12 * (size_t)fo;
+#pragma unroll
+            for (int i = 0; i < 12; ++i)
+            {
+                const float a = acc[i][w][lane];
+                if (a != 0.f) unsafeAtomicAdd(row + i, a);
+                acc[i][w][lane] = 0.f;
+            }
+        }
+        --c;
     }
 }
 

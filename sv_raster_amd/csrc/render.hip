@@ -35,7 +35,8 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
-    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T)
+    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
+    uint64_t* __restrict__ hit_masks)
 {
     __shared__ Stage st;
     __shared__ uint32_t s_last[NWAVE];
@@ -52,8 +53,10 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
     const uint2 range = ranges[tile];
 
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    const uint32_t last = composite_subtile<NS, EXACT>(st, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list,
-                                                       bboxes, centers, lengths, geos, rgbs, T, C0, C1, C2);
+    uint64_t* mask_out = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
+    const uint32_t last = composite_subtile<NS, EXACT, NWAVE, false, true>(
+        st, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list, bboxes, centers, lengths, geos, rgbs, T, C0, C1,
+        C2, nullptr, nullptr, mask_out);
 
     if (inside)
     {
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-    const uint32_t* __restrict__ orig, float* __restrict__ partial)
+    const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
     __shared__ Stage st;
     __shared__ float acc[12][NWAVE][64];
@@ -97,9 +100,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const size_t HW = (size_t)H * W;
     const size_t pid = inside ? (size_t)py * W + px : 0;
     if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
-    adjoint_subtile<NS, SYNC_MIN, EXACT>(st, acc, w, lane, W, H, tx, ty, sx0, sy0, sx1, sy1, inside, pid, HW, ray,
-                                         range, bg, vox_list, bboxes, centers, lengths, geos, rgbs, out_T, n_contrib,
-                                         dL_dcolor, dL_dT, orig, partial);
+    const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
+    adjoint_masked<NS, SYNC_MIN, EXACT>(st, acc, w, lane, W, H, tx, ty, inside, pid, HW, ray, range, bg, masks,
+                                        vox_list, bboxes, centers, lengths, geos, rgbs, out_T, n_contrib, dL_dcolor,
+                                        dL_dT, orig, partial);
 }
 
 // Per-voxel gradient = sum of the partial rows of its entries, which are
@@ -216,11 +220,11 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     prof_begin(PROF_RENDER_FWD, stream);
     render_fwd_kernel<NS, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, img.tile_last, img.n_contrib, out_color, out_T);
+        rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks);
     SVR_CHECK_LAUNCH();
     render_fwd_kernel<NS, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, img.tile_last, img.n_contrib, out_color, out_T);
+        rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;
@@ -238,13 +242,13 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int per = (T + 7) / 8;
     if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
-    render_bwd_kernel<NS, BWD_SYNC_MIN, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
+    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
+        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
-    render_bwd_kernel<NS, BWD_SYNC_MIN, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
+    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
+        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);
@@ -278,7 +282,7 @@ int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_
     g = GeomState::carve((void*)geom_state, N, nullptr);
     OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
     b = BinningState::carve(binning_state, R, nullptr);
-    img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr);
+    img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr, R);
     return binning_forward(W, H, N, g, ot, R, b, img, stream);
 }
 
@@ -307,7 +311,7 @@ int backward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64
     const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
     g = GeomState::carve((void*)geom_state, N, nullptr);
     b = BinningState::carve((void*)binning_state, R, nullptr);
-    img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr);
+    img = ImageState::carve((void*)image_state, (int64_t)W * H, T, nullptr, R);
     return SVR_OK;
 }
 

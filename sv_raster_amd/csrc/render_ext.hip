@@ -118,7 +118,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
                                          const float* __restrict__ centers, const float* __restrict__ lengths,
                                          const float* __restrict__ geos, const float* __restrict__ rgbs, int flags,
                                          float* __restrict__ max_w, float& T, float& C0, float& C1, float& C2,
-                                         ExtAcc& x)
+                                         ExtAcc& x, uint64_t* __restrict__ mask_out)
 {
     uint32_t last = 0;
     bool done = !inside;
@@ -153,6 +153,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
                        sy1, qpix);
         }
 
+        uint64_t hits = 0ull;
         while (hm)
         {
             const int l = __builtin_ctzll(hm);
@@ -168,6 +169,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
                                                      nullptr);
             const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
             if (alpha < MIN_ALPHA) continue;
+            hits |= 1ull << l;
             const float pt_w = T * alpha;
             C0 = __builtin_fmaf(pt_w, st.rgb[0][w][l], C0);
             C1 = __builtin_fmaf(pt_w, st.rgb[1][w][l], C1);
@@ -205,6 +207,9 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
                 break;
             }
         }
+        // the hit masks of the plain backward (a forward with options may be
+        // followed by a backward without them)
+        mask_out[(size_t)((base - range.x) >> 6) * (64 * NW)] = hits;
         base = nbase;
         if (__ballot(!done) == 0ull) break;
     }
@@ -257,7 +262,8 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, int flags, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
-    float* __restrict__ out_depth, float* __restrict__ out_normal, float* __restrict__ max_w)
+    float* __restrict__ out_depth, float* __restrict__ out_normal, float* __restrict__ max_w,
+    uint64_t* __restrict__ hit_masks)
 {
     __shared__ Stage st;
     __shared__ uint32_t sv[NWAVE][64];
@@ -276,9 +282,10 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
 
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
     ExtAcc x{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
+    uint64_t* mask_out = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
     const uint32_t last = composite_ext<NS, EXACT>(st, sv, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list,
                                                    bboxes, centers, lengths, geos, rgbs, flags, max_w, T, C0, C1, C2,
-                                                   x);
+                                                   x, mask_out);
     if (inside)
     {
         const size_t pid = (size_t)py * W + px;
@@ -622,11 +629,13 @@ int launch_fwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     prof_begin(PROF_RENDER_FWD, stream);
     render_fwd_ext_kernel<NS, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w);
+        rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w,
+        img.hit_masks);
     SVR_CHECK_LAUNCH();
     render_fwd_ext_kernel<NS, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
         W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w);
+        rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w,
+        img.hit_masks);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;

@@ -145,19 +145,32 @@ struct BinningState
     }
 };
 
-// Per-frame image state (backends/new_cuda/src/raster_state.h:61-68).
+// Hit-mask slots of a frame with R entries over T tiles: chunk c (entries
+// [range.x + 64c, range.x + 64c + 64)) of tile t uses slot
+// floor(range.x / 64) + t + c; these are distinct across tiles and all below
+// R / 64 + T + 1.
+inline int64_t hit_mask_slots(int64_t R, int64_t T) { return R > 0 ? R / 64 + T + 1 : 0; }
+
+// Per-frame image state (backends/new_cuda/src/raster_state.h:61-68), plus
+// the forward's hit masks: per slot, wave and lane (pixel) of a 16x16 tile,
+// bit l is set when entry l of the chunk contributed to the pixel (the ray
+// meets the box and alpha >= MIN_ALPHA, before early termination).  The
+// backward walks exactly those bits instead of re-testing every candidate.
+// The masks sit after the reference's fields, so R = 0 carves the prefix.
 struct ImageState
 {
     uint2* ranges;       // [T] [start, end) of each tile's entries
     uint32_t* tile_last; // [T] range.x + max n_contrib over the tile
     uint32_t* n_contrib; // [H*W]
-    static ImageState carve(void* p, int64_t P, int64_t T, size_t* bytes)
+    uint64_t* hit_masks; // [hit_mask_slots(R, T)][4 waves][64 lanes]
+    static ImageState carve(void* p, int64_t P, int64_t T, size_t* bytes, int64_t R = 0)
     {
         Carver c(p);
         ImageState s;
         s.ranges = c.take<uint2>(T);
         s.tile_last = c.take<uint32_t>(T);
         s.n_contrib = c.take<uint32_t>(P);
+        s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
         if (bytes) *bytes = c.size();
         return s;
     }

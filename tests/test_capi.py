@@ -42,7 +42,8 @@ def test_state_size_queries():
     L = _lib.load()
     assert L.svr_geom_state_size(0) == 256 or L.svr_geom_state_size(0) >= 0
     assert L.svr_geom_state_size(1_000_000) >= 1_000_000 * (4 + 4 + 8 + 4 + 4)
-    assert L.svr_image_state_size(1920, 1080) >= 1920 * 1080 * 4 + 8160 * 12
+    assert L.svr_image_state_size(1920, 1080, 0) >= 1920 * 1080 * 4 + 8160 * 12
+    assert L.svr_image_state_size(1920, 1080, 64000) >= L.svr_image_state_size(1920, 1080, 0) + (1000 + 8160) * 2048
     assert L.svr_binning_state_size(1000) >= 1000 * (8 + 8 + 4 + 4 + 4)
     assert L.svr_order_table_size(1000) >= 8000 * 8
     assert L.svr_backward_workspace_size(1000) >= 1000 * 48
