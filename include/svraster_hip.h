@@ -275,10 +275,11 @@ enum { SVR_TILE_16X16 = 0, SVR_TILE_8X8 = 1, SVR_TILE_32X8 = 2 };
 
 /* Sizes: geometry state of V views (its scan runs over V * N entries, so
  * svr_num_rendered(V * N, ...) returns the batch's total entry count R), the
- * batch image state for T_total tiles and P_total pixels, and the backward
+ * batch image state for T_total tiles, P_total pixels and R entries (the
+ * forward's hit masks, as in svr_image_state_size), and the backward
  * workspace (per-entry partial rows plus per-(view, voxel) gradient rows). */
 size_t svr_batch_geom_state_size(int32_t n_views, int64_t n_voxels);
-size_t svr_batch_image_state_size(int64_t total_tiles, int64_t total_pixels);
+size_t svr_batch_image_state_size(int64_t total_tiles, int64_t total_pixels, int64_t num_rendered);
 size_t svr_batch_backward_workspace_size(int32_t n_views, int64_t n_voxels, int64_t num_rendered);
 
 /* Preprocess of every (view, voxel) pair: out_n_duplicates [V, N] (int32) and

@@ -71,7 +71,7 @@ SIGNATURES = {
     "svr_ijk_2_octpath": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_octpath_2_ijk": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_batch_geom_state_size": (c_sz, [c_i32, c_i64]),
-    "svr_batch_image_state_size": (c_sz, [c_i64, c_i64]),
+    "svr_batch_image_state_size": (c_sz, [c_i64, c_i64, c_i64]),
     "svr_batch_backward_workspace_size": (c_sz, [c_i32, c_i64, c_i64]),
     "svr_batch_preprocess": (c_i32, [c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_batch_sh_compute": (c_i32, [c_i32, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),

@@ -152,7 +152,7 @@ def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_length
         raise Exception("capacity and overflow go together")
     n_dup, geom, R, rgbs = _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree, capacity)
     binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
-    img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels)), dtype=torch.uint8,
+    img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels, R)), dtype=torch.uint8,
                       device=dev)
     color = torch.empty(3 * views.total_pixels, dtype=torch.float32, device=dev)
     T = torch.empty(views.total_pixels, dtype=torch.float32, device=dev)
@@ -416,7 +416,7 @@ def compute_training_stat(views, octree_paths, vox_centers, vox_lengths, geos):
     _lib.check(_L.svr_num_rendered(V * N, _C._ptr(geom), st, ctypes.byref(R)), "num_rendered")
     R = int(R.value)
     binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
-    img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels)), dtype=torch.uint8,
+    img = torch.empty(int(_L.svr_batch_image_state_size(views.total_tiles, views.total_pixels, R)), dtype=torch.uint8,
                       device=dev)
     T = torch.empty(views.total_pixels, dtype=torch.float32, device=dev)
     pair_w = torch.empty(V * N, dtype=torch.float32, device=dev)

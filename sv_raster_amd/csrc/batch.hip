@@ -41,7 +41,8 @@ struct BatchImageState
     uint32_t* exact_list; // [T] tiles deferred to the exact ray-box launch
     uint32_t* qcounts;    // radix counts for the queue sort
     uint32_t* ctr;        // [8]: 0 fwd head, 1 exact count, 2 fwd exact head, 3 bwd head, 4 bwd exact head
-    static BatchImageState carve(void* p, int64_t T, int64_t P, size_t* bytes)
+    uint64_t* hit_masks;  // [hit_mask_slots(R, T)][up to 4 waves][64]: as in ImageState, global tile ids
+    static BatchImageState carve(void* p, int64_t T, int64_t P, size_t* bytes, int64_t R = 0)
     {
         Carver c(p);
         BatchImageState s;
@@ -55,6 +56,7 @@ struct BatchImageState
         s.exact_list = c.take<uint32_t>(T);
         s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
         s.ctr = c.take<uint32_t>(8);
+        s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
         if (bytes) *bytes = c.size();
         return s;
     }
@@ -276,7 +278,7 @@ __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t (*sv)[64], ui
                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
                                       uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
                                       float* __restrict__ out_color, float* __restrict__ out_T,
-                                      float* __restrict__ max_w)
+                                      float* __restrict__ max_w, uint64_t* __restrict__ hit_masks)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
@@ -291,9 +293,11 @@ __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t (*sv)[64], ui
     const uint2 range = ranges[g];
     const size_t vo = (size_t)t.u * (size_t)N;
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    const uint32_t last = composite_subtile<NS, EXACT, TS::NW, MAX_W>(
+    uint64_t* mask_out = MAX_W ? nullptr : hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
+    const uint32_t last = composite_subtile<NS, EXACT, TS::NW, MAX_W, !MAX_W>(
         st, w, lane, ray, t.inside, range, t.sx0, t.sy0, t.sx1, t.sy1, vox_list, bboxes + vo, centers, lengths, geos,
-        MAX_W ? rgbs : rgbs + 3 * vo, T, C0, C1, C2, sv, MAX_W ? reinterpret_cast<uint32_t*>(max_w + vo) : nullptr);
+        MAX_W ? rgbs : rgbs + 3 * vo, T, C0, C1, C2, sv, MAX_W ? reinterpret_cast<uint32_t*>(max_w + vo) : nullptr,
+        mask_out);
     if (t.inside)
     {
         const size_t HW = (size_t)t.H * t.W;
@@ -330,14 +334,15 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
-    float* __restrict__ max_w)
+    float* __restrict__ max_w, uint64_t* __restrict__ hit_masks)
 {
     __shared__ StageT<TS::NW> st;
     __shared__ uint32_t sv[MAX_W ? TS::NW : 1][64];
     __shared__ uint32_t s_last[TS::NW];
     const uint32_t g = order[blockIdx.x];
     batch_fwd_tile<NS, false, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes,
-                                         centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T, max_w);
+                                         centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T, max_w,
+                                         hit_masks);
 }
 
 // Exact form: a few workgroups loop over the deferred tiles (head ctr[2]).
@@ -347,7 +352,8 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
-    float* __restrict__ out_color, float* __restrict__ out_T, float* __restrict__ max_w)
+    float* __restrict__ out_color, float* __restrict__ out_T, float* __restrict__ max_w,
+    uint64_t* __restrict__ hit_masks)
 {
     __shared__ StageT<TS::NW> st;
     __shared__ uint32_t sv[MAX_W ? TS::NW : 1][64];
@@ -363,7 +369,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
         batch_fwd_tile<NS, true, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes,
                                             centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T,
-                                            max_w);
+                                            max_w, hit_masks);
         __syncthreads();
     }
 }
@@ -376,7 +382,8 @@ __device__ inline void batch_bwd_tile(StageT<TS::NW>& st, float (&acc)[12][TS::N
                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
                                       const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
                                       const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-                                      const uint32_t* __restrict__ orig, float* __restrict__ partial)
+                                      const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
+                                      float* __restrict__ partial)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
@@ -388,10 +395,12 @@ __device__ inline void batch_bwd_tile(StageT<TS::NW>& st, float (&acc)[12][TS::N
     const size_t pb = (size_t)vw.pix_base;
     const size_t HW = (size_t)t.H * t.W;
     const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
-    adjoint_subtile<NS, BWD_SYNC_MIN, EXACT, TS>(st, acc, w, lane, t.W, t.H, t.tx, t.ty, t.sx0, t.sy0, t.sx1, t.sy1,
-                                             t.inside, pid, HW, ray, ranges[g], vw.bg_color, vox_list, bboxes + vo,
-                                             centers, lengths, geos, rgbs + 3 * vo, out_T + pb, n_contrib + pb,
-                                             dL_dcolor + 3 * pb, dL_dT + pb, orig, partial);
+    const uint2 range = ranges[g];
+    const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
+    adjoint_masked<NS, BWD_SYNC_MIN_MASKED, EXACT, TS>(st, acc, w, lane, t.W, t.H, t.tx, t.ty, t.inside, pid, HW, ray,
+                                                       range, vw.bg_color, masks, vox_list, bboxes + vo, centers,
+                                                       lengths, geos, rgbs + 3 * vo, out_T + pb, n_contrib + pb,
+                                                       dL_dcolor + 3 * pb, dL_dT + pb, orig, partial);
 }
 
 template <int NS, typename TS>
@@ -401,13 +410,13 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-    float* __restrict__ partial)
+    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
     __shared__ StageT<TS::NW> st;
     __shared__ float acc[12][TS::NW][64];
     const uint32_t g = order[blockIdx.x];
     batch_bwd_tile<NS, false, TS>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs, out_T,
-                              n_contrib, dL_dcolor, dL_dT, orig, partial);
+                                  n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
 }
 
 // The forward's deferred tiles (exact_list, count ctr[1]; head ctr[4]).
@@ -418,7 +427,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-    const uint32_t* __restrict__ orig, float* __restrict__ partial)
+    const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
     __shared__ StageT<TS::NW> st;
     __shared__ float acc[12][TS::NW][64];
@@ -432,7 +441,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
         batch_bwd_tile<NS, true, TS>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
-                                 out_T, n_contrib, dL_dcolor, dL_dT, orig, partial);
+                                     out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
         __syncthreads();
     }
 }
@@ -634,11 +643,11 @@ int batch_fwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const B
 {
     batch_fwd_kernel<NS, TS, MAX_W><<<(unsigned)T, TS::THREADS, 0, stream>>>(
         V, views, N, img.order, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
-        img.tile_last, img.n_contrib, out_color, out_T, max_w);
+        img.tile_last, img.n_contrib, out_color, out_T, max_w, img.hit_masks);
     SVR_CHECK_LAUNCH();
     batch_fwd_exact_kernel<NS, TS, MAX_W><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
         V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
-        img.tile_last, img.n_contrib, out_color, out_T, max_w);
+        img.tile_last, img.n_contrib, out_color, out_T, max_w, img.hit_masks);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }
@@ -664,11 +673,11 @@ int batch_bwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const B
 {
     batch_bwd_kernel<NS, TS><<<(unsigned)T, TS::THREADS, 0, stream>>>(
         V, views, N, img.order, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T, img.n_contrib,
-        dL_dcolor, dL_dT, b.orig, partial);
+        dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     batch_bwd_exact_kernel<NS, TS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
         V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T,
-        img.n_contrib, dL_dcolor, dL_dT, b.orig, partial);
+        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }
@@ -686,10 +695,10 @@ size_t svr_batch_geom_state_size(int32_t V, int64_t N)
     return bytes;
 }
 
-size_t svr_batch_image_state_size(int64_t T, int64_t P)
+size_t svr_batch_image_state_size(int64_t T, int64_t P, int64_t R)
 {
     size_t bytes = 0;
-    BatchImageState::carve(nullptr, T > 0 ? T : 0, P > 0 ? P : 0, &bytes);
+    BatchImageState::carve(nullptr, T > 0 ? T : 0, P > 0 ? P : 0, &bytes, R > 0 ? R : 0);
     return bytes;
 }
 
@@ -788,7 +797,7 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
         SVR_REQUIRE(fill_u32(max_w, 0u, (int64_t)V * N, stream) == SVR_OK, SVR_ERR_HIP, "fill max_w failed");
     SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_batch_rasterize_forward: null binning state");
     SVR_REQUIRE(((uintptr_t)geos & 15u) == 0, SVR_ERR_INVALID, "geos must be 16-byte aligned");
-    BatchImageState img = BatchImageState::carve(image_state, T, P, nullptr);
+    BatchImageState img = BatchImageState::carve(image_state, T, P, nullptr, R);
     GeomState g = GeomState::carve((void*)geom_state, (int64_t)V * N, nullptr);
     OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
     BinningState b = BinningState::carve(binning_state, R, nullptr);
@@ -882,7 +891,7 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
     const int64_t VN = (int64_t)V * N;
     GeomState g = GeomState::carve((void*)geom_state, VN, nullptr);
     BinningState b = BinningState::carve((void*)binning_state, R, nullptr);
-    BatchImageState img = BatchImageState::carve((void*)image_state, T, P, nullptr);
+    BatchImageState img = BatchImageState::carve((void*)image_state, T, P, nullptr, R);
     BatchBwdWork wk = BatchBwdWork::carve(workspace, VN, R, nullptr);
 
     if (R > 0)
