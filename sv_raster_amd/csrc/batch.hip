@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __
     if (nd == 0 || scan[p] > cap) return;
     if (nd > BIG_DUP)
     {
-        big_list[atomicAdd(big_count, 1u)] = (uint32_t)p;
+        queue_large(big_list, big_count, (int64_t)gridDim.y * N, nd, (uint32_t)p);
         return;
     }
     const svr_view& vw = views[blockIdx.y];
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __
 }
 
 template <typename TS>
-__global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view* __restrict__ views, int64_t N,
+__global__ __launch_bounds__(256) void batch_duplicate_big_kernel(int V, const svr_view* __restrict__ views, int64_t N,
                                                                   int pos_bits, const uint32_t* __restrict__ n_dup,
                                                                   const uint32_t* __restrict__ scan,
                                                                   const uint2* __restrict__ bboxes,
@@ -193,10 +193,19 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view
                                                                   const uint32_t* __restrict__ big_list,
                                                                   const uint32_t* __restrict__ big_count)
 {
-    const uint32_t count = *big_count;
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    // large pairs one workgroup each, then the huge ones (queued from the end
+    // of big_list) spread over every workgroup
+    const uint32_t count = big_count[0], huge = big_count[1];
+    const int64_t n_items = (int64_t)V * N;
+    for (uint32_t pass = 0; pass < 2; ++pass)
     {
-        const uint32_t p = big_list[i];
+    const uint32_t n = pass == 0 ? count : huge;
+    const uint32_t i0 = pass == 0 ? blockIdx.x : 0u, di = pass == 0 ? gridDim.x : 1u;
+    const uint32_t e0 = pass == 0 ? threadIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t de = pass == 0 ? blockDim.x : gridDim.x * blockDim.x;
+    for (uint32_t i = i0; i < n; i += di)
+    {
+        const uint32_t p = pass == 0 ? big_list[i] : big_list[n_items - 1 - i];
         const uint32_t u = (uint32_t)(p / (uint64_t)N), v = (uint32_t)(p - (uint64_t)u * (uint64_t)N);
         const svr_view& vw = views[u];
         const int gx = tiles_along(vw.width, TS::TX), gy = tiles_along(vw.height, TS::TY);
@@ -207,7 +216,7 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view
         const uint32_t w = (uint32_t)(txmax - txmin + 1);
         const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
         const uint32_t bits = qbits[p];
-        for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x)
+        for (uint32_t e = e0; e < nd; e += de)
         {
             const uint32_t qi = e / tiles, rem = e - qi * tiles;
             uint32_t b = bits;
@@ -218,6 +227,7 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(const svr_view
             keys[off + e] = ((uint64_t)g << pos_bits) | (uint64_t)order_pos[8 * (size_t)v + q];
             vals[off + e] = off + e;
         }
+    }
     }
 }
 
@@ -488,15 +498,31 @@ __global__ __launch_bounds__(64) void batch_sh_fwd_kernel(int deg, int V, const 
     const int lane = threadIdx.x;
     const int64_t v0 = (int64_t)blockIdx.x * 64;
     const int nv = (int)min((int64_t)64, N - v0);
-    for (int e = lane; e < nv * 3; e += 64)
+    // the block's rows are contiguous: all loads in flight before the LDS stores
+    float r0[3], r1[45];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
     {
-        const int k = e / 3, f = e - 3 * k;
-        buf[k * BSH_PAD + f] = sh0[3 * v0 + e];
+        const int e = lane + 64 * j;
+        r0[j] = e < nv * 3 ? sh0[3 * v0 + e] : 0.f;
     }
-    for (int e = lane; e < nv * 45; e += 64)
+#pragma unroll
+    for (int j = 0; j < 45; ++j)
     {
-        const int k = e / 45, f = e - 45 * k;
-        buf[k * BSH_PAD + 3 + f] = shs[45 * v0 + e];
+        const int e = lane + 64 * j;
+        r1[j] = e < nv * 45 ? shs[45 * v0 + e] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        const int e = lane + 64 * j, k = e / 3, f = e - 3 * k;
+        if (e < nv * 3) buf[k * BSH_PAD + f] = r0[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 45; ++j)
+    {
+        const int e = lane + 64 * j, k = e / 45, f = e - 45 * k;
+        if (e < nv * 45) buf[k * BSH_PAD + 3 + f] = r1[j];
     }
     wave_sync();
     if (lane >= nv) return;
@@ -817,7 +843,7 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     }
     if (R > 0)
     {
-        SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
+        SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
         // emit into the buffer that makes the sorted result land in b.keys / b.orig
         const int nbits = pos_bits + tile_bits;
         const bool odd = ((nbits + RADIX_BITS - 1) / RADIX_BITS) & 1;
@@ -829,7 +855,7 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
                                                                      g.counters + 2, cap)));
         SVR_CHECK_LAUNCH();
         SVR_WITH_TILE(shape, (batch_duplicate_big_kernel<TS><<<1024, 256, 0, stream>>>(
-                                 views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0, g.big_list,
+                                 V, views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0, g.big_list,
                                  g.counters + 2)));
         SVR_CHECK_LAUNCH();
         uint64_t* sorted = nullptr;

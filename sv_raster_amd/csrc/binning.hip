@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int g
     if (nd == 0) return;
     if (nd > BIG_DUP)
     {
-        big_list[atomicAdd(big_count, 1u)] = (uint32_t)v;
+        queue_large(big_list, big_count, N, nd, (uint32_t)v);
         return;
     }
     uint32_t off = scan[v] - nd;
@@ -319,7 +319,24 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int g
     }
 }
 
-__global__ __launch_bounds__(256) void duplicate_big_kernel(int gx, int gy, int pos_bits,
+// Entry e of a voxel's run: quadrant (the qi-th set bit of bits) major, then
+// tile rows of its rectangle.
+__device__ inline void emit_one(uint32_t e, uint32_t off, uint32_t tiles, uint32_t w, int txmin, int tymin,
+                                uint32_t bits, int gx, int pos_bits, const uint32_t* __restrict__ pos8,
+                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t qi = e / tiles, rem = e - qi * tiles;
+    uint32_t b = bits;
+    for (uint32_t s = 0; s < qi; ++s) b &= b - 1u;
+    const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
+    const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
+    keys[off + e] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | (uint64_t)pos8[q];
+    vals[off + e] = off + e;
+}
+
+// Large voxels: one workgroup each; then the huge ones with every workgroup
+// taking a strided share of each.
+__global__ __launch_bounds__(256) void duplicate_big_kernel(int64_t N, int gx, int gy, int pos_bits,
                                                             const uint32_t* __restrict__ n_dup,
                                                             const uint32_t* __restrict__ scan,
                                                             const uint2* __restrict__ bboxes,
@@ -330,27 +347,25 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(int gx, int gy, int 
                                                             const uint32_t* __restrict__ big_list,
                                                             const uint32_t* __restrict__ big_count)
 {
-    const uint32_t count = *big_count;
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    const uint32_t count = big_count[0], huge = big_count[1];
+    for (uint32_t pass = 0; pass < 2; ++pass)
     {
-        const uint32_t v = big_list[i];
-        const uint32_t nd = n_dup[v];
-        const uint32_t off = scan[v] - nd;
-        int txmin, tymin, txmax, tymax;
-        rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
-        const uint32_t w = (uint32_t)(txmax - txmin + 1);
-        const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
-        const uint32_t bits = qbits[v];
-        for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x)
+        const uint32_t n = pass == 0 ? count : huge;
+        const uint32_t i0 = pass == 0 ? blockIdx.x : 0u, di = pass == 0 ? gridDim.x : 1u;
+        const uint32_t e0 = pass == 0 ? threadIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t de = pass == 0 ? blockDim.x : gridDim.x * blockDim.x;
+        for (uint32_t i = i0; i < n; i += di)
         {
-            const uint32_t qi = e / tiles, rem = e - qi * tiles;
-            // the qi-th set bit of bits
-            uint32_t b = bits;
-            for (uint32_t s = 0; s < qi; ++s) b &= b - 1u;
-            const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
-            const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
-            keys[off + e] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | (uint64_t)order_pos[8 * v + q];
-            vals[off + e] = off + e;
+            const uint32_t v = pass == 0 ? big_list[i] : big_list[N - 1 - i];
+            const uint32_t nd = n_dup[v];
+            const uint32_t off = scan[v] - nd;
+            int txmin, tymin, txmax, tymax;
+            rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
+            const uint32_t w = (uint32_t)(txmax - txmin + 1);
+            const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
+            const uint32_t bits = qbits[v];
+            for (uint32_t e = e0; e < nd; e += de)
+                emit_one(e, off, tiles, w, txmin, tymin, bits, gx, pos_bits, order_pos + 8 * (size_t)v, keys, vals);
         }
     }
 }
@@ -437,7 +452,7 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     }
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
-    SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 1, stream) == SVR_OK, SVR_ERR_HIP, "fill counter failed");
+    SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
     // the sort ping-pongs between the two buffers, one 8-bit digit per pass:
     // emit into the buffer that makes the sorted result land in b.keys / b.orig
     const int nbits = pos_bits + tile_bits;
@@ -448,8 +463,8 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
                                                                       g.qbits, ot.pos, k0, o0, g.big_list,
                                                                       g.counters + 2);
     SVR_CHECK_LAUNCH();
-    duplicate_big_kernel<<<1024, 256, 0, stream>>>(gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0,
-                                                   g.big_list, g.counters + 2);
+    duplicate_big_kernel<<<1024, 256, 0, stream>>>(N, gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0,
+                                                   o0, g.big_list, g.counters + 2);
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
     uint32_t* sorted_orig = nullptr;

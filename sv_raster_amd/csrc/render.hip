@@ -109,7 +109,9 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
 // Per-voxel gradient = sum of the partial rows of its entries, which are
 // contiguous in emission order: [scan[v] - n_dup[v], scan[v]).  Voxels with
 // more than BIG_DUP entries (listed in big_list by the duplication pass) are
-// summed by a whole workgroup.
+// summed by a whole workgroup, those with more than HUGE_DUP by every
+// workgroup of the launch (partial sums added with float atomics into rows
+// that grad_reduce_kernel zeroes first).
 
 __device__ inline void write_grad(size_t v, const float* a, float* __restrict__ dgeos, float* __restrict__ drgbs,
                                   float* __restrict__ dsub)
@@ -133,8 +135,15 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint3
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
     uint32_t nd = n_dup[v];
-    if (nd > BIG_DUP) return;
+    // entries beyond a fixed capacity were never emitted (nor queued)
     if (scan[v] > cap) nd = 0;
+    if (nd > HUGE_DUP)
+    {
+        const float z[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        write_grad((size_t)v, z, dgeos, drgbs, dsub);
+        return;
+    }
+    if (nd > BIG_DUP) return;
     float a[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) a[i] = 0.f;
@@ -150,7 +159,7 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint3
     write_grad((size_t)v, a, dgeos, drgbs, dsub);
 }
 
-__global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __restrict__ big_list,
+__global__ __launch_bounds__(256) void grad_reduce_big_kernel(int64_t N, const uint32_t* __restrict__ big_list,
                                                               const uint32_t* __restrict__ big_count,
                                                               const uint32_t* __restrict__ n_dup,
                                                               const uint32_t* __restrict__ scan, uint32_t cap,
@@ -160,17 +169,22 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __
 {
     __shared__ float red[12][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // one workgroup per voxel with many entries; blocks stride over the queue
-    const uint32_t count = *big_count;
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    // pass 0: one workgroup per large voxel, blocks striding over the queue;
+    // pass 1: the huge voxels (queued from the end), each split over all blocks
+    const uint32_t counts[2] = {big_count[0], big_count[1]};
+    for (uint32_t pass = 0; pass < 2; ++pass)
+    for (uint32_t i = pass == 0 ? blockIdx.x : 0u; i < counts[pass]; i += pass == 0 ? gridDim.x : 1u)
     {
-        const uint32_t v = big_list[i];
+        const uint32_t v = pass == 0 ? big_list[i] : big_list[N - 1 - i];
         const uint32_t nd = scan[v] > cap ? 0u : n_dup[v];
         const uint32_t off = scan[v] - nd;
+        const uint32_t e0 = pass == 0 ? threadIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t de = pass == 0 ? blockDim.x : gridDim.x * blockDim.x;
+        if (pass == 1 && blockIdx.x * blockDim.x >= nd) continue;  // no rows for this block
         float a[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) a[i] = 0.f;
-        for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x)
+        for (uint32_t e = e0; e < nd; e += de)
         {
             const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
             const float4 p0 = r[0], p1 = r[1], p2 = r[2];
@@ -190,7 +204,18 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(const uint32_t* __
             float t[12];
 #pragma unroll
             for (int i = 0; i < 12; ++i) t[i] = (red[i][0] + red[i][1]) + (red[i][2] + red[i][3]);
-            write_grad((size_t)v, t, dgeos, drgbs, dsub);
+            if (pass == 0)
+            {
+                write_grad((size_t)v, t, dgeos, drgbs, dsub);
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) unsafeAtomicAdd(dgeos + 8 * (size_t)v + k, t[k]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) unsafeAtomicAdd(drgbs + 3 * (size_t)v + k, t[8 + k]);
+                unsafeAtomicAdd(dsub + v, t[11]);
+            }
         }
         __syncthreads();
     }
@@ -202,8 +227,8 @@ int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, floa
     grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, g.n_dup, g.scan, cap, partial, dgeos,
                                                                        drgbs, dsub);
     SVR_CHECK_LAUNCH();
-    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(g.big_list, g.counters + 2, g.n_dup, g.scan, cap, partial, dgeos,
-                                                     drgbs, dsub);
+    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(N, g.big_list, g.counters + 2, g.n_dup, g.scan, cap, partial,
+                                                     dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }

@@ -78,11 +78,28 @@ __global__ __launch_bounds__(64) void sh_fwd16_kernel(int deg, int64_t cnt, cons
     if (lane < nv) myv = idx ? idx[t0 + lane] : t0 + lane;
     sv[lane] = myv;
     wave_sync();
-    for (int e = lane; e < nv * SH_F; e += 64)
+    // element e = lane + 64 j of the block's 64 x 48 coefficients: all 48
+    // loads are issued before the LDS stores, so they are in flight together
+    float r[SH_F];
+#pragma unroll
+    for (int j = 0; j < SH_F; ++j)
     {
+        const int e = lane + 64 * j;
         const int k = e / SH_F, f = e - k * SH_F;
-        const int64_t v = sv[k];
-        buf[k * SH_PAD + f] = (f < 3) ? sh0[3 * v + f] : shs[v * 45 + (f - 3)];
+        float x = 0.f;
+        if (k < nv)
+        {
+            const int64_t v = sv[k];
+            x = (f < 3) ? sh0[3 * v + f] : shs[v * 45 + (f - 3)];
+        }
+        r[j] = x;
+    }
+#pragma unroll
+    for (int j = 0; j < SH_F; ++j)
+    {
+        const int e = lane + 64 * j;
+        const int k = e / SH_F, f = e - k * SH_F;
+        buf[k * SH_PAD + f] = r[j];
     }
     wave_sync();
     if (lane >= nv) return;
@@ -138,6 +155,84 @@ __global__ __launch_bounds__(64) void sh_bwd16_kernel(int deg, int64_t cnt, cons
         const float val = buf[k * SH_PAD + f];
         if (f < 3) dsh0[3 * v + f] = val;
         else dshs[v * 45 + (f - 3)] = val;
+    }
+}
+
+// The degree-3 backward over every voxel, in blocks of 64 consecutive voxels:
+// rows of voxels listed in idx (marked by sh_mark_kernel with MARK_BITS in
+// the first word of their dL_dsh0 row) get the SH gradient, the others zeros,
+// and each block writes its contiguous 64 x 3 and 64 x 45 output spans with
+// 16-byte stores.  The same values as sh_bwd16_kernel over idx after zeroing
+// both outputs (rows listed twice are written once: the reference writes, it
+// does not accumulate), without the separate 12 N + 180 N byte zero fill.
+constexpr uint32_t MARK_BITS = 0x7FC0DEADu;  // a NaN payload no gradient produces
+
+__global__ __launch_bounds__(256) void sh_mark_kernel(int64_t cnt, const int64_t* __restrict__ idx,
+                                                      float* __restrict__ dsh0)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) reinterpret_cast<uint32_t*>(dsh0)[3 * idx[i]] = MARK_BITS;
+}
+
+__global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, const float* __restrict__ centers,
+                                                            const float* __restrict__ cam,
+                                                            const float* __restrict__ dL_drgbs,
+                                                            float* __restrict__ dsh0, float* __restrict__ dshs)
+{
+    __shared__ float buf[64 * SH_PAD];
+    const int lane = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int nv = (int)min((int64_t)64, N - v0);
+    const int64_t v = v0 + lane;
+    float* row = buf + lane * SH_PAD;
+    if (lane < nv)
+    {
+        const bool marked = reinterpret_cast<const uint32_t*>(dsh0)[3 * v] == MARK_BITS;
+        if (marked)
+        {
+            float x, y, z, b[16];
+            sh_dir(centers, v, cam, x, y, z);
+            const int nb = sh_basis(deg, x, y, z, b);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch)
+            {
+                const float g = dL_drgbs[3 * v + ch];
+                row[ch] = SH_C0 * g;
+                for (int k = 1; k < 16; ++k) row[3 * k + ch] = (k < nb) ? b[k] * g : 0.f;
+            }
+        }
+        else
+        {
+            for (int f = 0; f < SH_F; ++f) row[f] = 0.f;
+        }
+    }
+    wave_sync();
+    if (nv == 64)
+    {
+        // 16-byte aligned spans: 64 rows start at a multiple of 64 voxels
+        float4* o0 = reinterpret_cast<float4*>(dsh0 + 3 * v0);
+        for (int e = lane; e < 48; e += 64)
+        {
+            const int a = 4 * e;
+            float q[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) q[i] = buf[((a + i) / 3) * SH_PAD + (a + i) % 3];
+            o0[e] = make_float4(q[0], q[1], q[2], q[3]);
+        }
+        float4* o1 = reinterpret_cast<float4*>(dshs + 45 * v0);
+        for (int e = lane; e < 720; e += 64)
+        {
+            const int a = 4 * e;
+            float q[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) q[i] = buf[((a + i) / 45) * SH_PAD + 3 + (a + i) % 45];
+            o1[e] = make_float4(q[0], q[1], q[2], q[3]);
+        }
+    }
+    else
+    {
+        for (int e = lane; e < nv * 3; e += 64) dsh0[3 * v0 + e] = buf[(e / 3) * SH_PAD + e % 3];
+        for (int e = lane; e < nv * 45; e += 64) dshs[45 * v0 + e] = buf[(e / 45) * SH_PAD + 3 + e % 45];
     }
 }
 
@@ -266,6 +361,18 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
     SVR_REQUIRE(deg >= 0 && deg <= 3 && M >= (deg + 1) * (deg + 1), SVR_ERR_INVALID,
                 "sh degree %d needs %d coefficients, got %d", deg, (deg + 1) * (deg + 1), M);
     if (N == 0) return SVR_OK;
+    if (M == 16 && n_idx > 0 && ((uintptr_t)dsh0 & 15u) == 0 && ((uintptr_t)dshs & 15u) == 0)
+    {
+        if (fill_u32(dsh0, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+        prof_begin(PROF_SH_BWD, stream);
+        sh_mark_kernel<<<(unsigned)((n_idx + 255) / 256), 256, 0, stream>>>(n_idx, idx, dsh0);
+        SVR_CHECK_LAUNCH();
+        sh_bwd16_dense_kernel<<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, dL_drgbs, dsh0,
+                                                                           dshs);
+        SVR_CHECK_LAUNCH();
+        prof_end(PROF_SH_BWD, stream);
+        return SVR_OK;
+    }
     if (fill_u32(dsh0, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
     if (M > 1 && fill_u32(dshs, 0u, 3 * N * (int64_t)(M - 1), stream) != SVR_OK) return SVR_ERR_HIP;
     const int64_t cnt = n_idx ? n_idx : N;

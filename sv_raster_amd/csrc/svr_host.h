@@ -62,6 +62,21 @@ struct Carver
 // Voxels with more list entries than this are expanded / reduced by a whole
 // workgroup (they are queued in GeomState::big_list by the duplication pass).
 constexpr uint32_t BIG_DUP = 32;
+// Voxels with more entries than this (the outside shell: up to ~10^5 entries
+// on a 1080p frame) are spread over every workgroup of the launch instead, so
+// that no single workgroup becomes its tail; they are queued from the end of
+// big_list (count in counters[3]).
+constexpr uint32_t HUGE_DUP = 4096;
+
+// Queues item `id` with nd entries (nd > BIG_DUP) for the large-voxel passes.
+__device__ inline void queue_large(uint32_t* __restrict__ big_list, uint32_t* __restrict__ counts, int64_t n_items,
+                                   uint32_t nd, uint32_t id)
+{
+    if (nd > HUGE_DUP)
+        big_list[n_items - 1 - (int64_t)atomicAdd(counts + 1, 1u)] = id;
+    else
+        big_list[atomicAdd(counts, 1u)] = id;
+}
 
 constexpr int SCAN_BLOCK = 1024;
 constexpr int SCAN_ITEMS = 4;
@@ -76,6 +91,7 @@ struct GeomState
     uint32_t* qbits;       // [N] camera-quadrant bitset
     uint32_t* block_sums;  // [ceil(N / SCAN_TILE) + 1]
     uint32_t* counters;    // [8]: 0-1 R (total entries, 64-bit), 2 large-voxel queue length,
+                           // 3 huge-voxel queue length (from the end of big_list),
                            // 4-5 entries to sort (R, or 0 when R exceeds a fixed capacity)
     uint32_t* big_list;    // [N] voxels with many entries, expanded by whole workgroups
     static GeomState carve(void* p, int64_t N, size_t* bytes)
