@@ -197,14 +197,12 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(int V, const s
     // of big_list) spread over every workgroup
     const uint32_t count = big_count[0], huge = big_count[1];
     const int64_t n_items = (int64_t)V * N;
+    const uint32_t K = huge ? max(1u, gridDim.x / huge) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
     {
-    const uint32_t n = pass == 0 ? count : huge;
-    const uint32_t i0 = pass == 0 ? blockIdx.x : 0u, di = pass == 0 ? gridDim.x : 1u;
-    const uint32_t e0 = pass == 0 ? threadIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t de = pass == 0 ? blockDim.x : gridDim.x * blockDim.x;
-    for (uint32_t i = i0; i < n; i += di)
+    for (uint32_t it = blockIdx.x; it < (pass == 0 ? count : huge * K); it += gridDim.x)
     {
+        const uint32_t i = pass == 0 ? it : it % huge;
         const uint32_t p = pass == 0 ? big_list[i] : big_list[n_items - 1 - i];
         const uint32_t u = (uint32_t)(p / (uint64_t)N), v = (uint32_t)(p - (uint64_t)u * (uint64_t)N);
         const svr_view& vw = views[u];
@@ -216,7 +214,10 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(int V, const s
         const uint32_t w = (uint32_t)(txmax - txmin + 1);
         const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
         const uint32_t bits = qbits[p];
-        for (uint32_t e = e0; e < nd; e += de)
+        const uint32_t c = pass == 0 ? 0u : it / huge;
+        const uint32_t e_lo = pass == 0 ? 0u : (uint32_t)(((uint64_t)nd * c) / K);
+        const uint32_t e_hi = pass == 0 ? nd : (uint32_t)(((uint64_t)nd * (c + 1)) / K);
+        for (uint32_t e = e_lo + threadIdx.x; e < e_hi; e += blockDim.x)
         {
             const uint32_t qi = e / tiles, rem = e - qi * tiles;
             uint32_t b = bits;

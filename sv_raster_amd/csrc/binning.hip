@@ -347,24 +347,27 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(int64_t N, int gx, i
                                                             const uint32_t* __restrict__ big_list,
                                                             const uint32_t* __restrict__ big_count)
 {
+    // pass 0: one workgroup per large voxel; pass 1: each huge voxel split into
+    // K entry ranges taken by K different workgroups (item = chunk * huge + voxel)
     const uint32_t count = big_count[0], huge = big_count[1];
+    const uint32_t K = huge ? max(1u, gridDim.x / huge) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
     {
-        const uint32_t n = pass == 0 ? count : huge;
-        const uint32_t i0 = pass == 0 ? blockIdx.x : 0u, di = pass == 0 ? gridDim.x : 1u;
-        const uint32_t e0 = pass == 0 ? threadIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
-        const uint32_t de = pass == 0 ? blockDim.x : gridDim.x * blockDim.x;
-        for (uint32_t i = i0; i < n; i += di)
+        for (uint32_t it = blockIdx.x; it < (pass == 0 ? count : huge * K); it += gridDim.x)
         {
+            const uint32_t i = pass == 0 ? it : it % huge;
             const uint32_t v = pass == 0 ? big_list[i] : big_list[N - 1 - i];
             const uint32_t nd = n_dup[v];
             const uint32_t off = scan[v] - nd;
+            const uint32_t c = pass == 0 ? 0u : it / huge;
+            const uint32_t e_lo = pass == 0 ? 0u : (uint32_t)(((uint64_t)nd * c) / K);
+            const uint32_t e_hi = pass == 0 ? nd : (uint32_t)(((uint64_t)nd * (c + 1)) / K);
             int txmin, tymin, txmax, tymax;
             rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
             const uint32_t w = (uint32_t)(txmax - txmin + 1);
             const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
             const uint32_t bits = qbits[v];
-            for (uint32_t e = e0; e < nd; e += de)
+            for (uint32_t e = e_lo + threadIdx.x; e < e_hi; e += blockDim.x)
                 emit_one(e, off, tiles, w, txmin, tymin, bits, gx, pos_bits, order_pos + 8 * (size_t)v, keys, vals);
         }
     }

@@ -170,21 +170,24 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(int64_t N, const u
     __shared__ float red[12][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // pass 0: one workgroup per large voxel, blocks striding over the queue;
-    // pass 1: the huge voxels (queued from the end), each split over all blocks
+    // pass 1: the huge voxels (queued from the end), each split into K row
+    // ranges taken by K different workgroups (item = chunk * huge + voxel)
     const uint32_t counts[2] = {big_count[0], big_count[1]};
+    const uint32_t K = counts[1] ? max(1u, gridDim.x / counts[1]) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
-    for (uint32_t i = pass == 0 ? blockIdx.x : 0u; i < counts[pass]; i += pass == 0 ? gridDim.x : 1u)
+    for (uint32_t it = blockIdx.x; it < (pass == 0 ? counts[0] : counts[1] * K); it += gridDim.x)
     {
+        const uint32_t i = pass == 0 ? it : it % counts[1];
         const uint32_t v = pass == 0 ? big_list[i] : big_list[N - 1 - i];
         const uint32_t nd = scan[v] > cap ? 0u : n_dup[v];
         const uint32_t off = scan[v] - nd;
-        const uint32_t e0 = pass == 0 ? threadIdx.x : blockIdx.x * blockDim.x + threadIdx.x;
-        const uint32_t de = pass == 0 ? blockDim.x : gridDim.x * blockDim.x;
-        if (pass == 1 && blockIdx.x * blockDim.x >= nd) continue;  // no rows for this block
+        const uint32_t c = pass == 0 ? 0u : it / counts[1];
+        const uint32_t e_lo = pass == 0 ? 0u : (uint32_t)(((uint64_t)nd * c) / K);
+        const uint32_t e_hi = pass == 0 ? nd : (uint32_t)(((uint64_t)nd * (c + 1)) / K);
         float a[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) a[i] = 0.f;
-        for (uint32_t e = e0; e < nd; e += de)
+        for (uint32_t e = e_lo + threadIdx.x; e < e_hi; e += blockDim.x)
         {
             const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
             const float4 p0 = r[0], p1 = r[1], p2 = r[2];
