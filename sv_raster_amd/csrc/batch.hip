@@ -386,15 +386,14 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
 }
 
 template <int NS, bool EXACT, typename TS>
-__device__ inline void batch_bwd_tile(StageT<TS::NW>& st, float (&acc)[12][TS::NW][64], uint32_t g, int V,
-                                      const svr_view* __restrict__ views, int64_t N, const uint2* __restrict__ ranges,
-                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
-                                      const float* __restrict__ centers, const float* __restrict__ lengths,
-                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
-                                      const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
-                                      const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-                                      const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-                                      float* __restrict__ partial)
+__device__ inline void batch_bwd_tile(WindowLDS<BWD_WINDOW>& sh, uint32_t g, int V, const svr_view* __restrict__ views,
+                                      int64_t N, const uint2* __restrict__ ranges,
+                                      const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
+                                      const float* __restrict__ lengths, const float* __restrict__ geos,
+                                      const float* __restrict__ rgbs, const float* __restrict__ out_T,
+                                      const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
+                                      const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
+                                      const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
@@ -408,26 +407,25 @@ __device__ inline void batch_bwd_tile(StageT<TS::NW>& st, float (&acc)[12][TS::N
     const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
     const uint2 range = ranges[g];
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
-    adjoint_masked<NS, BWD_SYNC_MIN_MASKED, EXACT, TS>(st, acc, w, lane, t.W, t.H, t.tx, t.ty, t.inside, pid, HW, ray,
-                                                       range, vw.bg_color, masks, vox_list, bboxes + vo, centers,
-                                                       lengths, geos, rgbs + 3 * vo, out_T + pb, n_contrib + pb,
-                                                       dL_dcolor + 3 * pb, dL_dT + pb, orig, partial);
+    adjoint_window<NS, BWD_SYNC_MIN_MASKED, EXACT, TS>(sh, w, lane, t.inside, pid, HW, ray, range, vw.bg_color, masks,
+                                                       vox_list, centers, lengths, geos, rgbs + 3 * vo, out_T + pb,
+                                                       n_contrib + pb, dL_dcolor + 3 * pb, dL_dT + pb, orig,
+                                                       partial);
 }
 
 template <int NS, typename TS>
 __global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
-    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
-    const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
-    const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
+    const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
+    const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
+    const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
+    float* __restrict__ partial)
 {
-    __shared__ StageT<TS::NW> st;
-    __shared__ float acc[12][TS::NW][64];
+    __shared__ WindowLDS<BWD_WINDOW> sh;
     const uint32_t g = order[blockIdx.x];
-    batch_bwd_tile<NS, false, TS>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs, out_T,
-                                  n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+    batch_bwd_tile<NS, false, TS>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T, n_contrib,
+                                  dL_dcolor, dL_dT, orig, hit_masks, partial);
 }
 
 // The forward's deferred tiles (exact_list, count ctr[1]; head ctr[4]).
@@ -435,13 +433,12 @@ template <int NS, typename TS>
 __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
     const uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
-    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
-    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-    const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
+    const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
+    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
-    __shared__ StageT<TS::NW> st;
-    __shared__ float acc[12][TS::NW][64];
+    __shared__ WindowLDS<BWD_WINDOW> sh;
     __shared__ uint32_t s_i;
     const uint32_t count = ctr[1];
     while (true)
@@ -451,8 +448,8 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
         const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_bwd_tile<NS, true, TS>(st, acc, g, V, views, N, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
-                                     out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+        batch_bwd_tile<NS, true, TS>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
+                                     n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
         __syncthreads();
     }
 }
@@ -699,11 +696,11 @@ int batch_bwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const B
                      const float* dL_dT, float* partial, hipStream_t stream)
 {
     batch_bwd_kernel<NS, TS><<<(unsigned)T, TS::THREADS, 0, stream>>>(
-        V, views, N, img.order, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T, img.n_contrib,
-        dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
+        V, views, N, img.order, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T, img.n_contrib, dL_dcolor,
+        dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     batch_bwd_exact_kernel<NS, TS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
-        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs, out_T,
+        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T,
         img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     return SVR_OK;

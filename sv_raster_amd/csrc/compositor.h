@@ -259,28 +259,34 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
     return last;
 }
 
+// Staged payload of one entry for the backward, 4 float4 (one 16-byte LDS
+// read each): centre + length, densities 0-3, densities 4-7, colour + 1/length.
+struct EntryPay
+{
+    float4 q[4];
+};
+
 // Adjoint of one (pixel, entry) hit (backward.cu:259-527 without the
 // regularizers): recovers T before the entry, updates the running dL/dT and
 // writes the 12 gradient terms (8 densities, 3 colours, subdivision priority).
-template <int NS, bool EXACT, int NW>
-__device__ inline bool bwd_hit(const StageT<NW>& st, int w, int l, const PixRay& ray, float& T, float& last_dT, float dp0,
-                               float dp1, float dp2, float* gq)
+template <int NS, bool EXACT>
+__device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, float& last_dT, float dp0, float dp1,
+                               float dp2, float* gq)
 {
-    const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
+    const float4 a0 = p.q[0], a1 = p.q[1], a2 = p.q[2], a3 = p.q[3];
+    const float c0 = a0.x, c1 = a0.y, c2 = a0.z, len = a0.w;
     float a, b;
     ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
     if (a > b) return false;
-    float g[8], dI[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-    const float vol = vox_vol<NS, true>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, dI);
+    const float g[8] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+    float dI[8];
+    const float vol = vox_vol<NS, true>(c0, c1, c2, len, a3.w, g, a, b, ray, dI);
     const float e = svr_expf(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     if (alpha < MIN_ALPHA) return false;
     T = T / (1.f - alpha);
     const float pt_w = alpha * T;
-    const float r0 = st.rgb[0][w][l], r1 = st.rgb[1][w][l], r2 = st.rgb[2][w][l];
-    const float dpt = __builtin_fmaf(dp2, r2, __builtin_fmaf(dp1, r1, dp0 * r0));
+    const float dpt = __builtin_fmaf(dp2, a3.z, __builtin_fmaf(dp1, a3.y, dp0 * a3.x));
     const float diff = dpt - last_dT;
     const float dalpha = T * diff;
     last_dT = __builtin_fmaf(alpha, diff, last_dT);
@@ -294,311 +300,181 @@ __device__ inline bool bwd_hit(const StageT<NW>& st, int w, int l, const PixRay&
     return true;
 }
 
-// Reverse traversal of one wave's 8x8 sub-tile (backward.cu:175-527 for the
-// colour and T losses).  EXACT selects the ray-box min/max form (see
-// ray_aabb); it is uniform over the wave.
-//
-// Entries whose bbox covers at least SYNC_MIN pixels of the wave (coarse
-// voxels, the outside shell) are "sync slots": every pixel processes them at
-// the same step, at their place in its own order, and their 12 gradient terms
-// are summed across the wave by one butterfly (wave_sum12) instead of up to
-// 64-way conflicting LDS atomics.
-// Between two sync slots each lane walks its own remaining hits in lock-step
-// and accumulates with ds_add.
-//
-// The chunk loop is software-pipelined: chunk c+1 is loaded into registers
-// before chunk c's gradients are flushed with global atomics, because on this
-// architecture a later load's wait also waits for all earlier stores/atomics.
-template <int NS, int SYNC_MIN, bool EXACT, typename TS = Tile16>
-__device__ inline void adjoint_subtile(StageT<TS::NW>& st, float (&acc)[12][TS::NW][64], int w, int lane, int W, int H, int tx,
-                                       int ty, int sx0, int sy0, int sx1, int sy1, bool inside, size_t pid, size_t HW,
-                                       const PixRay& ray, uint2 range, float bg,
-                                       const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
-                                       const float* __restrict__ centers, const float* __restrict__ lengths,
-                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
-                                       const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
-                                       const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-                                       const uint32_t* __restrict__ orig, float* __restrict__ partial)
+// Workgroup-shared state of the windowed adjoint: for the KW chunks of a
+// window, the staged payload, the per-entry gradient accumulators, whether
+// some pixel of the tile used the entry, and the entry's emission-order row.
+template <int KW>
+struct WindowLDS
 {
-    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
-    uint32_t nc = 0;
-    if (inside)
-    {
-        T = out_T[pid];
-        nc = n_contrib[pid];
-        dp0 = dL_dcolor[pid];
-        dp1 = dL_dcolor[HW + pid];
-        dp2 = dL_dcolor[2 * HW + pid];
-        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
-    }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k][w][lane] = 0.f;
-    uint64_t qpix[8];
-#pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(inside && nc > 0 && ray.pq == q);
-    const int64_t rx = (int64_t)range.x;
-    int64_t end = rx + wave_max_u32(nc);
+    EntryPay pay[KW][64];
+    float acc[KW][12][64];
+    uint32_t used[KW][64];
+    uint32_t orow[KW][64];
+    uint32_t nch[16];  // per wave: chunks that hold its pixels' contributors
+};
+constexpr int BWD_WINDOW = 4;
 
-    EntryRegs cur;
-    uint32_t cur_o = 0;
-    {
-        const int64_t jl = end - 1 - lane;
-        const bool valid = jl >= rx;
-        load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
-                   qpix);
-        if (valid) cur_o = orig[jl];
-    }
-    while (end > rx)
-    {
-        wave_sync();
-        stage_entry(st, w, lane, cur);
-        uint64_t sync = __ballot(__popcll(cur.pm) >= SYNC_MIN);
-        wave_sync();
-        uint64_t hm = wave_transpose64(cur.pm, lane);
-        // entry end-1-l is behind this pixel's last contributor when
-        // end-1-l >= range.x + nc, i.e. l < end - range.x - nc
-        const int64_t cut = end - rx - (int64_t)nc;
-        if (cut >= 64) hm = 0ull;
-        else if (cut > 0) hm &= ~((1ull << cut) - 1ull);
-
-        while (true)
-        {
-            const int s = sync ? __builtin_ctzll(sync) : 64;
-            uint64_t seg = (s == 64) ? hm : (hm & ((1ull << s) - 1ull));
-            hm &= ~seg;
-            while (seg)
-            {
-                const int l = __builtin_ctzll(seg);
-                seg &= seg - 1ull;
-                float gq[12];
-                if (bwd_hit<NS, EXACT, TS::NW>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
-                {
-#pragma unroll
-                    for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
-                }
-            }
-            if (s == 64) break;
-            sync &= sync - 1ull;
-            const bool mine = (hm >> s) & 1ull;
-            hm &= ~(1ull << s);
-            float gq[12];
-#pragma unroll
-            for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-            bool hit = false;
-            if (mine) hit = bwd_hit<NS, EXACT, TS::NW>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
-            if (__ballot(hit) == 0ull) continue;
-            if (!hit)
-            {
-#pragma unroll
-                for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-            }
-            const float r = wave_sum12(gq, lane);
-            const int ri = (lane >> 2) & 15;
-            if ((lane & 3) == 0 && ri < 12) acc[ri][w][s] += r;
-        }
-
-        // prefetch the next chunk before this chunk's atomics are issued
-        const uint32_t fo = cur_o;
-        const bool fpm = cur.pm != 0ull;
-        const bool fsole = fpm && single_subtile<TS>(cur.bb, tx * TS::TX, ty * TS::TY, W, H);
-        const int64_t nend = end - 64;
-        {
-            const int64_t jl = nend - 1 - lane;
-            const bool valid = jl >= rx;
-            load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                       sy1, qpix);
-            if (valid) cur_o = orig[jl];
-        }
-        wave_sync();
-        // per-entry partial sums: an entry belongs to one tile, so at most the
-        // four waves of this workgroup add into its row; when the entry's bbox
-        // lies in this wave's sub-tile alone, the wave owns the (zeroed) row
-        // and writes it with plain stores
-        if (fsole)
-        {
-            float a[12];
-            bool any = false;
-#pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                a[i] = acc[i][w][lane];
-                any = any || (a[i] != 0.f);
-                acc[i][w][lane] = 0.f;
-            }
-            if (any)
-            {
-                float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)fo);
-                row[0] = make_float4(a[0], a[1], a[2], a[3]);
-                row[1] = make_float4(a[4], a[5], a[6], a[7]);
-                row[2] = make_float4(a[8], a[9], a[10], a[11]);
-            }
-        }
-        else if (fpm)
-        {
-            float* row = partial + 12 * (size_t)fo;
-#pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                const float a = acc[i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(row + i, a);
-                acc[i][w][lane] = 0.f;
-            }
-        }
-        end = nend;
-    }
-}
-
-// The reverse traversal driven by the forward's hit masks (see
-// composite_subtile's MASK): chunk c of the tile is read as the 64-bit mask of
-// entries that contributed to this lane's pixel, so a lane walks only its real
-// hits (no bbox, quadrant, ray-box or alpha rejections, no n_contrib cut) and
-// an entry's payload is loaded only when some pixel of the wave used it.  The
-// chunks are the forward's (aligned at range.x), walked from the wave's last
-// contributor down; within a chunk each lane takes its hits from the highest
-// bit down.  Sync slots, LDS accumulation and the per-entry row flush are as in
-// adjoint_subtile, with the slot test on real-hit counts.
-template <int NS, int SYNC_MIN, bool EXACT, typename TS = Tile16>
-__device__ inline void adjoint_masked(StageT<TS::NW>& st, float (&acc)[12][TS::NW][64], int w, int lane, int W, int H,
-                                      int tx, int ty, bool inside, size_t pid, size_t HW, const PixRay& ray,
-                                      uint2 range, float bg, const uint64_t* __restrict__ masks,
-                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
-                                      const float* __restrict__ centers, const float* __restrict__ lengths,
-                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
-                                      const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
-                                      const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-                                      const uint32_t* __restrict__ orig, float* __restrict__ partial)
+// Reverse traversal of a tile (backward.cu:175-527 for the colour and T
+// losses), driven by the forward's hit masks (composite_subtile's MASK): for
+// chunk c a lane reads the 64-bit mask of entries that contributed to its
+// pixel, so it walks only real hits (no bbox, quadrant, ray-box or alpha
+// rejections, no n_contrib cut).
+//
+// The tile's waves go through the chunks together, KW at a time (a window),
+// from the tile's last contributor down:
+//   1. each wave reads its masks for the window's chunks and marks the
+//      entries its pixels used;
+//   2. the workgroup stages the payload of the used entries, one entry per
+//      thread, once for all its waves;
+//   3. each wave walks its pixels' hits, entries in descending order.  Entries
+//      used by at least SYNC_MIN pixels of the wave are "sync slots": every
+//      lane reaches them at the same step, at their place in its own order,
+//      and their 12 terms are summed across the wave by one butterfly
+//      (wave_sum12).  Between sync slots each lane walks its own hits in
+//      lock-step.  Both add into the window's shared accumulators with LDS
+//      atomics (the waves of the tile meet there instead of in global memory);
+//   4. the workgroup writes one 12-float row per used entry into
+//      partial[orig] with plain stores: an entry belongs to one tile, so this
+//      is its only writer.  Rows of unused entries keep the caller's zeros.
+// EXACT selects the ray-box min/max form (see ray_aabb); it is uniform over
+// the workgroup.
+template <int NS, int SYNC_MIN, bool EXACT, typename TS = Tile16, int KW = BWD_WINDOW>
+__device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, bool inside, size_t pid, size_t HW,
+                                      const PixRay& ray, uint2 range, float bg, const uint64_t* __restrict__ masks,
+                                      const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
+                                      const float* __restrict__ lengths, const float* __restrict__ geos,
+                                      const float* __restrict__ rgbs, const float* __restrict__ out_T,
+                                      const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
+                                      const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
+                                      float* __restrict__ partial)
 {
-    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
-    uint32_t nc = 0;
-    if (inside)
-    {
-        T = out_T[pid];
-        nc = n_contrib[pid];
-        dp0 = dL_dcolor[pid];
-        dp1 = dL_dcolor[HW + pid];
-        dp2 = dL_dcolor[2 * HW + pid];
-        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
-    }
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k][w][lane] = 0.f;
-    const uint32_t rx = range.x;
-    int c = (int)((wave_max_u32(nc) + 63u) >> 6) - 1;
+    static_assert(TS::NW <= 16, "per-wave chunk counts are kept for at most 16 waves");
     constexpr size_t STRIDE = 64 * TS::NW;
-
-    // chunk c: this lane's hit mask, the entry-major transpose (bit p of lane
-    // l: pixel p used entry l) and entry l's payload if any pixel used it
-    EntryRegs cur;
-    uint64_t m = 0ull, er = 0ull;
-    uint32_t cur_o = 0;
-    auto fetch = [&](int cc) {
-        m = masks[(size_t)cc * STRIDE];
-        er = wave_transpose64(m, lane);
-        cur.pm = er;
-        if (er)
-        {
-            const uint32_t j = rx + ((uint32_t)cc << 6) + (uint32_t)lane;
-            const uint32_t val = vox_list[j];
-            const size_t v = val & VOX_ID_MASK;
-            cur.v = (uint32_t)v;
-            cur.bb = bboxes[v];
-            cur.c0 = centers[3 * v + 0];
-            cur.c1 = centers[3 * v + 1];
-            cur.c2 = centers[3 * v + 2];
-            cur.l = lengths[v];
-            cur.linv = 1.f / cur.l;
-            cur.g0 = *reinterpret_cast<const float4*>(geos + 8 * v);
-            cur.g1 = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
-            cur.r0 = rgbs[3 * v + 0];
-            cur.r1 = rgbs[3 * v + 1];
-            cur.r2 = rgbs[3 * v + 2];
-            cur_o = orig[j];
-        }
-    };
-    if (c >= 0) fetch(c);
-    while (c >= 0)
+    const int t = threadIdx.x;
+    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
+    uint32_t nc = 0;
+    if (inside)
     {
-        wave_sync();
-        stage_entry(st, w, lane, cur);
-        uint64_t sync = __ballot(__popcll(er) >= SYNC_MIN);
-        wave_sync();
-        uint64_t hm = m;
-        // this chunk's row-flush data, then the next chunk's loads, which
-        // complete while this chunk is traversed
-        const uint32_t fo = cur_o;
-        const bool fpm = er != 0ull;
-        const bool fsole = fpm && single_subtile<TS>(cur.bb, tx * TS::TX, ty * TS::TY, W, H);
-        if (c > 0) fetch(c - 1);
+        T = out_T[pid];
+        nc = n_contrib[pid];
+        dp0 = dL_dcolor[pid];
+        dp1 = dL_dcolor[HW + pid];
+        dp2 = dL_dcolor[2 * HW + pid];
+        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
+    }
+    const uint32_t my_chunks = (wave_max_u32(nc) + 63u) >> 6;
+    if (lane == 0) sh.nch[w] = my_chunks;
+    for (int i = t; i < KW * 12 * 64; i += TS::THREADS) (&sh.acc[0][0][0])[i] = 0.f;
+    for (int i = t; i < KW * 64; i += TS::THREADS) (&sh.used[0][0])[i] = 0u;
+    __syncthreads();
+    uint32_t C = 0;
+#pragma unroll
+    for (int i = 0; i < TS::NW; ++i) C = max(C, sh.nch[i]);
 
-        while (true)
+    for (int c0 = (int)C - 1; c0 >= 0; c0 -= KW)
+    {
+        const int nj = min(KW, c0 + 1);
+        // 1. this wave's masks, the entries they use, the sync slots
+        uint64_t m[KW], syncm[KW];
+#pragma unroll
+        for (int j = 0; j < KW; ++j)
         {
-            const int s = sync ? 63 - __builtin_clzll(sync) : -1;
-            // this lane's hits above the next sync slot, highest first
-            uint64_t seg = (s < 0) ? hm : (hm & ~((2ull << s) - 1ull));
-            if (s == 63) seg = 0ull;
-            hm &= ~seg;
-            while (seg)
+            m[j] = 0ull;
+            syncm[j] = 0ull;
+            if (j < nj && (uint32_t)(c0 - j) < my_chunks)
             {
-                const int l = 63 - __builtin_clzll(seg);
-                seg &= ~(1ull << l);
-                float gq[12];
-                if (bwd_hit<NS, EXACT, TS::NW>(st, w, l, ray, T, last_dT, dp0, dp1, dp2, gq))
-                {
-#pragma unroll
-                    for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
-                }
+                m[j] = masks[(size_t)(c0 - j) * STRIDE];
+                const uint64_t er = wave_transpose64(m[j], lane);
+                if (er) sh.used[j][lane] = 1u;
+                syncm[j] = __ballot(__popcll(er) >= SYNC_MIN);
             }
-            if (s < 0) break;
-            sync &= ~(1ull << s);
-            const bool mine = (hm >> s) & 1ull;
-            hm &= ~(1ull << s);
-            float gq[12];
-#pragma unroll
-            for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-            bool hit = false;
-            if (mine) hit = bwd_hit<NS, EXACT, TS::NW>(st, w, s, ray, T, last_dT, dp0, dp1, dp2, gq);
-            if (__ballot(hit) == 0ull) continue;
-            if (!hit)
+        }
+        __syncthreads();
+        // 2. stage the used entries' payload
+        for (int i = t; i < KW * 64; i += TS::THREADS)
+        {
+            const int j = i >> 6, l = i & 63;
+            if (j < nj && sh.used[j][l])
             {
+                const uint32_t e = range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l;
+                const size_t v = vox_list[e] & VOX_ID_MASK;
+                const float len = lengths[v];
+                EntryPay& p = sh.pay[j][l];
+                p.q[0] = make_float4(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], len);
+                p.q[1] = *reinterpret_cast<const float4*>(geos + 8 * v);
+                p.q[2] = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
+                p.q[3] = make_float4(rgbs[3 * v + 0], rgbs[3 * v + 1], rgbs[3 * v + 2], 1.f / len);
+                sh.orow[j][l] = orig[e];
+            }
+        }
+        __syncthreads();
+        // 3. walk this wave's hits
+#pragma unroll
+        for (int j = 0; j < KW; ++j)
+        {
+            if (j >= nj) break;
+            uint64_t hm = m[j], sync = syncm[j];
+            float (&acc)[12][64] = sh.acc[j];
+            while (true)
+            {
+                const int s = sync ? 63 - __builtin_clzll(sync) : -1;
+                // this lane's hits above the next sync slot, highest first
+                uint64_t seg = (s < 0) ? hm : (hm & ~((2ull << s) - 1ull));
+                if (s == 63) seg = 0ull;
+                hm &= ~seg;
+                while (seg)
+                {
+                    const int l = 63 - __builtin_clzll(seg);
+                    seg &= ~(1ull << l);
+                    float gq[12];
+                    if (bwd_hit<NS, EXACT>(sh.pay[j][l], ray, T, last_dT, dp0, dp1, dp2, gq))
+                    {
+#pragma unroll
+                        for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][l], gq[i]);
+                    }
+                }
+                if (s < 0) break;
+                sync &= ~(1ull << s);
+                const bool mine = (hm >> s) & 1ull;
+                hm &= ~(1ull << s);
+                float gq[12];
 #pragma unroll
                 for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-            }
-            const float r = wave_sum12(gq, lane);
-            const int ri = (lane >> 2) & 15;
-            if ((lane & 3) == 0 && ri < 12) acc[ri][w][s] += r;
-        }
-
-        wave_sync();
-        if (fsole)
-        {
-            float a[12];
-            bool any = false;
+                bool hit = false;
+                if (mine) hit = bwd_hit<NS, EXACT>(sh.pay[j][s], ray, T, last_dT, dp0, dp1, dp2, gq);
+                if (__ballot(hit) == 0ull) continue;
+                if (!hit)
+                {
 #pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                a[i] = acc[i][w][lane];
-                any = any || (a[i] != 0.f);
-                acc[i][w][lane] = 0.f;
+                    for (int i = 0; i < 12; ++i) gq[i] = 0.f;
+                }
+                const float r = wave_sum12(gq, lane);
+                const int ri = (lane >> 2) & 15;
+                if ((lane & 3) == 0 && ri < 12) atomicAdd(&acc[ri][s], r);
             }
-            if (any)
+        }
+        __syncthreads();
+        // 4. one row per used entry, then reset the window
+        for (int i = t; i < KW * 64; i += TS::THREADS)
+        {
+            const int j = i >> 6, l = i & 63;
+            if (j < nj && sh.used[j][l])
             {
-                float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)fo);
+                float a[12];
+#pragma unroll
+                for (int k = 0; k < 12; ++k)
+                {
+                    a[k] = sh.acc[j][k][l];
+                    sh.acc[j][k][l] = 0.f;
+                }
+                float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)sh.orow[j][l]);
                 row[0] = make_float4(a[0], a[1], a[2], a[3]);
                 row[1] = make_float4(a[4], a[5], a[6], a[7]);
                 row[2] = make_float4(a[8], a[9], a[10], a[11]);
+                sh.used[j][l] = 0u;
             }
         }
-        else if (fpm)
-        {
-            float* row = partial + 12 * (size_t)fo;
-#pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                const float a = acc[i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(row + i, a);
-                acc[i][w][lane] = 0.f;
-            }
-        }
-        --c;
+        __syncthreads();
     }
 }
 

@@ -85,15 +85,12 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
     const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
-    __shared__ Stage st;
-    __shared__ float acc[12][NWAVE][64];
+    __shared__ WindowLDS<BWD_WINDOW> sh;
     const int tile = xcd_tile(blockIdx.x, per_xcd);
     if (tile >= T_tiles) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
-    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
-    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
-    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+    const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
     const bool inside = px < W && py < H;
     const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
     const uint2 range = ranges[tile];
@@ -101,9 +98,8 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const size_t pid = inside ? (size_t)py * W + px : 0;
     if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
-    adjoint_masked<NS, SYNC_MIN, EXACT>(st, acc, w, lane, W, H, tx, ty, inside, pid, HW, ray, range, bg, masks,
-                                        vox_list, bboxes, centers, lengths, geos, rgbs, out_T, n_contrib, dL_dcolor,
-                                        dL_dT, orig, partial);
+    adjoint_window<NS, SYNC_MIN, EXACT>(sh, w, lane, inside, pid, HW, ray, range, bg, masks, vox_list, centers, lengths,
+                                        geos, rgbs, out_T, n_contrib, dL_dcolor, dL_dT, orig, partial);
 }
 
 // Per-voxel gradient = sum of the partial rows of its entries, which are
