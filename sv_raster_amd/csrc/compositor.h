@@ -10,15 +10,11 @@
 
 namespace svr {
 
-// Sync-slot thresholds (pixels of a wave that use an entry): bbox coverage
-// in the traversal that tests candidates, real hits in the one driven by the
-// forward's hit masks (measured on the 1080p scene: 8 is best there, the
-// remaining per-lane LDS atomics conflict less than wave-wide sums cost).
-#ifndef BWD_SYNC_MIN
-#define BWD_SYNC_MIN 24
-#endif
+// Sync-slot threshold of the backward: an entry used by at least this many
+// pixels of a wave is summed across the wave by a butterfly instead of
+// per-lane LDS atomics (measured on the 1080p scene: 12 beats 4, 8 and 16).
 #ifndef BWD_SYNC_MIN_MASKED
-#define BWD_SYNC_MIN_MASKED 8
+#define BWD_SYNC_MIN_MASKED 12
 #endif
 
 // Binning tile of TX x TY pixels; each wave of its workgroup owns one 8x8
@@ -312,7 +308,10 @@ struct WindowLDS
     uint32_t orow[KW][64];
     uint32_t nch[16];  // per wave: chunks that hold its pixels' contributors
 };
-constexpr int BWD_WINDOW = 4;
+#ifndef BWD_WINDOW_CHUNKS
+#define BWD_WINDOW_CHUNKS 4
+#endif
+constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 
 // Reverse traversal of a tile (backward.cu:175-527 for the colour and T
 // losses), driven by the forward's hit masks (composite_subtile's MASK): for
