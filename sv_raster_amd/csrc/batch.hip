@@ -398,19 +398,20 @@ __device__ inline void batch_bwd_tile(WindowLDS<BWD_WINDOW>& sh, uint32_t g, int
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
     const svr_view& vw = views[t.u];
-    const PixRay ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
-    const bool need_exact = __syncthreads_or(!ray_inv_finite(ray)) != 0;
+    PlainHit<NS, EXACT> hit;
+    hit.ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
+    const bool need_exact = __syncthreads_or(!ray_inv_finite(hit.ray)) != 0;
     if (!EXACT && need_exact) return;  // rendered by the exact launch
     const size_t vo = (size_t)t.u * (size_t)N;
     const size_t pb = (size_t)vw.pix_base;
     const size_t HW = (size_t)t.H * t.W;
     const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
+    const uint32_t nc = hit.init(t.inside, pid, HW, vw.bg_color, out_T + pb, n_contrib + pb, dL_dcolor + 3 * pb,
+                                 dL_dT + pb);
     const uint2 range = ranges[g];
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
-    adjoint_window<NS, BWD_SYNC_MIN_MASKED, EXACT, TS>(sh, w, lane, t.inside, pid, HW, ray, range, vw.bg_color, masks,
-                                                       vox_list, centers, lengths, geos, rgbs + 3 * vo, out_T + pb,
-                                                       n_contrib + pb, dL_dcolor + 3 * pb, dL_dT + pb, orig,
-                                                       partial);
+    adjoint_window<BWD_SYNC_MIN_MASKED, TS, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths,
+                                                        geos, rgbs + 3 * vo, orig, partial);
 }
 
 template <int NS, typename TS>

@@ -68,27 +68,6 @@ __device__ inline uint64_t select_q(const uint64_t* qpix, uint32_t q)
     return s;
 }
 
-// Number of 8-pixel cells k (cell k = [base + 8k, base + 8k + 7] clipped to
-// [0, limit]) among the first `cells` that the inclusive range [lo, hi] meets.
-__device__ inline int cells_met(int lo, int hi, int base, int cells, int limit)
-{
-    const int first = max(lo, base);
-    const int last = min(min(hi, base + 8 * cells - 1), limit);
-    if (first > last) return 0;
-    return (last - base) / 8 - (first - base) / 8 + 1;
-}
-
-// True when the packed bbox (inclusive) overlaps exactly one of the 8x8
-// sub-tiles of the TS tile at (tx0, ty0): then only that sub-tile's wave can
-// produce gradient terms for the entry.
-template <typename TS>
-__device__ inline bool single_subtile(uint2 bb, int tx0, int ty0, int W, int H)
-{
-    const int bxmin = (int)(bb.x >> 16), bymin = (int)(bb.x & 0xFFFFu);
-    const int bxmax = (int)(bb.y >> 16), bymax = (int)(bb.y & 0xFFFFu);
-    return cells_met(bxmin, bxmax, tx0, TS::WX, W - 1) * cells_met(bymin, bymax, ty0, TS::WY, H - 1) == 1;
-}
-
 __device__ inline uint32_t wave_max_u32(uint32_t x)
 {
 #pragma unroll
@@ -296,6 +275,34 @@ __device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, f
     return true;
 }
 
+// A pixel of the plain adjoint: its ray and running state; one call per hit,
+// in descending entry order.
+template <int NS, bool EXACT>
+struct PlainHit
+{
+    PixRay ray;
+    float T, last_dT, dp0, dp1, dp2;
+
+    // backward.cu:160-180: state after the pixel's last contributor
+    __device__ inline uint32_t init(bool inside, size_t pid, size_t HW, float bg, const float* __restrict__ out_T,
+                                    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
+                                    const float* __restrict__ dL_dT)
+    {
+        T = 0.f; dp0 = 0.f; dp1 = 0.f; dp2 = 0.f; last_dT = 0.f;
+        if (!inside) return 0u;
+        T = out_T[pid];
+        dp0 = dL_dcolor[pid];
+        dp1 = dL_dcolor[HW + pid];
+        dp2 = dL_dcolor[2 * HW + pid];
+        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
+        return n_contrib[pid];
+    }
+    __device__ inline bool operator()(const EntryPay& p, float* gq)
+    {
+        return bwd_hit<NS, EXACT>(p, ray, T, last_dT, dp0, dp1, dp2, gq);
+    }
+};
+
 // Workgroup-shared state of the windowed adjoint: for the KW chunks of a
 // window, the staged payload, the per-entry gradient accumulators, whether
 // some pixel of the tile used the entry, and the entry's emission-order row.
@@ -335,32 +342,18 @@ constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 //   4. the workgroup writes one 12-float row per used entry into
 //      partial[orig] with plain stores: an entry belongs to one tile, so this
 //      is its only writer.  Rows of unused entries keep the caller's zeros.
-// EXACT selects the ray-box min/max form (see ray_aabb); it is uniform over
-// the workgroup.
-template <int NS, int SYNC_MIN, bool EXACT, typename TS = Tile16, int KW = BWD_WINDOW>
-__device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, bool inside, size_t pid, size_t HW,
-                                      const PixRay& ray, uint2 range, float bg, const uint64_t* __restrict__ masks,
-                                      const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
-                                      const float* __restrict__ lengths, const float* __restrict__ geos,
-                                      const float* __restrict__ rgbs, const float* __restrict__ out_T,
-                                      const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
-                                      const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-                                      float* __restrict__ partial)
+// HIT holds the pixel's ray and running state and computes one hit's 12
+// terms (PlainHit here; the adjoint with the optional terms has its own).
+template <int SYNC_MIN, typename TS, int KW, typename HIT>
+__device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32_t nc, HIT& hit_fn, uint2 range,
+                                      const uint64_t* __restrict__ masks, const uint32_t* __restrict__ vox_list,
+                                      const float* __restrict__ centers, const float* __restrict__ lengths,
+                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                      const uint32_t* __restrict__ orig, float* __restrict__ partial)
 {
     static_assert(TS::NW <= 16, "per-wave chunk counts are kept for at most 16 waves");
     constexpr size_t STRIDE = 64 * TS::NW;
     const int t = threadIdx.x;
-    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
-    uint32_t nc = 0;
-    if (inside)
-    {
-        T = out_T[pid];
-        nc = n_contrib[pid];
-        dp0 = dL_dcolor[pid];
-        dp1 = dL_dcolor[HW + pid];
-        dp2 = dL_dcolor[2 * HW + pid];
-        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
-    }
     const uint32_t my_chunks = (wave_max_u32(nc) + 63u) >> 6;
     if (lane == 0) sh.nch[w] = my_chunks;
     for (int i = t; i < KW * 12 * 64; i += TS::THREADS) (&sh.acc[0][0][0])[i] = 0.f;
@@ -426,7 +419,7 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, bool i
                     const int l = 63 - __builtin_clzll(seg);
                     seg &= ~(1ull << l);
                     float gq[12];
-                    if (bwd_hit<NS, EXACT>(sh.pay[j][l], ray, T, last_dT, dp0, dp1, dp2, gq))
+                    if (hit_fn(sh.pay[j][l], gq))
                     {
 #pragma unroll
                         for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][l], gq[i]);
@@ -440,7 +433,7 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, bool i
 #pragma unroll
                 for (int i = 0; i < 12; ++i) gq[i] = 0.f;
                 bool hit = false;
-                if (mine) hit = bwd_hit<NS, EXACT>(sh.pay[j][s], ray, T, last_dT, dp0, dp1, dp2, gq);
+                if (mine) hit = hit_fn(sh.pay[j][s], gq);
                 if (__ballot(hit) == 0ull) continue;
                 if (!hit)
                 {

@@ -92,14 +92,16 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const int tx = tile % gx, ty = tile / gx;
     const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
     const bool inside = px < W && py < H;
-    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    PlainHit<NS, EXACT> hit;
+    hit.ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
     const uint2 range = ranges[tile];
     const size_t HW = (size_t)H * W;
     const size_t pid = inside ? (size_t)py * W + px : 0;
-    if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
+    if ((__syncthreads_or(!ray_inv_finite(hit.ray)) != 0) != EXACT) return;
+    const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT);
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
-    adjoint_window<NS, SYNC_MIN, EXACT>(sh, w, lane, inside, pid, HW, ray, range, bg, masks, vox_list, centers, lengths,
-                                        geos, rgbs, out_T, n_contrib, dL_dcolor, dL_dT, orig, partial);
+    adjoint_window<SYNC_MIN, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths, geos,
+                                                 rgbs, orig, partial);
 }
 
 // Per-voxel gradient = sum of the partial rows of its entries, which are

@@ -333,25 +333,25 @@ struct ExtCoef
 
 // Adjoint of one (pixel, entry) hit with the optional terms
 // (backward.cu:259-527, in the reference's order of accumulation).
-template <int NS, bool EXACT, int NW>
-__device__ inline bool bwd_hit_ext(const StageT<NW>& st, int w, int l, const PixRay& ray, const ExtCoef& k,
-                                   ExtPix& p, float& T, float& last_dT, float dp0, float dp1, float dp2, float* gq)
+template <int NS, bool EXACT>
+__device__ inline bool bwd_hit_ext(const EntryPay& pe, const PixRay& ray, const ExtCoef& k, ExtPix& p, float& T,
+                                   float& last_dT, float dp0, float dp1, float dp2, float* gq)
 {
-    const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
+    const float4 a0 = pe.q[0], a1 = pe.q[1], a2 = pe.q[2], a3 = pe.q[3];
+    const float c0 = a0.x, c1 = a0.y, c2 = a0.z, len = a0.w;
     float a, b;
     ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
     if (a > b) return false;
-    float g[8], dI[8], lv[NS], edI[NS][8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-    const float linv = st.linv[w][l];
+    const float g[8] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+    float dI[8], lv[NS], edI[NS][8];
+    const float linv = a3.w;
     const float vol = vox_samples<NS, true>(c0, c1, c2, len, linv, g, a, b, ray, lv, edI, dI);
     const float e = svr_expf(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     if (alpha < MIN_ALPHA) return false;
     T = T / (1.f - alpha);
     const float pt_w = alpha * T;
-    const float col[3] = {st.rgb[0][w][l], st.rgb[1][w][l], st.rgb[2][w][l]};
+    const float col[3] = {a3.x, a3.y, a3.z};
     float dpt = __builtin_fmaf(dp2, col[2], __builtin_fmaf(dp1, col[1], dp0 * col[0]));
     float dg[8];
 #pragma unroll
@@ -454,31 +454,27 @@ __device__ inline bool bwd_hit_ext(const StageT<NW>& st, int w, int l, const Pix
     return true;
 }
 
-// Reverse traversal of one wave's sub-tile with the optional terms: each lane
-// walks its own hits and accumulates with LDS atomics (the distortion
-// prefix / suffix sums make every hit depend on the previous one of the same
-// pixel, so there are no wave-wide sync slots here); the per-entry partial
-// rows are flushed as in the plain adjoint.
+// A pixel of the adjoint with the optional terms (the per-hit work of
+// bwd_hit_ext; traversal, sync slots and row writes are adjoint_window's).
 template <int NS, bool EXACT>
-__device__ inline void adjoint_ext(Stage& st, float (&acc)[12][NWAVE][64], int w, int lane, int W, int H, int tx, int ty,
-                                   int sx0, int sy0, int sx1, int sy1, bool inside, size_t pid, size_t HW,
-                                   const PixRay& ray, uint2 range, float bg, const ExtCoef& k,
-                                   const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
-                                   const float* __restrict__ centers, const float* __restrict__ lengths,
-                                   const float* __restrict__ geos, const float* __restrict__ rgbs,
-                                   const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
-                                   const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-                                   const float* __restrict__ dL_ddepth, const float* __restrict__ dL_dnormal,
-                                   const float* __restrict__ gt, const float* __restrict__ out_depth,
-                                   const uint32_t* __restrict__ orig, float* __restrict__ partial)
+struct ExtHit
 {
-    float T = 0.f, dp0 = 0.f, dp1 = 0.f, dp2 = 0.f, last_dT = 0.f;
-    uint32_t nc = 0;
-    ExtPix p{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (inside)
+    PixRay ray;
+    ExtCoef k;
+    ExtPix p;
+    float T, last_dT, dp0, dp1, dp2;
+
+    // backward.cu:160-211: state after the pixel's last contributor
+    __device__ inline uint32_t init(bool inside, size_t pid, size_t HW, float bg, const float* __restrict__ out_T,
+                                    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
+                                    const float* __restrict__ dL_dT, const float* __restrict__ dL_ddepth,
+                                    const float* __restrict__ dL_dnormal, const float* __restrict__ gt,
+                                    const float* __restrict__ out_depth)
     {
+        T = 0.f; dp0 = 0.f; dp1 = 0.f; dp2 = 0.f; last_dT = 0.f;
+        p = ExtPix{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (!inside) return 0u;
         T = out_T[pid];
-        nc = n_contrib[pid];
         dp0 = dL_dcolor[pid];
         dp1 = dL_dcolor[HW + pid];
         dp2 = dL_dcolor[2 * HW + pid];
@@ -501,120 +497,44 @@ __device__ inline void adjoint_ext(Stage& st, float (&acc)[12][NWAVE][64], int w
             p.prefix_wm = out_depth[HW + pid];
             p.prefix_w = 1.f - T;
         }
+        return n_contrib[pid];
     }
-#pragma unroll
-    for (int i = 0; i < 12; ++i) acc[i][w][lane] = 0.f;
-    uint64_t qpix[8];
-#pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(inside && nc > 0 && ray.pq == q);
-    const int64_t rx = (int64_t)range.x;
-    int64_t end = rx + wave_max_u32(nc);
-
-    EntryRegs cur;
-    uint32_t cur_o = 0;
+    __device__ inline bool operator()(const EntryPay& pe, float* gq)
     {
-        const int64_t jl = end - 1 - lane;
-        const bool valid = jl >= rx;
-        load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
-                   qpix);
-        if (valid) cur_o = orig[jl];
+        return bwd_hit_ext<NS, EXACT>(pe, ray, k, p, T, last_dT, dp0, dp1, dp2, gq);
     }
-    while (end > rx)
-    {
-        wave_sync();
-        stage_entry(st, w, lane, cur);
-        wave_sync();
-        uint64_t hm = wave_transpose64(cur.pm, lane);
-        const int64_t cut = end - rx - (int64_t)nc;
-        if (cut >= 64) hm = 0ull;
-        else if (cut > 0) hm &= ~((1ull << cut) - 1ull);
-        while (hm)
-        {
-            const int l = __builtin_ctzll(hm);
-            hm &= hm - 1ull;
-            float gq[12];
-            if (bwd_hit_ext<NS, EXACT, NWAVE>(st, w, l, ray, k, p, T, last_dT, dp0, dp1, dp2, gq))
-            {
-#pragma unroll
-                for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][w][l], gq[i]);
-            }
-        }
-
-        const uint32_t fo = cur_o;
-        const bool fpm = cur.pm != 0ull;
-        const bool fsole = fpm && single_subtile<Tile16>(cur.bb, tx * TILE_X, ty * TILE_Y, W, H);
-        const int64_t nend = end - 64;
-        {
-            const int64_t jl = nend - 1 - lane;
-            const bool valid = jl >= rx;
-            load_entry(cur, valid, valid ? vox_list[jl] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                       sy1, qpix);
-            if (valid) cur_o = orig[jl];
-        }
-        wave_sync();
-        if (fsole)
-        {
-            float a[12];
-            bool any = false;
-#pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                a[i] = acc[i][w][lane];
-                any = any || (a[i] != 0.f);
-                acc[i][w][lane] = 0.f;
-            }
-            if (any)
-            {
-                float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)fo);
-                row[0] = make_float4(a[0], a[1], a[2], a[3]);
-                row[1] = make_float4(a[4], a[5], a[6], a[7]);
-                row[2] = make_float4(a[8], a[9], a[10], a[11]);
-            }
-        }
-        else if (fpm)
-        {
-            float* row = partial + 12 * (size_t)fo;
-#pragma unroll
-            for (int i = 0; i < 12; ++i)
-            {
-                const float a = acc[i][w][lane];
-                if (a != 0.f) unsafeAtomicAdd(row + i, a);
-                acc[i][w][lane] = 0.f;
-            }
-        }
-        end = nend;
-    }
-}
+};
 
 template <int NS, bool EXACT>
 __global__ __launch_bounds__(256) void render_bwd_ext_kernel(
     int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, ExtCoef k, const uint2* __restrict__ ranges,
-    const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float* __restrict__ centers,
-    const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
-    const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
-    const float* __restrict__ dL_dT, const float* __restrict__ dL_ddepth, const float* __restrict__ dL_dnormal,
-    const float* __restrict__ gt, const float* __restrict__ out_depth, const uint32_t* __restrict__ orig,
+    const uint32_t* __restrict__ vox_list, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
+    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+    const float* __restrict__ dL_ddepth, const float* __restrict__ dL_dnormal, const float* __restrict__ gt,
+    const float* __restrict__ out_depth, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
     float* __restrict__ partial)
 {
-    __shared__ Stage st;
-    __shared__ float acc[12][NWAVE][64];
+    __shared__ WindowLDS<BWD_WINDOW> sh;
     const int tile = xcd_tile(blockIdx.x, per_xcd);
     if (tile >= T_tiles) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
-    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
-    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
-    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+    const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
     const bool inside = px < W && py < H;
-    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    ExtHit<NS, EXACT> hit;
+    hit.ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    hit.k = k;
     const uint2 range = ranges[tile];
     const size_t HW = (size_t)H * W;
     const size_t pid = inside ? (size_t)py * W + px : 0;
-    if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
-    adjoint_ext<NS, EXACT>(st, acc, w, lane, W, H, tx, ty, sx0, sy0, sx1, sy1, inside, pid, HW, ray, range, bg, k,
-                           vox_list, bboxes, centers, lengths, geos, rgbs, out_T, n_contrib, dL_dcolor, dL_dT,
-                           dL_ddepth, dL_dnormal, gt, out_depth, orig, partial);
+    if ((__syncthreads_or(!ray_inv_finite(hit.ray)) != 0) != EXACT) return;
+    const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt,
+                                 out_depth);
+    const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
+    adjoint_window<BWD_SYNC_MIN_MASKED, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers,
+                                                            lengths, geos, rgbs, orig, partial);
 }
 
 template <int NS>
@@ -655,12 +575,12 @@ int launch_bwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
     render_bwd_ext_kernel<NS, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
-        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, g.bbox, centers, lengths,
-        geos, rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, partial);
+        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
+        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     render_bwd_ext_kernel<NS, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
-        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, g.bbox, centers, lengths,
-        geos, rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, partial);
+        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
+        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);
