@@ -102,6 +102,10 @@ __global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* __restrict__ co
     if (t == 0) counts[(int64_t)RADIX * nb + blockIdx.x] = carry_s;
 }
 
+// Ranks each key of the block (stable within its digit), reorders the block's
+// keys in LDS by digit, then writes each digit's run to its global position
+// with consecutive threads on consecutive addresses (coalesced stores instead
+// of one scattered 12-byte store per key).
 template <bool HAS_VALS>
 __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
     const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
@@ -111,8 +115,10 @@ __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
     const int64_t n = sort_count(n_cap, n_dev);
     constexpr int NW = SORT_THREADS / 64;
     __shared__ uint32_t wcnt[NW][RADIX];
-    __shared__ uint32_t gbase[RADIX];
+    __shared__ uint32_t gbase[RADIX];  // global start of this block's run of digit d
+    __shared__ uint32_t lbase[RADIX];  // start of digit d in the block's reordered keys
     __shared__ uint32_t wsum[NW];
+    __shared__ uint64_t lkeys[SORT_TILE];  // reordered keys, then (as 32-bit words) values
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
 
     // digit base = exclusive prefix over digit totals + this block's row offset
@@ -135,7 +141,8 @@ __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
     }
     __syncthreads();
 
-    const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)w * (SORT_ITEMS * 64);
+    const int64_t blk0 = (int64_t)blockIdx.x * SORT_TILE;
+    const int64_t base = blk0 + (int64_t)w * (SORT_ITEMS * 64);
     uint64_t k[SORT_ITEMS];
     uint32_t v[SORT_ITEMS];
     uint32_t rk[SORT_ITEMS];
@@ -156,27 +163,74 @@ __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
         if (valid && lane == 63 - __clzll(m)) wcnt[w][d] = cnt + (uint32_t)__popcll(m);
     }
     __syncthreads();
-    {
-        uint32_t s = 0;
+    // per digit: exclusive offsets of the waves, and the block's digit total
+    uint32_t dtot = 0;
 #pragma unroll
-        for (int i = 0; i < NW; ++i)
+    for (int i = 0; i < NW; ++i)
+    {
+        const uint32_t c = wcnt[i][t];
+        wcnt[i][t] = dtot;
+        dtot += c;
+    }
+    // exclusive scan of the digit totals over the block (digit = thread)
+    {
+        uint32_t s = dtot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
         {
-            const uint32_t c = wcnt[i][t];
-            wcnt[i][t] = s;
-            s += c;
+            const uint32_t y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
         }
+        __syncthreads();
+        if (lane == 63) wsum[w] = s;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int q = 0; q < w; ++q) pre += wsum[q];
+        lbase[t] = pre + s - dtot;
     }
     __syncthreads();
+    uint32_t lp[SORT_ITEMS];
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r)
     {
         const int64_t idx = base + r * 64 + lane;
-        if (idx < n)
+        const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
+        lp[r] = lbase[d] + wcnt[w][d] + rk[r];
+        if (idx < n) lkeys[lp[r]] = k[r];
+    }
+    __syncthreads();
+    const int nk = (int)min((int64_t)SORT_TILE, n - blk0);
+    // the digit of reordered position i, kept for the values pass
+    uint32_t dd[SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r)
+    {
+        const int i = t + r * SORT_THREADS;
+        dd[r] = 0;
+        if (i < nk)
         {
-            const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
-            const uint32_t dst = gbase[d] + wcnt[w][d] + rk[r];
-            keys_out[dst] = k[r];
-            if (HAS_VALS) vals_out[dst] = v[r];
+            const uint64_t key = lkeys[i];
+            const uint32_t d = (uint32_t)(key >> shift) & dmask;
+            dd[r] = d;
+            keys_out[gbase[d] + (uint32_t)i - lbase[d]] = key;
+        }
+    }
+    if (HAS_VALS)
+    {
+        uint32_t* lv = reinterpret_cast<uint32_t*>(lkeys);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < SORT_ITEMS; ++r)
+        {
+            const int64_t idx = base + r * 64 + lane;
+            if (idx < n) lv[lp[r]] = v[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < SORT_ITEMS; ++r)
+        {
+            const int i = t + r * SORT_THREADS;
+            if (i < nk) vals_out[gbase[dd[r]] + (uint32_t)i - lbase[dd[r]]] = lv[i];
         }
     }
 }
