@@ -72,9 +72,11 @@ int svr_rasterize_preprocess(int32_t width, int32_t height, float tanfovx, float
                              const float* vox_centers, const float* vox_lengths, int32_t* out_n_duplicates,
                              void* geom_state, svr_stream_t stream);
 
-/* Inclusive scan of n_duplicates and the number of rendered entries R
- * (forward.cu:649-663).  Synchronises `stream` to return R to the host, as
- * the reference does before sizing its binning buffer. */
+/* The number of rendered entries R, the sum of n_duplicates (forward.cu:
+ * 649-663, whose inclusive scan places each voxel's entries; here the
+ * forward places them in rank order itself).  Synchronises `stream` to
+ * return R to the host, as the reference does before sizing its binning
+ * buffer. */
 int svr_num_rendered(int64_t n_voxels, void* geom_state, svr_stream_t stream, int64_t* num_rendered_host);
 
 /* Builds the rank-order table of an octree: the position of every
@@ -273,7 +275,7 @@ typedef struct svr_view {
  * tile ranges are relative to the batch's tiles. */
 enum { SVR_TILE_16X16 = 0, SVR_TILE_8X8 = 1, SVR_TILE_32X8 = 2 };
 
-/* Sizes: geometry state of V views (its scan runs over V * N entries, so
+/* Sizes: geometry state of V views (its count runs over V * N entries, so
  * svr_num_rendered(V * N, ...) returns the batch's total entry count R), the
  * batch image state for T_total tiles, P_total pixels and R entries (the
  * forward's hit masks, as in svr_image_state_size), and the backward
@@ -323,7 +325,7 @@ int svr_batch_rasterize_backward(int32_t n_samp, int32_t n_views, const svr_view
                                  float* subdiv_p_bw, svr_stream_t stream);
 
 /* Capture-safe forms for hipGraph replay of a batch training step: no host
- * synchronisation.  svr_scan_entries runs the scan of svr_num_rendered and
+ * synchronisation.  svr_scan_entries runs the sum of svr_num_rendered and
  * leaves the entry count on the device.  The capacity forms take buffers
  * sized for `capacity` entries (binning state, backward workspace) and read
  * the count on the device; when it exceeds the capacity, *overflow_dev

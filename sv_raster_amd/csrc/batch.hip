@@ -129,73 +129,59 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
     qbits_out[p] = bits;
 }
 
+// forward.cu:523-577 per (view, voxel) pair: each quadrant's run at its
+// rank-ordered offset within the view (recorded in qoff); pairs with many
+// entries are queued (as u * N + v) for batch_duplicate_big_kernel.  Runs that
+// would end beyond `cap` (a frame over a fixed capacity) emit nothing.
 template <typename TS>
-__device__ inline void emit_entries(uint32_t pair_v, uint32_t off, uint2 bb, uint32_t bits, int gx, int gy,
-                                    uint32_t tile_base, int pos_bits, const uint32_t* __restrict__ order_pos,
-                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+__global__ __launch_bounds__(256) void batch_duplicate_kernel(GeomState g, const svr_view* __restrict__ views,
+                                                              int64_t N, int pos_bits,
+                                                              const uint32_t* __restrict__ order_pos,
+                                                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                              uint32_t cap)
 {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
+    const uint32_t nd = g.n_dup[p];
+    if (nd == 0) return;
+    const bool large = nd > BIG_DUP;
+    const svr_view& vw = views[blockIdx.y];
+    const int gx = tiles_along(vw.width, TS::TX), gy = tiles_along(vw.height, TS::TY);
     int txmin, tymin, txmax, tymax;
-    rect_of<TS::TX, TS::TY>(bb, gx, gy, txmin, tymin, txmax, tymax);
-    for (uint32_t q = 0; q < 8; ++q)
+    rect_of<TS::TX, TS::TY>(g.bbox[p], gx, gy, txmin, tymin, txmax, tymax);
+    const uint32_t bits = g.qbits[p];
+    const uint32_t tiles = nd / (uint32_t)__popc(bits);
+    const uint32_t* blk = g.pos_off + 8 * (size_t)blockIdx.y * (size_t)N;
+    for (uint32_t b = bits; b; b &= b - 1u)
     {
-        if (!(bits & (1u << q))) continue;
-        const uint64_t pos = order_pos[8 * (size_t)pair_v + q];
+        const uint32_t q = (uint32_t)__builtin_ctz(b);
+        const uint64_t pos = order_pos[8 * v + q];
+        uint32_t off = blk[pos];
+        set_run_offset(g, p, bits, q, off);
+        if (large || (uint64_t)off + tiles > cap) continue;
         for (int y = tymin; y <= tymax; ++y)
             for (int x = txmin; x <= txmax; ++x)
             {
-                const uint32_t g = tile_base + (uint32_t)y * (uint32_t)gx + (uint32_t)x;
+                const uint32_t g = (uint32_t)vw.tile_base + (uint32_t)y * (uint32_t)gx + (uint32_t)x;
                 keys[off] = ((uint64_t)g << pos_bits) | pos;
                 vals[off] = off;
                 ++off;
             }
     }
-}
-
-// forward.cu:523-577 per (view, voxel) pair; pairs with many entries are
-// queued (as u * N + v) for batch_duplicate_big_kernel.  Pairs whose entries
-// would end beyond `cap` (a frame over a fixed capacity) emit nothing.
-template <typename TS>
-__global__ __launch_bounds__(256) void batch_duplicate_kernel(const svr_view* __restrict__ views, int64_t N,
-                                                              int pos_bits, const uint32_t* __restrict__ n_dup,
-                                                              const uint32_t* __restrict__ scan,
-                                                              const uint2* __restrict__ bboxes,
-                                                              const uint32_t* __restrict__ qbits,
-                                                              const uint32_t* __restrict__ order_pos,
-                                                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                              uint32_t* __restrict__ big_list,
-                                                              uint32_t* __restrict__ big_count, uint32_t cap)
-{
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
-    const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
-    const uint32_t nd = n_dup[p];
-    if (nd == 0 || scan[p] > cap) return;
-    if (nd > BIG_DUP)
-    {
-        queue_large(big_list, big_count, (int64_t)gridDim.y * N, nd, (uint32_t)p);
-        return;
-    }
-    const svr_view& vw = views[blockIdx.y];
-    emit_entries<TS>((uint32_t)v, scan[p] - nd, bboxes[p], qbits[p], tiles_along(vw.width, TS::TX),
-                     tiles_along(vw.height, TS::TY),
-                 (uint32_t)vw.tile_base, pos_bits, order_pos, keys, vals);
+    if (large) queue_large(g.big_list, g.counters + 2, (int64_t)gridDim.y * N, nd, (uint32_t)p);
 }
 
 template <typename TS>
-__global__ __launch_bounds__(256) void batch_duplicate_big_kernel(int V, const svr_view* __restrict__ views, int64_t N,
-                                                                  int pos_bits, const uint32_t* __restrict__ n_dup,
-                                                                  const uint32_t* __restrict__ scan,
-                                                                  const uint2* __restrict__ bboxes,
-                                                                  const uint32_t* __restrict__ qbits,
+__global__ __launch_bounds__(256) void batch_duplicate_big_kernel(GeomState g, int V, const svr_view* __restrict__ views,
+                                                                  int64_t N, int pos_bits,
                                                                   const uint32_t* __restrict__ order_pos,
                                                                   uint64_t* __restrict__ keys,
-                                                                  uint32_t* __restrict__ vals,
-                                                                  const uint32_t* __restrict__ big_list,
-                                                                  const uint32_t* __restrict__ big_count)
+                                                                  uint32_t* __restrict__ vals, uint32_t cap)
 {
     // large pairs one workgroup each, then the huge ones (queued from the end
     // of big_list) spread over every workgroup
-    const uint32_t count = big_count[0], huge = big_count[1];
+    const uint32_t count = g.counters[2], huge = g.counters[3];
     const int64_t n_items = (int64_t)V * N;
     const uint32_t K = huge ? max(1u, gridDim.x / huge) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
@@ -203,30 +189,37 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(int V, const s
     for (uint32_t it = blockIdx.x; it < (pass == 0 ? count : huge * K); it += gridDim.x)
     {
         const uint32_t i = pass == 0 ? it : it % huge;
-        const uint32_t p = pass == 0 ? big_list[i] : big_list[n_items - 1 - i];
+        const uint32_t p = pass == 0 ? g.big_list[i] : g.big_list[n_items - 1 - i];
         const uint32_t u = (uint32_t)(p / (uint64_t)N), v = (uint32_t)(p - (uint64_t)u * (uint64_t)N);
         const svr_view& vw = views[u];
         const int gx = tiles_along(vw.width, TS::TX), gy = tiles_along(vw.height, TS::TY);
-        const uint32_t nd = n_dup[p];
-        const uint32_t off = scan[p] - nd;
+        const uint32_t nd = g.n_dup[p];
         int txmin, tymin, txmax, tymax;
-        rect_of<TS::TX, TS::TY>(bboxes[p], gx, gy, txmin, tymin, txmax, tymax);
+        rect_of<TS::TX, TS::TY>(g.bbox[p], gx, gy, txmin, tymin, txmax, tymax);
         const uint32_t w = (uint32_t)(txmax - txmin + 1);
         const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
-        const uint32_t bits = qbits[p];
+        const uint32_t bits = g.qbits[p];
         const uint32_t c = pass == 0 ? 0u : it / huge;
         const uint32_t e_lo = pass == 0 ? 0u : (uint32_t)(((uint64_t)nd * c) / K);
         const uint32_t e_hi = pass == 0 ? nd : (uint32_t)(((uint64_t)nd * (c + 1)) / K);
-        for (uint32_t e = e_lo + threadIdx.x; e < e_hi; e += blockDim.x)
+        uint32_t qi = 0;
+        for (uint32_t b = bits; b; b &= b - 1u, ++qi)
         {
-            const uint32_t qi = e / tiles, rem = e - qi * tiles;
-            uint32_t b = bits;
-            for (uint32_t s = 0; s < qi; ++s) b &= b - 1u;
-            const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
-            const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
-            const uint32_t g = (uint32_t)vw.tile_base + y * (uint32_t)gx + x;
-            keys[off + e] = ((uint64_t)g << pos_bits) | (uint64_t)order_pos[8 * (size_t)v + q];
-            vals[off + e] = off + e;
+            // the part of [e_lo, e_hi) in this quadrant's run
+            const uint32_t q = (uint32_t)__builtin_ctz(b);
+            const uint32_t r_lo = max(e_lo, qi * tiles), r_hi = min(e_hi, (qi + 1) * tiles);
+            if (r_lo >= r_hi) continue;
+            const uint32_t off = run_offset(g, p, bits, q);
+            if ((uint64_t)off + tiles > cap) continue;
+            const uint64_t pos = order_pos[8 * (size_t)v + q];
+            for (uint32_t e = r_lo + threadIdx.x; e < r_hi; e += blockDim.x)
+            {
+                const uint32_t rem = e - qi * tiles;
+                const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
+                const uint32_t gt = (uint32_t)vw.tile_base + y * (uint32_t)gx + x;
+                keys[off + rem] = ((uint64_t)gt << pos_bits) | pos;
+                vals[off + rem] = off + rem;
+            }
         }
     }
     }
@@ -843,24 +836,24 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     if (R > 0)
     {
         SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
-        // emit into the buffer that makes the sorted result land in b.keys / b.orig
-        const int nbits = pos_bits + tile_bits;
-        const bool odd = ((nbits + RADIX_BITS - 1) / RADIX_BITS) & 1;
+        rc = pair_offsets_launch(V, N, g, ot.pos, stream);
+        if (rc != SVR_OK) return rc;
+        // entries are emitted view by view in rank order, so the sort only
+        // orders the (global) tile bits; emit into the buffer that makes the
+        // sorted result land in b.keys / b.orig
+        const bool odd = radix_passes(tile_bits) & 1;
         uint64_t* k0 = odd ? b.keys_alt : b.keys;
         uint32_t* o0 = odd ? b.orig_alt : b.orig;
         SVR_WITH_TILE(shape, (batch_duplicate_kernel<TS><<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0,
-                                                           stream>>>(views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox,
-                                                                     g.qbits, ot.pos, k0, o0, g.big_list,
-                                                                     g.counters + 2, cap)));
+                                                           stream>>>(g, views_dev, N, pos_bits, ot.pos, k0, o0, cap)));
         SVR_CHECK_LAUNCH();
-        SVR_WITH_TILE(shape, (batch_duplicate_big_kernel<TS><<<1024, 256, 0, stream>>>(
-                                 V, views_dev, N, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0, o0, g.big_list,
-                                 g.counters + 2)));
+        SVR_WITH_TILE(shape, (batch_duplicate_big_kernel<TS><<<1024, 256, 0, stream>>>(g, V, views_dev, N, pos_bits,
+                                                                                       ot.pos, k0, o0, cap)));
         SVR_CHECK_LAUNCH();
         uint64_t* sorted = nullptr;
         uint32_t* sorted_orig = nullptr;
-        rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, nbits, b.counts, stream,
-                           &sorted, &sorted_orig, R_dev);
+        rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, tile_bits, b.counts,
+                           stream, &sorted, &sorted_orig, R_dev, pos_bits);
         if (rc != SVR_OK) return rc;
         SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
         rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream, R_dev);

@@ -235,22 +235,29 @@ __global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
     }
 }
 
-// Stable LSD radix sort over bits [0, nbits) in 8-bit digits.  The sorted
-// data ends in the buffers returned through out_keys / out_vals.
+// Stable LSD radix sort over bits [lo_bit, lo_bit + nbits) in digits of at
+// most 8 bits.  The sorted data ends in the buffers returned through
+// out_keys / out_vals.
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
-                  const uint32_t* n_dev)
+                  const uint32_t* n_dev, int lo_bit)
 {
     *out_keys = keys;
     if (out_vals) *out_vals = vals;
     if (n <= 0 || nbits <= 0) return SVR_OK;
+    SVR_REQUIRE(lo_bit >= 0 && lo_bit + nbits <= 64, SVR_ERR_INVALID, "radix sort bits [%d, %d) out of range", lo_bit,
+                lo_bit + nbits);
     const int64_t nb = sort_blocks(n);
     SVR_REQUIRE(nb < (1ll << 31), SVR_ERR_INVALID, "radix sort of %lld keys too large", (long long)n);
     uint64_t* kin = keys; uint64_t* kout = keys_alt;
     uint32_t* vin = vals; uint32_t* vout = vals_alt;
-    for (int shift = 0; shift < nbits; shift += RADIX_BITS)
+    // radix_passes(nbits) passes with the bits spread evenly over them
+    const int passes = radix_passes(nbits);
+    const int step = (nbits + passes - 1) / passes;
+    for (int done = 0; done < nbits; done += step)
     {
-        const int bits = (nbits - shift) < RADIX_BITS ? (nbits - shift) : RADIX_BITS;
+        const int shift = lo_bit + done;
+        const int bits = (nbits - done) < step ? (nbits - done) : step;
         const uint32_t dmask = (1u << bits) - 1u;
         radix_upsweep<<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, n, n_dev, shift, dmask, counts, nb);
         SVR_CHECK_LAUNCH();
@@ -335,34 +342,60 @@ struct OrderScratch
 
 // ---- per-frame duplication, ranges and key export --------------------
 
-// forward.cu:523-577, emitting (tile << pos_bits | pos) keys.
-__global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int gy, int pos_bits,
-                                                        const uint32_t* __restrict__ n_dup,
-                                                        const uint32_t* __restrict__ scan,
-                                                        const uint2* __restrict__ bboxes,
-                                                        const uint32_t* __restrict__ qbits,
-                                                        const uint32_t* __restrict__ order_pos,
-                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                        uint32_t* __restrict__ big_list,
-                                                        uint32_t* __restrict__ big_count)
+// Each active (item, quadrant) pair's tile count at its rank position, in
+// view blockIdx.y's block of pos_off (pos_off cleared beforehand).
+__global__ __launch_bounds__(256) void pair_count_kernel(int64_t N, const uint32_t* __restrict__ n_dup,
+                                                         const uint32_t* __restrict__ qbits,
+                                                         const uint32_t* __restrict__ order_pos,
+                                                         uint32_t* __restrict__ pos_off)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
-    const uint32_t nd = n_dup[v];
+    const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
+    const uint32_t nd = n_dup[p];
     if (nd == 0) return;
-    if (nd > BIG_DUP)
+    const uint32_t bits = qbits[p];
+    const uint32_t tiles = nd / (uint32_t)__popc(bits);
+    uint32_t* blk = pos_off + 8 * (size_t)blockIdx.y * (size_t)N;
+    for (uint32_t b = bits; b; b &= b - 1u)
     {
-        queue_large(big_list, big_count, N, nd, (uint32_t)v);
-        return;
+        const uint32_t q = (uint32_t)__builtin_ctz(b);
+        blk[order_pos[8 * v + q]] = tiles;
     }
-    uint32_t off = scan[v] - nd;
+}
+
+int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream)
+{
+    const int64_t n8 = 8 * (int64_t)V * N;
+    if (n8 == 0) return SVR_OK;
+    if (fill_u32(g.pos_off, 0u, n8, stream) != SVR_OK) return SVR_ERR_HIP;
+    pair_count_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(N, g.n_dup, g.qbits,
+                                                                                         order_pos, g.pos_off);
+    SVR_CHECK_LAUNCH();
+    return pair_scan_launch(g.pos_off, n8, g.pos_sums, stream);
+}
+
+// forward.cu:523-577, emitting (tile << pos_bits | pos) keys: each quadrant's
+// run at its rank-ordered offset (recorded in qoff for the later passes).
+__global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
+                                                        const uint32_t* __restrict__ order_pos,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const uint32_t nd = g.n_dup[v];
+    if (nd == 0) return;
+    const bool large = nd > BIG_DUP;
     int txmin, tymin, txmax, tymax;
-    rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
-    const uint32_t bits = qbits[v];
-    for (uint32_t q = 0; q < 8; ++q)
+    rect_of(g.bbox[v], gx, gy, txmin, tymin, txmax, tymax);
+    const uint32_t bits = g.qbits[v];
+    for (uint32_t b = bits; b; b &= b - 1u)
     {
-        if (!(bits & (1u << q))) continue;
+        const uint32_t q = (uint32_t)__builtin_ctz(b);
         const uint64_t pos = order_pos[8 * v + q];
+        uint32_t off = g.pos_off[pos];
+        set_run_offset(g, (size_t)v, bits, q, off);
+        if (large) continue;
         for (int y = tymin; y <= tymax; ++y)
             for (int x = txmin; x <= txmax; ++x)
             {
@@ -371,58 +404,54 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int64_t N, int gx, int g
                 ++off;
             }
     }
-}
-
-// Entry e of a voxel's run: quadrant (the qi-th set bit of bits) major, then
-// tile rows of its rectangle.
-__device__ inline void emit_one(uint32_t e, uint32_t off, uint32_t tiles, uint32_t w, int txmin, int tymin,
-                                uint32_t bits, int gx, int pos_bits, const uint32_t* __restrict__ pos8,
-                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
-{
-    const uint32_t qi = e / tiles, rem = e - qi * tiles;
-    uint32_t b = bits;
-    for (uint32_t s = 0; s < qi; ++s) b &= b - 1u;
-    const uint32_t q = (uint32_t)__ffs((int)b) - 1u;
-    const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
-    keys[off + e] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | (uint64_t)pos8[q];
-    vals[off + e] = off + e;
+    if (large) queue_large(g.big_list, g.counters + 2, N, nd, (uint32_t)v);
 }
 
 // Large voxels: one workgroup each; then the huge ones with every workgroup
-// taking a strided share of each.
-__global__ __launch_bounds__(256) void duplicate_big_kernel(int64_t N, int gx, int gy, int pos_bits,
-                                                            const uint32_t* __restrict__ n_dup,
-                                                            const uint32_t* __restrict__ scan,
-                                                            const uint2* __restrict__ bboxes,
-                                                            const uint32_t* __restrict__ qbits,
+// taking a strided share of each.  Entry e of a voxel: quadrant (the qi-th set
+// bit of its bits) major, then the tile rows of its rectangle, written into
+// that quadrant's run.
+__global__ __launch_bounds__(256) void duplicate_big_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
                                                             const uint32_t* __restrict__ order_pos,
                                                             uint64_t* __restrict__ keys,
-                                                            uint32_t* __restrict__ vals,
-                                                            const uint32_t* __restrict__ big_list,
-                                                            const uint32_t* __restrict__ big_count)
+                                                            uint32_t* __restrict__ vals)
 {
     // pass 0: one workgroup per large voxel; pass 1: each huge voxel split into
     // K entry ranges taken by K different workgroups (item = chunk * huge + voxel)
-    const uint32_t count = big_count[0], huge = big_count[1];
+    const uint32_t count = g.counters[2], huge = g.counters[3];
     const uint32_t K = huge ? max(1u, gridDim.x / huge) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
     {
         for (uint32_t it = blockIdx.x; it < (pass == 0 ? count : huge * K); it += gridDim.x)
         {
             const uint32_t i = pass == 0 ? it : it % huge;
-            const uint32_t v = pass == 0 ? big_list[i] : big_list[N - 1 - i];
-            const uint32_t nd = n_dup[v];
-            const uint32_t off = scan[v] - nd;
+            const uint32_t v = pass == 0 ? g.big_list[i] : g.big_list[N - 1 - i];
+            const uint32_t nd = g.n_dup[v];
             const uint32_t c = pass == 0 ? 0u : it / huge;
             const uint32_t e_lo = pass == 0 ? 0u : (uint32_t)(((uint64_t)nd * c) / K);
             const uint32_t e_hi = pass == 0 ? nd : (uint32_t)(((uint64_t)nd * (c + 1)) / K);
             int txmin, tymin, txmax, tymax;
-            rect_of(bboxes[v], gx, gy, txmin, tymin, txmax, tymax);
+            rect_of(g.bbox[v], gx, gy, txmin, tymin, txmax, tymax);
             const uint32_t w = (uint32_t)(txmax - txmin + 1);
             const uint32_t tiles = w * (uint32_t)(tymax - tymin + 1);
-            const uint32_t bits = qbits[v];
-            for (uint32_t e = e_lo + threadIdx.x; e < e_hi; e += blockDim.x)
-                emit_one(e, off, tiles, w, txmin, tymin, bits, gx, pos_bits, order_pos + 8 * (size_t)v, keys, vals);
+            const uint32_t bits = g.qbits[v];
+            uint32_t qi = 0;
+            for (uint32_t b = bits; b; b &= b - 1u, ++qi)
+            {
+                // the part of [e_lo, e_hi) in this quadrant's run
+                const uint32_t q = (uint32_t)__builtin_ctz(b);
+                const uint32_t r_lo = max(e_lo, qi * tiles), r_hi = min(e_hi, (qi + 1) * tiles);
+                if (r_lo >= r_hi) continue;
+                const uint32_t off = run_offset(g, v, bits, q);
+                const uint64_t pos = order_pos[8 * (size_t)v + q];
+                for (uint32_t e = r_lo + threadIdx.x; e < r_hi; e += blockDim.x)
+                {
+                    const uint32_t rem = e - qi * tiles;
+                    const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
+                    keys[off + rem] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | pos;
+                    vals[off + rem] = off + rem;
+                }
+            }
         }
     }
 }
@@ -510,23 +539,22 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
     SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
-    // the sort ping-pongs between the two buffers, one 8-bit digit per pass:
-    // emit into the buffer that makes the sorted result land in b.keys / b.orig
-    const int nbits = pos_bits + tile_bits;
-    const bool odd = ((nbits + RADIX_BITS - 1) / RADIX_BITS) & 1;
+    int rc = pair_offsets_launch(1, N, g, ot.pos, stream);
+    if (rc != SVR_OK) return rc;
+    // entries are emitted in rank order, so the sort only orders the tile
+    // bits; it ping-pongs between the two buffers, one digit per pass: emit
+    // into the buffer that makes the sorted result land in b.keys / b.orig
+    const bool odd = radix_passes(tile_bits) & 1;
     uint64_t* k0 = odd ? b.keys_alt : b.keys;
     uint32_t* o0 = odd ? b.orig_alt : b.orig;
-    duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, gx, gy, pos_bits, g.n_dup, g.scan, g.bbox,
-                                                                      g.qbits, ot.pos, k0, o0, g.big_list,
-                                                                      g.counters + 2);
+    duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0, o0);
     SVR_CHECK_LAUNCH();
-    duplicate_big_kernel<<<1024, 256, 0, stream>>>(N, gx, gy, pos_bits, g.n_dup, g.scan, g.bbox, g.qbits, ot.pos, k0,
-                                                   o0, g.big_list, g.counters + 2);
+    duplicate_big_kernel<<<1024, 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0, o0);
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
     uint32_t* sorted_orig = nullptr;
-    int rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, nbits, b.counts, stream,
-                           &sorted, &sorted_orig);
+    rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, tile_bits, b.counts, stream,
+                       &sorted, &sorted_orig, nullptr, pos_bits);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
     rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);

@@ -111,28 +111,88 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scan_sums_kernel(uint32_t* __restr
     }
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n,
-                                                                const uint32_t* __restrict__ block_sums,
-                                                                uint32_t* __restrict__ out)
+// ---- exclusive scan of the rank-ordered pair counts, in place ---------
+
+// PSCAN_ITEMS consecutive elements of thread t (16-byte aligned runs).
+__device__ inline void pscan_load(const uint32_t* __restrict__ c, int64_t n, int64_t base, uint32_t (&v)[PSCAN_ITEMS])
+{
+    if (base + PSCAN_ITEMS <= n)
+    {
+#pragma unroll
+        for (int i = 0; i < PSCAN_ITEMS / 4; ++i)
+        {
+            const uint4 q = *reinterpret_cast<const uint4*>(c + base + 4 * i);
+            v[4 * i + 0] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < PSCAN_ITEMS; ++i) v[i] = (base + i < n) ? c[base + i] : 0u;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void pscan_reduce_kernel(const uint32_t* __restrict__ c, int64_t n,
+                                                                  uint32_t* __restrict__ sums)
 {
     __shared__ uint32_t lds[SCAN_BLOCK / 64];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
+    uint32_t v[PSCAN_ITEMS];
+    pscan_load(c, n, (int64_t)blockIdx.x * PSCAN_TILE + (int64_t)threadIdx.x * PSCAN_ITEMS, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i)
-    {
-        v[i] = (base + i < n) ? in[base + i] : 0u;
-        s += v[i];
-    }
+    for (int i = 0; i < PSCAN_ITEMS; ++i) s += v[i];
     uint32_t total;
-    uint32_t run = block_incl_scan(s, lds, &total) - s + block_sums[blockIdx.x];
+    block_incl_scan(s, lds, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Every element becomes its exclusive prefix (only those of nonzero counts
+// are read later; writing whole 16-byte runs keeps the stores dense).
+__global__ __launch_bounds__(SCAN_BLOCK) void pscan_apply_kernel(uint32_t* __restrict__ c, int64_t n,
+                                                                 const uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t lds[SCAN_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * PSCAN_TILE + (int64_t)threadIdx.x * PSCAN_ITEMS;
+    uint32_t v[PSCAN_ITEMS];
+    pscan_load(c, n, base, v);
+    uint32_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i)
+    for (int i = 0; i < PSCAN_ITEMS; ++i) s += v[i];
+    uint32_t total;
+    uint32_t run = block_incl_scan(s, lds, &total) - s + sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < PSCAN_ITEMS; ++i)
     {
-        run += v[i];
-        if (base + i < n) out[base + i] = run;
+        const uint32_t x = v[i];
+        v[i] = run;
+        run += x;
     }
+    if (base + PSCAN_ITEMS <= n)
+    {
+#pragma unroll
+        for (int i = 0; i < PSCAN_ITEMS / 4; ++i)
+            *reinterpret_cast<uint4*>(c + base + 4 * i) = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < PSCAN_ITEMS; ++i)
+            if (base + i < n) c[base + i] = v[i];
+    }
+}
+
+int pair_scan_launch(uint32_t* c, int64_t n, uint32_t* sums, hipStream_t stream)
+{
+    if (n <= 0) return SVR_OK;
+    const int64_t nb = (n + PSCAN_TILE - 1) / PSCAN_TILE;
+    SVR_REQUIRE(nb < (1ll << 31), SVR_ERR_INVALID, "pair scan of %lld elements too large", (long long)n);
+    pscan_reduce_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(c, n, sums);
+    SVR_CHECK_LAUNCH();
+    scan_sums_kernel<<<1, SCAN_BLOCK, 0, stream>>>(sums, nb, sums + nb);
+    SVR_CHECK_LAUNCH();
+    pscan_apply_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(c, n, sums);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
 }
 
 }  // namespace svr
@@ -177,8 +237,6 @@ int svr_scan_entries(int64_t N, void* geom_state, svr_stream_t stream)
     scan_reduce_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums);
     SVR_CHECK_LAUNCH();
     scan_sums_kernel<<<1, SCAN_BLOCK, 0, stream>>>(g.block_sums, nb, g.counters);
-    SVR_CHECK_LAUNCH();
-    scan_apply_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums, g.scan);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_SCAN, stream);
     return SVR_OK;

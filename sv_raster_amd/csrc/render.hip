@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
                                                  rgbs, orig, partial);
 }
 
-// Per-voxel gradient = sum of the partial rows of its entries, which are
-// contiguous in emission order: [scan[v] - n_dup[v], scan[v]).  Voxels with
+// Per-voxel gradient = sum of the partial rows of its entries: one run of
+// consecutive rows per quadrant (run_offset), quadrants ascending.  Voxels with
 // more than BIG_DUP entries (listed in big_list by the duplication pass) are
 // summed by a whole workgroup, those with more than HUGE_DUP by every
 // workgroup of the launch (partial sums added with float atomics into rows
@@ -122,19 +122,26 @@ __device__ inline void write_grad(size_t v, const float* a, float* __restrict__ 
     dsub[v] = a[11];
 }
 
-// Entries at or beyond `cap` were never written (a frame over a fixed
-// capacity): such voxels get zero gradients.
-__global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint32_t* __restrict__ n_dup,
-                                                          const uint32_t* __restrict__ scan, uint32_t cap,
+__device__ inline void add_row(float (&a)[12], const float* __restrict__ partial, uint32_t row)
+{
+    const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)row);
+    const float4 p0 = r[0], p1 = r[1], p2 = r[2];
+    a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
+    a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
+    a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+}
+
+// One thread per item with at most BIG_DUP entries; rows of the huge items
+// are zeroed here for grad_reduce_big_kernel's atomics.  A run ending beyond
+// `cap` was never emitted (a frame over a fixed capacity) and counts as zero.
+__global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N, uint32_t cap,
                                                           const float* __restrict__ partial,
                                                           float* __restrict__ dgeos, float* __restrict__ drgbs,
                                                           float* __restrict__ dsub)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
-    uint32_t nd = n_dup[v];
-    // entries beyond a fixed capacity were never emitted (nor queued)
-    if (scan[v] > cap) nd = 0;
+    const uint32_t nd = g.n_dup[v];
     if (nd > HUGE_DUP)
     {
         const float z[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -145,22 +152,21 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(int64_t N, const uint3
     float a[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) a[i] = 0.f;
-    const uint32_t off = scan[v] - nd;
-    for (uint32_t e = 0; e < nd; ++e)
+    if (nd > 0)
     {
-        const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
-        const float4 p0 = r[0], p1 = r[1], p2 = r[2];
-        a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
-        a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
-        a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+        const uint32_t bits = g.qbits[v];
+        const uint32_t tiles = nd / (uint32_t)__popc(bits);
+        for (uint32_t b = bits; b; b &= b - 1u)
+        {
+            const uint32_t off = run_offset(g, (size_t)v, bits, (uint32_t)__builtin_ctz(b));
+            if ((uint64_t)off + tiles > cap) continue;
+            for (uint32_t e = 0; e < tiles; ++e) add_row(a, partial, off + e);
+        }
     }
     write_grad((size_t)v, a, dgeos, drgbs, dsub);
 }
 
-__global__ __launch_bounds__(256) void grad_reduce_big_kernel(int64_t N, const uint32_t* __restrict__ big_list,
-                                                              const uint32_t* __restrict__ big_count,
-                                                              const uint32_t* __restrict__ n_dup,
-                                                              const uint32_t* __restrict__ scan, uint32_t cap,
+__global__ __launch_bounds__(256) void grad_reduce_big_kernel(GeomState g, int64_t N, uint32_t cap,
                                                               const float* __restrict__ partial,
                                                               float* __restrict__ dgeos, float* __restrict__ drgbs,
                                                               float* __restrict__ dsub)
@@ -170,28 +176,32 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(int64_t N, const u
     // pass 0: one workgroup per large voxel, blocks striding over the queue;
     // pass 1: the huge voxels (queued from the end), each split into K row
     // ranges taken by K different workgroups (item = chunk * huge + voxel)
-    const uint32_t counts[2] = {big_count[0], big_count[1]};
+    const uint32_t counts[2] = {g.counters[2], g.counters[3]};
     const uint32_t K = counts[1] ? max(1u, gridDim.x / counts[1]) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
     for (uint32_t it = blockIdx.x; it < (pass == 0 ? counts[0] : counts[1] * K); it += gridDim.x)
     {
         const uint32_t i = pass == 0 ? it : it % counts[1];
-        const uint32_t v = pass == 0 ? big_list[i] : big_list[N - 1 - i];
-        const uint32_t nd = scan[v] > cap ? 0u : n_dup[v];
-        const uint32_t off = scan[v] - nd;
+        const uint32_t v = pass == 0 ? g.big_list[i] : g.big_list[N - 1 - i];
+        const uint32_t nd = g.n_dup[v];
+        const uint32_t bits = g.qbits[v];
+        const uint32_t tiles = nd / (uint32_t)__popc(bits);
         const uint32_t c = pass == 0 ? 0u : it / counts[1];
         const uint32_t e_lo = pass == 0 ? 0u : (uint32_t)(((uint64_t)nd * c) / K);
         const uint32_t e_hi = pass == 0 ? nd : (uint32_t)(((uint64_t)nd * (c + 1)) / K);
         float a[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) a[i] = 0.f;
-        for (uint32_t e = e_lo + threadIdx.x; e < e_hi; e += blockDim.x)
+        uint32_t qi = 0;
+        for (uint32_t b = bits; b; b &= b - 1u, ++qi)
         {
-            const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
-            const float4 p0 = r[0], p1 = r[1], p2 = r[2];
-            a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
-            a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
-            a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+            // the part of [e_lo, e_hi) in this quadrant's run
+            const uint32_t r_lo = max(e_lo, qi * tiles), r_hi = min(e_hi, (qi + 1) * tiles);
+            if (r_lo >= r_hi) continue;
+            const uint32_t off = run_offset(g, v, bits, (uint32_t)__builtin_ctz(b));
+            if ((uint64_t)off + tiles > cap) continue;
+            const uint32_t row0 = off - qi * tiles;
+            for (uint32_t e = r_lo + threadIdx.x; e < r_hi; e += blockDim.x) add_row(a, partial, row0 + e);
         }
 #pragma unroll
         for (int i = 0; i < 12; ++i)
@@ -225,11 +235,9 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(int64_t N, const u
 int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs, float* dsub,
                        hipStream_t stream, uint32_t cap)
 {
-    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, g.n_dup, g.scan, cap, partial, dgeos,
-                                                                       drgbs, dsub);
+    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
-    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(N, g.big_list, g.counters + 2, g.n_dup, g.scan, cap, partial,
-                                                     dgeos, drgbs, dsub);
+    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }

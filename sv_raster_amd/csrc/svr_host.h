@@ -82,33 +82,75 @@ constexpr int SCAN_BLOCK = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
+// The pair scan (entry offsets in rank-order position order) uses wider tiles.
+constexpr int PSCAN_ITEMS = 16;
+constexpr int PSCAN_TILE = SCAN_BLOCK * PSCAN_ITEMS;
+
 // Geometry state (one per preprocess call): per-voxel results + scan scratch.
+// Items are voxels (one view) or (view, voxel) pairs u * N + v (a batch).
+//
+// Entry emission order: the entries of the (item, quadrant) pairs are laid
+// out pair after pair in the rank order of the pairs (the order table's
+// position), each pair's run covering its tile rectangle row by row.  Every
+// pair has at most one entry per tile, so a stable sort of the entries on the
+// tile bits alone leaves each tile's entries in rank order: the sorted list
+// equals the sort on the full (tile << pos_bits | pos) key.
 struct GeomState
 {
-    uint32_t* n_dup;       // [N]
-    uint32_t* scan;        // [N] inclusive prefix sum of n_dup
-    uint2* bbox;           // [N] packed pixel bbox (x<<16 | y) min / max
-    uint32_t* qbits;       // [N] camera-quadrant bitset
-    uint32_t* block_sums;  // [ceil(N / SCAN_TILE) + 1]
+    uint32_t* n_dup;       // [n] entries of the item (tiles x quadrants)
+    uint32_t* run0;        // [n] first entry of the item's run when it has one quadrant
+    uint2* bbox;           // [n] packed pixel bbox (x<<16 | y) min / max
+    uint32_t* qbits;       // [n] camera-quadrant bitset
+    uint32_t* block_sums;  // [ceil(n / SCAN_TILE) + 1]
     uint32_t* counters;    // [8]: 0-1 R (total entries, 64-bit), 2 large-voxel queue length,
                            // 3 huge-voxel queue length (from the end of big_list),
                            // 4-5 entries to sort (R, or 0 when R exceeds a fixed capacity)
-    uint32_t* big_list;    // [N] voxels with many entries, expanded by whole workgroups
-    static GeomState carve(void* p, int64_t N, size_t* bytes)
+    uint32_t* big_list;    // [n] voxels with many entries, expanded by whole workgroups
+    uint32_t* pos_off;     // [8n] per view, indexed by rank-order position: the pair's
+                           // entry count, then (scanned in place) its first entry
+    uint32_t* qoff;        // [8n] first entry of pair (item p, quadrant q) at 8p + q,
+                           // written for the items with several quadrants only
+    uint32_t* pos_sums;    // [ceil(8n / PSCAN_TILE) + 2]
+    static GeomState carve(void* p, int64_t n, size_t* bytes)
     {
         Carver c(p);
         GeomState g;
-        g.n_dup = c.take<uint32_t>(N);
-        g.scan = c.take<uint32_t>(N);
-        g.bbox = c.take<uint2>(N);
-        g.qbits = c.take<uint32_t>(N);
-        g.block_sums = c.take<uint32_t>((N + SCAN_TILE - 1) / SCAN_TILE + 1);
+        g.n_dup = c.take<uint32_t>(n);
+        g.run0 = c.take<uint32_t>(n);
+        g.bbox = c.take<uint2>(n);
+        g.qbits = c.take<uint32_t>(n);
+        g.block_sums = c.take<uint32_t>((n + SCAN_TILE - 1) / SCAN_TILE + 1);
         g.counters = c.take<uint32_t>(8);
-        g.big_list = c.take<uint32_t>(N);
+        g.big_list = c.take<uint32_t>(n);
+        g.pos_off = c.take<uint32_t>(8 * n);
+        g.qoff = c.take<uint32_t>(8 * n);
+        g.pos_sums = c.take<uint32_t>((8 * n + PSCAN_TILE - 1) / PSCAN_TILE + 2);
         if (bytes) *bytes = c.size();
         return g;
     }
 };
+
+// Quadrant of the qi-th set bit of bits.
+__host__ __device__ inline uint32_t nth_set_bit(uint32_t bits, uint32_t qi)
+{
+    for (uint32_t s = 0; s < qi; ++s) bits &= bits - 1u;
+    return (uint32_t)__builtin_ctz(bits);
+}
+
+// First entry of the run of (item p, quadrant q); bits = the item's quadrants.
+__device__ inline uint32_t run_offset(const GeomState& g, size_t p, uint32_t bits, uint32_t q)
+{
+    return (bits & (bits - 1u)) ? g.qoff[8 * p + q] : g.run0[p];
+}
+
+// Records the run offset of (item p, quadrant q).
+__device__ inline void set_run_offset(const GeomState& g, size_t p, uint32_t bits, uint32_t q, uint32_t off)
+{
+    if (bits & (bits - 1u))
+        g.qoff[8 * p + q] = off;
+    else
+        g.run0[p] = off;
+}
 
 // Per-octree order table: the rank-order position of every (voxel, quadrant) pair.
 struct OrderTable
@@ -196,13 +238,22 @@ __host__ __device__ inline int tiles_x(int W) { return (W + 15) / 16; }
 __host__ __device__ inline int tiles_y(int H) { return (H + 15) / 16; }
 
 // Stable LSD radix sort of 64-bit keys (optionally with 32-bit values) over
-// bits [0, nbits).  The sorted data ends in the buffers returned through
+// bits [lo_bit, lo_bit + nbits), in radix_passes(nbits) passes of at most
+// RADIX_BITS bits.  The sorted data ends in the buffers returned through
 // out_keys / out_vals; counts needs RADIX * sort_blocks(n) + RADIX words.
 // With n_dev, the kernels sort min(n, *n_dev) keys (a 64-bit count stored as
 // two 32-bit words on the device); n then only sizes the launch.
+inline int radix_passes(int nbits) { return (nbits + RADIX_BITS - 1) / RADIX_BITS; }
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
-                  const uint32_t* n_dev = nullptr);
+                  const uint32_t* n_dev = nullptr, int lo_bit = 0);
+
+// Entry offsets of the (item, quadrant) pairs in rank order (GeomState's
+// emission order): pos_off cleared, filled with each active pair's tile count
+// at view u's block [8 N u, 8 N (u + 1)) indexed by rank position, then
+// exclusive-scanned in place (only the active positions are written).
+int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream);
+int pair_scan_launch(uint32_t* c, int64_t n, uint32_t* sums, hipStream_t stream);
 
 // Key duplication, sort, tile ranges and the sorted entry list of one frame.
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
@@ -220,7 +271,9 @@ int ranges_launch(const uint64_t* keys, int64_t R, int pos_bits, const uint32_t*
 int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hipStream_t stream);
 
 // Per-voxel (or per (view, voxel) pair) sums of the per-entry partial
-// gradient rows over each pair's contiguous emission-order segment.
+// gradient rows over the item's quadrant runs (GeomState::qoff), quadrants
+// ascending and each run in emission order.  A run ending beyond `cap` was
+// never emitted and counts as zero.
 int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
                        float* dsub, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
 

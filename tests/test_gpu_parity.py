@@ -55,10 +55,13 @@ def test_parity_small(name, seed):
                                             ("A", (0.01, -0.02, -1.0))])
 def test_parity_multi_quadrant(name, direction):
     """Camera close to a coordinate axis: the view straddles axis planes, so pixels
-    of one tile have different ray quadrants and tile lists interleave quadrants."""
+    of one tile have different ray quadrants and tile lists interleave quadrants;
+    voxels cut by those planes have several quadrants (one entry run each)."""
     case = make_case(name, seed=3, direction=direction)
     o = run_oracle(case)
     assert np.unique(o["vox_list"] >> 29).size >= 4
+    live_bits = o["qbits"][o["n_dup"] > 0].astype(np.uint32)
+    assert (np.unpackbits(live_bits.view(np.uint8)).reshape(-1, 32).sum(1) >= 2).sum() >= 100
     g = run_gpu(case)
     _check_forward(g, o)
     _check_backward(g, o)
