@@ -225,20 +225,6 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(GeomState g, i
     }
 }
 
-// Queue keys: tiles bucketed by list length, longest bucket first
-// (bucket = bit length of the length, key = 32 - bucket).
-__global__ __launch_bounds__(256) void queue_keys_kernel(int64_t T, const uint2* __restrict__ ranges,
-                                                         uint64_t* __restrict__ qkeys, uint32_t* __restrict__ order)
-{
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= T) return;
-    const uint2 r = ranges[g];
-    const uint32_t len = r.y - r.x;
-    const uint32_t bucket = len ? 32u - (uint32_t)__clz((int)len) : 0u;
-    qkeys[g] = 32u - bucket;
-    order[g] = (uint32_t)g;
-}
-
 // ---- compositor over the tile queue ---------------------------------------
 
 struct TileGeom
@@ -859,17 +845,9 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
         rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream, R_dev);
         if (rc != SVR_OK) return rc;
     }
-    // queue order: longest tile lists first (one 8-bit radix pass on the
-    // length bucket, from the alternate buffers into img.order)
-    queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, img.ranges, img.qkeys_alt, img.order_alt);
-    SVR_CHECK_LAUNCH();
-    {
-        uint64_t* qk = nullptr;
-        uint32_t* qo = nullptr;
-        rc = radix_sort_ex(img.qkeys_alt, img.qkeys, img.order_alt, img.order, T, 8, img.qcounts, stream, &qk, &qo);
-        if (rc != SVR_OK) return rc;
-        SVR_REQUIRE(qo == img.order, SVR_ERR_HIP, "queue sort ended in the wrong buffer");
-    }
+    // queue order: longest tile lists first
+    rc = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt, img.qcounts, stream);
+    if (rc != SVR_OK) return rc;
     prof_end(PROF_BINNING, stream);
 
     prof_begin(PROF_RENDER_FWD, stream);

@@ -524,6 +524,32 @@ int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hip
     return SVR_OK;
 }
 
+// Tile queue keys: 255 - length_bucket (an ascending sort puts the longest
+// lists first), values the tile ids.
+__global__ __launch_bounds__(256) void tile_queue_keys_kernel(int64_t T, const uint2* __restrict__ ranges,
+                                                              uint64_t* __restrict__ qkeys,
+                                                              uint32_t* __restrict__ order)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= T) return;
+    const uint2 r = ranges[g];
+    qkeys[g] = 255u - length_bucket(r.y - r.x);
+    order[g] = (uint32_t)g;
+}
+
+int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t* qkeys_alt, uint32_t* order,
+                      uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream)
+{
+    tile_queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, ranges, qkeys_alt, order_alt);
+    SVR_CHECK_LAUNCH();
+    uint64_t* qk = nullptr;
+    uint32_t* qo = nullptr;
+    const int rc = radix_sort_ex(qkeys_alt, qkeys, order_alt, order, T, 8, qcounts, stream, &qk, &qo);
+    if (rc != SVR_OK) return rc;
+    SVR_REQUIRE(qo == order, SVR_ERR_HIP, "tile queue sort ended in the wrong buffer");
+    return SVR_OK;
+}
+
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
                     BinningState& b, ImageState& img, hipStream_t stream)
 {
@@ -533,8 +559,10 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     SVR_REQUIRE(fill_u32(img.ranges, 0u, 2 * T, stream) == SVR_OK, SVR_ERR_HIP, "fill ranges failed");
     if (R == 0)
     {
+        const int rc0 = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt,
+                                          img.qcounts, stream);
         prof_end(PROF_BINNING, stream);
-        return SVR_OK;
+        return rc0;
     }
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
@@ -558,6 +586,8 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
     rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
+    if (rc != SVR_OK) return rc;
+    rc = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt, img.qcounts, stream);
     if (rc != SVR_OK) return rc;
     prof_end(PROF_BINNING, stream);
     return SVR_OK;

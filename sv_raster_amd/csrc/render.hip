@@ -31,7 +31,7 @@ namespace svr {
 // differs); each launch returns at once from the other's tiles.
 template <int NS, bool EXACT>
 __global__ __launch_bounds__(256) void render_fwd_kernel(
-    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
+    int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
@@ -40,8 +40,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
 {
     __shared__ Stage st;
     __shared__ uint32_t s_last[NWAVE];
-    const int tile = xcd_tile(blockIdx.x, per_xcd);
-    if (tile >= T_tiles) return;
+    const int tile = (int)tile_order[blockIdx.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
     const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
@@ -78,7 +77,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
 // Launched twice per frame, split by tiles like render_fwd_kernel.
 template <int NS, int SYNC_MIN, bool EXACT>
 __global__ __launch_bounds__(256) void render_bwd_kernel(
-    int W, int H, int gx, int T_tiles, int per_xcd, float tanfovx, float tanfovy, float cx, float cy,
+    int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
@@ -86,8 +85,7 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
     __shared__ WindowLDS<BWD_WINDOW> sh;
-    const int tile = xcd_tile(blockIdx.x, per_xcd);
-    if (tile >= T_tiles) return;
+    const int tile = (int)tile_order[blockIdx.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
     const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
@@ -250,14 +248,13 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
 {
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
-    const int per = (T + 7) / 8;
     prof_begin(PROF_RENDER_FWD, stream);
-    render_fwd_kernel<NS, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
-        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+    render_fwd_kernel<NS, false><<<(unsigned)T, 256, 0, stream>>>(
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks);
     SVR_CHECK_LAUNCH();
-    render_fwd_kernel<NS, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
-        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+    render_fwd_kernel<NS, true><<<(unsigned)T, 256, 0, stream>>>(
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
@@ -273,15 +270,14 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
 {
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
-    const int per = (T + 7) / 8;
     if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
-    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, false><<<(unsigned)(8 * per), 256, 0, stream>>>(
-        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, false><<<(unsigned)T, 256, 0, stream>>>(
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
-    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, true><<<(unsigned)(8 * per), 256, 0, stream>>>(
-        W, H, gx, T, per, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, true><<<(unsigned)T, 256, 0, stream>>>(
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);

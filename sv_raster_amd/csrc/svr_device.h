@@ -359,8 +359,16 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Block -> tile mapping that keeps runs of neighbouring tiles on one XCD
-// (workgroups are dispatched round-robin over the 8 XCDs).
-__device__ inline int xcd_tile(int b, int per_xcd) { return (b & 7) * per_xcd + (b >> 3); }
+// Scheduling bucket of a tile's list length: 8 sub-buckets per octave (the
+// bit length and the next three bits), 0..255, larger for longer lists.  The
+// compositors take tiles longest bucket first: per-tile work spans three
+// orders of magnitude, and a long list that starts late is the frame's tail.
+__device__ inline uint32_t length_bucket(uint32_t len)
+{
+    if (len == 0) return 0u;
+    const uint32_t b = 32u - (uint32_t)__clz((int)len);
+    const uint32_t top4 = b >= 4 ? (len >> (b - 4)) : (len << (4 - b));
+    return 8u * (b - 1u) + (top4 & 7u);
+}
 
 }  // namespace svr

@@ -220,6 +220,11 @@ struct ImageState
     uint2* ranges;       // [T] [start, end) of each tile's entries
     uint32_t* tile_last; // [T] range.x + max n_contrib over the tile
     uint32_t* n_contrib; // [H*W]
+    uint32_t* order;     // [T] tile queue: longest lists first (workgroup b renders tile order[b])
+    uint32_t* order_alt; // [T]
+    uint64_t* qkeys;     // [T] queue sort keys
+    uint64_t* qkeys_alt; // [T]
+    uint32_t* qcounts;   // radix counts of the queue sort
     uint64_t* hit_masks; // [hit_mask_slots(R, T)][4 waves][64 lanes]
     static ImageState carve(void* p, int64_t P, int64_t T, size_t* bytes, int64_t R = 0)
     {
@@ -228,6 +233,11 @@ struct ImageState
         s.ranges = c.take<uint2>(T);
         s.tile_last = c.take<uint32_t>(T);
         s.n_contrib = c.take<uint32_t>(P);
+        s.order = c.take<uint32_t>(T);
+        s.order_alt = c.take<uint32_t>(T);
+        s.qkeys = c.take<uint64_t>(T);
+        s.qkeys_alt = c.take<uint64_t>(T);
+        s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
         s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
         if (bytes) *bytes = c.size();
         return s;
@@ -260,6 +270,12 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
                     BinningState& b, ImageState& img, hipStream_t stream);
 
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
+
+// Tile queue: order[0..T) = tile ids sorted by length_bucket of their list,
+// longest first, stable (keys and order are built in the _alt buffers, one
+// 8-bit radix pass lands them in qkeys / order).
+int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t* qkeys_alt, uint32_t* order,
+                      uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream);
 
 // Tile ranges and the sorted entry list from sorted (tile << pos_bits | pos)
 // keys; ranges must be zeroed first.
