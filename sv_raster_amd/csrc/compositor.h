@@ -131,6 +131,27 @@ __device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryR
     st.rgb[0][w][lane] = e.r0; st.rgb[1][w][lane] = e.r1; st.rgb[2][w][lane] = e.r2;
 }
 
+// Candidates (bit l: staged entry l) whose box the pixel's ray meets, i.e.
+// ray_aabb gives !(a > b).  A pre-pass over the bbox candidates with the
+// ray-box test alone, so that the lanes of the compositing loop step over
+// real intersections only: on the 1080p scene a third of the candidates
+// miss the box, and in the longest tiles two thirds of the lock-step
+// iterations were such misses.
+template <bool EXACT, int NW>
+__device__ inline uint64_t raybox_filter(const StageT<NW>& st, int w, uint64_t hm, const PixRay& ray)
+{
+    uint64_t keep = 0ull;
+    while (hm)
+    {
+        const int l = __builtin_ctzll(hm);
+        hm &= hm - 1ull;
+        float a, b;
+        ray_aabb<EXACT>(st.c[0][w][l], st.c[1][w][l], st.c[2][w][l], st.l[w][l], ray, a, b);
+        if (!(a > b)) keep |= 1ull << l;
+    }
+    return keep;
+}
+
 // Front-to-back compositing of one wave's 8x8 sub-tile over the tile's list
 // (forward.cu:150-328 for colour and T).  Returns the pixel's n_contrib and
 // leaves its transmittance and colour in T, C0..C2.  EXACT selects the
@@ -192,6 +213,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
                                sx1, sy1, qpix);
         }
 
+        hm = raybox_filter<EXACT>(st, w, hm, ray);
         uint64_t hits = 0ull;
         while (hm)
         {
@@ -199,8 +221,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             hm &= hm - 1ull;
             const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
-            if (a > b) continue;
+            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);  // same values as in raybox_filter: !(a > b)
             float g[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
