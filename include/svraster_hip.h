@@ -79,6 +79,15 @@ int svr_rasterize_preprocess(int32_t width, int32_t height, float tanfovx, float
  * buffer. */
 int svr_num_rendered(int64_t n_voxels, void* geom_state, svr_stream_t stream, int64_t* num_rendered_host);
 
+/* The in-frustum voxel list and R with one synchronisation: replaces the two
+ * host round trips of rasterize_voxels between preprocess and the raster,
+ * `torch.where(n_duplicates > 0)[0]` (renderer.py:81) and num_rendered
+ * (forward.cu:649-663).  Writes the ascending ids of the voxels with
+ * n_duplicates > 0 into idx_out[0 .. *num_visible_host) (int64, room for
+ * n_voxels ids) and returns R and that count on the host. */
+int svr_visible_voxels(int64_t n_voxels, void* geom_state, int64_t* idx_out, svr_stream_t stream,
+                       int64_t* num_rendered_host, int64_t* num_visible_host);
+
 /* Builds the rank-order table of an octree: the position of every
  * (voxel, quadrant) pair in the order of the reference's 63-bit order rank
  * octpath ^ repeat(q) (auxiliary.h:57-60), ties by (voxel, quadrant).  Valid

@@ -127,10 +127,25 @@ def rasterize_preprocess(image_width, image_height, tan_fovx, tan_fovy, cx, cy, 
     return out_n_dup, geom
 
 
+def visible_voxels(geomBuffer, n_voxels, device):
+    """In-frustum voxel ids (int64, ascending; what renderer.py:81 gets from torch.where(n_duplicates > 0)[0])
+    and the number of rendered entries R, from the geometry state of rasterize_preprocess, with one host
+    synchronisation for both."""
+    N = int(n_voxels)
+    if N == 0:
+        return torch.empty(0, dtype=torch.int64, device=device), 0
+    idx = torch.empty(N, dtype=torch.int64, device=device)
+    R, nv = ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.check(_L.svr_visible_voxels(N, _ptr(geomBuffer), _ptr(idx), _stream(device), ctypes.byref(R),
+                                     ctypes.byref(nv)), "visible_voxels")
+    return idx[:int(nv.value)], int(R.value)
+
+
 # binding.cpp:26 -> forward.cu:792-879
 def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix, c2w_matrix,
                      bg_color, need_depth, need_distortion, need_normal, track_max_w, octree_paths, vox_centers,
-                     vox_lengths, geos, rgbs, geomBuffer, debug=False):
+                     vox_lengths, geos, rgbs, geomBuffer, debug=False, num_rendered=None):
+    """num_rendered: R when the caller already has it (visible_voxels); otherwise it is read here."""
     if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
         raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
     if rgbs.ndimension() != 2 or rgbs.size(1) != 3:
@@ -148,10 +163,13 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
     out_depth = torch.empty((3, H, W), dtype=torch.float32, device=dev) if need_depth or need_distortion else empty
     out_normal = torch.empty((3, H, W), dtype=torch.float32, device=dev) if need_normal else empty
     max_w = torch.empty((N, 1), dtype=torch.float32, device=dev) if track_max_w else empty
-    R = ctypes.c_int64(0)
-    if N > 0:
-        _lib.check(_L.svr_num_rendered(N, _ptr(geomBuffer), _stream(dev), ctypes.byref(R)), "num_rendered")
-    R = int(R.value)
+    if num_rendered is None:
+        R = ctypes.c_int64(0)
+        if N > 0:
+            _lib.check(_L.svr_num_rendered(N, _ptr(geomBuffer), _stream(dev), ctypes.byref(R)), "num_rendered")
+        R = int(R.value)
+    else:
+        R = int(num_rendered)
     img = torch.empty(int(_L.svr_image_state_size(W, H, R)), dtype=torch.uint8, device=dev)
     binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
     table = order_table_cache.get(octree_paths) if R > 0 else None

@@ -84,8 +84,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scan_reduce_kernel(const uint32_t*
 }
 
 // Exclusive scan of the block sums in place (one workgroup); total -> counters[0..1] (u64).
-__global__ __launch_bounds__(SCAN_BLOCK) void scan_sums_kernel(uint32_t* __restrict__ block_sums, int64_t nb,
-                                                               uint32_t* __restrict__ counters)
+__device__ inline void scan_sums_block(uint32_t* __restrict__ block_sums, int64_t nb, uint32_t* __restrict__ counters)
 {
     __shared__ uint32_t lds[SCAN_BLOCK / 64];
     __shared__ uint64_t carry_s;
@@ -109,6 +108,69 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scan_sums_kernel(uint32_t* __restr
         counters[0] = (uint32_t)R;
         counters[1] = (uint32_t)(R >> 32);
     }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void scan_sums_kernel(uint32_t* __restrict__ block_sums, int64_t nb,
+                                                               uint32_t* __restrict__ counters)
+{
+    scan_sums_block(block_sums, nb, counters);
+}
+
+// Workgroup 0 scans the entry-count block sums (total -> counters[0..1]),
+// workgroup 1 the in-frustum block counts (total -> counters[6..7]).
+__global__ __launch_bounds__(SCAN_BLOCK) void scan_sums2_kernel(uint32_t* __restrict__ block_sums,
+                                                                uint32_t* __restrict__ vis_sums, int64_t nb,
+                                                                uint32_t* __restrict__ counters)
+{
+    if (blockIdx.x == 0) scan_sums_block(block_sums, nb, counters);
+    else scan_sums_block(vis_sums, nb, counters + 6);
+}
+
+// Per scan block: the entry total (block_sums) and the number of in-frustum
+// voxels, n_dup > 0 (vis_sums).
+__global__ __launch_bounds__(SCAN_BLOCK) void count_reduce_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                  uint32_t* __restrict__ block_sums,
+                                                                  uint32_t* __restrict__ vis_sums)
+{
+    __shared__ uint32_t lds[SCAN_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0, c = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+    {
+        const uint32_t x = (base + i < n) ? in[base + i] : 0u;
+        s += x;
+        c += (x != 0u);
+    }
+    uint32_t total;
+    block_incl_scan(s, lds, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+    block_incl_scan(c, lds, &total);
+    if (threadIdx.x == 0) vis_sums[blockIdx.x] = total;
+}
+
+// Ascending ids of the voxels with n_dup > 0; scan block b writes from
+// vis_sums[b] (exclusive-scanned) on (renderer.py:81, torch.where(n_dup > 0)).
+__global__ __launch_bounds__(SCAN_BLOCK) void visible_write_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                   const uint32_t* __restrict__ vis_sums,
+                                                                   int64_t* __restrict__ idx)
+{
+    __shared__ uint32_t lds[SCAN_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    bool f[SCAN_ITEMS];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+    {
+        f[i] = (base + i < n) && in[base + i] != 0u;
+        c += f[i];
+    }
+    uint32_t total;
+    const uint32_t incl = block_incl_scan(c, lds, &total);
+    int64_t o = (int64_t)vis_sums[blockIdx.x] + (int64_t)(incl - c);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (f[i]) idx[o++] = base + i;
 }
 
 // ---- exclusive scan of the rank-ordered pair counts, in place ---------
@@ -257,6 +319,36 @@ int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* 
     SVR_REQUIRE(R < (1ull << 32), SVR_ERR_CAPACITY, "number of rendered entries %llu exceeds 2^32 - 1",
                 (unsigned long long)R);
     *R_host = (int64_t)R;
+    return SVR_OK;
+}
+
+int svr_visible_voxels(int64_t N, void* geom_state, int64_t* idx_out, svr_stream_t stream, int64_t* R_host,
+                       int64_t* n_visible_host)
+{
+    SVR_REQUIRE(R_host != nullptr && n_visible_host != nullptr, SVR_ERR_INVALID, "svr_visible_voxels: null output");
+    *R_host = 0;
+    *n_visible_host = 0;
+    SVR_REQUIRE(N >= 0 && N < (1ll << 32), SVR_ERR_INVALID, "voxel count out of range");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(geom_state != nullptr && idx_out != nullptr, SVR_ERR_INVALID, "svr_visible_voxels: null pointer");
+    GeomState g = GeomState::carve(geom_state, N, nullptr);
+    const int64_t nb = (N + SCAN_TILE - 1) / SCAN_TILE;
+    prof_begin(PROF_SCAN, stream);
+    count_reduce_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums, g.vis_sums);
+    SVR_CHECK_LAUNCH();
+    scan_sums2_kernel<<<2, SCAN_BLOCK, 0, stream>>>(g.block_sums, g.vis_sums, nb, g.counters);
+    SVR_CHECK_LAUNCH();
+    visible_write_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.vis_sums, idx_out);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_SCAN, stream);
+    uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+    SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    const uint64_t R = ((uint64_t)cnt[1] << 32) | cnt[0];
+    SVR_REQUIRE(R < (1ull << 32), SVR_ERR_CAPACITY, "number of rendered entries %llu exceeds 2^32 - 1",
+                (unsigned long long)R);
+    *R_host = (int64_t)R;
+    *n_visible_host = (int64_t)(((uint64_t)cnt[7] << 32) | cnt[6]);
     return SVR_OK;
 }
 

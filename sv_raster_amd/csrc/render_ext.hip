@@ -153,6 +153,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
                        sy1, qpix);
         }
 
+        hm = raybox_filter<EXACT>(st, w, hm, ray);
         uint64_t hits = 0ull;
         while (hm)
         {
@@ -160,8 +161,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
             hm &= hm - 1ull;
             const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
-            if (a > b) continue;
+            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);  // same values as in raybox_filter: !(a > b)
             float g[8], lv[NS];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];

@@ -102,9 +102,11 @@ struct GeomState
     uint2* bbox;           // [n] packed pixel bbox (x<<16 | y) min / max
     uint32_t* qbits;       // [n] camera-quadrant bitset
     uint32_t* block_sums;  // [ceil(n / SCAN_TILE) + 1]
+    uint32_t* vis_sums;    // [ceil(n / SCAN_TILE) + 1] in-frustum items per scan block
     uint32_t* counters;    // [8]: 0-1 R (total entries, 64-bit), 2 large-voxel queue length,
                            // 3 huge-voxel queue length (from the end of big_list),
-                           // 4-5 entries to sort (R, or 0 when R exceeds a fixed capacity)
+                           // 4-5 entries to sort (R, or 0 when R exceeds a fixed capacity),
+                           // 6-7 in-frustum items (n_dup > 0)
     uint32_t* big_list;    // [n] voxels with many entries, expanded by whole workgroups
     uint32_t* pos_off;     // [8n] per view, indexed by rank-order position: the pair's
                            // entry count, then (scanned in place) its first entry
@@ -120,6 +122,7 @@ struct GeomState
         g.bbox = c.take<uint2>(n);
         g.qbits = c.take<uint32_t>(n);
         g.block_sums = c.take<uint32_t>((n + SCAN_TILE - 1) / SCAN_TILE + 1);
+        g.vis_sums = c.take<uint32_t>((n + SCAN_TILE - 1) / SCAN_TILE + 1);
         g.counters = c.take<uint32_t>(8);
         g.big_list = c.take<uint32_t>(n);
         g.pos_off = c.take<uint32_t>(8 * n);

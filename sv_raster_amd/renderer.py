@@ -76,7 +76,8 @@ def rasterize_voxels(
         vox_lengths,
         raster_settings.debug,
     )
-    in_frusts_idx = torch.where(n_duplicates > 0)[0]
+    # torch.where(n_duplicates > 0)[0] and the entry count R, read with one host synchronisation
+    in_frusts_idx, num_rendered = _C.visible_voxels(geomBuffer, N, device)
 
     cam_pos = raster_settings.c2w_matrix[:3, 3]
     vox_params = vox_fn(in_frusts_idx, cam_pos, raster_settings.color_mode)
@@ -116,6 +117,7 @@ def rasterize_voxels(
         geos,
         rgbs,
         subdiv_p,
+        num_rendered,
     )
 
 
@@ -124,7 +126,8 @@ class _RasterizeVoxels(torch.autograd.Function):
     returns gradients for geos, rgbs and the subdivision priority."""
 
     @staticmethod
-    def forward(ctx, raster_settings, geomBuffer, octree_paths, vox_centers, vox_lengths, geos, rgbs, subdiv_p):
+    def forward(ctx, raster_settings, geomBuffer, octree_paths, vox_centers, vox_lengths, geos, rgbs, subdiv_p,
+                num_rendered=None):
         need_distortion = raster_settings.lambda_dist > 0
         num_rendered, binningBuffer, imgBuffer, out_color, out_depth, out_normal, out_T, max_w = \
             _C.rasterize_voxels(
@@ -149,6 +152,7 @@ class _RasterizeVoxels(torch.autograd.Function):
                 rgbs,
                 geomBuffer,
                 raster_settings.debug,
+                num_rendered=num_rendered,
             )
         ctx.raster_settings = raster_settings
         ctx.num_rendered = num_rendered
@@ -202,7 +206,7 @@ class _RasterizeVoxels(torch.autograd.Function):
             out_normal,
             raster_settings.debug,
         )
-        return None, None, None, None, None, dL_dgeos, dL_drgbs, subdiv_p_bw
+        return None, None, None, None, None, dL_dgeos, dL_drgbs, subdiv_p_bw, None
 
 
 class SH_eval(torch.autograd.Function):

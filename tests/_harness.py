@@ -52,12 +52,13 @@ def run_gpu(case, bg=0.0, n_samp=1, backward=True, device="cuda"):
     N = centers.shape[0]
     n_dup, geom = _C.rasterize_preprocess(W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w,
                                           cam["near"], octpath, centers, lengths)
-    idx = torch.where(n_dup > 0)[0]
+    idx, R_vis = _C.visible_voxels(geom, N, centers.device)
     cam_pos = c2w[:3, 3]
     rgbs = _C.sh_compute(3, idx, centers, cam_pos, sh0, shs)
     R, binning, img, color, _, _, T, _ = _C.rasterize_voxels(
         n_samp, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w, bg, False, False, False, False,
         octpath, centers, lengths, geos, rgbs, geom)
+    assert R == R_vis, (R, R_vis)
     ranges, tile_last, n_contrib = _C.unpack_ImageState(W, H, img)
     hi, lo, vals = _C.export_sort_keys(W, H, R, binning, octpath)
     gnd, gbb, gqb = _C.unpack_geom_state(N, geom)

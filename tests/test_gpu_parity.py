@@ -22,6 +22,7 @@ def _check_forward(g, o):
     assert np.array_equal(g["bbox"][live], o["bbox"][live])
     assert np.array_equal(g["qbits"][live], o["qbits"][live])
     assert g["R"] == o["vox_list"].size
+    assert np.array_equal(g["idx"], o["idx"])
     assert np.array_equal(g["keys_hi"], o["keys_hi"])
     assert np.array_equal(g["keys_lo"], o["keys_lo"])
     assert np.array_equal(g["vox_list"], o["vox_list"])
@@ -114,3 +115,30 @@ def test_parity_axis_aligned_rays():
     g = run_gpu(case)
     _check_forward(g, o)
     _check_backward(g, o)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,direction", [(5000, None), (4096, None), (8193, None), (3000, (0.0, 0.0, -1.0))])
+def test_visible_voxels(n, direction):
+    """svr_visible_voxels: the in-frustum list equals torch.where(n_dup > 0)[0] and R equals svr_num_rendered,
+    at voxel counts on and off the 4096-voxel scan block, including a camera that sees nothing of the scene
+    (placed outside it, looking away)."""
+    import torch
+    from sv_raster_amd import _C, synthetic as S
+    sc = S.make_scene("A", seed=4, n=n)
+    cam = S.make_camera(128, 96, seed=4, direction=direction)
+    if direction is not None:  # look away from the scene: every voxel behind the camera
+        cam["w2c"] = np.linalg.inv(cam["c2w"].astype(np.float64) @ np.diag([-1.0, 1.0, -1.0, 1.0])).astype(np.float32)
+        cam["c2w"] = np.linalg.inv(cam["w2c"].astype(np.float64)).astype(np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    centers, lengths, octpath = t(sc["vox_centers"]), t(sc["vox_lengths"]), t(sc["octpath"])
+    N = centers.shape[0]
+    print(f"visible_voxels N={N} direction={direction}", flush=True)
+    n_dup, geom = _C.rasterize_preprocess(cam["W"], cam["H"], cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"],
+                                          t(cam["w2c"]), t(cam["c2w"]), cam["near"], octpath, centers, lengths)
+    idx, R = _C.visible_voxels(geom, N, centers.device)
+    ref = torch.where(n_dup > 0)[0]
+    assert idx.dtype == torch.int64 and torch.equal(idx, ref)
+    assert R == int(n_dup.long().sum().item())
+    if direction is not None:
+        assert idx.numel() == 0 and R == 0
