@@ -26,7 +26,16 @@ MAX_RENDER_TILES = int(_consts.max_render_tiles)
 TILE = int(_consts.tile_x)
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(device):
+    """The current torch stream of `device` as a hipStream_t."""
+    if not isinstance(device, torch.device):
+        device = torch.device(device)
+    if _raw_stream is not None and device.type == "cuda":
+        idx = device.index if device.index is not None else torch.cuda.current_device()
+        return ctypes.c_void_p(_raw_stream(idx))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
