@@ -238,6 +238,31 @@ __global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, 
 
 // adam_step.cu:21-64 (unbiased) and 67-102 (biased); lerp(v0, v1, t) = fma(t, v1, fma(-t, v0, v0))
 // With coef (the device-step form), step_size and rb2 come from coef[0..1].
+inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > cap) b = cap;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+template <bool SPARSE, bool BIASED>
+__device__ inline void adam_elem(float& p, float& m, float& v, float gv, float step_size, float beta1, float beta2,
+                                 float rb2, float eps)
+{
+    if (SPARSE && gv == 0.0f) return;
+    const float mv = __builtin_fmaf(beta1, m, __builtin_fmaf(-beta1, gv, gv));
+    const float gg = gv * gv;
+    const float vv = __builtin_fmaf(beta2, v, __builtin_fmaf(-beta2, gg, gg));
+    float denom;
+    if (BIASED)
+        denom = __builtin_sqrtf(vv) + eps;
+    else
+        denom = __builtin_fmaf(__builtin_sqrtf(vv), rb2, eps);
+    p -= step_size * (mv / denom);
+    m = mv;
+    v = vv;
+}
+
 template <bool SPARSE, bool BIASED>
 __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
                                                    float* __restrict__ m, float* __restrict__ v, float step_size,
@@ -254,17 +279,85 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict_
     {
         const float gv = g[i];
         if (SPARSE && gv == 0.0f) continue;
-        const float mv = __builtin_fmaf(beta1, m[i], __builtin_fmaf(-beta1, gv, gv));
-        const float gg = gv * gv;
-        const float vv = __builtin_fmaf(beta2, v[i], __builtin_fmaf(-beta2, gg, gg));
-        float denom;
-        if (BIASED)
-            denom = __builtin_sqrtf(vv) + eps;
-        else
-            denom = __builtin_fmaf(__builtin_sqrtf(vv), rb2, eps);
-        p[i] -= step_size * (mv / denom);
+        float pv = p[i], mv = m[i], vv = v[i];
+        adam_elem<false, BIASED>(pv, mv, vv, gv, step_size, beta1, beta2, rb2, eps);
+        p[i] = pv;
         m[i] = mv;
         v[i] = vv;
+    }
+}
+
+// Same update on 16-byte vectors (all four arrays 16-byte aligned): one
+// float4 gradient load per thread and iteration; in the sparse form a vector
+// of zero gradients is skipped whole, and inside a mixed vector the elements
+// with a zero gradient keep their values (they are stored back unchanged).
+// The last n % 4 elements are updated one per thread.
+template <bool SPARSE, bool BIASED>
+__global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                                                    float* __restrict__ m, float* __restrict__ v, float step_size,
+                                                    float beta1, float beta2, float rb2, float eps,
+                                                    const float* __restrict__ coef)
+{
+    if (coef)
+    {
+        step_size = coef[0];
+        rb2 = coef[1];
+    }
+    const int64_t n4 = n >> 2;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float4* __restrict__ P = reinterpret_cast<float4*>(p);
+    const float4* __restrict__ G = reinterpret_cast<const float4*>(g);
+    float4* __restrict__ M = reinterpret_cast<float4*>(m);
+    float4* __restrict__ V = reinterpret_cast<float4*>(v);
+    for (int64_t i = gid; i < n4; i += stride)
+    {
+        const float4 g4 = G[i];
+        if (SPARSE && g4.x == 0.f && g4.y == 0.f && g4.z == 0.f && g4.w == 0.f) continue;
+        float4 p4 = P[i], m4 = M[i], v4 = V[i];
+        adam_elem<SPARSE, BIASED>(p4.x, m4.x, v4.x, g4.x, step_size, beta1, beta2, rb2, eps);
+        adam_elem<SPARSE, BIASED>(p4.y, m4.y, v4.y, g4.y, step_size, beta1, beta2, rb2, eps);
+        adam_elem<SPARSE, BIASED>(p4.z, m4.z, v4.z, g4.z, step_size, beta1, beta2, rb2, eps);
+        adam_elem<SPARSE, BIASED>(p4.w, m4.w, v4.w, g4.w, step_size, beta1, beta2, rb2, eps);
+        P[i] = p4;
+        M[i] = m4;
+        V[i] = v4;
+    }
+    const int64_t t = 4 * n4 + gid;
+    if (t < n)
+    {
+        float pv = p[t], mv = m[t], vv = v[t];
+        adam_elem<SPARSE, BIASED>(pv, mv, vv, g[t], step_size, beta1, beta2, rb2, eps);
+        p[t] = pv;
+        m[t] = mv;
+        v[t] = vv;
+    }
+}
+
+template <bool BIASED>
+void adam_launch(bool sparse, int64_t n, float* p, const float* g, float* m, float* v, float step_size, float beta1,
+                 float beta2, float rb2, float eps, const float* coef, hipStream_t stream)
+{
+    const bool vec = ((((uintptr_t)p) | ((uintptr_t)g) | ((uintptr_t)m) | ((uintptr_t)v)) & 15u) == 0;
+    if (vec)
+    {
+        const unsigned grid = grid_for((n >> 2) > 0 ? (n >> 2) : 1, 8192);
+        if (sparse)
+            adam4_kernel<true, BIASED><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, beta1, beta2, rb2, eps,
+                                                                 coef);
+        else
+            adam4_kernel<false, BIASED><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, beta1, beta2, rb2, eps,
+                                                                  coef);
+    }
+    else
+    {
+        const unsigned grid = grid_for(n, 8192);
+        if (sparse)
+            adam_kernel<true, BIASED><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, beta1, beta2, rb2, eps,
+                                                                coef);
+        else
+            adam_kernel<false, BIASED><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, beta1, beta2, rb2, eps,
+                                                                 coef);
     }
 }
 
@@ -320,12 +413,6 @@ __global__ void path2ijk_kernel(int64_t n, const int64_t* __restrict__ paths, co
     out[3 * t] = i; out[3 * t + 1] = j; out[3 * t + 2] = k;
 }
 
-inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20)
-{
-    int64_t b = (n + 255) / 256;
-    if (b > cap) b = cap;
-    return (unsigned)(b < 1 ? 1 : b);
-}
 
 }  // namespace svr
 
@@ -398,14 +485,8 @@ int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, 
     const double bc2 = 1.0 - pow(beta2, step);
     const float step_size = (float)(lr / bc1);
     const float rb2 = (float)(1.0 / sqrt(bc2));
-    const unsigned grid = grid_for(n, 8192);
     prof_begin(PROF_ADAM, stream);
-    if (sparse)
-        adam_kernel<true, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2, rb2,
-                                                           eps, nullptr);
-    else
-        adam_kernel<false, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, (float)beta1, (float)beta2,
-                                                            rb2, eps, nullptr);
+    adam_launch<false>(sparse != 0, n, p, g, m, v, step_size, (float)beta1, (float)beta2, rb2, eps, nullptr, stream);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_ADAM, stream);
     return SVR_OK;
@@ -417,16 +498,10 @@ int svr_unbiased_adam_step_device(int32_t sparse, int64_t n, float* p, const flo
 {
     if (n == 0) return SVR_OK;
     SVR_REQUIRE(step_dev && coef_dev, SVR_ERR_INVALID, "svr_unbiased_adam_step_device: null step or coefficients");
-    const unsigned grid = grid_for(n, 8192);
     prof_begin(PROF_ADAM, stream);
     adam_coef_kernel<<<1, 64, 0, stream>>>(step_dev, lr, beta1, beta2, coef_dev);
     SVR_CHECK_LAUNCH();
-    if (sparse)
-        adam_kernel<true, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps,
-                                                           coef_dev);
-    else
-        adam_kernel<false, false><<<grid, 256, 0, stream>>>(n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps,
-                                                            coef_dev);
+    adam_launch<false>(sparse != 0, n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps, coef_dev, stream);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_ADAM, stream);
     return SVR_OK;
@@ -436,12 +511,8 @@ int svr_biased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, fl
                          float beta1, float beta2, float eps, svr_stream_t stream)
 {
     if (n == 0) return SVR_OK;
-    const unsigned grid = grid_for(n, 8192);
     prof_begin(PROF_ADAM, stream);
-    if (sparse)
-        adam_kernel<true, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps, nullptr);
-    else
-        adam_kernel<false, true><<<grid, 256, 0, stream>>>(n, p, g, m, v, lr, beta1, beta2, 1.f, eps, nullptr);
+    adam_launch<true>(sparse != 0, n, p, g, m, v, lr, beta1, beta2, 1.f, eps, nullptr, stream);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_ADAM, stream);
     return SVR_OK;
