@@ -69,6 +69,7 @@ struct BatchBwdWork
     float* partial;   // [R][12]
     float* pair_geo;  // [V*N][8]
     float* pair_sub;  // [V*N]
+    float* huge;      // [HUGE_SPLIT_ROWS][12] partial sums of split huge pairs
     static BatchBwdWork carve(void* p, int64_t VN, int64_t R, size_t* bytes)
     {
         Carver c(p);
@@ -76,6 +77,7 @@ struct BatchBwdWork
         w.partial = c.take<float>(12 * R);
         w.pair_geo = c.take<float>(8 * VN);
         w.pair_sub = c.take<float>(VN);
+        w.huge = c.take<float>(12 * HUGE_SPLIT_ROWS);
         if (bytes) *bytes = c.size();
         return w;
     }
@@ -910,7 +912,7 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
     prof_begin(PROF_GRAD_REDUCE, stream);
     if (R > 0)
     {
-        rc = grad_reduce_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, stream,
+        rc = grad_reduce_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, wk.huge, stream,
                                 capacity_mode ? (uint32_t)R : 0xFFFFFFFFu);
         if (rc != SVR_OK) return rc;
     }

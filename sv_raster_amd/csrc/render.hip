@@ -105,9 +105,10 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
 // Per-voxel gradient = sum of the partial rows of its entries: one run of
 // consecutive rows per quadrant (run_offset), quadrants ascending.  Voxels with
 // more than BIG_DUP entries (listed in big_list by the duplication pass) are
-// summed by a whole workgroup, those with more than HUGE_DUP by every
-// workgroup of the launch (partial sums added with float atomics into rows
-// that grad_reduce_kernel zeroes first).
+// summed by a whole workgroup; those with more than HUGE_DUP are split over
+// several workgroups, whose partial sums land in scratch rows that
+// grad_reduce_huge_kernel adds in a fixed order (no atomics: the result does
+// not depend on scheduling).
 
 __device__ inline void write_grad(size_t v, const float* a, float* __restrict__ dgeos, float* __restrict__ drgbs,
                                   float* __restrict__ dsub)
@@ -129,8 +130,7 @@ __device__ inline void add_row(float (&a)[12], const float* __restrict__ partial
     a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
 }
 
-// One thread per item with at most BIG_DUP entries; rows of the huge items
-// are zeroed here for grad_reduce_big_kernel's atomics.  A run ending beyond
+// One thread per item with at most BIG_DUP entries.  A run ending beyond
 // `cap` was never emitted (a frame over a fixed capacity) and counts as zero.
 __global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N, uint32_t cap,
                                                           const float* __restrict__ partial,
@@ -140,12 +140,6 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
     const uint32_t nd = g.n_dup[v];
-    if (nd > HUGE_DUP)
-    {
-        const float z[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        write_grad((size_t)v, z, dgeos, drgbs, dsub);
-        return;
-    }
     if (nd > BIG_DUP) return;
     float a[12];
 #pragma unroll
@@ -164,18 +158,23 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N
     write_grad((size_t)v, a, dgeos, drgbs, dsub);
 }
 
+// Row ranges per huge item (HUGE_SPLIT_ROWS / items, at least 1).
+__device__ inline uint32_t huge_split(uint32_t n_huge) { return n_huge ? max(1u, HUGE_SPLIT_ROWS / n_huge) : 1u; }
+
+// Launched with HUGE_SPLIT_ROWS workgroups.
 __global__ __launch_bounds__(256) void grad_reduce_big_kernel(GeomState g, int64_t N, uint32_t cap,
                                                               const float* __restrict__ partial,
                                                               float* __restrict__ dgeos, float* __restrict__ drgbs,
-                                                              float* __restrict__ dsub)
+                                                              float* __restrict__ dsub, float* __restrict__ huge_rows)
 {
     __shared__ float red[12][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // pass 0: one workgroup per large voxel, blocks striding over the queue;
     // pass 1: the huge voxels (queued from the end), each split into K row
-    // ranges taken by K different workgroups (item = chunk * huge + voxel)
+    // ranges taken by K different workgroups (item = chunk * huge + voxel);
+    // with K > 1 item `it` writes its sums to scratch row `it`
     const uint32_t counts[2] = {g.counters[2], g.counters[3]};
-    const uint32_t K = counts[1] ? max(1u, gridDim.x / counts[1]) : 1u;
+    const uint32_t K = huge_split(counts[1]);
     for (uint32_t pass = 0; pass < 2; ++pass)
     for (uint32_t it = blockIdx.x; it < (pass == 0 ? counts[0] : counts[1] * K); it += gridDim.x)
     {
@@ -213,29 +212,54 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(GeomState g, int64
             float t[12];
 #pragma unroll
             for (int i = 0; i < 12; ++i) t[i] = (red[i][0] + red[i][1]) + (red[i][2] + red[i][3]);
-            if (pass == 0)
+            if (pass == 0 || K == 1)
             {
                 write_grad((size_t)v, t, dgeos, drgbs, dsub);
             }
             else
             {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) unsafeAtomicAdd(dgeos + 8 * (size_t)v + k, t[k]);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) unsafeAtomicAdd(drgbs + 3 * (size_t)v + k, t[8 + k]);
-                unsafeAtomicAdd(dsub + v, t[11]);
+                float4* row = reinterpret_cast<float4*>(huge_rows + 12 * (size_t)it);
+                row[0] = make_float4(t[0], t[1], t[2], t[3]);
+                row[1] = make_float4(t[4], t[5], t[6], t[7]);
+                row[2] = make_float4(t[8], t[9], t[10], t[11]);
             }
         }
         __syncthreads();
     }
 }
 
+// Huge items split over K > 1 workgroups: item i adds its K scratch rows
+// (c * huge + i, c ascending), one wave per item.
+__global__ __launch_bounds__(64) void grad_reduce_huge_kernel(GeomState g, int64_t N,
+                                                              const float* __restrict__ huge_rows,
+                                                              float* __restrict__ dgeos, float* __restrict__ drgbs,
+                                                              float* __restrict__ dsub)
+{
+    const uint32_t n_huge = g.counters[3];
+    const uint32_t K = huge_split(n_huge);
+    if (K == 1) return;
+    const int lane = threadIdx.x;
+    for (uint32_t i = blockIdx.x; i < n_huge; i += gridDim.x)
+    {
+        float a[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) a[k] = 0.f;
+        for (uint32_t c = (uint32_t)lane; c < K; c += 64) add_row(a, huge_rows, c * n_huge + i);
+        float t[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) t[k] = wave_sum_f32(a[k]);
+        if (lane == 0) write_grad((size_t)g.big_list[N - 1 - i], t, dgeos, drgbs, dsub);
+    }
+}
+
 int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs, float* dsub,
-                       hipStream_t stream, uint32_t cap)
+                       float* huge_rows, hipStream_t stream, uint32_t cap)
 {
     grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
-    grad_reduce_big_kernel<<<1024, 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
+    grad_reduce_big_kernel<<<HUGE_SPLIT_ROWS, 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub, huge_rows);
+    SVR_CHECK_LAUNCH();
+    grad_reduce_huge_kernel<<<64, 64, 0, stream>>>(g, N, huge_rows, dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }
@@ -282,7 +306,7 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);
-    const int rc = grad_reduce_launch(N, g, partial, dgeos, drgbs, dsub, stream);
+    const int rc = grad_reduce_launch(N, g, partial, dgeos, drgbs, dsub, huge_rows(partial, R), stream);
     prof_end(PROF_GRAD_REDUCE, stream);
     return rc;
 }
@@ -374,7 +398,7 @@ int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, f
 
 size_t svr_backward_workspace_size(int64_t R)
 {
-    return align_up(sizeof(float) * 12 * (size_t)(R > 0 ? R : 0), 256);
+    return align_up(sizeof(float) * 12 * (size_t)(R > 0 ? R : 0), 256) + sizeof(float) * 12 * HUGE_SPLIT_ROWS;
 }
 
 int svr_rasterize_backward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx, float cy,

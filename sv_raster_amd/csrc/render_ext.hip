@@ -580,7 +580,7 @@ int launch_bwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);
-    const int rc = grad_reduce_launch(N, g, partial, dgeos, drgbs, dsub, stream);
+    const int rc = grad_reduce_launch(N, g, partial, dgeos, drgbs, dsub, huge_rows(partial, R), stream);
     prof_end(PROF_GRAD_REDUCE, stream);
     return rc;
 }

@@ -67,6 +67,10 @@ constexpr uint32_t BIG_DUP = 32;
 // that no single workgroup becomes its tail; they are queued from the end of
 // big_list (count in counters[3]).
 constexpr uint32_t HUGE_DUP = 4096;
+// Workgroups of grad_reduce_big_kernel; a huge item is split into up to
+// HUGE_SPLIT_ROWS / (number of huge items) row ranges, whose partial sums go
+// to a scratch of HUGE_SPLIT_ROWS 12-float rows and are added in a fixed order.
+constexpr uint32_t HUGE_SPLIT_ROWS = 1024;
 
 // Queues item `id` with nd entries (nd > BIG_DUP) for the large-voxel passes.
 __device__ inline void queue_large(uint32_t* __restrict__ big_list, uint32_t* __restrict__ counts, int64_t n_items,
@@ -293,8 +297,15 @@ int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hip
 // gradient rows over the item's quadrant runs (GeomState::qoff), quadrants
 // ascending and each run in emission order.  A run ending beyond `cap` was
 // never emitted and counts as zero.
+// Scratch rows of the huge-item split in the single-view backward workspace:
+// after the R partial rows (svr_backward_workspace_size).
+inline float* huge_rows(float* partial, int64_t R)
+{
+    return partial + align_up(sizeof(float) * 12 * (size_t)R, 256) / sizeof(float);
+}
+
 int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
-                       float* dsub, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
+                       float* dsub, float* huge_rows, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
 
 // Argument checks, state carving and (forward) binning shared by the plain and
 // the extended compositor entry points.  backward_setup returns 1 after
