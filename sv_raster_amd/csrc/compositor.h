@@ -374,6 +374,7 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
 {
     static_assert(TS::NW <= 16, "per-wave chunk counts are kept for at most 16 waves");
     constexpr size_t STRIDE = 64 * TS::NW;
+    constexpr int PER = (KW * 64 + TS::THREADS - 1) / TS::THREADS;  // window entries per thread
     const int t = threadIdx.x;
     const uint32_t my_chunks = (wave_max_u32(nc) + 63u) >> 6;
     if (lane == 0) sh.nch[w] = my_chunks;
@@ -384,6 +385,30 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
 #pragma unroll
     for (int i = 0; i < TS::NW; ++i) C = max(C, sh.nch[i]);
 
+    // Software pipeline over windows: the hit masks of the next window and the
+    // list values / emission rows of its entries are loaded while the current
+    // window is staged and walked, so a window starts with its masks in
+    // registers and its payload loads need one round trip.
+    uint64_t m_pf[KW];
+    uint32_t val_pf[PER], orow_pf[PER];
+    auto prefetch = [&](int c0) {
+        const int nj = min(KW, c0 + 1);
+#pragma unroll
+        for (int j = 0; j < KW; ++j)
+            m_pf[j] = (j < nj && (uint32_t)(c0 - j) < my_chunks) ? masks[(size_t)(c0 - j) * STRIDE] : 0ull;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+        {
+            const int i = t + k * TS::THREADS;
+            const int j = i >> 6, l = i & 63;
+            const uint32_t e = range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l;
+            const bool ok = i < KW * 64 && j < nj && e < range.y;
+            val_pf[k] = ok ? vox_list[e] : 0u;
+            orow_pf[k] = ok ? orig[e] : 0u;
+        }
+    };
+    if (C > 0) prefetch((int)C - 1);
+
     for (int c0 = (int)C - 1; c0 >= 0; c0 -= KW)
     {
         const int nj = min(KW, c0 + 1);
@@ -392,11 +417,10 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
 #pragma unroll
         for (int j = 0; j < KW; ++j)
         {
-            m[j] = 0ull;
+            m[j] = m_pf[j];
             syncm[j] = 0ull;
             if (j < nj && (uint32_t)(c0 - j) < my_chunks)
             {
-                m[j] = masks[(size_t)(c0 - j) * STRIDE];
                 const uint64_t er = wave_transpose64(m[j], lane);
                 if (er) sh.used[j][lane] = 1u;
                 syncm[j] = __ballot(__popcll(er) >= SYNC_MIN);
@@ -404,22 +428,24 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
         }
         __syncthreads();
         // 2. stage the used entries' payload
-        for (int i = t; i < KW * 64; i += TS::THREADS)
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
         {
+            const int i = t + k * TS::THREADS;
             const int j = i >> 6, l = i & 63;
-            if (j < nj && sh.used[j][l])
+            if (i < KW * 64 && j < nj && sh.used[j][l])
             {
-                const uint32_t e = range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l;
-                const size_t v = vox_list[e] & VOX_ID_MASK;
+                const size_t v = val_pf[k] & VOX_ID_MASK;
                 const float len = lengths[v];
                 EntryPay& p = sh.pay[j][l];
                 p.q[0] = make_float4(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], len);
                 p.q[1] = *reinterpret_cast<const float4*>(geos + 8 * v);
                 p.q[2] = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
                 p.q[3] = make_float4(rgbs[3 * v + 0], rgbs[3 * v + 1], rgbs[3 * v + 2], 1.f / len);
-                sh.orow[j][l] = orig[e];
+                sh.orow[j][l] = orow_pf[k];
             }
         }
+        if (c0 - KW >= 0) prefetch(c0 - KW);
         __syncthreads();
         // 3. walk this wave's hits
 #pragma unroll
