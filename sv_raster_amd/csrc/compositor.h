@@ -226,7 +226,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
             const float vol = vox_vol<NS, false>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, nullptr);
-            const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
+            const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf_core(-vol));
             if (alpha < MIN_ALPHA) continue;
             if constexpr (MASK) hits |= 1ull << l;
             const float pt_w = T * alpha;
@@ -275,18 +275,20 @@ __device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, f
     ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
     if (a > b) return false;
     const float g[8] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
-    float dI[8];
-    const float vol = vox_vol<NS, true>(c0, c1, c2, len, a3.w, g, a, b, ray, dI);
-    const float e = svr_expf(-vol);
+    // NS == 1: dI[i] = dd * w[i] is folded into the gradient scale below
+    float dI[8], dd = 1.f;
+    const float vol = (NS == 1) ? vox_vol1_w(c0, c1, c2, len, a3.w, g, a, b, ray, dI, dd)
+                                : vox_vol<NS, true>(c0, c1, c2, len, a3.w, g, a, b, ray, dI);
+    const float e = svr_expf_core(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     if (alpha < MIN_ALPHA) return false;
-    T = T / (1.f - alpha);
+    T = div_refined(T, 1.f - alpha);
     const float pt_w = alpha * T;
     const float dpt = __builtin_fmaf(dp2, a3.z, __builtin_fmaf(dp1, a3.y, dp0 * a3.x));
     const float diff = dpt - last_dT;
     const float dalpha = T * diff;
     last_dT = __builtin_fmaf(alpha, diff, last_dT);
-    const float dIs = dalpha * e;
+    const float dIs = (dalpha * e) * dd;
 #pragma unroll
     for (int i = 0; i < 8; ++i) gq[i] = dIs * dI[i];
     gq[8] = pt_w * dp0;

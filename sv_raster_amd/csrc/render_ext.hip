@@ -56,9 +56,9 @@ __device__ inline float entry_depth(float a, float b, float alpha, const float* 
     if constexpr (NS == 3)
     {
         const float st = 0.3333333f * (b - a);
-        const float a0 = fminf(MAX_ALPHA, 1.f - svr_expf(-lv[0]));
-        const float a1 = fminf(MAX_ALPHA, 1.f - svr_expf(-lv[1]));
-        const float a2 = fminf(MAX_ALPHA, 1.f - svr_expf(-lv[2]));
+        const float a0 = fminf(MAX_ALPHA, 1.f - svr_expf_core(-lv[0]));
+        const float a1 = fminf(MAX_ALPHA, 1.f - svr_expf_core(-lv[1]));
+        const float a2 = fminf(MAX_ALPHA, 1.f - svr_expf_core(-lv[2]));
         const float t0 = a + 0.5f * st, t1 = a + 1.5f * st, t2 = a + 2.5f * st;
         if (dD)
         {
@@ -71,8 +71,8 @@ __device__ inline float entry_depth(float a, float b, float alpha, const float* 
     else if constexpr (NS == 2)
     {
         const float st = 0.5f * (b - a);
-        const float a0 = fminf(MAX_ALPHA, 1.f - svr_expf(-lv[0]));
-        const float a1 = fminf(MAX_ALPHA, 1.f - svr_expf(-lv[1]));
+        const float a0 = fminf(MAX_ALPHA, 1.f - svr_expf_core(-lv[0]));
+        const float a1 = fminf(MAX_ALPHA, 1.f - svr_expf_core(-lv[1]));
         const float t0 = a + 0.5f * st, t1 = a + 1.5f * st;
         if (dD)
         {
@@ -167,7 +167,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
             const float vol = vox_samples<NS, false>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, lv, nullptr,
                                                      nullptr);
-            const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf(-vol));
+            const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf_core(-vol));
             if (alpha < MIN_ALPHA) continue;
             hits |= 1ull << l;
             const float pt_w = T * alpha;
@@ -345,7 +345,7 @@ __device__ inline bool bwd_hit_ext(const EntryPay& pe, const PixRay& ray, const 
     float dI[8], lv[NS], edI[NS][8];
     const float linv = a3.w;
     const float vol = vox_samples<NS, true>(c0, c1, c2, len, linv, g, a, b, ray, lv, edI, dI);
-    const float e = svr_expf(-vol);
+    const float e = svr_expf_core(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     if (alpha < MIN_ALPHA) return false;
     T = T / (1.f - alpha);

@@ -49,7 +49,12 @@ __host__ __device__ inline uint32_t required_bits(uint64_t n)
     return b;
 }
 
-__device__ inline float svr_expf(float x)
+// svr_expf without its overflow and NaN checks: the same value for every
+// x <= 88.72283935546875 (above that svr_expf returns +inf and this one the
+// ldexp overflow, +inf, up to the largest reduced arguments); a NaN input
+// gives a NaN either way.  The compositors call it on -vol <= 0 and on the
+// exp_linear_11 argument, which is <= 0.1 wherever its value is used.
+__device__ inline float svr_expf_core(float x)
 {
     const float kf = __builtin_rintf(x * 1.44269502162933349609375f);
     float r = __builtin_fmaf(-kf, 0.693145751953125f, x);
@@ -63,9 +68,14 @@ __device__ inline float svr_expf(float x)
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
     // exact scaling by 2^k with a single rounding (also into the denormal range)
-    float res = __builtin_ldexpf(p, (int)kf);
+    const float res = __builtin_ldexpf(p, (int)kf);
+    return (x < -103.97208404541015625f) ? 0.f : res;
+}
+
+__device__ inline float svr_expf(float x)
+{
+    float res = svr_expf_core(x);
     res = (x > 88.72283935546875f) ? __uint_as_float(0x7f800000u) : res;
-    res = (x < -103.97208404541015625f) ? 0.f : res;
     res = (x != x) ? x : res;
     return res;
 }
@@ -194,7 +204,7 @@ __device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_i
 #pragma unroll
         for (int i = 1; i < 8; ++i) d = __builtin_fmaf(geo[i], w[i], d);
         // exp_linear_11 and its derivative share one exponential
-        const float ex = svr_expf(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
+        const float ex = svr_expf_core(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
         const bool lin = d > 1.1f;
         vol = vol + scale * (lin ? d : ex);
         if (DI)
@@ -206,6 +216,44 @@ __device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_i
         qt0 = qt0 + qs0; qt1 = qt1 + qs1; qt2 = qt2 + qs2;
     }
     return vol;
+}
+
+// vox_vol<1, true> with the density derivative kept apart: w receives the 8
+// trilinear weights and dd the factor 100*step*exp_linear_11'(d), so that
+// dI[i] = dd * w[i] (the same vol; the backward scales w by dL/dvol * dd once).
+__device__ inline float vox_vol1_w(float c0, float c1, float c2, float l, float l_inv, const float* geo, float a,
+                                   float b, const PixRay& ray, float* w, float& dd)
+{
+    const float step_sz = b - a;
+    const float t0 = __builtin_fmaf(0.5f, step_sz, a);
+    const float r = 0.5f * l;
+    const float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - (c0 - r)) * l_inv;
+    const float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - (c1 - r)) * l_inv;
+    const float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - (c2 - r)) * l_inv;
+    const float scale = STEP_SZ_SCALE * step_sz;
+    const float wx[2] = {1.f - qt0, qt0};
+    const float wy[2] = {1.f - qt1, qt1};
+    const float wz[2] = {1.f - qt2, qt2};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = (wx[(i >> 2) & 1] * wy[(i >> 1) & 1]) * wz[i & 1];
+    float d = geo[0] * w[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) d = __builtin_fmaf(geo[i], w[i], d);
+    const float ex = svr_expf_core(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
+    const bool lin = d > 1.1f;
+    dd = scale * (lin ? 1.0f : 0.909090909091f * ex);
+    return 0.f + scale * (lin ? d : ex);
+}
+
+// x / y from the hardware reciprocal and one Newton correction (within an
+// ulp or so of the IEEE quotient; y is a normal positive number here).  The
+// backward recovers the transmittance before a hit with it; only gradients,
+// held to a relative tolerance, depend on it.
+__device__ inline float div_refined(float x, float y)
+{
+    const float r = __builtin_amdgcn_rcpf(y);
+    const float q = x * r;
+    return __builtin_fmaf(__builtin_fmaf(-q, y, x), r, q);
 }
 
 // Per-sample form of vox_vol (same vol and dI): lv[k] is sample k's volume
@@ -243,7 +291,7 @@ __device__ inline float vox_samples(float c0, float c1, float c2, float l, float
         float d = geo[0] * w[0];
 #pragma unroll
         for (int i = 1; i < 8; ++i) d = __builtin_fmaf(geo[i], w[i], d);
-        const float ex = svr_expf(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
+        const float ex = svr_expf_core(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
         const bool lin = d > 1.1f;
         const float term = scale * (lin ? d : ex);
         lv[k] = term;
