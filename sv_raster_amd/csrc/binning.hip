@@ -537,9 +537,68 @@ __global__ __launch_bounds__(256) void tile_queue_keys_kernel(int64_t T, const u
     order[g] = (uint32_t)g;
 }
 
+// The whole queue in one workgroup (frames of at most QUEUE_ONE_BLOCK_MAX
+// tiles): bucket histogram in LDS, bucket offsets longest first, then every
+// tile takes the next slot of its bucket.  The order inside a bucket follows
+// the LDS atomics (it only affects scheduling, never a result).
+constexpr int64_t QUEUE_ONE_BLOCK_MAX = 32768;
+__global__ __launch_bounds__(1024) void tile_queue_block_kernel(int64_t T, const uint2* __restrict__ ranges,
+                                                                uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t base[256];
+    const int t = threadIdx.x;
+    if (t < 256) hist[t] = 0u;
+    __syncthreads();
+    for (int64_t g = t; g < T; g += blockDim.x)
+    {
+        const uint2 r = ranges[g];
+        atomicAdd(&hist[length_bucket(r.y - r.x)], 1u);
+    }
+    __syncthreads();
+    if (t < 64)
+    {
+        // exclusive offsets over buckets 255, 254, ..., 0 (lane t owns buckets 255 - 4t - k)
+        uint32_t c[4], s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            c[k] = hist[255 - 4 * t - k];
+            s += c[k];
+        }
+        uint32_t incl = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (t >= o) incl += y;
+        }
+        uint32_t run = incl - s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            base[255 - 4 * t - k] = run;
+            run += c[k];
+        }
+    }
+    __syncthreads();
+    for (int64_t g = t; g < T; g += blockDim.x)
+    {
+        const uint2 r = ranges[g];
+        const uint32_t slot = atomicAdd(&base[length_bucket(r.y - r.x)], 1u);
+        order[slot] = (uint32_t)g;
+    }
+}
+
 int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t* qkeys_alt, uint32_t* order,
                       uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream)
 {
+    if (T <= QUEUE_ONE_BLOCK_MAX)
+    {
+        tile_queue_block_kernel<<<1, 1024, 0, stream>>>(T, ranges, order);
+        SVR_CHECK_LAUNCH();
+        return SVR_OK;
+    }
     tile_queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, ranges, qkeys_alt, order_alt);
     SVR_CHECK_LAUNCH();
     uint64_t* qk = nullptr;

@@ -279,8 +279,9 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
 
 // Tile queue: order[0..T) = tile ids sorted by length_bucket of their list,
-// longest first, stable (keys and order are built in the _alt buffers, one
-// 8-bit radix pass lands them in qkeys / order).
+// longest first.  Up to 32768 tiles one workgroup builds it (histogram and
+// scatter in LDS); larger batches sort the keys (built in the _alt buffers,
+// one stable 8-bit radix pass lands them in qkeys / order).
 int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t* qkeys_alt, uint32_t* order,
                       uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream);
 
