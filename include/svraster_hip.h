@@ -84,9 +84,16 @@ int svr_num_rendered(int64_t n_voxels, void* geom_state, svr_stream_t stream, in
  * `torch.where(n_duplicates > 0)[0]` (renderer.py:81) and num_rendered
  * (forward.cu:649-663).  Writes the ascending ids of the voxels with
  * n_duplicates > 0 into idx_out[0 .. *num_visible_host) (int64, room for
- * n_voxels ids) and returns R and that count on the host. */
-int svr_visible_voxels(int64_t n_voxels, void* geom_state, int64_t* idx_out, svr_stream_t stream,
-                       int64_t* num_rendered_host, int64_t* num_visible_host);
+ * n_voxels ids) and returns R and that count on the host.  With an order
+ * table (svr_build_order_table of the same octree; may be NULL) it also
+ * queues the part of the forward's binning that does not need R, to run on
+ * the device while the host goes on: a svr_rasterize_forward(_ext) of this
+ * geometry state then passes SVR_FWD_PAIRS_READY. */
+int svr_visible_voxels(int64_t n_voxels, void* geom_state, const void* order_table, int64_t* idx_out,
+                       svr_stream_t stream, int64_t* num_rendered_host, int64_t* num_visible_host);
+
+/* flags of svr_rasterize_forward(_ext) */
+enum { SVR_FWD_PAIRS_READY = 1 /* svr_visible_voxels already prepared the binning with this order table */ };
 
 /* Builds the rank-order table of an octree: the position of every
  * (voxel, quadrant) pair in the order of the reference's 63-bit order rank
@@ -104,7 +111,7 @@ int svr_rasterize_forward(int32_t n_samp, int32_t width, int32_t height, float t
                           float cy, const float* c2w, float bg_color, int64_t n_voxels, const float* vox_centers,
                           const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,
                           const void* order_table, int64_t num_rendered, void* binning_state, void* image_state,
-                          float* out_color, float* out_T, svr_stream_t stream);
+                          float* out_color, float* out_T, int32_t flags, svr_stream_t stream);
 
 /* Workspace of svr_rasterize_backward for R entries: one 12-float partial
  * gradient row per list entry. */
@@ -136,7 +143,7 @@ int svr_rasterize_forward_ext(int32_t n_samp, int32_t width, int32_t height, flo
                               const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,
                               const void* order_table, int64_t num_rendered, void* binning_state, void* image_state,
                               float* out_color, float* out_T, float* out_depth, float* out_normal, float* max_w,
-                              svr_stream_t stream);
+                              int32_t flags, svr_stream_t stream);
 
 /* Replaces `_C.rasterize_voxels_backward` (backward.cu:654-772) with the
  * optional terms (backward.cu:160-510): need_depth back-propagates

@@ -24,6 +24,7 @@ MAX_SORT_KEY_BITS = int(_consts.max_sort_key_bits)
 MAX_PACKED_IMAGE_DIM = int(_consts.max_packed_image_dim)
 MAX_RENDER_TILES = int(_consts.max_render_tiles)
 TILE = int(_consts.tile_x)
+SVR_FWD_PAIRS_READY = 1  # flags of svr_rasterize_forward(_ext), include/svraster_hip.h
 
 
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
@@ -136,25 +137,27 @@ def rasterize_preprocess(image_width, image_height, tan_fovx, tan_fovy, cx, cy, 
     return out_n_dup, geom
 
 
-def visible_voxels(geomBuffer, n_voxels, device):
+def visible_voxels(geomBuffer, n_voxels, device, order_table=None):
     """In-frustum voxel ids (int64, ascending; what renderer.py:81 gets from torch.where(n_duplicates > 0)[0])
     and the number of rendered entries R, from the geometry state of rasterize_preprocess, with one host
-    synchronisation for both."""
+    synchronisation for both.  With the octree's order table the binning work that does not need R is queued
+    as well; the forward of this geometry state is then called with pairs_ready=True."""
     N = int(n_voxels)
     if N == 0:
         return torch.empty(0, dtype=torch.int64, device=device), 0
     idx = torch.empty(N, dtype=torch.int64, device=device)
     R, nv = ctypes.c_int64(0), ctypes.c_int64(0)
-    _lib.check(_L.svr_visible_voxels(N, _ptr(geomBuffer), _ptr(idx), _stream(device), ctypes.byref(R),
-                                     ctypes.byref(nv)), "visible_voxels")
+    _lib.check(_L.svr_visible_voxels(N, _ptr(geomBuffer), _ptr(order_table), _ptr(idx), _stream(device),
+                                     ctypes.byref(R), ctypes.byref(nv)), "visible_voxels")
     return idx[:int(nv.value)], int(R.value)
 
 
 # binding.cpp:26 -> forward.cu:792-879
 def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix, c2w_matrix,
                      bg_color, need_depth, need_distortion, need_normal, track_max_w, octree_paths, vox_centers,
-                     vox_lengths, geos, rgbs, geomBuffer, debug=False, num_rendered=None):
-    """num_rendered: R when the caller already has it (visible_voxels); otherwise it is read here."""
+                     vox_lengths, geos, rgbs, geomBuffer, debug=False, num_rendered=None, pairs_ready=False):
+    """num_rendered: R when the caller already has it (visible_voxels); otherwise it is read here.
+    pairs_ready: visible_voxels ran with this octree's order table on geomBuffer."""
     if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
         raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
     if rgbs.ndimension() != 2 or rgbs.size(1) != 3:
@@ -182,6 +185,7 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
     img = torch.empty(int(_L.svr_image_state_size(W, H, R)), dtype=torch.uint8, device=dev)
     binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
     table = order_table_cache.get(octree_paths) if R > 0 else None
+    flags = SVR_FWD_PAIRS_READY if pairs_ready else 0
     centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
     c2w = _f32(c2w_matrix)
     if ext:
@@ -191,12 +195,13 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
             float(bg_color), int(bool(need_depth)), int(bool(need_distortion)), int(bool(need_normal)),
             int(bool(track_max_w)), N, _ptr(centers), _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer),
             _ptr(table), R, _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), _ptr(out_depth),
-            _ptr(out_normal), _ptr(max_w), _stream(dev)), "rasterize_voxels")
+            _ptr(out_normal), _ptr(max_w), flags, _stream(dev)), "rasterize_voxels")
     else:
         _lib.check(_L.svr_rasterize_forward(int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx),
                                             float(cy), _ptr(c2w), float(bg_color), N, _ptr(centers), _ptr(lengths),
                                             _ptr(geos_c), _ptr(rgbs_c), _ptr(geomBuffer), _ptr(table), R,
-                                            _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), _stream(dev)),
+                                            _ptr(binning), _ptr(img), _ptr(out_color), _ptr(out_T), flags,
+                                            _stream(dev)),
                    "rasterize_voxels")
     _debug(debug, dev, "rasterize_voxels")
     return R, binning, img, out_color, out_depth, out_normal, out_T, max_w

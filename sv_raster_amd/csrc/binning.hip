@@ -609,8 +609,14 @@ int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t*
     return SVR_OK;
 }
 
+int pair_prepare(int64_t N, const GeomState& g, const OrderTable& ot, hipStream_t stream)
+{
+    SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
+    return pair_offsets_launch(1, N, g, ot.pos, stream);
+}
+
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
-                    BinningState& b, ImageState& img, hipStream_t stream)
+                    BinningState& b, ImageState& img, hipStream_t stream, bool pairs_ready)
 {
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int64_t T = (int64_t)gx * gy;
@@ -625,8 +631,7 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     }
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
-    SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
-    int rc = pair_offsets_launch(1, N, g, ot.pos, stream);
+    int rc = pairs_ready ? SVR_OK : pair_prepare(N, g, ot, stream);
     if (rc != SVR_OK) return rc;
     // entries are emitted in rank order, so the sort only orders the tile
     // bits; it ping-pongs between the two buffers, one digit per pass: emit

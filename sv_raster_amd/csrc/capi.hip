@@ -36,6 +36,23 @@ int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream)
     return SVR_OK;
 }
 
+hipError_t thread_event(hipEvent_t* ev)
+{
+    constexpr int MAX_DEV = 64;
+    thread_local hipEvent_t events[MAX_DEV] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEV) return hipErrorInvalidDevice;
+    if (!events[dev])
+    {
+        e = hipEventCreateWithFlags(&events[dev], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    *ev = events[dev];
+    return hipSuccess;
+}
+
 struct ProfPair
 {
     hipEvent_t start, stop;

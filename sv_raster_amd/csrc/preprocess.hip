@@ -322,8 +322,8 @@ int svr_num_rendered(int64_t N, void* geom_state, svr_stream_t stream, int64_t* 
     return SVR_OK;
 }
 
-int svr_visible_voxels(int64_t N, void* geom_state, int64_t* idx_out, svr_stream_t stream, int64_t* R_host,
-                       int64_t* n_visible_host)
+int svr_visible_voxels(int64_t N, void* geom_state, const void* order_table, int64_t* idx_out, svr_stream_t stream,
+                       int64_t* R_host, int64_t* n_visible_host)
 {
     SVR_REQUIRE(R_host != nullptr && n_visible_host != nullptr, SVR_ERR_INVALID, "svr_visible_voxels: null output");
     *R_host = 0;
@@ -343,7 +343,22 @@ int svr_visible_voxels(int64_t N, void* geom_state, int64_t* idx_out, svr_stream
     prof_end(PROF_SCAN, stream);
     uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
-    SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    if (order_table)
+    {
+        // wait for the copy only: the pair offsets queued behind it keep the
+        // device busy while the caller evaluates the colours of the list
+        hipEvent_t ev = nullptr;
+        SVR_CHECK_HIP(thread_event(&ev));
+        SVR_CHECK_HIP(hipEventRecord(ev, stream));
+        OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
+        const int rc = pair_prepare(N, g, ot, stream);
+        if (rc != SVR_OK) return rc;
+        SVR_CHECK_HIP(hipEventSynchronize(ev));
+    }
+    else
+    {
+        SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    }
     const uint64_t R = ((uint64_t)cnt[1] << 32) | cnt[0];
     SVR_REQUIRE(R < (1ull << 32), SVR_ERR_CAPACITY, "number of rendered entries %llu exceeds 2^32 - 1",
                 (unsigned long long)R);

@@ -320,7 +320,8 @@ namespace svr {
 int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* centers,
                   const float* lengths, const float* geos, const float* rgbs, const void* geom_state,
                   const void* order_table, int64_t R, void* binning_state, void* image_state, float* out_color,
-                  float* out_T, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream)
+                  float* out_T, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream,
+                  int32_t flags)
 {
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
@@ -337,7 +338,7 @@ int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_
     OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
     b = BinningState::carve(binning_state, R, nullptr);
     img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr, R);
-    return binning_forward(W, H, N, g, ot, R, b, img, stream);
+    return binning_forward(W, H, N, g, ot, R, b, img, stream, (flags & SVR_FWD_PAIRS_READY) != 0);
 }
 
 // Returns 1 (with the outputs zeroed) when there is nothing to composite.
@@ -377,13 +378,13 @@ int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, f
                           const float* c2w, float bg, int64_t N, const float* centers, const float* lengths,
                           const float* geos, const float* rgbs, const void* geom_state, const void* order_table,
                           int64_t R, void* binning_state, void* image_state, float* out_color, float* out_T,
-                          svr_stream_t stream)
+                          int32_t flags, svr_stream_t stream)
 {
     GeomState g;
     BinningState b;
     ImageState img;
     const int rc = forward_setup(n_samp, W, H, c2w, N, centers, lengths, geos, rgbs, geom_state, order_table, R,
-                                 binning_state, image_state, out_color, out_T, g, b, img, stream);
+                                 binning_state, image_state, out_color, out_T, g, b, img, stream, flags);
     if (rc != SVR_OK) return rc;
     switch (n_samp)
     {

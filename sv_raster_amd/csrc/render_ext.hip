@@ -597,7 +597,7 @@ int svr_rasterize_forward_ext(int32_t n_samp, int32_t W, int32_t H, float tanfov
                               const float* lengths, const float* geos, const float* rgbs, const void* geom_state,
                               const void* order_table, int64_t R, void* binning_state, void* image_state,
                               float* out_color, float* out_T, float* out_depth, float* out_normal, float* max_w,
-                              svr_stream_t stream)
+                              int32_t fwd_flags, svr_stream_t stream)
 {
     SVR_REQUIRE(!(need_depth || need_distortion) || out_depth, SVR_ERR_INVALID,
                 "svr_rasterize_forward_ext: need_depth / need_distortion without out_depth");
@@ -608,7 +608,7 @@ int svr_rasterize_forward_ext(int32_t n_samp, int32_t W, int32_t H, float tanfov
     BinningState b;
     ImageState img;
     int rc = forward_setup(n_samp, W, H, c2w, N, centers, lengths, geos, rgbs, geom_state, order_table, R,
-                           binning_state, image_state, out_color, out_T, g, b, img, stream);
+                           binning_state, image_state, out_color, out_T, g, b, img, stream, fwd_flags);
     if (rc != SVR_OK) return rc;
     const int flags = (need_depth ? EXT_DEPTH : 0) | (need_distortion ? EXT_DIST : 0) |
                       (need_normal ? EXT_NORMAL : 0) | (track_max_w && N > 0 ? EXT_MAX_W : 0);

@@ -273,10 +273,20 @@ int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* or
 int pair_scan_launch(uint32_t* c, int64_t n, uint32_t* sums, hipStream_t stream);
 
 // Key duplication, sort, tile ranges and the sorted entry list of one frame.
+// pairs_ready: pair_prepare already ran on this geometry state (svr_visible_voxels
+// with an order table, SVR_FWD_PAIRS_READY).
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
-                    BinningState& b, ImageState& img, hipStream_t stream);
+                    BinningState& b, ImageState& img, hipStream_t stream, bool pairs_ready = false);
+
+// Clears the large-voxel queue counters and builds the rank-ordered pair
+// offsets of one view (the part of binning that does not need R).
+int pair_prepare(int64_t N, const GeomState& g, const OrderTable& ot, hipStream_t stream);
 
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
+
+// A timing-disabled event of the calling thread for the current device,
+// created on first use and kept until the process ends.
+hipError_t thread_event(hipEvent_t* ev);
 
 // Tile queue: order[0..T) = tile ids sorted by length_bucket of their list,
 // longest first.  Up to 32768 tiles one workgroup builds it (histogram and
@@ -314,7 +324,8 @@ int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial
 int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* centers,
                   const float* lengths, const float* geos, const float* rgbs, const void* geom_state,
                   const void* order_table, int64_t R, void* binning_state, void* image_state, float* out_color,
-                  float* out_T, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream);
+                  float* out_T, GeomState& g, BinningState& b, ImageState& img, hipStream_t stream,
+                  int32_t flags = 0);
 int backward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_t N, const float* geos,
                    const void* geom_state, int64_t R, const void* binning_state, const void* image_state,
                    const float* out_T, const float* dL_dcolor, const float* dL_dT, void* workspace, float* dgeos,

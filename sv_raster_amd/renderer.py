@@ -76,8 +76,10 @@ def rasterize_voxels(
         vox_lengths,
         raster_settings.debug,
     )
-    # torch.where(n_duplicates > 0)[0] and the entry count R, read with one host synchronisation
-    in_frusts_idx, num_rendered = _C.visible_voxels(geomBuffer, N, device)
+    # torch.where(n_duplicates > 0)[0] and the entry count R, read with one host synchronisation; the part of
+    # the binning that does not need R is queued behind it and runs while vox_fn evaluates the colours
+    table = _C.order_table_cache.get(octree_paths) if N > 0 else None
+    in_frusts_idx, num_rendered = _C.visible_voxels(geomBuffer, N, device, table)
 
     cam_pos = raster_settings.c2w_matrix[:3, 3]
     vox_params = vox_fn(in_frusts_idx, cam_pos, raster_settings.color_mode)
@@ -118,6 +120,7 @@ def rasterize_voxels(
         rgbs,
         subdiv_p,
         num_rendered,
+        table is not None,
     )
 
 
@@ -127,7 +130,7 @@ class _RasterizeVoxels(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, raster_settings, geomBuffer, octree_paths, vox_centers, vox_lengths, geos, rgbs, subdiv_p,
-                num_rendered=None):
+                num_rendered=None, pairs_ready=False):
         need_distortion = raster_settings.lambda_dist > 0
         num_rendered, binningBuffer, imgBuffer, out_color, out_depth, out_normal, out_T, max_w = \
             _C.rasterize_voxels(
@@ -153,6 +156,7 @@ class _RasterizeVoxels(torch.autograd.Function):
                 geomBuffer,
                 raster_settings.debug,
                 num_rendered=num_rendered,
+                pairs_ready=pairs_ready,
             )
         ctx.raster_settings = raster_settings
         ctx.num_rendered = num_rendered
@@ -206,7 +210,7 @@ class _RasterizeVoxels(torch.autograd.Function):
             out_normal,
             raster_settings.debug,
         )
-        return None, None, None, None, None, dL_dgeos, dL_drgbs, subdiv_p_bw, None
+        return None, None, None, None, None, dL_dgeos, dL_drgbs, subdiv_p_bw, None, None
 
 
 class SH_eval(torch.autograd.Function):

@@ -55,7 +55,7 @@ def test_invalid_arguments_fail_without_device_work():
     rc = L.svr_rasterize_preprocess(0, 10, 0.5, 0.5, 5.0, 5.0, None, None, 0.02, 10, None, None, None, None, None)
     assert rc == 1 and b"image size" in L.svr_last_error()
     rc = L.svr_rasterize_forward(4, 16, 16, 0.5, 0.5, 8.0, 8.0, None, 0.0, 0, None, None, None, None, None, None, 0,
-                                 None, None, None, None, None)
+                                 None, None, None, None, 0, None)
     assert rc == 1 and b"n_samp_per_vox" in L.svr_last_error()
     with pytest.raises(Exception, match="image size"):
         _lib.check(1 if L.svr_rasterize_preprocess(70000, 10, 0.5, 0.5, 5.0, 5.0, None, None, 0.02, 1, None, None,

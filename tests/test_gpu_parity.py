@@ -142,3 +142,40 @@ def test_visible_voxels(n, direction):
     assert R == int(n_dup.long().sum().item())
     if direction is not None:
         assert idx.numel() == 0 and R == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,seed", [("C", 5), ("A", 1)])
+def test_renderer_prepared_binning_matches_oracle(name, seed):
+    """The drop-in renderer.rasterize_voxels, whose visible_voxels call queues the pair offsets before the host
+    synchronisation (forward called with SVR_FWD_PAIRS_READY): image, T and the density / SH gradients of one
+    MSE step against the oracle, twice in a row on the same octree (cached order table)."""
+    import torch
+    from oracle import oracle as O
+    from sv_raster_amd import renderer, synthetic as S
+    case = make_case(name, seed=seed)
+    sc, cam = case["scene"], case["cam"]
+    print(f"renderer prepared binning {name} seed {seed}", flush=True)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
+    geos, sh0, shs = t(sc["geos"]).requires_grad_(), t(sc["sh0"]).requires_grad_(), t(sc["shs"]).requires_grad_()
+    settings = renderer.RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=cam["W"],
+                                       image_height=cam["H"], tanfovx=cam["tanfovx"], tanfovy=cam["tanfovy"],
+                                       cx=cam["cx"], cy=cam["cy"], w2c_matrix=t(cam["w2c"]), c2w_matrix=t(cam["c2w"]))
+
+    def vox_fn(idx, cam_pos, color_mode):
+        rgbs = renderer.SH_eval.apply(3, idx, centers, cam_pos, None, sh0, shs)
+        return {"geos": geos, "rgbs": rgbs, "subdiv_p": torch.ones((centers.shape[0], 1), device="cuda")}
+
+    o = run_oracle(case)
+    for _ in range(2):
+        geos.grad = sh0.grad = shs.grad = None
+        color, _, _, T, _ = renderer.rasterize_voxels(settings, octpath, centers, lengths, vox_fn)
+        loss = (color * t(case["dL_dcolor"])).sum() + (T * t(case["dL_dT"])).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        assert rel_err(color.detach().cpu().numpy(), o["color"], 0) <= IMG_RTOL
+        assert rel_err(T.detach().cpu().numpy().reshape(o["T"].shape), o["T"], 0) <= IMG_RTOL
+        assert rel_err(geos.grad.cpu().numpy(), o["dgeos"]) <= GRAD_RTOL
+        assert rel_err(sh0.grad.cpu().numpy(), o["dsh0"]) <= GRAD_RTOL
+        assert rel_err(shs.grad.cpu().numpy(), o["dshs"]) <= GRAD_RTOL
