@@ -72,6 +72,11 @@ def load_traffic(kernel, config, n_vox, width, height):
         return None
 
 
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def collect_stages(L):
     out = {}
     for i, name in enumerate(STAGES):
@@ -272,6 +277,7 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    progress(f"{args.warmup} warm-up steps done")
 
     # snapshot of parameters and optimizer state, so that the instrumented pass below repeats the same steps
     params = (geos, sh0, shs)
@@ -286,6 +292,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    progress(f"{args.steps} timed steps: {ms_per_step:.4f} ms per step")
     # the same steps again (parameters and optimizer state restored) with HIP events around every stage,
     # recorded on the launch stream: per-stage kernel times and the roofline of the dominant kernel
     with torch.no_grad():
@@ -306,6 +313,7 @@ def main():
     stages = collect_stages(L)
     _lib.check(L.svr_profile_enable(0), "profile")
     ms_per_step_instrumented = (t3 - t2) * 1e3 / args.steps
+    progress(f"{args.steps} instrumented steps: {ms_per_step_instrumented:.4f} ms per step")
 
     # forward only (the FPS path): preprocess + SH + binning + compositor
     with torch.no_grad():
@@ -357,10 +365,12 @@ def main():
 
     batched = None
     if args.batch_config.lower() != "none":
+        progress("multi-view figure")
         batched = batch_vs_plain(args, dev)
 
     cpu = None
     if args.cpu_steps > 0:
+        progress("CPU baseline")
         cpu = cpu_baseline(case, args.cpu_steps)
 
     result = {

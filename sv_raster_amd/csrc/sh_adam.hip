@@ -64,6 +64,16 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int deg, int64_t cnt, const
 constexpr int SH_F = 48;
 constexpr int SH_PAD = 49;
 
+// Linear staging of one block's 64 rows: s0 holds the 64 x 3 sh0 values,
+// s1 the 64 x 45 shs values, row k at 3 k and 45 k (odd strides: lane k
+// touching row k is bank-conflict free), so that the block's contiguous
+// output spans are plain 16-byte copies out of LDS.
+struct ShStage
+{
+    alignas(16) float s0[64 * 3 + 4];
+    alignas(16) float s1[64 * 45 + 4];
+};
+
 __global__ __launch_bounds__(64) void sh_fwd16_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,
                                                       const float* __restrict__ centers,
                                                       const float* __restrict__ cam, const float* __restrict__ sh0,
@@ -179,12 +189,13 @@ __global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, 
                                                             const float* __restrict__ dL_drgbs,
                                                             float* __restrict__ dsh0, float* __restrict__ dshs)
 {
-    __shared__ float buf[64 * SH_PAD];
+    __shared__ ShStage st;
     const int lane = threadIdx.x;
     const int64_t v0 = (int64_t)blockIdx.x * 64;
     const int nv = (int)min((int64_t)64, N - v0);
     const int64_t v = v0 + lane;
-    float* row = buf + lane * SH_PAD;
+    float* r0 = st.s0 + 3 * lane;
+    float* r1 = st.s1 + 45 * lane;
     if (lane < nv)
     {
         const bool marked = reinterpret_cast<const uint32_t*>(dsh0)[3 * v] == MARK_BITS;
@@ -197,42 +208,29 @@ __global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, 
             for (int ch = 0; ch < 3; ++ch)
             {
                 const float g = dL_drgbs[3 * v + ch];
-                row[ch] = SH_C0 * g;
-                for (int k = 1; k < 16; ++k) row[3 * k + ch] = (k < nb) ? b[k] * g : 0.f;
+                r0[ch] = SH_C0 * g;
+                for (int k = 1; k < 16; ++k) r1[3 * (k - 1) + ch] = (k < nb) ? b[k] * g : 0.f;
             }
         }
         else
         {
-            for (int f = 0; f < SH_F; ++f) row[f] = 0.f;
+            for (int f = 0; f < 3; ++f) r0[f] = 0.f;
+            for (int f = 0; f < 45; ++f) r1[f] = 0.f;
         }
     }
     wave_sync();
     if (nv == 64)
     {
-        // 16-byte aligned spans: 64 rows start at a multiple of 64 voxels
+        // 64 rows starting at a multiple of 64 voxels: 16-byte aligned spans
         float4* o0 = reinterpret_cast<float4*>(dsh0 + 3 * v0);
-        for (int e = lane; e < 48; e += 64)
-        {
-            const int a = 4 * e;
-            float q[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) q[i] = buf[((a + i) / 3) * SH_PAD + (a + i) % 3];
-            o0[e] = make_float4(q[0], q[1], q[2], q[3]);
-        }
+        for (int e = lane; e < 48; e += 64) o0[e] = reinterpret_cast<const float4*>(st.s0)[e];
         float4* o1 = reinterpret_cast<float4*>(dshs + 45 * v0);
-        for (int e = lane; e < 720; e += 64)
-        {
-            const int a = 4 * e;
-            float q[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) q[i] = buf[((a + i) / 45) * SH_PAD + 3 + (a + i) % 45];
-            o1[e] = make_float4(q[0], q[1], q[2], q[3]);
-        }
+        for (int e = lane; e < 720; e += 64) o1[e] = reinterpret_cast<const float4*>(st.s1)[e];
     }
     else
     {
-        for (int e = lane; e < nv * 3; e += 64) dsh0[3 * v0 + e] = buf[(e / 3) * SH_PAD + e % 3];
-        for (int e = lane; e < nv * 45; e += 64) dshs[45 * v0 + e] = buf[(e / 45) * SH_PAD + 3 + e % 45];
+        for (int e = lane; e < nv * 3; e += 64) dsh0[3 * v0 + e] = st.s0[e];
+        for (int e = lane; e < nv * 45; e += 64) dshs[45 * v0 + e] = st.s1[e];
     }
 }
 
