@@ -25,6 +25,17 @@
 #include "svr_host.h"
 #include "voxel_ops.h"
 
+// Backward window and sync-slot threshold of the batch compositor, measured
+// apart from the single-view kernel's: on the 16-view 2 M-voxel workload 4
+// chunks and 8 pixels beat (6, 8), (5, 12) and (4, 12) (the 8x8 binning tile
+// has one wave per workgroup, so larger windows stage more entries per thread).
+#ifndef BATCH_BWD_WINDOW
+#define BATCH_BWD_WINDOW 4
+#endif
+#ifndef BATCH_BWD_SYNC
+#define BATCH_BWD_SYNC 8
+#endif
+
 namespace svr {
 
 // Batch image state: per global tile ranges / tile_last, per pixel n_contrib,
@@ -367,7 +378,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
 }
 
 template <int NS, bool EXACT, typename TS>
-__device__ inline void batch_bwd_tile(WindowLDS<BWD_WINDOW>& sh, uint32_t g, int V, const svr_view* __restrict__ views,
+__device__ inline void batch_bwd_tile(WindowLDS<BATCH_BWD_WINDOW>& sh, uint32_t g, int V, const svr_view* __restrict__ views,
                                       int64_t N, const uint2* __restrict__ ranges,
                                       const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
                                       const float* __restrict__ lengths, const float* __restrict__ geos,
@@ -391,7 +402,7 @@ __device__ inline void batch_bwd_tile(WindowLDS<BWD_WINDOW>& sh, uint32_t g, int
                                  dL_dT + pb);
     const uint2 range = ranges[g];
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
-    adjoint_window<BWD_SYNC_MIN_MASKED, TS, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths,
+    adjoint_window<BATCH_BWD_SYNC, TS, BATCH_BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths,
                                                         geos, rgbs + 3 * vo, orig, partial);
 }
 
@@ -404,7 +415,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
     const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
     float* __restrict__ partial)
 {
-    __shared__ WindowLDS<BWD_WINDOW> sh;
+    __shared__ WindowLDS<BATCH_BWD_WINDOW> sh;
     const uint32_t g = order[blockIdx.x];
     batch_bwd_tile<NS, false, TS>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T, n_contrib,
                                   dL_dcolor, dL_dT, orig, hit_masks, partial);
@@ -420,7 +431,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
     const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
     const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
 {
-    __shared__ WindowLDS<BWD_WINDOW> sh;
+    __shared__ WindowLDS<BATCH_BWD_WINDOW> sh;
     __shared__ uint32_t s_i;
     const uint32_t count = ctr[1];
     while (true)

@@ -12,9 +12,10 @@ namespace svr {
 
 // Sync-slot threshold of the backward: an entry used by at least this many
 // pixels of a wave is summed across the wave by a butterfly instead of
-// per-lane LDS atomics (measured on the 1080p scene: 12 beats 4, 8 and 16).
+// per-lane LDS atomics (measured on the 1080p scene with 6-chunk windows:
+// 8 beats 6, 10, 12 and 16).
 #ifndef BWD_SYNC_MIN_MASKED
-#define BWD_SYNC_MIN_MASKED 12
+#define BWD_SYNC_MIN_MASKED 8
 #endif
 
 // Binning tile of TX x TY pixels; each wave of its workgroup owns one 8x8
@@ -338,8 +339,11 @@ struct WindowLDS
     uint32_t orow[KW][64];
     uint32_t nch[16];  // per wave: chunks that hold its pixels' contributors
 };
+// Chunks per window of the backward (measured with the window prefetch on the
+// 1080p scene: 6 beats 2-5, 7 and 8; 7 and more leave room in LDS for only
+// two workgroups per CU).
 #ifndef BWD_WINDOW_CHUNKS
-#define BWD_WINDOW_CHUNKS 4
+#define BWD_WINDOW_CHUNKS 6
 #endif
 constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 
