@@ -92,8 +92,12 @@ int svr_num_rendered(int64_t n_voxels, void* geom_state, svr_stream_t stream, in
 int svr_visible_voxels(int64_t n_voxels, void* geom_state, const void* order_table, int64_t* idx_out,
                        svr_stream_t stream, int64_t* num_rendered_host, int64_t* num_visible_host);
 
-/* flags of svr_rasterize_forward(_ext) */
-enum { SVR_FWD_PAIRS_READY = 1 /* svr_visible_voxels already prepared the binning with this order table */ };
+/* flags of svr_rasterize_binning and svr_rasterize_forward(_ext) */
+enum
+{
+    SVR_FWD_PAIRS_READY = 1, /* svr_visible_voxels already prepared the binning with this order table */
+    SVR_FWD_BINNED = 2       /* svr_rasterize_binning already filled the binning and image states */
+};
 
 /* Builds the rank-order table of an octree: the position of every
  * (voxel, quadrant) pair in the order of the reference's 63-bit order rank
@@ -102,6 +106,15 @@ enum { SVR_FWD_PAIRS_READY = 1 /* svr_visible_voxels already prepared the binnin
  * bytes.  Synchronises `stream` once (reads the finest used level). */
 int svr_build_order_table(int64_t n_voxels, const int64_t* octree_paths, void* order_table, void* scratch,
                           svr_stream_t stream);
+
+/* The binning half of the forward (forward.cu:665-753: key duplication, sort,
+ * tile ranges) on its own, so that it can be queued before the per-voxel
+ * colours exist (it needs only the geometry): a svr_rasterize_forward(_ext)
+ * on the same states then passes SVR_FWD_BINNED and only composites.  flags:
+ * SVR_FWD_PAIRS_READY as for the forward. */
+int svr_rasterize_binning(int32_t width, int32_t height, int64_t n_voxels, const void* geom_state,
+                          const void* order_table, int64_t num_rendered, void* binning_state, void* image_state,
+                          int32_t flags, svr_stream_t stream);
 
 /* Replaces `_C.rasterize_voxels` (forward.cu:792-879) for n_samp in [1,3]
  * without depth / normal / distortion / max_w: key duplication, sort, tile

@@ -24,7 +24,8 @@ MAX_SORT_KEY_BITS = int(_consts.max_sort_key_bits)
 MAX_PACKED_IMAGE_DIM = int(_consts.max_packed_image_dim)
 MAX_RENDER_TILES = int(_consts.max_render_tiles)
 TILE = int(_consts.tile_x)
-SVR_FWD_PAIRS_READY = 1  # flags of svr_rasterize_forward(_ext), include/svraster_hip.h
+SVR_FWD_PAIRS_READY = 1  # flags of svr_rasterize_binning / svr_rasterize_forward(_ext), include/svraster_hip.h
+SVR_FWD_BINNED = 2
 
 
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
@@ -152,12 +153,30 @@ def visible_voxels(geomBuffer, n_voxels, device, order_table=None):
     return idx[:int(nv.value)], int(R.value)
 
 
+def rasterize_binning(image_width, image_height, octree_paths, geomBuffer, num_rendered, pairs_ready=False):
+    """The binning half of the forward (key duplication, sort, tile ranges) for R = num_rendered entries, queued
+    before the per-voxel colours exist; returns the (binningBuffer, imageBuffer) that rasterize_voxels then
+    composites with binned=(binningBuffer, imageBuffer)."""
+    dev = geomBuffer.device
+    N = octree_paths.numel()
+    W, H, R = int(image_width), int(image_height), int(num_rendered)
+    img = torch.empty(int(_L.svr_image_state_size(W, H, R)), dtype=torch.uint8, device=dev)
+    binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
+    table = order_table_cache.get(octree_paths) if N > 0 else None
+    _lib.check(_L.svr_rasterize_binning(W, H, N, _ptr(geomBuffer), _ptr(table), R, _ptr(binning), _ptr(img),
+                                        SVR_FWD_PAIRS_READY if pairs_ready else 0, _stream(dev)),
+               "rasterize_binning")
+    return binning, img
+
+
 # binding.cpp:26 -> forward.cu:792-879
 def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fovy, cx, cy, w2c_matrix, c2w_matrix,
                      bg_color, need_depth, need_distortion, need_normal, track_max_w, octree_paths, vox_centers,
-                     vox_lengths, geos, rgbs, geomBuffer, debug=False, num_rendered=None, pairs_ready=False):
+                     vox_lengths, geos, rgbs, geomBuffer, debug=False, num_rendered=None, pairs_ready=False,
+                     binned=None):
     """num_rendered: R when the caller already has it (visible_voxels); otherwise it is read here.
-    pairs_ready: visible_voxels ran with this octree's order table on geomBuffer."""
+    pairs_ready: visible_voxels ran with this octree's order table on geomBuffer.
+    binned: (binningBuffer, imageBuffer) from rasterize_binning of the same R; only the compositor runs."""
     if vox_centers.ndimension() != 2 or vox_centers.size(1) != 3:
         raise RuntimeError("vox_centers must have dimensions (num_points, 3)")
     if rgbs.ndimension() != 2 or rgbs.size(1) != 3:
@@ -182,10 +201,14 @@ def rasterize_voxels(n_samp_per_vox, image_width, image_height, tan_fovx, tan_fo
         R = int(R.value)
     else:
         R = int(num_rendered)
-    img = torch.empty(int(_L.svr_image_state_size(W, H, R)), dtype=torch.uint8, device=dev)
-    binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
+    if binned is not None:
+        binning, img = binned
+        flags = SVR_FWD_BINNED
+    else:
+        img = torch.empty(int(_L.svr_image_state_size(W, H, R)), dtype=torch.uint8, device=dev)
+        binning = torch.empty(int(_L.svr_binning_state_size(R)) if R > 0 else 0, dtype=torch.uint8, device=dev)
+        flags = SVR_FWD_PAIRS_READY if pairs_ready else 0
     table = order_table_cache.get(octree_paths) if R > 0 else None
-    flags = SVR_FWD_PAIRS_READY if pairs_ready else 0
     centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
     c2w = _f32(c2w_matrix)
     if ext:

@@ -44,6 +44,7 @@ SIGNATURES = {
     "svr_num_rendered": (c_i32, [c_i64, c_p, c_p, ctypes.POINTER(c_i64)]),
     "svr_visible_voxels": (c_i32, [c_i64, c_p, c_p, c_p, c_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     "svr_build_order_table": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
+    "svr_rasterize_binning": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_i64, c_p, c_p, c_i32, c_p]),
     "svr_rasterize_forward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
                                       c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_i32, c_p]),
     "svr_backward_workspace_size": (c_sz, [c_i64]),

@@ -53,6 +53,20 @@ hipError_t thread_event(hipEvent_t* ev)
     return hipSuccess;
 }
 
+hipError_t thread_pinned_words(uint32_t** words)
+{
+    thread_local uint32_t* buf = nullptr;
+    if (!buf)
+    {
+        void* p = nullptr;
+        const hipError_t e = hipHostMalloc(&p, 64 * sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        buf = (uint32_t*)p;
+    }
+    *words = buf;
+    return hipSuccess;
+}
+
 struct ProfPair
 {
     hipEvent_t start, stop;

@@ -341,8 +341,11 @@ int svr_visible_voxels(int64_t N, void* geom_state, const void* order_table, int
     visible_write_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.vis_sums, idx_out);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_SCAN, stream);
-    uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+    // pinned staging: the copy is truly asynchronous, so the pair offsets
+    // below are queued at once behind it
+    uint32_t* cnt = nullptr;
+    SVR_CHECK_HIP(thread_pinned_words(&cnt));
+    SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     if (order_table)
     {
         // wait for the copy only: the pair offsets queued behind it keep the

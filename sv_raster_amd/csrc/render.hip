@@ -338,6 +338,7 @@ int forward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64_
     OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
     b = BinningState::carve(binning_state, R, nullptr);
     img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr, R);
+    if (flags & SVR_FWD_BINNED) return SVR_OK;
     return binning_forward(W, H, N, g, ot, R, b, img, stream, (flags & SVR_FWD_PAIRS_READY) != 0);
 }
 
@@ -395,6 +396,23 @@ int svr_rasterize_forward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, f
     default: return launch_fwd<3>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, g, b, img, centers, lengths, geos, rgbs,
                                   out_color, out_T, stream);
     }
+}
+
+int svr_rasterize_binning(int32_t W, int32_t H, int64_t N, const void* geom_state, const void* order_table,
+                          int64_t R, void* binning_state, void* image_state, int32_t flags, svr_stream_t stream)
+{
+    SVR_REQUIRE(W > 0 && H > 0 && W <= MAX_PACKED_IMAGE_DIM && H <= MAX_PACKED_IMAGE_DIM, SVR_ERR_INVALID,
+                "image size %dx%d outside [1, %d]", W, H, MAX_PACKED_IMAGE_DIM);
+    SVR_REQUIRE(R >= 0 && R < (1ll << 32), SVR_ERR_INVALID, "num_rendered out of range");
+    SVR_REQUIRE(image_state, SVR_ERR_INVALID, "svr_rasterize_binning: null image state");
+    SVR_REQUIRE(N == 0 || (geom_state && order_table), SVR_ERR_INVALID, "svr_rasterize_binning: null pointer");
+    SVR_REQUIRE(R == 0 || binning_state, SVR_ERR_INVALID, "svr_rasterize_binning: null binning state");
+    const int64_t T = (int64_t)tiles_x(W) * tiles_y(H);
+    GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
+    BinningState b = BinningState::carve(binning_state, R, nullptr);
+    ImageState img = ImageState::carve(image_state, (int64_t)W * H, T, nullptr, R);
+    return binning_forward(W, H, N, g, ot, R, b, img, stream, (flags & SVR_FWD_PAIRS_READY) != 0);
 }
 
 size_t svr_backward_workspace_size(int64_t R)

@@ -288,6 +288,10 @@ int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
 // created on first use and kept until the process ends.
 hipError_t thread_event(hipEvent_t* ev);
 
+// 64 words of pinned host memory of the calling thread (device-to-host
+// staging of counts), allocated on first use and kept until the process ends.
+hipError_t thread_pinned_words(uint32_t** words);
+
 // Tile queue: order[0..T) = tile ids sorted by length_bucket of their list,
 // longest first.  Up to 32768 tiles one workgroup builds it (histogram and
 // scatter in LDS); larger batches sort the keys (built in the _alt buffers,

@@ -61,6 +61,10 @@ def rasterize_voxels(
             vox_lengths.device != device:
         raise Exception("Device mismatch.")
 
+    # the camera position handed to vox_fn (renderer.py:85 passes the view c2w[:3, 3]); made contiguous here,
+    # before the host synchronisation below, so that the colour evaluation does not copy it on the critical path
+    cam_pos = raster_settings.c2w_matrix[:3, 3].contiguous()
+
     n_duplicates, geomBuffer = _C.rasterize_preprocess(
         raster_settings.image_width,
         raster_settings.image_height,
@@ -77,11 +81,13 @@ def rasterize_voxels(
         raster_settings.debug,
     )
     # torch.where(n_duplicates > 0)[0] and the entry count R, read with one host synchronisation; the part of
-    # the binning that does not need R is queued behind it and runs while vox_fn evaluates the colours
+    # the binning that does not need R is queued behind it, and the rest right after it, before vox_fn, so the
+    # device bins while the host evaluates the colours (binning needs the geometry only)
     table = _C.order_table_cache.get(octree_paths) if N > 0 else None
     in_frusts_idx, num_rendered = _C.visible_voxels(geomBuffer, N, device, table)
+    binned = _C.rasterize_binning(raster_settings.image_width, raster_settings.image_height, octree_paths,
+                                  geomBuffer, num_rendered, pairs_ready=table is not None)
 
-    cam_pos = raster_settings.c2w_matrix[:3, 3]
     vox_params = vox_fn(in_frusts_idx, cam_pos, raster_settings.color_mode)
     geos = vox_params['geos']
     rgbs = vox_params['rgbs']
@@ -120,7 +126,7 @@ def rasterize_voxels(
         rgbs,
         subdiv_p,
         num_rendered,
-        table is not None,
+        binned,
     )
 
 
@@ -130,7 +136,7 @@ class _RasterizeVoxels(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, raster_settings, geomBuffer, octree_paths, vox_centers, vox_lengths, geos, rgbs, subdiv_p,
-                num_rendered=None, pairs_ready=False):
+                num_rendered=None, binned=None):
         need_distortion = raster_settings.lambda_dist > 0
         num_rendered, binningBuffer, imgBuffer, out_color, out_depth, out_normal, out_T, max_w = \
             _C.rasterize_voxels(
@@ -156,7 +162,7 @@ class _RasterizeVoxels(torch.autograd.Function):
                 geomBuffer,
                 raster_settings.debug,
                 num_rendered=num_rendered,
-                pairs_ready=pairs_ready,
+                binned=binned,
             )
         ctx.raster_settings = raster_settings
         ctx.num_rendered = num_rendered
