@@ -179,9 +179,15 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
     // The next chunk is loaded (into registers) while the current one is
     // composited; its pixel masks use the quadrant masks of the lanes active
     // at load time, and lanes that finish meanwhile drop their candidates.
-    uint64_t qpix[8];
+    // pixels of each ray quadrant (fixed), intersected with the active ones
+    uint64_t qall[8], qpix[8];
 #pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
+    for (uint32_t q = 0; q < 8; ++q) qall[q] = __ballot(ray.pq == q);
+    {
+        const uint64_t act = __ballot(!done);
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) qpix[q] = qall[q] & act;
+    }
     EntryRegs cur;
     uint32_t base = range.x;
     {
@@ -206,8 +212,9 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
         const uint32_t nbase = base + 64;
         if (nbase < range.y)
         {
+            const uint64_t act = __ballot(!done);
 #pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
+            for (uint32_t q = 0; q < 8; ++q) qpix[q] = qall[q] & act;
             const uint32_t j = nbase + (uint32_t)lane;
             const bool valid = j < range.y;
             load_entry<!MAX_W>(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0,

@@ -206,7 +206,9 @@ __device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_i
         // exp_linear_11 and its derivative share one exponential
         const float ex = svr_expf_core(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
         const bool lin = d > 1.1f;
-        vol = vol + scale * (lin ? d : ex);
+        // the first sample's term is the sum itself: 0 + x == x for every
+        // x >= +0 or NaN (scale and the activation are never negative)
+        vol = (k == 0) ? scale * (lin ? d : ex) : vol + scale * (lin ? d : ex);
         if (DI)
         {
             const float dd = scale * (lin ? 1.0f : 0.909090909091f * ex);
@@ -242,7 +244,7 @@ __device__ inline float vox_vol1_w(float c0, float c1, float c2, float l, float 
     const float ex = svr_expf_core(__builtin_fmaf(0.909090909091f, d, -0.904689820196f));
     const bool lin = d > 1.1f;
     dd = scale * (lin ? 1.0f : 0.909090909091f * ex);
-    return 0.f + scale * (lin ? d : ex);
+    return scale * (lin ? d : ex);  // == vox_vol<1>: 0 + x is x for x >= +0
 }
 
 // x / y from the hardware reciprocal and one Newton correction (within an
