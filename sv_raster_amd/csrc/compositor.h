@@ -349,6 +349,12 @@ struct WindowLDS
 // Chunks per window of the backward (measured with the window prefetch on the
 // 1080p scene: 6 beats 2-5, 7 and 8; 7 and more leave room in LDS for only
 // two workgroups per CU).
+// Backward hit walk: 1 = the merged walk (other lanes take their own hits
+// in a sync slot's step), 0 = segments between sync slots in lock-step.
+#ifndef BWD_MERGED_WALK
+#define BWD_MERGED_WALK 1
+#endif
+
 #ifndef BWD_WINDOW_CHUNKS
 #define BWD_WINDOW_CHUNKS 6
 #endif
@@ -467,6 +473,53 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
             if (j >= nj) break;
             uint64_t hm = m[j], sync = syncm[j];
             float (&acc)[12][64] = sh.acc[j];
+#if BWD_MERGED_WALK
+            // One hit per lane and step.  A lane's next hits are those above
+            // its next sync slot ("allowed"); the wave's highest remaining
+            // sync slot s is taken in the step where none of its users has an
+            // allowed hit left, and in that step the other lanes go on with
+            // their own allowed hits, so they do not idle while s is summed.
+            while (__ballot(hm != 0ull) != 0ull)
+            {
+                const int s = sync ? 63 - __builtin_clzll(sync) : -1;
+                const uint64_t mine_sync = hm & sync;
+                const int ns = mine_sync ? 63 - __builtin_clzll(mine_sync) : -1;
+                const uint64_t allowed = (ns < 0) ? hm : (ns == 63 ? 0ull : (hm & ~((2ull << ns) - 1ull)));
+                const bool owner = s >= 0 && ((hm >> s) & 1ull);
+                const bool ready = s >= 0 && __ballot(owner && allowed != 0ull) == 0ull;
+                int l = -1;
+                const bool is_sync = ready && owner;
+                if (is_sync) l = s;
+                else if (allowed) l = 63 - __builtin_clzll(allowed);
+                float gq[12];
+#pragma unroll
+                for (int i = 0; i < 12; ++i) gq[i] = 0.f;
+                bool hit = false;
+                if (l >= 0)
+                {
+                    hm &= ~(1ull << l);
+                    hit = hit_fn(sh.pay[j][l], gq);
+                }
+                if (ready)
+                {
+                    sync &= ~(1ull << s);
+                    if (__ballot(is_sync && hit) != 0ull)
+                    {
+                        float gs[12];
+#pragma unroll
+                        for (int i = 0; i < 12; ++i) gs[i] = (is_sync && hit) ? gq[i] : 0.f;
+                        const float r = wave_sum12(gs, lane);
+                        const int ri = (lane >> 2) & 15;
+                        if ((lane & 3) == 0 && ri < 12) atomicAdd(&acc[ri][s], r);
+                    }
+                }
+                if (hit && !is_sync)
+                {
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][l], gq[i]);
+                }
+            }
+#else
             while (true)
             {
                 const int s = sync ? 63 - __builtin_clzll(sync) : -1;
@@ -504,6 +557,7 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
                 const int ri = (lane >> 2) & 15;
                 if ((lane & 3) == 0 && ri < 12) atomicAdd(&acc[ri][s], r);
             }
+#endif
         }
         __syncthreads();
         // 4. one row per used entry, then reset the window
