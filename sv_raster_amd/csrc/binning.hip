@@ -543,12 +543,14 @@ __global__ __launch_bounds__(256) void tile_queue_keys_kernel(int64_t T, const u
 // the LDS atomics (it only affects scheduling, never a result).
 constexpr int64_t QUEUE_ONE_BLOCK_MAX = 32768;
 __global__ __launch_bounds__(1024) void tile_queue_block_kernel(int64_t T, const uint2* __restrict__ ranges,
-                                                                uint32_t* __restrict__ order)
+                                                                uint32_t* __restrict__ order,
+                                                                uint32_t* __restrict__ clear8)
 {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t base[256];
     const int t = threadIdx.x;
     if (t < 256) hist[t] = 0u;
+    if (clear8 && t < 8) clear8[t] = 0u;
     __syncthreads();
     for (int64_t g = t; g < T; g += blockDim.x)
     {
@@ -591,14 +593,15 @@ __global__ __launch_bounds__(1024) void tile_queue_block_kernel(int64_t T, const
 }
 
 int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t* qkeys_alt, uint32_t* order,
-                      uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream)
+                      uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream, uint32_t* clear8)
 {
     if (T <= QUEUE_ONE_BLOCK_MAX)
     {
-        tile_queue_block_kernel<<<1, 1024, 0, stream>>>(T, ranges, order);
+        tile_queue_block_kernel<<<1, 1024, 0, stream>>>(T, ranges, order, clear8);
         SVR_CHECK_LAUNCH();
         return SVR_OK;
     }
+    if (clear8 && fill_u32(clear8, 0u, 8, stream) != SVR_OK) return SVR_ERR_HIP;
     tile_queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, ranges, qkeys_alt, order_alt);
     SVR_CHECK_LAUNCH();
     uint64_t* qk = nullptr;
@@ -625,7 +628,7 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     if (R == 0)
     {
         const int rc0 = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt,
-                                          img.qcounts, stream);
+                                          img.qcounts, stream, img.ctr);
         prof_end(PROF_BINNING, stream);
         return rc0;
     }
@@ -651,7 +654,8 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
     rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
     if (rc != SVR_OK) return rc;
-    rc = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt, img.qcounts, stream);
+    rc = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt, img.qcounts, stream,
+                           img.ctr);
     if (rc != SVR_OK) return rc;
     prof_end(PROF_BINNING, stream);
     return SVR_OK;

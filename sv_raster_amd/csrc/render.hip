@@ -25,30 +25,31 @@
 
 namespace svr {
 
-// Launched twice per frame: EXACT = false handles the tiles whose 256 rays
-// all have finite 1/rd (hardware min/max in the ray-box test), EXACT = true
-// the remaining tiles (the reference's ternary min/max, whose NaN handling
-// differs); each launch returns at once from the other's tiles.
+// Two launches per frame.  render_fwd_kernel takes the tiles in queue order
+// (workgroup b: tile order[b]) and composites those whose 256 rays all have
+// finite 1/rd (hardware min/max in the ray-box test); a tile with another
+// ray is appended to exact_list.  render_fwd_exact_kernel then loops a few
+// workgroups over that list with the reference's ternary min/max (whose NaN
+// handling differs).  The backward launches walk the same list, so neither
+// direction spends a launch of T workgroups on the rare exact tiles.
+constexpr unsigned EXACT_GRID = 64;
+
 template <int NS, bool EXACT>
-__global__ __launch_bounds__(256) void render_fwd_kernel(
-    int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
-    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
-    const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
-    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
-    uint64_t* __restrict__ hit_masks)
+__device__ inline void fwd_tile(Stage& st, uint32_t (&s_last)[NWAVE], int tile, const PixRay& ray, int W, int H,
+                                int gx, float bg, const uint2* __restrict__ ranges,
+                                const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                const float* __restrict__ centers, const float* __restrict__ lengths,
+                                const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
+                                float* __restrict__ out_color, float* __restrict__ out_T,
+                                uint64_t* __restrict__ hit_masks)
 {
-    __shared__ Stage st;
-    __shared__ uint32_t s_last[NWAVE];
-    const int tile = (int)tile_order[blockIdx.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
     const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
     const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
     const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
     const bool inside = px < W && py < H;
-    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
-    if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
     const uint2 range = ranges[tile];
 
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
@@ -74,32 +75,120 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
         tile_last[tile] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
 }
 
-// Launched twice per frame, split by tiles like render_fwd_kernel.
-template <int NS, int SYNC_MIN, bool EXACT>
-__global__ __launch_bounds__(256) void render_bwd_kernel(
+__device__ inline PixRay tile_ray(int tile, int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy,
+                                  const float* __restrict__ c2w)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tx = tile % gx, ty = tile / gx;
+    const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
+    return make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void render_fwd_kernel(
     int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
     const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
-    const float* __restrict__ geos, const float* __restrict__ rgbs, const float* __restrict__ out_T,
-    const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
-    const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+    const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
+    uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
+    uint64_t* __restrict__ hit_masks, uint32_t* __restrict__ exact_ctr, uint32_t* __restrict__ exact_list)
 {
-    __shared__ WindowLDS<BWD_WINDOW> sh;
+    __shared__ Stage st;
+    __shared__ uint32_t s_last[NWAVE];
     const int tile = (int)tile_order[blockIdx.x];
+    const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
+    if (__syncthreads_or(!ray_inv_finite(ray)) != 0)
+    {
+        if (threadIdx.x == 0) exact_list[atomicAdd(exact_ctr, 1u)] = (uint32_t)tile;
+        return;
+    }
+    fwd_tile<NS, false>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
+                        tile_last, n_contrib, out_color, out_T, hit_masks);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void render_fwd_exact_kernel(
+    int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy, const float* __restrict__ c2w, float bg,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
+    const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
+    float* __restrict__ out_color, float* __restrict__ out_T, uint64_t* __restrict__ hit_masks,
+    const uint32_t* __restrict__ exact_ctr, const uint32_t* __restrict__ exact_list)
+{
+    __shared__ Stage st;
+    __shared__ uint32_t s_last[NWAVE];
+    const uint32_t count = exact_ctr[0];
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    {
+        const int tile = (int)exact_list[i];
+        const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
+        fwd_tile<NS, true>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
+                           tile_last, n_contrib, out_color, out_T, hit_masks);
+        __syncthreads();
+    }
+}
+
+template <int NS, int SYNC_MIN, bool EXACT>
+__device__ inline void bwd_tile(WindowLDS<BWD_WINDOW>& sh, int tile, const PixRay& ray, int W, int H, int gx,
+                                float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+                                const float* __restrict__ centers, const float* __restrict__ lengths,
+                                const float* __restrict__ geos, const float* __restrict__ rgbs,
+                                const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+                                const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
+                                const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
+                                float* __restrict__ partial)
+{
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
     const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
     const bool inside = px < W && py < H;
     PlainHit<NS, EXACT> hit;
-    hit.ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+    hit.ray = ray;
     const uint2 range = ranges[tile];
     const size_t HW = (size_t)H * W;
     const size_t pid = inside ? (size_t)py * W + px : 0;
-    if ((__syncthreads_or(!ray_inv_finite(hit.ray)) != 0) != EXACT) return;
     const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT);
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
     adjoint_window<SYNC_MIN, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths, geos,
                                                  rgbs, orig, partial);
+}
+
+template <int NS, int SYNC_MIN>
+__global__ __launch_bounds__(256) void render_bwd_kernel(
+    int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
+    const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
+    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
+    const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
+    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+{
+    __shared__ WindowLDS<BWD_WINDOW> sh;
+    const int tile = (int)tile_order[blockIdx.x];
+    const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
+    if (__syncthreads_or(!ray_inv_finite(ray)) != 0) return;  // the forward listed it for the exact launch
+    bwd_tile<NS, SYNC_MIN, false>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
+                                  n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+}
+
+template <int NS, int SYNC_MIN>
+__global__ __launch_bounds__(256) void render_bwd_exact_kernel(
+    int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy, const float* __restrict__ c2w, float bg,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
+    const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
+    const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
+    const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
+    float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr, const uint32_t* __restrict__ exact_list)
+{
+    __shared__ WindowLDS<BWD_WINDOW> sh;
+    const uint32_t count = exact_ctr[0];
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
+    {
+        const int tile = (int)exact_list[i];
+        const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
+        bwd_tile<NS, SYNC_MIN, true>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs,
+                                     out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+        __syncthreads();
+    }
 }
 
 // Per-voxel gradient = sum of the partial rows of its entries: one run of
@@ -273,13 +362,13 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
     prof_begin(PROF_RENDER_FWD, stream);
-    render_fwd_kernel<NS, false><<<(unsigned)T, 256, 0, stream>>>(
+    render_fwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks);
+        rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
-    render_fwd_kernel<NS, true><<<(unsigned)T, 256, 0, stream>>>(
-        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks);
+    render_fwd_exact_kernel<NS><<<EXACT_GRID, 256, 0, stream>>>(
+        W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+        img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;
@@ -296,13 +385,13 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int T = gx * gy;
     if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
-    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, false><<<(unsigned)T, 256, 0, stream>>>(
-        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
+    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED><<<(unsigned)T, 256, 0, stream>>>(
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
+        out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
     SVR_CHECK_LAUNCH();
-    render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED, true><<<(unsigned)T, 256, 0, stream>>>(
-        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
-        rgbs, out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
+    render_bwd_exact_kernel<NS, BWD_SYNC_MIN_MASKED><<<EXACT_GRID, 256, 0, stream>>>(
+        W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T,
+        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);

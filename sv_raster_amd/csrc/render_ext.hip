@@ -255,6 +255,8 @@ __device__ inline float median_refine(const PixRay& ray, int v, float med_T, con
     return Dmed;
 }
 
+constexpr unsigned EXT_EXACT_GRID = 64;
+
 template <int NS, bool EXACT>
 __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
@@ -263,12 +265,18 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     const float* __restrict__ geos, const float* __restrict__ rgbs, int flags, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     float* __restrict__ out_depth, float* __restrict__ out_normal, float* __restrict__ max_w,
-    uint64_t* __restrict__ hit_masks)
+    uint64_t* __restrict__ hit_masks, uint32_t* __restrict__ exact_ctr, uint32_t* __restrict__ exact_list)
 {
     __shared__ Stage st;
     __shared__ uint32_t sv[NWAVE][64];
     __shared__ uint32_t s_last[NWAVE];
-    const int tile = (int)tile_order[blockIdx.x];
+    // EXACT = false: workgroup b takes tile order[b] and lists it for the
+    // exact launch if one of its rays has a non-finite 1/rd; EXACT = true: a
+    // few workgroups loop over that list (render.hip's protocol)
+    const uint32_t count = EXACT ? exact_ctr[0] : 1u;
+    for (uint32_t it = EXACT ? blockIdx.x : 0u; it < count; it += EXACT ? gridDim.x : 1u)
+    {
+    const int tile = EXACT ? (int)exact_list[it] : (int)tile_order[blockIdx.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
     const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
@@ -276,7 +284,11 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
     const bool inside = px < W && py < H;
     const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
-    if ((__syncthreads_or(!ray_inv_finite(ray)) != 0) != EXACT) return;
+    if (!EXACT && __syncthreads_or(!ray_inv_finite(ray)) != 0)
+    {
+        if (threadIdx.x == 0) exact_list[atomicAdd(exact_ctr, 1u)] = (uint32_t)tile;
+        return;
+    }
     const uint2 range = ranges[tile];
 
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
@@ -315,6 +327,8 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     __syncthreads();
     if (threadIdx.x == 0)
         tile_last[tile] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+    __syncthreads();
+    }
 }
 
 // Per-pixel inputs and running sums of the adjoint.
@@ -513,10 +527,15 @@ __global__ __launch_bounds__(256) void render_bwd_ext_kernel(
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
     const float* __restrict__ dL_ddepth, const float* __restrict__ dL_dnormal, const float* __restrict__ gt,
     const float* __restrict__ out_depth, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-    float* __restrict__ partial)
+    float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr, const uint32_t* __restrict__ exact_list)
 {
     __shared__ WindowLDS<BWD_WINDOW> sh;
-    const int tile = (int)tile_order[blockIdx.x];
+    // tiles split between the two launches as in render_fwd_ext_kernel (the
+    // forward of the frame built exact_list)
+    const uint32_t count = EXACT ? exact_ctr[0] : 1u;
+    for (uint32_t it = EXACT ? blockIdx.x : 0u; it < count; it += EXACT ? gridDim.x : 1u)
+    {
+    const int tile = EXACT ? (int)exact_list[it] : (int)tile_order[blockIdx.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
     const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
@@ -527,12 +546,14 @@ __global__ __launch_bounds__(256) void render_bwd_ext_kernel(
     const uint2 range = ranges[tile];
     const size_t HW = (size_t)H * W;
     const size_t pid = inside ? (size_t)py * W + px : 0;
-    if ((__syncthreads_or(!ray_inv_finite(hit.ray)) != 0) != EXACT) return;
+    if (!EXACT && __syncthreads_or(!ray_inv_finite(hit.ray)) != 0) return;
     const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt,
                                  out_depth);
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
     adjoint_window<BWD_SYNC_MIN_MASKED, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers,
                                                             lengths, geos, rgbs, orig, partial);
+    __syncthreads();
+    }
 }
 
 template <int NS>
@@ -547,12 +568,12 @@ int launch_fwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     render_fwd_ext_kernel<NS, false><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w,
-        img.hit_masks);
+        img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
-    render_fwd_ext_kernel<NS, true><<<(unsigned)T, 256, 0, stream>>>(
+    render_fwd_ext_kernel<NS, true><<<EXT_EXACT_GRID, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
         rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w,
-        img.hit_masks);
+        img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;
@@ -572,11 +593,13 @@ int launch_bwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     prof_begin(PROF_RENDER_BWD, stream);
     render_bwd_ext_kernel<NS, false><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
-        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial);
+        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial,
+        img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
-    render_bwd_ext_kernel<NS, true><<<(unsigned)T, 256, 0, stream>>>(
+    render_bwd_ext_kernel<NS, true><<<EXT_EXACT_GRID, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
-        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial);
+        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial,
+        img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);

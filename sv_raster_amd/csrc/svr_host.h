@@ -232,6 +232,8 @@ struct ImageState
     uint64_t* qkeys;     // [T] queue sort keys
     uint64_t* qkeys_alt; // [T]
     uint32_t* qcounts;   // radix counts of the queue sort
+    uint32_t* ctr;       // [8]: 0 tiles in exact_list (cleared with the tile queue)
+    uint32_t* exact_list;// [T] tiles with a ray whose 1/rd is not finite (the exact ray-box launches)
     uint64_t* hit_masks; // [hit_mask_slots(R, T)][4 waves][64 lanes]
     static ImageState carve(void* p, int64_t P, int64_t T, size_t* bytes, int64_t R = 0)
     {
@@ -245,6 +247,8 @@ struct ImageState
         s.qkeys = c.take<uint64_t>(T);
         s.qkeys_alt = c.take<uint64_t>(T);
         s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
+        s.ctr = c.take<uint32_t>(8);
+        s.exact_list = c.take<uint32_t>(T);
         s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
         if (bytes) *bytes = c.size();
         return s;
@@ -295,9 +299,10 @@ hipError_t thread_pinned_words(uint32_t** words);
 // Tile queue: order[0..T) = tile ids sorted by length_bucket of their list,
 // longest first.  Up to 32768 tiles one workgroup builds it (histogram and
 // scatter in LDS); larger batches sort the keys (built in the _alt buffers,
-// one stable 8-bit radix pass lands them in qkeys / order).
+// one stable 8-bit radix pass lands them in qkeys / order).  clear8, when
+// given, has 8 words zeroed on the way (the frame's counters).
 int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t* qkeys_alt, uint32_t* order,
-                      uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream);
+                      uint32_t* order_alt, uint32_t* qcounts, hipStream_t stream, uint32_t* clear8 = nullptr);
 
 // Tile ranges and the sorted entry list from sorted (tile << pos_bits | pos)
 // keys; ranges must be zeroed first.
