@@ -276,58 +276,58 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     const uint32_t count = EXACT ? exact_ctr[0] : 1u;
     for (uint32_t it = EXACT ? blockIdx.x : 0u; it < count; it += EXACT ? gridDim.x : 1u)
     {
-    const int tile = EXACT ? (int)exact_list[it] : (int)tile_order[blockIdx.x];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tx = tile % gx, ty = tile / gx;
-    const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
-    const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
-    const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
-    const bool inside = px < W && py < H;
-    const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
-    if (!EXACT && __syncthreads_or(!ray_inv_finite(ray)) != 0)
-    {
-        if (threadIdx.x == 0) exact_list[atomicAdd(exact_ctr, 1u)] = (uint32_t)tile;
-        return;
-    }
-    const uint2 range = ranges[tile];
+        const int tile = EXACT ? (int)exact_list[it] : (int)tile_order[blockIdx.x];
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int tx = tile % gx, ty = tile / gx;
+        const int sx0 = tx * TILE_X + (w & 1) * 8, sy0 = ty * TILE_Y + (w >> 1) * 8;
+        const int sx1 = min(sx0 + 7, W - 1), sy1 = min(sy0 + 7, H - 1);
+        const int px = sx0 + (lane & 7), py = sy0 + (lane >> 3);
+        const bool inside = px < W && py < H;
+        const PixRay ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+        if (!EXACT && __syncthreads_or(!ray_inv_finite(ray)) != 0)
+        {
+            if (threadIdx.x == 0) exact_list[atomicAdd(exact_ctr, 1u)] = (uint32_t)tile;
+            return;
+        }
+        const uint2 range = ranges[tile];
 
-    float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    ExtAcc x{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
-    uint64_t* mask_out = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
-    const uint32_t last = composite_ext<NS, EXACT>(st, sv, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list,
-                                                   bboxes, centers, lengths, geos, rgbs, flags, max_w, T, C0, C1, C2,
-                                                   x, mask_out);
-    if (inside)
-    {
-        const size_t pid = (size_t)py * W + px;
-        const size_t HW = (size_t)H * W;
-        n_contrib[pid] = last;
-        out_color[pid] = C0 + T * bg;
-        out_color[HW + pid] = C1 + T * bg;
-        out_color[2 * HW + pid] = C2 + T * bg;
-        out_T[pid] = T;
-        if (out_depth)
+        float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+        ExtAcc x{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
+        uint64_t* mask_out = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
+        const uint32_t last = composite_ext<NS, EXACT>(st, sv, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list,
+                                                       bboxes, centers, lengths, geos, rgbs, flags, max_w, T, C0, C1, C2,
+                                                       x, mask_out);
+        if (inside)
         {
-            float Dmed = 0.f;
-            if ((flags & EXT_DEPTH) && x.med_v >= 0)
-                Dmed = median_refine<EXACT>(ray, x.med_v, x.med_T, centers, lengths, geos);
-            out_depth[pid] = (flags & EXT_DEPTH) ? x.D * ray.rd_norm_inv : 0.f;
-            out_depth[HW + pid] = (flags & EXT_DIST) ? x.Dd : 0.f;
-            out_depth[2 * HW + pid] = (flags & EXT_DEPTH) ? Dmed * ray.rd_norm_inv : 0.f;
+            const size_t pid = (size_t)py * W + px;
+            const size_t HW = (size_t)H * W;
+            n_contrib[pid] = last;
+            out_color[pid] = C0 + T * bg;
+            out_color[HW + pid] = C1 + T * bg;
+            out_color[2 * HW + pid] = C2 + T * bg;
+            out_T[pid] = T;
+            if (out_depth)
+            {
+                float Dmed = 0.f;
+                if ((flags & EXT_DEPTH) && x.med_v >= 0)
+                    Dmed = median_refine<EXACT>(ray, x.med_v, x.med_T, centers, lengths, geos);
+                out_depth[pid] = (flags & EXT_DEPTH) ? x.D * ray.rd_norm_inv : 0.f;
+                out_depth[HW + pid] = (flags & EXT_DIST) ? x.Dd : 0.f;
+                out_depth[2 * HW + pid] = (flags & EXT_DEPTH) ? Dmed * ray.rd_norm_inv : 0.f;
+            }
+            if (flags & EXT_NORMAL)
+            {
+                out_normal[pid] = x.N0;
+                out_normal[HW + pid] = x.N1;
+                out_normal[2 * HW + pid] = x.N2;
+            }
         }
-        if (flags & EXT_NORMAL)
-        {
-            out_normal[pid] = x.N0;
-            out_normal[HW + pid] = x.N1;
-            out_normal[2 * HW + pid] = x.N2;
-        }
-    }
-    const uint32_t wm = wave_max_u32(inside ? last : 0u);
-    if (lane == 0) s_last[w] = wm;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        tile_last[tile] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
-    __syncthreads();
+        const uint32_t wm = wave_max_u32(inside ? last : 0u);
+        if (lane == 0) s_last[w] = wm;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            tile_last[tile] = range.x + max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+        __syncthreads();
     }
 }
 
@@ -535,24 +535,24 @@ __global__ __launch_bounds__(256) void render_bwd_ext_kernel(
     const uint32_t count = EXACT ? exact_ctr[0] : 1u;
     for (uint32_t it = EXACT ? blockIdx.x : 0u; it < count; it += EXACT ? gridDim.x : 1u)
     {
-    const int tile = EXACT ? (int)exact_list[it] : (int)tile_order[blockIdx.x];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tx = tile % gx, ty = tile / gx;
-    const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
-    const bool inside = px < W && py < H;
-    ExtHit<NS, EXACT> hit;
-    hit.ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
-    hit.k = k;
-    const uint2 range = ranges[tile];
-    const size_t HW = (size_t)H * W;
-    const size_t pid = inside ? (size_t)py * W + px : 0;
-    if (!EXACT && __syncthreads_or(!ray_inv_finite(hit.ray)) != 0) return;
-    const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt,
-                                 out_depth);
-    const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
-    adjoint_window<BWD_SYNC_MIN_MASKED, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers,
-                                                            lengths, geos, rgbs, orig, partial);
-    __syncthreads();
+        const int tile = EXACT ? (int)exact_list[it] : (int)tile_order[blockIdx.x];
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int tx = tile % gx, ty = tile / gx;
+        const int px = tx * TILE_X + (w & 1) * 8 + (lane & 7), py = ty * TILE_Y + (w >> 1) * 8 + (lane >> 3);
+        const bool inside = px < W && py < H;
+        ExtHit<NS, EXACT> hit;
+        hit.ray = make_ray(px, py, W, H, tanfovx, tanfovy, cx, cy, c2w);
+        hit.k = k;
+        const uint2 range = ranges[tile];
+        const size_t HW = (size_t)H * W;
+        const size_t pid = inside ? (size_t)py * W + px : 0;
+        if (!EXACT && __syncthreads_or(!ray_inv_finite(hit.ray)) != 0) return;
+        const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt,
+                                     out_depth);
+        const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
+        adjoint_window<BWD_SYNC_MIN_MASKED, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers,
+                                                                lengths, geos, rgbs, orig, partial);
+        __syncthreads();
     }
 }
 
