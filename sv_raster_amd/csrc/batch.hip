@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
                                                                int32_t* __restrict__ out_n_dup,
                                                                uint32_t* __restrict__ n_dup,
                                                                uint2* __restrict__ bboxes,
-                                                               uint32_t* __restrict__ qbits_out)
+                                                               uint32_t* __restrict__ qbits_out,
+                                                               float4* __restrict__ octs)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -132,14 +133,16 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
     }
     uint32_t nd, bits;
     uint2 bb;
+    float4 oc;
     preprocess_voxel<TS::TX, TS::TY>(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W,
                                      H, tiles_along(W, TS::TX), tiles_along(H, TS::TY), focal_x, focal_y, vw.cx,
-                                     vw.cy, m, cw, vw.near_plane, nd, bb, bits);
+                                     vw.cy, m, cw, vw.near_plane, nd, bb, bits, oc);
     const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
     out_n_dup[p] = (int32_t)nd;
     n_dup[p] = nd;
     bboxes[p] = bb;
     qbits_out[p] = bits;
+    octs[p] = oc;
 }
 
 // forward.cu:523-577 per (view, voxel) pair: each quadrant's run at its
@@ -276,7 +279,7 @@ template <int NS, bool EXACT, typename TS, bool MAX_W>
 __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t (*sv)[64], uint32_t* s_last, uint32_t g, int V,
                                       const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
                                       uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges,
-                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
                                       const float* __restrict__ centers, const float* __restrict__ lengths,
                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
                                       uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
@@ -298,7 +301,7 @@ __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t (*sv)[64], ui
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
     uint64_t* mask_out = MAX_W ? nullptr : hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
     const uint32_t last = composite_subtile<NS, EXACT, TS::NW, MAX_W, !MAX_W>(
-        st, w, lane, ray, t.inside, range, t.sx0, t.sy0, t.sx1, t.sy1, vox_list, bboxes + vo, centers, lengths, geos,
+        st, w, lane, ray, t.inside, range, t.sx0, t.sy0, t.sx1, t.sy1, vox_list, bboxes + vo, octs + vo, centers, lengths, geos,
         MAX_W ? rgbs : rgbs + 3 * vo, T, C0, C1, C2, sv, MAX_W ? reinterpret_cast<uint32_t*>(max_w + vo) : nullptr,
         mask_out);
     if (t.inside)
@@ -334,7 +337,7 @@ template <int NS, typename TS, bool MAX_W>
 __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order, uint32_t* __restrict__ ctr,
     uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const uint2* __restrict__ bboxes, const float4* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     float* __restrict__ max_w, uint64_t* __restrict__ hit_masks)
@@ -343,7 +346,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
     __shared__ uint32_t sv[MAX_W ? TS::NW : 1][64];
     __shared__ uint32_t s_last[TS::NW];
     const uint32_t g = order[blockIdx.x];
-    batch_fwd_tile<NS, false, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes,
+    batch_fwd_tile<NS, false, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, octs,
                                          centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T, max_w,
                                          hit_masks);
 }
@@ -352,7 +355,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
 template <int NS, typename TS, bool MAX_W>
 __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr, uint32_t* __restrict__ exact_list,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
     float* __restrict__ out_color, float* __restrict__ out_T, float* __restrict__ max_w,
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
         const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_fwd_tile<NS, true, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes,
+        batch_fwd_tile<NS, true, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, octs,
                                             centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T,
                                             max_w, hit_masks);
         __syncthreads();
@@ -659,11 +662,11 @@ int batch_fwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const B
                      hipStream_t stream)
 {
     batch_fwd_kernel<NS, TS, MAX_W><<<(unsigned)T, TS::THREADS, 0, stream>>>(
-        V, views, N, img.order, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+        V, views, N, img.order, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos, rgbs,
         img.tile_last, img.n_contrib, out_color, out_T, max_w, img.hit_masks);
     SVR_CHECK_LAUNCH();
     batch_fwd_exact_kernel<NS, TS, MAX_W><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
-        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos, rgbs,
         img.tile_last, img.n_contrib, out_color, out_T, max_w, img.hit_masks);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
@@ -744,7 +747,7 @@ int svr_batch_preprocess(int32_t V, const svr_view* views_host, const svr_view* 
     prof_begin(PROF_PREPROCESS, stream);
     SVR_WITH_TILE(shape, (batch_preprocess_kernel<TS><<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0,
                                                         stream>>>(views_dev, N, centers, lengths, out_n_dup, g.n_dup,
-                                                                  g.bbox, g.qbits)));
+                                                                  g.bbox, g.qbits, g.oct)));
     SVR_CHECK_LAUNCH();
     prof_end(PROF_PREPROCESS, stream);
     return SVR_OK;

@@ -43,6 +43,7 @@ struct StageT
     float linv[NW][64];
     float g[8][NW][64];
     float rgb[3][NW][64];
+    uint64_t diag[NW][32];  // each wave's copy of the diagonal half-plane masks (init_diag)
 };
 using Stage = StageT<NWAVE>;
 
@@ -59,6 +60,41 @@ __device__ inline uint64_t bbox_pixel_mask(uint2 bb, int sx0, int sy0, int sx1, 
     const uint64_t rows_hi = (y1 >= 7) ? ~0ull : ((1ull << (8 * (y1 + 1))) - 1ull);
     const uint64_t rows = rows_hi & ~((1ull << (8 * y0)) - 1ull);
     return (col * 0x0101010101010101ull) & rows & qsel;
+}
+
+// Diagonal half-plane masks of an 8x8 sub-tile (bit 8 * row + column):
+// entry k + 1 (k = -1 .. 14) holds the pixels with column + row <= k, entry
+// 24 + k (k = -8 .. 7) those with column - row <= k.  Lanes 0..31 of a wave
+// write one entry each into the wave's table.
+__device__ inline void init_diag(uint64_t* diag, int lane)
+{
+    if (lane < 32)
+    {
+        const int k = lane < 16 ? lane - 1 : lane - 24;
+        uint64_t m = 0ull;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+        {
+            const int t = lane < 16 ? k - r : k + r;  // last column kept in row r
+            const uint32_t row = t < 0 ? 0u : (t >= 7 ? 0xFFu : (2u << t) - 1u);
+            m |= (uint64_t)row << (8 * r);
+        }
+        diag[lane] = m;
+    }
+}
+
+// Pixels of the sub-tile whose centre lies in the projected octagon o
+// (bounds of x + y and of x - y in the pixel frame, the sub-tile's first
+// pixel at (sx0, sy0)): lo <= S0 + col + row <= hi and likewise for x - y.
+__device__ inline uint64_t oct_pixel_mask(const uint64_t* diag, float4 o, int sx0, int sy0)
+{
+    const float s0 = (float)(sx0 + sy0), d0 = (float)(sx0 - sy0);
+    const auto cl = [](float x) { return fminf(fmaxf(x, -16.f), 32.f); };
+    const int a0 = (int)ceilf(cl(o.x - s0)), a1 = (int)floorf(cl(o.y - s0));
+    const int b0 = (int)ceilf(cl(o.z - d0)), b1 = (int)floorf(cl(o.w - d0));
+    const uint64_t sm = diag[min(max(a1, -1), 14) + 1] & ~diag[min(max(a0 - 1, -1), 14) + 1];
+    const uint64_t dm = diag[min(max(b1, -8), 7) + 24] & ~diag[min(max(b0 - 1, -8), 7) + 24];
+    return sm & dm;
 }
 
 __device__ inline uint64_t select_q(const uint64_t* qpix, uint32_t q)
@@ -89,17 +125,24 @@ struct EntryRegs
 };
 
 // RGB = false: the colour is not read (weights-only passes) and staged as 0.
-template <bool RGB = true>
-__device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes,
+// OCT: the candidate mask is also cut to the projected octagon.  Only for
+// rays with finite 1 / direction: where a direction component is exactly
+// zero, the ray-box test of the reference yields NaN slab bounds for boxes
+// with a face in the ray's plane and counts them as hits (forward.cu's
+// !(a > b)), whether or not the ray meets the box, so those pixels keep the
+// bbox candidates.
+template <bool RGB = true, bool OCT = true>
+__device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
                                   const float* __restrict__ centers, const float* __restrict__ lengths,
                                   const float* __restrict__ geos, const float* __restrict__ rgbs, int sx0, int sy0,
-                                  int sx1, int sy1, const uint64_t* qpix)
+                                  int sx1, int sy1, const uint64_t* qpix, const uint64_t* diag)
 {
     e.pm = 0ull;
     e.v = val & VOX_ID_MASK;
     if (!valid) return;
     e.bb = bboxes[e.v];
     e.pm = bbox_pixel_mask(e.bb, sx0, sy0, sx1, sy1, select_q(qpix, val >> 29));
+    if constexpr (OCT) e.pm &= oct_pixel_mask(diag, octs[e.v], sx0, sy0);
     if (!e.pm) return;
     const size_t v = e.v;
     e.c0 = centers[3 * v + 0];
@@ -132,27 +175,6 @@ __device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryR
     st.rgb[0][w][lane] = e.r0; st.rgb[1][w][lane] = e.r1; st.rgb[2][w][lane] = e.r2;
 }
 
-// Candidates (bit l: staged entry l) whose box the pixel's ray meets, i.e.
-// ray_aabb gives !(a > b).  A pre-pass over the bbox candidates with the
-// ray-box test alone, so that the lanes of the compositing loop step over
-// real intersections only: on the 1080p scene a third of the candidates
-// miss the box, and in the longest tiles two thirds of the lock-step
-// iterations were such misses.
-template <bool EXACT, int NW>
-__device__ inline uint64_t raybox_filter(const StageT<NW>& st, int w, uint64_t hm, const PixRay& ray)
-{
-    uint64_t keep = 0ull;
-    while (hm)
-    {
-        const int l = __builtin_ctzll(hm);
-        hm &= hm - 1ull;
-        float a, b;
-        ray_aabb<EXACT>(st.c[0][w][l], st.c[1][w][l], st.c[2][w][l], st.l[w][l], ray, a, b);
-        if (!(a > b)) keep |= 1ull << l;
-    }
-    return keep;
-}
-
 // Front-to-back compositing of one wave's 8x8 sub-tile over the tile's list
 // (forward.cu:150-328 for colour and T).  Returns the pixel's n_contrib and
 // leaves its transmittance and colour in T, C0..C2.  EXACT selects the
@@ -166,7 +188,7 @@ __device__ inline uint64_t raybox_filter(const StageT<NW>& st, int w, uint64_t h
 template <int NS, bool EXACT, int NW, bool MAX_W = false, bool MASK = false>
 __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
                                              int sx0, int sy0, int sx1, int sy1,
-                                             const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                             const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
                                              const float* __restrict__ centers, const float* __restrict__ lengths,
                                              const float* __restrict__ geos, const float* __restrict__ rgbs,
                                              float& T, float& C0, float& C1, float& C2,
@@ -175,6 +197,8 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
 {
     uint32_t last = 0;
     bool done = !inside;
+    init_diag(st.diag[w], lane);
+    wave_sync();
 
     // The next chunk is loaded (into registers) while the current one is
     // composited; its pixel masks use the quadrant masks of the lanes active
@@ -193,8 +217,8 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
     {
         const uint32_t j = base + (uint32_t)lane;
         const bool valid = j < range.y;
-        load_entry<!MAX_W>(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                           sy1, qpix);
+        load_entry<!MAX_W, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0, sx1,
+                           sy1, qpix, st.diag[w]);
     }
     while (base < range.y)
     {
@@ -217,11 +241,10 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             for (uint32_t q = 0; q < 8; ++q) qpix[q] = qall[q] & act;
             const uint32_t j = nbase + (uint32_t)lane;
             const bool valid = j < range.y;
-            load_entry<!MAX_W>(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0,
-                               sx1, sy1, qpix);
+            load_entry<!MAX_W, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0,
+                               sx1, sy1, qpix, st.diag[w]);
         }
 
-        hm = raybox_filter<EXACT>(st, w, hm, ray);
         uint64_t hits = 0ull;
         while (hm)
         {
@@ -229,7 +252,8 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             hm &= hm - 1ull;
             const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);  // same values as in raybox_filter: !(a > b)
+            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
+            if (a > b) continue;
             float g[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];

@@ -175,9 +175,11 @@ __global__ __launch_bounds__(256) void filter_geom_kernel(int64_t n, const int64
                                                           const uint32_t* __restrict__ n_dup,
                                                           const uint2* __restrict__ bbox,
                                                           const uint32_t* __restrict__ qbits,
+                                                          const float4* __restrict__ oct,
                                                           uint32_t* __restrict__ n_dup_out,
                                                           uint2* __restrict__ bbox_out,
-                                                          uint32_t* __restrict__ qbits_out)
+                                                          uint32_t* __restrict__ qbits_out,
+                                                          float4* __restrict__ oct_out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -185,6 +187,7 @@ __global__ __launch_bounds__(256) void filter_geom_kernel(int64_t n, const int64
     n_dup_out[i] = n_dup[src];
     bbox_out[i] = bbox[src];
     qbits_out[i] = qbits[src];
+    oct_out[i] = oct[src];
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -322,7 +325,7 @@ int svr_filter_geom_state(int64_t n_orig, int64_t n_keep, const int64_t* indices
     GeomState src = GeomState::carve((void*)geom_state, n_orig, nullptr);
     GeomState dst = GeomState::carve(new_geom_state, n_keep, nullptr);
     filter_geom_kernel<<<blocks_for(n_keep), 256, 0, stream>>>(n_keep, indices, src.n_dup, src.bbox, src.qbits,
-                                                               dst.n_dup, dst.bbox, dst.qbits);
+                                                               src.oct, dst.n_dup, dst.bbox, dst.qbits, dst.oct);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }

@@ -14,7 +14,7 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
     const float* __restrict__ w2c, const float* __restrict__ c2w, float near_plane,
     const float* __restrict__ centers, const float* __restrict__ lengths,
     int32_t* __restrict__ out_n_dup, uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes,
-    uint32_t* __restrict__ qbits_out)
+    uint32_t* __restrict__ qbits_out, float4* __restrict__ octs)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -27,8 +27,10 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
     }
     uint32_t nd, bits;
     uint2 bb;
+    float4 oc;
     preprocess_voxel(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W, H, gx, gy,
-                     focal_x, focal_y, cx, cy, m, cw, near_plane, nd, bb, bits);
+                     focal_x, focal_y, cx, cy, m, cw, near_plane, nd, bb, bits, oc);
+    octs[v] = oc;
     out_n_dup[v] = (int32_t)nd;
     n_dup[v] = nd;
     bboxes[v] = bb;
@@ -282,7 +284,7 @@ int svr_rasterize_preprocess(int32_t W, int32_t H, float tanfovx, float tanfovy,
     prof_begin(PROF_PREPROCESS, stream);
     preprocess_kernel<<<(unsigned)blocks, 256, 0, stream>>>(N, W, H, tiles_x(W), tiles_y(H), focal_x, focal_y, cx,
                                                             cy, w2c, c2w, near_plane, centers, lengths, out_n_dup,
-                                                            g.n_dup, g.bbox, g.qbits);
+                                                            g.n_dup, g.bbox, g.qbits, g.oct);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_PREPROCESS, stream);
     return SVR_OK;

@@ -37,7 +37,7 @@ constexpr unsigned EXACT_GRID = 64;
 template <int NS, bool EXACT>
 __device__ inline void fwd_tile(Stage& st, uint32_t (&s_last)[NWAVE], int tile, const PixRay& ray, int W, int H,
                                 int gx, float bg, const uint2* __restrict__ ranges,
-                                const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
                                 const float* __restrict__ centers, const float* __restrict__ lengths,
                                 const float* __restrict__ geos, const float* __restrict__ rgbs,
                                 uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
@@ -55,7 +55,7 @@ __device__ inline void fwd_tile(Stage& st, uint32_t (&s_last)[NWAVE], int tile, 
     float T = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
     uint64_t* mask_out = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
     const uint32_t last = composite_subtile<NS, EXACT, NWAVE, false, true>(
-        st, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list, bboxes, centers, lengths, geos, rgbs, T, C0, C1,
+        st, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list, bboxes, octs, centers, lengths, geos, rgbs, T, C0, C1,
         C2, nullptr, nullptr, mask_out);
 
     if (inside)
@@ -88,7 +88,7 @@ template <int NS>
 __global__ __launch_bounds__(256) void render_fwd_kernel(
     int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const uint2* __restrict__ bboxes, const float4* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     uint64_t* __restrict__ hit_masks, uint32_t* __restrict__ exact_ctr, uint32_t* __restrict__ exact_list)
@@ -102,14 +102,14 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
         if (threadIdx.x == 0) exact_list[atomicAdd(exact_ctr, 1u)] = (uint32_t)tile;
         return;
     }
-    fwd_tile<NS, false>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
+    fwd_tile<NS, false>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, octs, centers, lengths, geos, rgbs,
                         tile_last, n_contrib, out_color, out_T, hit_masks);
 }
 
 template <int NS>
 __global__ __launch_bounds__(256) void render_fwd_exact_kernel(
     int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy, const float* __restrict__ c2w, float bg,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
     float* __restrict__ out_color, float* __restrict__ out_T, uint64_t* __restrict__ hit_masks,
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void render_fwd_exact_kernel(
     {
         const int tile = (int)exact_list[i];
         const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
-        fwd_tile<NS, true>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, centers, lengths, geos, rgbs,
+        fwd_tile<NS, true>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, octs, centers, lengths, geos, rgbs,
                            tile_last, n_contrib, out_color, out_T, hit_masks);
         __syncthreads();
     }
@@ -363,11 +363,11 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     const int T = gx * gy;
     prof_begin(PROF_RENDER_FWD, stream);
     render_fwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(
-        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos,
         rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     render_fwd_exact_kernel<NS><<<EXACT_GRID, 256, 0, stream>>>(
-        W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos, rgbs,
+        W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos, rgbs,
         img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);

@@ -114,7 +114,7 @@ struct ExtAcc
 template <int NS, bool EXACT, int NW>
 __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64], int w, int lane, const PixRay& ray,
                                          bool inside, uint2 range, int sx0, int sy0, int sx1, int sy1,
-                                         const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes,
+                                         const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
                                          const float* __restrict__ centers, const float* __restrict__ lengths,
                                          const float* __restrict__ geos, const float* __restrict__ rgbs, int flags,
                                          float* __restrict__ max_w, float& T, float& C0, float& C1, float& C2,
@@ -122,6 +122,8 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
 {
     uint32_t last = 0;
     bool done = !inside;
+    init_diag(st.diag[w], lane);
+    wave_sync();
     uint64_t qpix[8];
 #pragma unroll
     for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
@@ -130,8 +132,8 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
     {
         const uint32_t j = base + (uint32_t)lane;
         const bool valid = j < range.y;
-        load_entry(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
-                   qpix);
+        load_entry<true, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
+                   qpix, st.diag[w]);
     }
     while (base < range.y)
     {
@@ -149,11 +151,10 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
             for (uint32_t q = 0; q < 8; ++q) qpix[q] = __ballot(!done && ray.pq == q);
             const uint32_t j = nbase + (uint32_t)lane;
             const bool valid = j < range.y;
-            load_entry(cur, valid, valid ? vox_list[j] : 0u, bboxes, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                       sy1, qpix);
+            load_entry<true, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0, sx1,
+                       sy1, qpix, st.diag[w]);
         }
 
-        hm = raybox_filter<EXACT>(st, w, hm, ray);
         uint64_t hits = 0ull;
         while (hm)
         {
@@ -161,7 +162,8 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
             hm &= hm - 1ull;
             const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);  // same values as in raybox_filter: !(a > b)
+            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
+            if (a > b) continue;
             float g[8], lv[NS];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
@@ -261,7 +263,7 @@ template <int NS, bool EXACT>
 __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const uint2* __restrict__ bboxes, const float4* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, int flags, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     float* __restrict__ out_depth, float* __restrict__ out_normal, float* __restrict__ max_w,
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
         ExtAcc x{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
         uint64_t* mask_out = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
         const uint32_t last = composite_ext<NS, EXACT>(st, sv, w, lane, ray, inside, range, sx0, sy0, sx1, sy1, vox_list,
-                                                       bboxes, centers, lengths, geos, rgbs, flags, max_w, T, C0, C1, C2,
+                                                       bboxes, octs, centers, lengths, geos, rgbs, flags, max_w, T, C0, C1, C2,
                                                        x, mask_out);
         if (inside)
         {
@@ -566,12 +568,12 @@ int launch_fwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     const int T = gx * gy;
     prof_begin(PROF_RENDER_FWD, stream);
     render_fwd_ext_kernel<NS, false><<<(unsigned)T, 256, 0, stream>>>(
-        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos,
         rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w,
         img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     render_fwd_ext_kernel<NS, true><<<EXT_EXACT_GRID, 256, 0, stream>>>(
-        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, centers, lengths, geos,
+        W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos,
         rgbs, flags, img.tile_last, img.n_contrib, out_color, out_T, out_depth, out_normal, max_w,
         img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();

@@ -10,6 +10,13 @@
 
 namespace svr {
 
+// Widening of the projected octagon, in pixels: a constant part plus a part
+// proportional to the magnitudes in play (projected coordinates and focal
+// lengths), far above the float rounding of the projection and of the
+// ray-box test, so that a pixel whose ray meets the box is never cut.
+constexpr float OCT_MARGIN = 0.05f;
+constexpr float OCT_MARGIN_REL = 4e-6f;
+
 // preprocess.cu:24-132 for one voxel (centre c0..c2, half size r) seen by one
 // camera: near-sphere cull, 8-corner projection, packed pixel bbox, quadrant
 // bitset and the number of list entries (tiles x quadrants).  The arithmetic
@@ -20,12 +27,20 @@ template <int TX = TILE_X, int TY = TILE_Y>
 __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, int W, int H, int gx, int gy,
                                         float focal_x, float focal_y, float cx, float cy, const float* w2c,
                                         const float* c2w, float near_plane, uint32_t& nd_out, uint2& bb_out,
-                                        uint32_t& bits_out)
+                                        uint32_t& bits_out, float4& oct_out)
 {
     const float ro0 = c2w[3], ro1 = c2w[7], ro2 = c2w[11];
     const float* m = w2c;
     uint32_t nd = 0, bits = 0;
     uint2 bb = make_uint2(0u, 0u);
+    // octagon of the projection in the pixel frame (pixel (px, py) centred
+    // at (px, py)): bounds of x + y and x - y over the projected corners,
+    // widened by OCT_MARGIN pixels; unbounded if a corner is behind the
+    // near plane (the projection of the box is then not their hull)
+    float os0 = 1e30f, os1 = -1e30f, od0 = 1e30f, od1 = -1e30f;
+    float pmax = 0.f;
+    bool all_front = true;
+    const float cxh_o = cx - 0.5f, cyh_o = cy - 0.5f;
     const float rel0 = c0 - ro0, rel1 = c1 - ro1, rel2 = c2 - ro2;
     if (!(rel0 * rel0 + rel1 * rel1 + rel2 * rel2 < near_plane * near_plane))
     {
@@ -40,9 +55,19 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
             const float camx = m[0] * p0 + m[1] * p1 + m[2] * p2 + m[3];
             const float camy = m[4] * p0 + m[5] * p1 + m[6] * p2 + m[7];
             const float camz = m[8] * p0 + m[9] * p1 + m[10] * p2 + m[11];
-            if (camz < near_plane) continue;
+            if (camz < near_plane)
+            {
+                all_front = false;
+                continue;
+            }
             const float iz = 1.0f / camz;
             const float u = camx * iz, vv = camy * iz;
+            {
+                const float up = __builtin_fmaf(focal_x, u, cxh_o), vp = __builtin_fmaf(focal_y, vv, cyh_o);
+                os0 = fminf(os0, up + vp); os1 = fmaxf(os1, up + vp);
+                od0 = fminf(od0, up - vp); od1 = fmaxf(od1, up - vp);
+                pmax = fmaxf(pmax, fabsf(up) + fabsf(vp));
+            }
             const uint32_t q = ((p0 < ro0) << 2) | ((p1 < ro1) << 1) | (p2 < ro2);
             umin = smin(umin, u); vmin = smin(vmin, vv);
             umax = smax(umax, u); vmax = smax(vmax, vv);
@@ -77,6 +102,9 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
     nd_out = nd;
     bb_out = bb;
     bits_out = bits;
+    const float mg = OCT_MARGIN + OCT_MARGIN_REL * (pmax + focal_x + focal_y);
+    oct_out = all_front ? make_float4(os0 - mg, os1 + mg, od0 - mg, od1 + mg)
+                        : make_float4(-1e30f, 1e30f, -1e30f, 1e30f);
 }
 
 // Tile rectangle of a packed bbox (auxiliary.h:150-170) for TX x TY tiles.
