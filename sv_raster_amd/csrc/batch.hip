@@ -908,7 +908,6 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
 
     if (R > 0)
     {
-        if (fill_u32(wk.partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
         if (fill_u32(img.ctr + 3, 0u, 2, stream) != SVR_OK) return SVR_ERR_HIP;
         prof_begin(PROF_RENDER_BWD, stream);
         SVR_WITH_TILE(shape, (rc = n_samp == 1   ? batch_bwd_launch<1, TS>(V, views_dev, N, T, b, img, g, centers,

@@ -18,10 +18,20 @@ void set_error(const char* fmt, ...)
     va_end(ap);
 }
 
-__global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value, int64_t count)
+// 16-byte stores over the aligned body of the range; the elements before the
+// first 16-byte boundary (head) and after the last one (tail) are stored
+// singly by the first threads of the grid.
+__global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value, int64_t count, int64_t head)
 {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = value;
+    const int64_t nvec = (count - head) >> 2;
+    uint4* __restrict__ body = reinterpret_cast<uint4*>(dst + head);
+    const uint4 v4 = make_uint4(value, value, value, value);
+    for (int64_t i = t0; i < nvec; i += stride) body[i] = v4;
+    if (t0 < head) dst[t0] = value;
+    const int64_t tail0 = head + 4 * nvec;
+    if (t0 < count - tail0) dst[tail0 + t0] = value;
 }
 
 // Clears device memory with a kernel instead of hipMemsetAsync so that a
@@ -29,9 +39,13 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value, int6
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return SVR_OK;
-    int64_t blocks = (count + 255) / 256;
+    SVR_REQUIRE(((uintptr_t)dst & 3u) == 0, SVR_ERR_INVALID, "fill: destination not 4-byte aligned");
+    int64_t head = (int64_t)(((16u - ((uintptr_t)dst & 15u)) & 15u) >> 2);
+    if (head > count) head = count;
+    int64_t blocks = ((count - head) / 4 + 255) / 256;
+    if (blocks < 1) blocks = 1;  // head and tail (at most 3 elements each) need the first threads only
     if (blocks > 4096) blocks = 4096;
-    fill_u32_kernel<<<(unsigned)blocks, 256, 0, stream>>>((uint32_t*)dst, value, count);
+    fill_u32_kernel<<<(unsigned)blocks, 256, 0, stream>>>((uint32_t*)dst, value, count, head);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }

@@ -403,9 +403,11 @@ constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 //      (wave_sum12).  Between sync slots each lane walks its own hits in
 //      lock-step.  Both add into the window's shared accumulators with LDS
 //      atomics (the waves of the tile meet there instead of in global memory);
-//   4. the workgroup writes one 12-float row per used entry into
-//      partial[orig] with plain stores: an entry belongs to one tile, so this
-//      is its only writer.  Rows of unused entries keep the caller's zeros.
+//   4. the workgroup writes the 12-float row of every entry of the window
+//      into partial[orig] with plain stores (zeros for entries no pixel
+//      used): an entry belongs to one tile, so this is its only writer.
+// After the last window, the entries past the tile's last contributor get
+// zero rows, so partial needs no clearing beforehand.
 // HIT holds the pixel's ray and running state and computes one hit's 12
 // terms (PlainHit here; the adjoint with the optional terms has its own).
 template <int SYNC_MIN, typename TS, int KW, typename HIT>
@@ -476,6 +478,8 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
         {
             const int i = t + k * TS::THREADS;
             const int j = i >> 6, l = i & 63;
+            if (i < KW * 64 && j < nj && range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l < range.y)
+                sh.orow[j][l] = orow_pf[k];
             if (i < KW * 64 && j < nj && sh.used[j][l])
             {
                 const size_t v = val_pf[k] & VOX_ID_MASK;
@@ -485,7 +489,6 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
                 p.q[1] = *reinterpret_cast<const float4*>(geos + 8 * v);
                 p.q[2] = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
                 p.q[3] = make_float4(rgbs[3 * v + 0], rgbs[3 * v + 1], rgbs[3 * v + 2], 1.f / len);
-                sh.orow[j][l] = orow_pf[k];
             }
         }
         if (c0 - KW >= 0) prefetch(c0 - KW);
@@ -584,11 +587,11 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
 #endif
         }
         __syncthreads();
-        // 4. one row per used entry, then reset the window
+        // 4. one row per entry (accumulators of unused entries are zero), then reset the window
         for (int i = t; i < KW * 64; i += TS::THREADS)
         {
             const int j = i >> 6, l = i & 63;
-            if (j < nj && sh.used[j][l])
+            if (j < nj && range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l < range.y)
             {
                 float a[12];
 #pragma unroll
@@ -605,6 +608,15 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
             }
         }
         __syncthreads();
+    }
+    // entries past every pixel's last contributor
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t e = range.x + (C << 6) + (uint32_t)t; e < range.y; e += TS::THREADS)
+    {
+        float4* row = reinterpret_cast<float4*>(partial + 12 * (size_t)orig[e]);
+        row[0] = z;
+        row[1] = z;
+        row[2] = z;
     }
 }
 

@@ -383,7 +383,6 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
 {
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int T = gx * gy;
-    if (fill_u32(partial, 0u, 12 * R, stream) != SVR_OK) return SVR_ERR_HIP;
     prof_begin(PROF_RENDER_BWD, stream);
     render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
