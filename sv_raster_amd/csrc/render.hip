@@ -241,7 +241,21 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N
         {
             const uint32_t off = run_offset(g, (size_t)v, bits, (uint32_t)__builtin_ctz(b));
             if ((uint64_t)off + tiles > cap) continue;
-            for (uint32_t e = 0; e < tiles; ++e) add_row(a, partial, off + e);
+            // rows in pairs: both rows' loads are in flight before the adds
+            // (which keep the row order)
+            uint32_t e = 0;
+            for (; e + 2 <= tiles; e += 2)
+            {
+                const float4* r0 = reinterpret_cast<const float4*>(partial + 12 * (size_t)(off + e));
+                const float4 p0 = r0[0], p1 = r0[1], p2 = r0[2], q0 = r0[3], q1 = r0[4], q2 = r0[5];
+                a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
+                a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
+                a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+                a[0] += q0.x; a[1] += q0.y; a[2] += q0.z; a[3] += q0.w;
+                a[4] += q1.x; a[5] += q1.y; a[6] += q1.z; a[7] += q1.w;
+                a[8] += q2.x; a[9] += q2.y; a[10] += q2.z; a[11] += q2.w;
+            }
+            if (e < tiles) add_row(a, partial, off + e);
         }
     }
     write_grad((size_t)v, a, dgeos, drgbs, dsub);
