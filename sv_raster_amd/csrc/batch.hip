@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
                                                                uint32_t* __restrict__ n_dup,
                                                                uint2* __restrict__ bboxes,
                                                                uint32_t* __restrict__ qbits_out,
-                                                               float4* __restrict__ octs)
+                                                               OctBox* __restrict__ octs)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
     }
     uint32_t nd, bits;
     uint2 bb;
-    float4 oc;
+    OctBox oc;
     preprocess_voxel<TS::TX, TS::TY>(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W,
                                      H, tiles_along(W, TS::TX), tiles_along(H, TS::TY), focal_x, focal_y, vw.cx,
                                      vw.cy, m, cw, vw.near_plane, nd, bb, bits, oc);
@@ -279,7 +279,7 @@ template <int NS, bool EXACT, typename TS, bool MAX_W>
 __device__ inline void batch_fwd_tile(StageT<TS::NW>& st, uint32_t (*sv)[64], uint32_t* s_last, uint32_t g, int V,
                                       const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
                                       uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges,
-                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+                                      const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
                                       const float* __restrict__ centers, const float* __restrict__ lengths,
                                       const float* __restrict__ geos, const float* __restrict__ rgbs,
                                       uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
@@ -337,7 +337,7 @@ template <int NS, typename TS, bool MAX_W>
 __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order, uint32_t* __restrict__ ctr,
     uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float4* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     float* __restrict__ max_w, uint64_t* __restrict__ hit_masks)
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_kernel(
 template <int NS, typename TS, bool MAX_W>
 __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
     int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr, uint32_t* __restrict__ exact_list,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
     float* __restrict__ out_color, float* __restrict__ out_T, float* __restrict__ max_w,

@@ -118,7 +118,6 @@ struct EntryRegs
 {
     uint32_t v;
     uint64_t pm;
-    uint2 bb;
     float c0, c1, c2, l, linv;
     float4 g0, g1;
     float r0, r1, r2;
@@ -132,7 +131,7 @@ struct EntryRegs
 // !(a > b)), whether or not the ray meets the box, so those pixels keep the
 // bbox candidates.
 template <bool RGB = true, bool OCT = true>
-__device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+__device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
                                   const float* __restrict__ centers, const float* __restrict__ lengths,
                                   const float* __restrict__ geos, const float* __restrict__ rgbs, int sx0, int sy0,
                                   int sx1, int sy1, const uint64_t* qpix, const uint64_t* diag)
@@ -140,9 +139,16 @@ __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const 
     e.pm = 0ull;
     e.v = val & VOX_ID_MASK;
     if (!valid) return;
-    e.bb = bboxes[e.v];
-    e.pm = bbox_pixel_mask(e.bb, sx0, sy0, sx1, sy1, select_q(qpix, val >> 29));
-    if constexpr (OCT) e.pm &= oct_pixel_mask(diag, octs[e.v], sx0, sy0);
+    if constexpr (OCT)
+    {
+        const OctBox& ob = octs[e.v];
+        const float4 o = ob.o;
+        e.pm = bbox_pixel_mask(ob.tb, sx0, sy0, sx1, sy1, select_q(qpix, val >> 29)) & oct_pixel_mask(diag, o, sx0, sy0);
+    }
+    else
+    {
+        e.pm = bbox_pixel_mask(bboxes[e.v], sx0, sy0, sx1, sy1, select_q(qpix, val >> 29));
+    }
     if (!e.pm) return;
     const size_t v = e.v;
     e.c0 = centers[3 * v + 0];
@@ -188,7 +194,7 @@ __device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryR
 template <int NS, bool EXACT, int NW, bool MAX_W = false, bool MASK = false>
 __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, const PixRay& ray, bool inside, uint2 range,
                                              int sx0, int sy0, int sx1, int sy1,
-                                             const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+                                             const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
                                              const float* __restrict__ centers, const float* __restrict__ lengths,
                                              const float* __restrict__ geos, const float* __restrict__ rgbs,
                                              float& T, float& C0, float& C1, float& C2,

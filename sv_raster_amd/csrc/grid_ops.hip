@@ -175,11 +175,11 @@ __global__ __launch_bounds__(256) void filter_geom_kernel(int64_t n, const int64
                                                           const uint32_t* __restrict__ n_dup,
                                                           const uint2* __restrict__ bbox,
                                                           const uint32_t* __restrict__ qbits,
-                                                          const float4* __restrict__ oct,
+                                                          const OctBox* __restrict__ oct,
                                                           uint32_t* __restrict__ n_dup_out,
                                                           uint2* __restrict__ bbox_out,
                                                           uint32_t* __restrict__ qbits_out,
-                                                          float4* __restrict__ oct_out)
+                                                          OctBox* __restrict__ oct_out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

@@ -14,7 +14,7 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
     const float* __restrict__ w2c, const float* __restrict__ c2w, float near_plane,
     const float* __restrict__ centers, const float* __restrict__ lengths,
     int32_t* __restrict__ out_n_dup, uint32_t* __restrict__ n_dup, uint2* __restrict__ bboxes,
-    uint32_t* __restrict__ qbits_out, float4* __restrict__ octs)
+    uint32_t* __restrict__ qbits_out, OctBox* __restrict__ octs)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void preprocess_kernel(
     }
     uint32_t nd, bits;
     uint2 bb;
-    float4 oc;
+    OctBox oc;
     preprocess_voxel(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], 0.5f * lengths[v], W, H, gx, gy,
                      focal_x, focal_y, cx, cy, m, cw, near_plane, nd, bb, bits, oc);
     octs[v] = oc;

@@ -37,7 +37,7 @@ constexpr unsigned EXACT_GRID = 64;
 template <int NS, bool EXACT>
 __device__ inline void fwd_tile(Stage& st, uint32_t (&s_last)[NWAVE], int tile, const PixRay& ray, int W, int H,
                                 int gx, float bg, const uint2* __restrict__ ranges,
-                                const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+                                const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
                                 const float* __restrict__ centers, const float* __restrict__ lengths,
                                 const float* __restrict__ geos, const float* __restrict__ rgbs,
                                 uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
@@ -88,7 +88,7 @@ template <int NS>
 __global__ __launch_bounds__(256) void render_fwd_kernel(
     int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float4* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     uint64_t* __restrict__ hit_masks, uint32_t* __restrict__ exact_ctr, uint32_t* __restrict__ exact_list)
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
 template <int NS>
 __global__ __launch_bounds__(256) void render_fwd_exact_kernel(
     int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy, const float* __restrict__ c2w, float bg,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
     float* __restrict__ out_color, float* __restrict__ out_T, uint64_t* __restrict__ hit_masks,

@@ -114,7 +114,7 @@ struct ExtAcc
 template <int NS, bool EXACT, int NW>
 __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64], int w, int lane, const PixRay& ray,
                                          bool inside, uint2 range, int sx0, int sy0, int sx1, int sy1,
-                                         const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const float4* __restrict__ octs,
+                                         const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
                                          const float* __restrict__ centers, const float* __restrict__ lengths,
                                          const float* __restrict__ geos, const float* __restrict__ rgbs, int flags,
                                          float* __restrict__ max_w, float& T, float& C0, float& C1, float& C2,
@@ -263,7 +263,7 @@ template <int NS, bool EXACT>
 __global__ __launch_bounds__(256) void render_fwd_ext_kernel(
     int W, int H, int gx, const uint32_t* __restrict__ tile_order, float tanfovx, float tanfovy, float cx, float cy,
     const float* __restrict__ c2w, float bg, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const uint2* __restrict__ bboxes, const float4* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
+    const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs, const float* __restrict__ centers, const float* __restrict__ lengths,
     const float* __restrict__ geos, const float* __restrict__ rgbs, int flags, uint32_t* __restrict__ tile_last,
     uint32_t* __restrict__ n_contrib, float* __restrict__ out_color, float* __restrict__ out_T,
     float* __restrict__ out_depth, float* __restrict__ out_normal, float* __restrict__ max_w,

@@ -14,6 +14,19 @@
 
 namespace svr {
 
+// Per-voxel candidate bounds of the forward, in the pixel frame (pixel
+// (x, y) centred at (x, y)): o = bounds of x + y and of x - y over the
+// projected corners (the octagon), tb = the pixel box [ceil(lo), floor(hi)]
+// of the projected x and y extents, packed like the bbox (x << 16 | y), both
+// widened by a rounding margin.  32 bytes, so one voxel's bounds sit in one
+// cache line.
+struct alignas(16) OctBox
+{
+    float4 o;
+    uint2 tb;
+    uint2 pad;
+};
+
 // backends/new_cuda/src/config.h:14-34
 constexpr int TILE_X = 16;
 constexpr int TILE_Y = 16;

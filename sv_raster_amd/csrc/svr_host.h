@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "../../include/svraster_hip.h"
+#include "svr_device.h"
 
 namespace svr {
 
@@ -104,7 +105,7 @@ struct GeomState
     uint32_t* n_dup;       // [n] entries of the item (tiles x quadrants)
     uint32_t* run0;        // [n] first entry of the item's run when it has one quadrant
     uint2* bbox;           // [n] packed pixel bbox (x<<16 | y) min / max
-    float4* oct;           // [n] projected octagon: bounds of x + y and of x - y (pixel frame)
+    OctBox* oct;           // [n] projected octagon and pixel box (forward candidate bounds)
     uint32_t* qbits;       // [n] camera-quadrant bitset
     uint32_t* block_sums;  // [ceil(n / SCAN_TILE) + 1]
     uint32_t* vis_sums;    // [ceil(n / SCAN_TILE) + 1] in-frustum items per scan block
@@ -125,7 +126,7 @@ struct GeomState
         g.n_dup = c.take<uint32_t>(n);
         g.run0 = c.take<uint32_t>(n);
         g.bbox = c.take<uint2>(n);
-        g.oct = c.take<float4>(n);
+        g.oct = c.take<OctBox>(n);
         g.qbits = c.take<uint32_t>(n);
         g.block_sums = c.take<uint32_t>((n + SCAN_TILE - 1) / SCAN_TILE + 1);
         g.vis_sums = c.take<uint32_t>((n + SCAN_TILE - 1) / SCAN_TILE + 1);

@@ -27,7 +27,7 @@ template <int TX = TILE_X, int TY = TILE_Y>
 __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, int W, int H, int gx, int gy,
                                         float focal_x, float focal_y, float cx, float cy, const float* w2c,
                                         const float* c2w, float near_plane, uint32_t& nd_out, uint2& bb_out,
-                                        uint32_t& bits_out, float4& oct_out)
+                                        uint32_t& bits_out, OctBox& oct_out)
 {
     const float ro0 = c2w[3], ro1 = c2w[7], ro2 = c2w[11];
     const float* m = w2c;
@@ -38,7 +38,7 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
     // widened by OCT_MARGIN pixels; unbounded if a corner is behind the
     // near plane (the projection of the box is then not their hull)
     float os0 = 1e30f, os1 = -1e30f, od0 = 1e30f, od1 = -1e30f;
-    float pmax = 0.f;
+    float pmax = 0.f, ox0 = 1e30f, ox1 = -1e30f, oy0 = 1e30f, oy1 = -1e30f;
     bool all_front = true;
     const float cxh_o = cx - 0.5f, cyh_o = cy - 0.5f;
     const float rel0 = c0 - ro0, rel1 = c1 - ro1, rel2 = c2 - ro2;
@@ -67,6 +67,8 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
                 os0 = fminf(os0, up + vp); os1 = fmaxf(os1, up + vp);
                 od0 = fminf(od0, up - vp); od1 = fmaxf(od1, up - vp);
                 pmax = fmaxf(pmax, fabsf(up) + fabsf(vp));
+                ox0 = fminf(ox0, up); ox1 = fmaxf(ox1, up);
+                oy0 = fminf(oy0, vp); oy1 = fmaxf(oy1, vp);
             }
             const uint32_t q = ((p0 < ro0) << 2) | ((p1 < ro1) << 1) | (p2 < ro2);
             umin = smin(umin, u); vmin = smin(vmin, vv);
@@ -103,8 +105,19 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
     bb_out = bb;
     bits_out = bits;
     const float mg = OCT_MARGIN + OCT_MARGIN_REL * (pmax + focal_x + focal_y);
-    oct_out = all_front ? make_float4(os0 - mg, os1 + mg, od0 - mg, od1 + mg)
-                        : make_float4(-1e30f, 1e30f, -1e30f, 1e30f);
+    if (all_front && nd > 0)
+    {
+        const auto px16 = [](float x) { return (uint32_t)fminf(fmaxf(x, 0.f), 65535.f); };
+        oct_out.o = make_float4(os0 - mg, os1 + mg, od0 - mg, od1 + mg);
+        oct_out.tb = make_uint2((px16(ceilf(ox0 - mg)) << 16) | px16(ceilf(oy0 - mg)),
+                                (px16(floorf(ox1 + mg)) << 16) | px16(floorf(oy1 + mg)));
+    }
+    else
+    {
+        oct_out.o = make_float4(-1e30f, 1e30f, -1e30f, 1e30f);
+        oct_out.tb = make_uint2(0u, 0xFFFFFFFFu);
+    }
+    oct_out.pad = make_uint2(0u, 0u);
 }
 
 // Tile rectangle of a packed bbox (auxiliary.h:150-170) for TX x TY tiles.
