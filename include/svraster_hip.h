@@ -213,13 +213,25 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
 int svr_unbiased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
                            float* exp_avg_sq, double step, double lr, double beta1, double beta2, float eps,
                            svr_stream_t stream);
+/* The bias terms of step `step` exactly as svr_unbiased_adam_step computes
+ * them on the host (adam_step.cu:119-124): step_size = lr / (1 - beta1^step),
+ * rb2 = 1 / sqrt(1 - beta2^step), each rounded to float once. */
+int svr_adam_bias_terms(double step, double lr, double beta1, double beta2, float* step_size_host,
+                        float* rb2_host);
 /* Capture-safe form of svr_unbiased_adam_step for hipGraph replay (no
  * reference counterpart): increments the step count *step_dev (int32, device)
- * and computes the bias terms from it on the device, in double as the host
- * form does; coef_dev is 2 floats of device scratch. */
+ * and takes the bias terms of the new count from `bias_table` (device:
+ * int32 first step, int32 count, then `count` (step_size, rb2) float pairs
+ * filled with svr_adam_bias_terms), so the update is bit-equal to the
+ * host-step form; a count outside the table falls back to pow() in double on
+ * the device.  bias_table may be NULL (always the fallback).  skip_dev
+ * (uint32, device, may be NULL): when nonzero the call changes nothing, the
+ * step count included (a frame over its entry capacity).  coef_dev is 3
+ * floats of device scratch. */
 int svr_unbiased_adam_step_device(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
-                                  float* exp_avg_sq, int32_t* step_dev, float* coef_dev, double lr, double beta1,
-                                  double beta2, float eps, svr_stream_t stream);
+                                  float* exp_avg_sq, int32_t* step_dev, const void* bias_table,
+                                  const uint32_t* skip_dev, float* coef_dev, double lr, double beta1, double beta2,
+                                  float eps, svr_stream_t stream);
 /* Replaces `_C.biased_adam_step` (adam_step.cu:139-162). */
 int svr_biased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
                          float* exp_avg_sq, float lr, float beta1, float beta2, float eps, svr_stream_t stream);
@@ -357,11 +369,13 @@ int svr_batch_rasterize_backward(int32_t n_samp, int32_t n_views, const svr_view
  * synchronisation.  svr_scan_entries runs the sum of svr_num_rendered and
  * leaves the entry count on the device.  The capacity forms take buffers
  * sized for `capacity` entries (binning state, backward workspace) and read
- * the count on the device; when it exceeds the capacity, *overflow_dev
- * (uint32, device) is set to 1 and the frame renders no entries (background
- * colour, T = 1, zero gradients).  The flag is only ever set, never cleared:
- * the caller zeroes it, checks it after any number of replays and
- * re-sizes. */
+ * the count on the device.  overflow_dev is 2 uint32 words on the device:
+ * when the count exceeds the capacity, overflow_dev[0] is set to 1 and the
+ * frame renders no entries (background colour, T = 1, zero gradients);
+ * overflow_dev[1] is rewritten every frame (1: this frame was over the
+ * capacity, else 0; svr_unbiased_adam_step_device takes it as skip_dev).
+ * Word 0 is only ever set, never cleared: the caller zeroes it, checks it
+ * after any number of replays and re-sizes. */
 int svr_scan_entries(int64_t n_counts, void* geom_state, svr_stream_t stream);
 int svr_batch_rasterize_forward_capacity(int32_t n_samp, int32_t n_views, const svr_view* views_host,
                                          const svr_view* views_dev, int64_t n_voxels, const float* vox_centers,

@@ -9,6 +9,8 @@ step runs in libsvraster_hip.so.
 import ctypes
 import weakref
 
+import numpy as np
+
 import torch
 
 from . import _lib
@@ -361,14 +363,33 @@ def unbiased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, step, lr, beta1
                                          float(eps), _stream(param.device)), "unbiased_adam_step")
 
 
-def unbiased_adam_step_device(sparse, param, grad, exp_avg, exp_avg_sq, step_dev, coef_dev, lr, beta1, beta2, eps):
+def adam_bias_table(first_step, count, lr, beta1, beta2, device):
+    """Device table of the host-form bias terms (svr_adam_bias_terms) of steps first_step .. first_step + count - 1:
+    int32 [first_step, count] followed by count (step_size, rb2) float32 pairs, in one int32 tensor."""
+    terms = np.empty((count, 2), dtype=np.float32)
+    a, b = ctypes.c_float(0), ctypes.c_float(0)
+    for i in range(count):
+        _lib.check(_L.svr_adam_bias_terms(float(first_step + i), float(lr), float(beta1), float(beta2),
+                                          ctypes.byref(a), ctypes.byref(b)), "adam_bias_terms")
+        terms[i] = (a.value, b.value)
+    head = np.array([first_step, count], dtype=np.int32)
+    return torch.from_numpy(np.concatenate([head, terms.reshape(-1).view(np.int32)])).to(device)
+
+
+def unbiased_adam_step_device(sparse, param, grad, exp_avg, exp_avg_sq, step_dev, coef_dev, lr, beta1, beta2, eps,
+                              bias_table=None, skip_dev=None):
     """Capture-safe form (no reference counterpart): the step count lives in step_dev (int32 [1]) and is
-    incremented on the device; coef_dev (float32 [2]) is scratch for the bias terms."""
+    incremented on the device; the bias terms of the new count come from bias_table (adam_bias_table; pow() on
+    the device outside it); coef_dev (float32 [3]) is scratch.  skip_dev (int32 [1], device): nonzero leaves
+    everything unchanged."""
     _check_adam(param, grad, exp_avg, exp_avg_sq)
+    if coef_dev.numel() < 3:
+        raise RuntimeError("coef_dev needs 3 floats")
     _lib.check(_L.svr_unbiased_adam_step_device(int(bool(sparse)), param.numel(), _ptr(param), _ptr(grad),
-                                                _ptr(exp_avg), _ptr(exp_avg_sq), _ptr(step_dev), _ptr(coef_dev),
-                                                float(lr), float(beta1), float(beta2), float(eps),
-                                                _stream(param.device)), "unbiased_adam_step_device")
+                                                _ptr(exp_avg), _ptr(exp_avg_sq), _ptr(step_dev), _ptr(bias_table),
+                                                _ptr(skip_dev), _ptr(coef_dev), float(lr), float(beta1),
+                                                float(beta2), float(eps), _stream(param.device)),
+               "unbiased_adam_step_device")
 
 
 def biased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, lr, beta1, beta2, eps):

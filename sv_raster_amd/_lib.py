@@ -88,8 +88,9 @@ SIGNATURES = {
                                                      c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_batch_rasterize_backward_capacity": (c_i32, [c_i32, c_i32, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
                                                       c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
-    "svr_unbiased_adam_step_device": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_f64, c_f64, c_f64,
-                                              c_f32, c_p]),
+    "svr_unbiased_adam_step_device": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_f64, c_f64,
+                                              c_f64, c_f32, c_p]),
+    "svr_adam_bias_terms": (c_i32, [c_f64, c_f64, c_f64, c_f64, c_p, c_p]),
     "svr_gather_geo_params": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_gather_geo_params_bw": (c_i32, [c_i64, c_p, c_p, c_p, c_p, c_p]),
     "svr_gather_feat_params": (c_i32, [c_i64, c_i32, c_p, c_p, c_p, c_p, c_p]),

@@ -139,13 +139,14 @@ def _state(views, vox_centers, vox_lengths, sh0, shs, active_sh_degree, capacity
 
 
 def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, capacity=None,
-                  overflow=None):
+                  overflow=None, table=None):
     """Forward of all views; returns the state the backward needs (also used by the tests):
     color [3 * P_total], T [P_total], R, rgbs [V * N, 3], n_dup [V * N] and the opaque buffers.
 
-    capacity / overflow (an int32 [1] device tensor): the capture-safe form, with buffers sized for
-    `capacity` entries and no host synchronisation; a frame over the capacity sets overflow to 1 and renders
-    no entries."""
+    capacity / overflow (an int32 [2] device tensor): the capture-safe form, with buffers sized for
+    `capacity` entries and no host synchronisation; a frame over the capacity sets overflow[0] (sticky) and
+    overflow[1] (this frame) to 1 and renders no entries.  table: the octree's rank-order table
+    (_C.build_order_table); by default the cached one of octree_paths."""
     dev = vox_centers.device
     V, N = views.n_views, vox_centers.size(0)
     if (capacity is None) != (overflow is None):
@@ -156,7 +157,10 @@ def batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_length
                       device=dev)
     color = torch.empty(3 * views.total_pixels, dtype=torch.float32, device=dev)
     T = torch.empty(views.total_pixels, dtype=torch.float32, device=dev)
-    table = _C.order_table_cache.get(octree_paths) if N > 0 else None
+    if table is None and N > 0:
+        table = _C.order_table_cache.get(octree_paths)
+    if overflow is not None and overflow.numel() < 2:
+        raise Exception("overflow needs 2 int32 words")
     centers, lengths, geos_c = _C._f32(vox_centers), _C._f32(vox_lengths), _C._f32_a16(geos)
     if capacity is None:
         _lib.check(_L.svr_batch_rasterize_forward(views.n_samp, V, views.host_ptr(), _C._ptr(views.dev), N,
@@ -271,11 +275,17 @@ class GraphTrainStep:
     One replay = forward of all views of `views`, the gradient of the summed per-view MSE to `targets`
     (one [3, H, W] tensor per view; dL/dcolor = 2 (color - target) / (3 H W) per view, the backward of
     torch's mse_loss), the batch backward and `optimizer.step()` (a SparseAdam switched to device step
-    counts).  Nothing in a replay synchronises with the host: the list buffers are sized for `capacity`
-    entries (by default 1.25 x the count of the current parameters plus a margin), and a frame over it sets
-    `overflow` and leaves the parameters unchanged (zero gradients); `overflowed()` reads the flag and
-    `recapture()` re-sizes.  Parameters are updated in place; `color` holds the last replay's images
-    (flat, per-view blocks as in ViewBatch.split).
+    counts, whose bias terms equal the host-step ones bit for bit).  Nothing in a replay synchronises with the
+    host: the list buffers are sized for `capacity` entries (by default 1.25 x the count of the current
+    parameters plus a margin).  A frame over the capacity renders no entries and its Adam update is skipped
+    (parameters, moments and step counts unchanged, with sparse or dense Adam); `overflowed()` reads the
+    sticky flag and `recapture()` re-sizes.  Parameters are updated in place; `color` holds the last replay's
+    images (flat, per-view blocks as in ViewBatch.split).
+
+    The graph reads the tensors it was captured with by address, so the trainer holds them: the octree's
+    rank-order table, the gradient tensors it installed as geos.grad / sh0.grad / shs.grad (the optimizer
+    reads them), the targets and the views.  Replacing or freeing those gradients (for example
+    zero_grad(set_to_none=True)) does not reach the graph, which keeps using the held tensors.
     """
 
     def __init__(self, views, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, optimizer, targets,
@@ -293,26 +303,27 @@ class GraphTrainStep:
         if self.target.numel() != 3 * views.total_pixels:
             raise Exception("target sizes do not match the views")
         self.zero_dT = torch.zeros(views.total_pixels, dtype=torch.float32, device=dev)
-        self.overflow = torch.zeros(1, dtype=torch.int32, device=dev)
+        # [0]: sticky overflow flag, [1]: this frame's (the Adam update's skip flag)
+        self.overflow = torch.zeros(2, dtype=torch.int32, device=dev)
         for p in (geos, sh0, shs):
             p.grad = torch.zeros_like(p)
+        self.grads = (geos.grad, sh0.grad, shs.grad)
         self.dsub = torch.zeros((self.centers.size(0), 1), dtype=torch.float32, device=dev)
-        optimizer.use_device_steps()
-        _C.order_table_cache.get(octree_paths)
+        optimizer.use_device_steps(skip=self.overflow[1:2])
+        self.table = _C.order_table_cache.get(octree_paths) if self.centers.size(0) > 0 else None
         self.recapture(capacity)
 
     def _measure(self):
         with torch.no_grad():
             st = batch_forward(self.views, self.deg, self.octree_paths, self.centers, self.lengths, self.geos,
-                               self.sh0, self.shs)
+                               self.sh0, self.shs, table=self.table)
         return st["R"]
 
     def _fwd_bwd(self):
         st = batch_forward(self.views, self.deg, self.octree_paths, self.centers, self.lengths, self.geos, self.sh0,
-                           self.shs, capacity=self.capacity, overflow=self.overflow)
+                           self.shs, capacity=self.capacity, overflow=self.overflow, table=self.table)
         dcol = (st["color"] - self.target) * self.weight
-        batch_backward(self.views, st, dcol, self.zero_dT,
-                       out=(self.geos.grad, self.sh0.grad, self.shs.grad, self.dsub))
+        batch_backward(self.views, st, dcol, self.zero_dT, out=(self.grads[0], self.grads[1], self.grads[2], self.dsub))
         self.color = st["color"]
 
     def recapture(self, capacity=None):
@@ -320,7 +331,11 @@ class GraphTrainStep:
         if capacity is None:
             capacity = int(self._measure() * self.headroom) + 65536
         self.capacity = int(capacity)
+        self.overflow.zero_()
         with torch.no_grad():
+            # the captured optimizer step reads the held gradient tensors
+            for p, g in zip((self.geos, self.sh0, self.shs), self.grads):
+                p.grad = g
             self._fwd_bwd()  # outside the graph: allocator and first-use warm-up
             torch.cuda.synchronize()
             self.graph = torch.cuda.CUDAGraph()
@@ -331,10 +346,11 @@ class GraphTrainStep:
 
     def step(self):
         self.graph.replay()
+        self.optimizer.note_device_steps(1)
 
     def overflowed(self):
         """True if a replay since the last capture exceeded the capacity (synchronises)."""
-        return bool(int(self.overflow.item()))
+        return bool(int(self.overflow[0].item()))
 
 
 _TUNED = {}
@@ -449,7 +465,9 @@ def _mark(cameras, octree_paths, vox_centers, vox_lengths, near, rate):
         return out
     views = _camera_views(cameras, near)
     centers, lengths, n_dup, _ = _pair_preprocess(views, vox_centers, vox_lengths)
-    pix = _pix_size(views)
+    # the cameras' own pixel size where they carry one (cameras.py:54-55), rounded to float once
+    ps = [float(getattr(c, "pix_size", 2.0 * float(c.tanfovx) / int(c.image_width))) for c in cameras]
+    pix = torch.tensor(ps, dtype=torch.float32).to(dev)
     _lib.check(_L.svr_batch_mark_samp(views.n_views, _C._ptr(views.dev), N, _C._ptr(centers), _C._ptr(lengths),
                                       _C._ptr(pix), _C._ptr(n_dup), _C._ptr(out) if rate else None,
                                       None if rate else _C._ptr(out), _C._stream(dev)), "batch_mark_samp")

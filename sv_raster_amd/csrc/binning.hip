@@ -514,7 +514,8 @@ __global__ void clamp_count_kernel(uint32_t* __restrict__ counters, uint64_t cap
     const uint64_t eff = R <= cap ? R : 0ull;
     counters[4] = (uint32_t)eff;
     counters[5] = (uint32_t)(eff >> 32);
-    if (R > cap) overflow[0] = 1u;
+    if (R > cap) overflow[0] = 1u;  // sticky
+    overflow[1] = R > cap ? 1u : 0u;  // this frame
 }
 
 int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hipStream_t stream)

@@ -269,6 +269,7 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict_
 {
     if (coef)
     {
+        if (coef[2] != 0.f) return;
         step_size = coef[0];
         rb2 = coef[1];
     }
@@ -298,6 +299,7 @@ __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict
 {
     if (coef)
     {
+        if (coef[2] != 0.f) return;
         step_size = coef[0];
         rb2 = coef[1];
     }
@@ -359,14 +361,36 @@ void adam_launch(bool sparse, int64_t n, float* p, const float* g, float* m, flo
     }
 }
 
-// Device-step form of the host bias terms (adam_step.cu:119-124): t = ++step,
-// coef = {lr / (1 - beta1^t), 1 / sqrt(1 - beta2^t)}, computed in double.
-__global__ void adam_coef_kernel(int32_t* __restrict__ step, double lr, double beta1, double beta2,
+// Device-step form of the bias terms (adam_step.cu:119-124): t = ++step and
+// coef = {lr / (1 - beta1^t), 1 / sqrt(1 - beta2^t), skip}.  The terms come
+// from a table the host filled with the host form (svr_adam_bias_terms), so
+// they are bit-equal to the host-step update: table = {int32 first step,
+// int32 count, then count (step_size, rb2) float pairs}.  A step outside the
+// table falls back to pow() in double on the device (the caller refills the
+// table before that happens).  *skip != 0 (a frame over its entry capacity)
+// leaves the step count alone and sets coef[2], which makes the update kernels
+// return without touching any element.
+__global__ void adam_coef_kernel(int32_t* __restrict__ step, const int32_t* __restrict__ table,
+                                 const uint32_t* __restrict__ skip, double lr, double beta1, double beta2,
                                  float* __restrict__ coef)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (skip && skip[0] != 0u)
+    {
+        coef[2] = 1.f;
+        return;
+    }
     const int32_t t = step[0] + 1;
     step[0] = t;
+    coef[2] = 0.f;
+    const int32_t i = table ? t - table[0] : -1;
+    if (table && i >= 0 && i < table[1])
+    {
+        const float* terms = reinterpret_cast<const float*>(table + 2);
+        coef[0] = terms[2 * i];
+        coef[1] = terms[2 * i + 1];
+        return;
+    }
     const double bc1 = 1.0 - pow(beta1, (double)t);
     const double bc2 = 1.0 - pow(beta2, (double)t);
     coef[0] = (float)(lr / bc1);
@@ -474,15 +498,24 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
     return SVR_OK;
 }
 
+int svr_adam_bias_terms(double step, double lr, double beta1, double beta2, float* step_size_host,
+                        float* rb2_host)
+{
+    SVR_REQUIRE(step_size_host && rb2_host, SVR_ERR_INVALID, "svr_adam_bias_terms: null output");
+    // adam_step.cu:119-124 (bias terms in double on the host)
+    const double bc1 = 1.0 - pow(beta1, step);
+    const double bc2 = 1.0 - pow(beta2, step);
+    *step_size_host = (float)(lr / bc1);
+    *rb2_host = (float)(1.0 / sqrt(bc2));
+    return SVR_OK;
+}
+
 int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, float* m, float* v, double step,
                            double lr, double beta1, double beta2, float eps, svr_stream_t stream)
 {
     if (n == 0) return SVR_OK;
-    // adam_step.cu:119-124 (bias terms in double on the host)
-    const double bc1 = 1.0 - pow(beta1, step);
-    const double bc2 = 1.0 - pow(beta2, step);
-    const float step_size = (float)(lr / bc1);
-    const float rb2 = (float)(1.0 / sqrt(bc2));
+    float step_size = 0.f, rb2 = 0.f;
+    svr_adam_bias_terms(step, lr, beta1, beta2, &step_size, &rb2);
     prof_begin(PROF_ADAM, stream);
     adam_launch<false>(sparse != 0, n, p, g, m, v, step_size, (float)beta1, (float)beta2, rb2, eps, nullptr, stream);
     SVR_CHECK_LAUNCH();
@@ -491,16 +524,19 @@ int svr_unbiased_adam_step(int32_t sparse, int64_t n, float* p, const float* g, 
 }
 
 int svr_unbiased_adam_step_device(int32_t sparse, int64_t n, float* p, const float* g, float* m, float* v,
-                                  int32_t* step_dev, float* coef_dev, double lr, double beta1, double beta2, float eps,
-                                  svr_stream_t stream)
+                                  int32_t* step_dev, const void* bias_table, const uint32_t* skip_dev, float* coef_dev,
+                                  double lr, double beta1, double beta2, float eps, svr_stream_t stream)
 {
-    if (n == 0) return SVR_OK;
     SVR_REQUIRE(step_dev && coef_dev, SVR_ERR_INVALID, "svr_unbiased_adam_step_device: null step or coefficients");
     prof_begin(PROF_ADAM, stream);
-    adam_coef_kernel<<<1, 64, 0, stream>>>(step_dev, lr, beta1, beta2, coef_dev);
+    adam_coef_kernel<<<1, 64, 0, stream>>>(step_dev, (const int32_t*)bias_table, skip_dev, lr, beta1, beta2,
+                                           coef_dev);
     SVR_CHECK_LAUNCH();
-    adam_launch<false>(sparse != 0, n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps, coef_dev, stream);
-    SVR_CHECK_LAUNCH();
+    if (n > 0)
+    {
+        adam_launch<false>(sparse != 0, n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps, coef_dev, stream);
+        SVR_CHECK_LAUNCH();
+    }
     prof_end(PROF_ADAM, stream);
     return SVR_OK;
 }

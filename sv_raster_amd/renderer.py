@@ -219,128 +219,98 @@ class _RasterizeVoxels(torch.autograd.Function):
         return None, None, None, None, None, dL_dgeos, dL_drgbs, subdiv_p_bw, None, None
 
 
+def _no_grad_input(*tensors):
+    """SH_eval differentiates the coefficients only (renderer.py:265-270 raise NotImplementedError for the
+    geometry inputs)."""
+    if any(torch.is_tensor(t) and t.requires_grad for t in tensors):
+        raise NotImplementedError
+
+
 class SH_eval(torch.autograd.Function):
-    """renderer.py:251-316: view-dependent colour from SH coefficients."""
+    """renderer.py:251-316: rgbs [N, 3] = SH(active_sh_degree) of the direction cam_pos -> voxel centre (or of
+    `viewdir` [N, 3] when given), + 0.5, clamped at 0; rows outside `idx` are zero (idx None: all voxels).
+    Gradients flow to sh0 and shs."""
 
     @staticmethod
     def forward(ctx, active_sh_degree, idx, vox_centers, cam_pos, viewdir, sh0, shs):
-        if torch.is_tensor(vox_centers) and vox_centers.requires_grad:
-            raise NotImplementedError
-        if torch.is_tensor(cam_pos) and cam_pos.requires_grad:
-            raise NotImplementedError
-        if torch.is_tensor(viewdir) and viewdir.requires_grad:
-            raise NotImplementedError
-
-        if idx is None:
-            idx = torch.empty(0, dtype=torch.int64)
-
-        if viewdir is not None:
-            vox_centers = viewdir
-            cam_pos = torch.zeros_like(cam_pos)
-
-        rgbs = _C.sh_compute(active_sh_degree, idx, vox_centers, cam_pos, sh0, shs)
-
-        ctx.active_sh_degree = active_sh_degree
-        ctx.M = 1 + shs.shape[1]
-        ctx.save_for_backward(idx, vox_centers, cam_pos, rgbs)
+        _no_grad_input(vox_centers, cam_pos, viewdir)
+        # a given view direction is evaluated as the direction from the origin to it
+        points, origin = (vox_centers, cam_pos) if viewdir is None else (viewdir, torch.zeros_like(cam_pos))
+        sel = torch.empty(0, dtype=torch.int64) if idx is None else idx
+        rgbs = _C.sh_compute(active_sh_degree, sel, points, origin, sh0, shs)
+        ctx.active_sh_degree, ctx.M = active_sh_degree, shs.shape[1] + 1
+        ctx.save_for_backward(sel, points, origin, rgbs)
         return rgbs
 
     @staticmethod
     def backward(ctx, dL_drgbs):
-        idx, vox_centers, cam_pos, rgbs = ctx.saved_tensors
-        dL_dsh0, dL_dshs = _C.sh_compute_bw(ctx.active_sh_degree, ctx.M, idx, vox_centers, cam_pos, rgbs, dL_drgbs)
-        return None, None, None, None, None, dL_dsh0, dL_dshs
+        sel, points, origin, rgbs = ctx.saved_tensors
+        grads = _C.sh_compute_bw(ctx.active_sh_degree, ctx.M, sel, points, origin, rgbs, dL_drgbs)
+        return (None,) * 5 + tuple(grads)
+
+
+def _check_gather(vox_key, care_idx):
+    assert vox_key.dim() == 2 and vox_key.shape[1] == 8
+    assert care_idx.dim() == 1
 
 
 class GatherGeoParams(torch.autograd.Function):
-    """renderer.py:319-345: per-voxel corner values [N, 8] gathered from grid points (rows of care_idx only)."""
+    """renderer.py:319-345: geo_params[idx, k] = grid_pts[vox_key[idx, k]] ([N, 8]) for idx in care_idx;
+    the backward scatters the rows back onto the grid points."""
 
     @staticmethod
     def forward(ctx, vox_key, care_idx, grid_pts):
-        assert len(vox_key.shape) == 2 and vox_key.shape[1] == 8
-        assert len(care_idx.shape) == 1
-        assert grid_pts.shape[0] == grid_pts.numel()
-        geo_params = _C.gather_triinterp_geo_params(vox_key, care_idx, grid_pts)
-        ctx.num_grid_pts = grid_pts.numel()
+        _check_gather(vox_key, care_idx)
+        assert grid_pts.numel() == grid_pts.shape[0]  # one value per grid point
         ctx.grid_shape = grid_pts.shape
         ctx.save_for_backward(vox_key, care_idx)
-        return geo_params
+        return _C.gather_triinterp_geo_params(vox_key, care_idx, grid_pts)
 
     @staticmethod
     def backward(ctx, dL_dgeo_params):
         vox_key, care_idx = ctx.saved_tensors
-        dL_dgrid_pts = _C.gather_triinterp_geo_params_bw(vox_key, care_idx, ctx.num_grid_pts, dL_dgeo_params)
-        # [M, 1] as the reference returns; also accepted for a flat [M] grid
-        return None, None, dL_dgrid_pts.view(ctx.grid_shape)
+        n_pts = ctx.grid_shape[0]
+        dL_dgrid = _C.gather_triinterp_geo_params_bw(vox_key, care_idx, n_pts, dL_dgeo_params)
+        # shaped like the grid ([M, 1] in the reference's models; a flat [M] grid also works)
+        return None, None, dL_dgrid.view(ctx.grid_shape)
 
 
 class GatherFeatParams(torch.autograd.Function):
-    """renderer.py:348-374: per-voxel corner features [N, 8, 3] gathered from grid points."""
+    """renderer.py:348-374: feat_params[idx, k] = grid_pts[vox_key[idx, k]] ([N, 8, C]) for idx in care_idx."""
 
     @staticmethod
     def forward(ctx, vox_key, care_idx, grid_pts):
-        assert len(vox_key.shape) == 2 and vox_key.shape[1] == 8
-        assert len(care_idx.shape) == 1
-        assert len(grid_pts.shape) == 2
-        feat_params = _C.gather_triinterp_feat_params(vox_key, care_idx, grid_pts)
-        ctx.num_grid_pts = len(grid_pts)
+        _check_gather(vox_key, care_idx)
+        assert grid_pts.dim() == 2
+        ctx.n_pts = grid_pts.shape[0]
         ctx.save_for_backward(vox_key, care_idx)
-        return feat_params
+        return _C.gather_triinterp_feat_params(vox_key, care_idx, grid_pts)
 
     @staticmethod
     def backward(ctx, dL_dfeat_params):
         vox_key, care_idx = ctx.saved_tensors
-        dL_dgrid_pts = _C.gather_triinterp_feat_params_bw(vox_key, care_idx, ctx.num_grid_pts, dL_dfeat_params)
-        return None, None, dL_dgrid_pts
+        return None, None, _C.gather_triinterp_feat_params_bw(vox_key, care_idx, ctx.n_pts, dL_dfeat_params)
 
 
-def mark_n_duplicates(
-        image_width, image_height,
-        tanfovx, tanfovy,
-        cx, cy,
-        w2c_matrix, c2w_matrix, near,
-        octree_paths, vox_centers, vox_lengths,
-        return_buffer=False,
-        debug=False):
-    """renderer.py:377-405."""
-    n_duplicates, geomBuffer = _C.rasterize_preprocess(
-        image_width, image_height, tanfovx, tanfovy, cx, cy, w2c_matrix, c2w_matrix, near,
-        octree_paths, vox_centers, vox_lengths, debug)
-    if return_buffer:
-        return n_duplicates, geomBuffer
-    return n_duplicates
+def mark_n_duplicates(image_width, image_height, tanfovx, tanfovy, cx, cy, w2c_matrix, c2w_matrix, near,
+                      octree_paths, vox_centers, vox_lengths, return_buffer=False, debug=False):
+    """renderer.py:377-405: the preprocess alone; n_duplicates [N] (int32), and the geometry state when
+    return_buffer is set."""
+    out = _C.rasterize_preprocess(image_width, image_height, tanfovx, tanfovy, cx, cy, w2c_matrix, c2w_matrix,
+                                  near, octree_paths, vox_centers, vox_lengths, debug)
+    return out if return_buffer else out[0]
 
 
 def mark_max_samp_rate(cameras, octree_paths, vox_centers, vox_lengths, near=0.02):
-    """renderer.py:408-423."""
-    max_samp_rate = torch.zeros([len(octree_paths)], dtype=torch.float32, device=vox_centers.device)
-    for cam in cameras:
-        n_duplicates = mark_n_duplicates(
-            image_width=cam.image_width, image_height=cam.image_height,
-            tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
-            cx=cam.cx, cy=cam.cy,
-            w2c_matrix=cam.w2c, c2w_matrix=cam.c2w, near=near,
-            octree_paths=octree_paths, vox_centers=vox_centers, vox_lengths=vox_lengths)
-        zdist = ((vox_centers - cam.position) * cam.lookat).sum(-1)
-        vis_idx = torch.where((n_duplicates > 0) & (zdist > near))[0]
-        zdist = zdist[vis_idx]
-        samp_interval = zdist * cam.pix_size
-        samp_rate = vox_lengths.squeeze(1)[vis_idx] / samp_interval
-        max_samp_rate[vis_idx] = torch.maximum(max_samp_rate[vis_idx], samp_rate)
-    return max_samp_rate
+    """renderer.py:408-423: per voxel, the largest length / (camera-axis distance x pixel size) over the
+    cameras that see it beyond `near` ([N] float32, 0 if none).  All cameras in one batch pass
+    (batch.mark_max_samp_rate) instead of a preprocess per camera."""
+    from . import batch
+    return batch.mark_max_samp_rate(cameras, octree_paths, vox_centers, vox_lengths, near=near)
 
 
 def mark_near(cameras, octree_paths, vox_centers, vox_lengths, near=0.2):
-    """renderer.py:426-438."""
-    is_near = torch.zeros([len(octree_paths)], dtype=torch.bool, device=vox_centers.device)
-    for cam in cameras:
-        n_duplicates = mark_n_duplicates(
-            image_width=cam.image_width, image_height=cam.image_height,
-            tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
-            cx=cam.cx, cy=cam.cy,
-            w2c_matrix=cam.w2c, c2w_matrix=cam.c2w, near=near,
-            octree_paths=octree_paths, vox_centers=vox_centers, vox_lengths=vox_lengths)
-        vis_idx = torch.where(n_duplicates > 0)[0]
-        zdist = ((vox_centers[vis_idx] - cam.position) * cam.lookat).sum(-1)
-        is_near[vis_idx] |= (zdist <= near + vox_lengths.squeeze(1)[vis_idx])
-    return is_near
+    """renderer.py:426-438: True for voxels that some camera sees within `near` + the voxel's length along its
+    axis ([N] bool).  All cameras in one batch pass (batch.mark_near)."""
+    from . import batch
+    return batch.mark_near(cameras, octree_paths, vox_centers, vox_lengths, near=near)

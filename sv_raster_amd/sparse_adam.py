@@ -26,14 +26,21 @@ class SparseAdam(torch.optim.Optimizer):
     def __setstate__(self, state):
         super(SparseAdam, self).__setstate__(state)
 
+    # steps covered by each parameter's device table of bias terms (refilled ahead of use)
+    BIAS_TABLE_STEPS = 1024
+
     @torch.no_grad()
-    def use_device_steps(self):
+    def use_device_steps(self, skip=None):
         """Switch to the capture-safe update (no reference counterpart): each parameter's step count moves to
         an int32 device tensor that the update itself increments, so a captured step can be replayed.  The
-        optimizer state (moments, step count) is created here, outside any capture.  Unbiased form only."""
+        bias terms of each step come from a device table filled on the host with the host-step formula, so the
+        replayed update is bit-equal to the host-step one.  `skip` (an int32 [1] device tensor, optional): a
+        nonzero value makes the update leave everything unchanged, the step count included.  The optimizer
+        state (moments, step count) is created here, outside any capture.  Unbiased form only."""
         if self.biased:
             raise NotImplementedError("device steps are implemented for the unbiased update")
         for group in self.param_groups:
+            beta1, beta2 = group['betas']
             for param in group['params']:
                 state = self.state[param]
                 if len(state) == 0:
@@ -42,8 +49,31 @@ class SparseAdam(torch.optim.Optimizer):
                     state['exp_avg_sq'] = torch.zeros_like(param, memory_format=torch.preserve_format)
                 if 'step_dev' not in state:
                     state['step_dev'] = torch.full((1,), int(state['step']), dtype=torch.int32, device=param.device)
-                    state['coef_dev'] = torch.zeros(2, dtype=torch.float32, device=param.device)
+                    state['coef_dev'] = torch.zeros(3, dtype=torch.float32, device=param.device)
+                    state['bias_table'] = _C.adam_bias_table(int(state['step']) + 1, self.BIAS_TABLE_STEPS,
+                                                             group['lr'], beta1, beta2, param.device)
+        self.device_skip = skip
+        self._calls_since_fill = 0
         self.device_steps = True
+
+    @torch.no_grad()
+    def note_device_steps(self, n=1):
+        """Records n device-step updates run outside step() (graph replays of a captured step).  When the
+        tables of bias terms may run out, reads the device step counts (one synchronisation) and refills the
+        tables in place from there, so replays keep taking host-form terms."""
+        self._calls_since_fill += n
+        if self._calls_since_fill < self.BIAS_TABLE_STEPS // 2:
+            return
+        for group in self.param_groups:
+            beta1, beta2 = group['betas']
+            for param in group['params']:
+                state = self.state[param]
+                if 'bias_table' not in state:
+                    continue
+                t = int(state['step_dev'].item())
+                state['bias_table'].copy_(_C.adam_bias_table(t + 1, self.BIAS_TABLE_STEPS, group['lr'], beta1, beta2,
+                                                             param.device))
+        self._calls_since_fill = 0
 
     @torch.no_grad()
     def step(self):
@@ -57,7 +87,10 @@ class SparseAdam(torch.optim.Optimizer):
                     state = self.state[param]
                     _C.unbiased_adam_step_device(self.sparse, param, param.grad, state['exp_avg'],
                                                  state['exp_avg_sq'], state['step_dev'], state['coef_dev'], lr,
-                                                 beta1, beta2, group['eps'])
+                                                 beta1, beta2, group['eps'], bias_table=state['bias_table'],
+                                                 skip_dev=self.device_skip)
+            if not torch.cuda.is_current_stream_capturing():
+                self.note_device_steps(1)
             return
         for group in self.param_groups:
             lr = group['lr']

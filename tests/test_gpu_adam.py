@@ -52,21 +52,66 @@ def test_adam_matches_oracle(n, offset, sparse, biased):
 
 
 @pytest.mark.gpu
-def test_adam_device_step_matches_host_step():
-    """The capture-safe form (step count and bias terms on the device) equals the host-step form."""
+@pytest.mark.parametrize("sparse", [True, False])
+def test_adam_device_step_matches_host_step(sparse):
+    """The capture-safe form (step count on the device, bias terms from the host-filled table) equals the
+    host-step form and the oracle bit for bit, also across table refills (a 3-step table over 10 steps)."""
     import torch
-    from sv_raster_amd import _C
+    from oracle import oracle as O
+    from sv_raster_amd import sparse_adam
+    print(f"adam device steps sparse={sparse}", flush=True)
     p, g, m, v = _case(10001, seed=3, density=0.5)
-    t = lambda a: torch.from_numpy(a.copy()).cuda()  # noqa: E731
-    a = [t(x) for x in (p, g, m, v)]
-    b = [t(x) for x in (p, g, m, v)]
-    step_dev = torch.zeros(1, dtype=torch.int32, device="cuda")
-    coef = torch.zeros(2, dtype=torch.float32, device="cuda")
-    for step in (1, 2, 3, 4):
-        _C.unbiased_adam_step(True, a[0], a[1], a[2], a[3], step, 0.025, 0.1, 0.99, 1e-15)
-        _C.unbiased_adam_step_device(True, b[0], b[1], b[2], b[3], step_dev, coef, 0.025, 0.1, 0.99, 1e-15)
+    ph, pd = torch.from_numpy(p.copy()).cuda(), torch.from_numpy(p.copy()).cuda()
+    oh = sparse_adam.SparseAdam([ph], lr=0.025, betas=(0.1, 0.99), eps=1e-15, sparse=sparse)
+    od = sparse_adam.SparseAdam([pd], lr=0.025, betas=(0.1, 0.99), eps=1e-15, sparse=sparse)
+    od.BIAS_TABLE_STEPS = 3
+    od.use_device_steps()
+    po, mo, vo = p.copy(), np.zeros_like(p), np.zeros_like(p)
+    rng = np.random.default_rng(11)
+    for step in range(1, 11):
+        gs = (g * rng.normal(1.0, 0.5, g.size)).astype(np.float32)
+        ph.grad = torch.from_numpy(gs).cuda()
+        pd.grad = torch.from_numpy(gs).cuda()
+        oh.step()
+        od.step()
+        O.adam_step(po, gs, mo, vo, step, 0.025, 0.1, 0.99, 1e-15, sparse=sparse)
     torch.cuda.synchronize()
-    assert int(step_dev.item()) == 4
-    assert torch.equal(a[2], b[2]) and torch.equal(a[3], b[3])  # moments do not use the bias terms
-    # the bias terms come from pow() on the device instead of the host: equal up to float rounding
-    assert torch.allclose(a[0], b[0], rtol=1e-6, atol=0.0)
+    assert int(od.state[pd]["step_dev"].item()) == 10
+    for a, b in ((ph, pd), (oh.state[ph]["exp_avg"], od.state[pd]["exp_avg"]),
+                 (oh.state[ph]["exp_avg_sq"], od.state[pd]["exp_avg_sq"])):
+        assert torch.equal(a, b)
+    assert np.array_equal(pd.cpu().numpy().view(np.uint32), po.view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sparse", [True, False])
+def test_adam_device_step_skip_changes_nothing(sparse):
+    """A nonzero skip word (a frame over its entry capacity) leaves parameters, moments and the step count as
+    they were, for sparse and dense Adam; the next step without skip is the host form's next step."""
+    import torch
+    from sv_raster_amd import sparse_adam
+    print(f"adam skip sparse={sparse}", flush=True)
+    p, g, m, v = _case(4099, seed=5, density=0.5)
+    skip = torch.zeros(1, dtype=torch.int32, device="cuda")
+    pd, ph = torch.from_numpy(p.copy()).cuda(), torch.from_numpy(p.copy()).cuda()
+    od = sparse_adam.SparseAdam([pd], lr=0.025, betas=(0.1, 0.99), eps=1e-15, sparse=sparse)
+    oh = sparse_adam.SparseAdam([ph], lr=0.025, betas=(0.1, 0.99), eps=1e-15, sparse=sparse)
+    od.use_device_steps(skip=skip)
+    pd.grad = torch.from_numpy(g).cuda()
+    ph.grad = torch.from_numpy(g).cuda()
+    od.step()
+    oh.step()
+    before = [t.clone() for t in (pd, od.state[pd]["exp_avg"], od.state[pd]["exp_avg_sq"])]
+    skip.fill_(1)
+    pd.grad = torch.zeros_like(pd)  # what an overflowing frame produces
+    od.step()
+    torch.cuda.synchronize()
+    assert int(od.state[pd]["step_dev"].item()) == 1
+    for a, b in zip((pd, od.state[pd]["exp_avg"], od.state[pd]["exp_avg_sq"]), before):
+        assert torch.equal(a, b)
+    skip.fill_(0)
+    pd.grad = torch.from_numpy(g).cuda()
+    od.step()
+    oh.step()
+    torch.cuda.synchronize()
+    assert torch.equal(pd, ph)

@@ -94,6 +94,29 @@ def test_training_stat_matches_per_camera_loop_and_oracle():
     assert _rel(g["min_samp_interval"], msi.cpu().numpy()) <= 1e-6
 
 
+def _loop_marks(cams, octpath, centers, lengths, near_rate, near_near):
+    """The per-camera loops of NC renderer.py:408-438 in torch on the preprocess of the drop-in path: the
+    maximum sampling rate over the cameras that see a voxel beyond near_rate, and whether some camera sees it
+    within near_near + its length."""
+    import torch
+    from sv_raster_amd import renderer
+    n = centers.shape[0]
+    rate = torch.zeros(n, dtype=torch.float32, device=centers.device)
+    close = torch.zeros(n, dtype=torch.bool, device=centers.device)
+    length = lengths.reshape(-1)
+    for cam in cams:
+        z = ((centers - cam.position) * cam.lookat).sum(-1)
+        seen = renderer.mark_n_duplicates(cam.image_width, cam.image_height, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy,
+                                          cam.w2c, cam.c2w, near_rate, octpath, centers, lengths) > 0
+        vis = seen & (z > near_rate)
+        cand = torch.where(vis, length / (z * cam.pix_size), torch.zeros_like(rate))
+        rate = torch.maximum(rate, cand)
+        seen = renderer.mark_n_duplicates(cam.image_width, cam.image_height, cam.tanfovx, cam.tanfovy, cam.cx, cam.cy,
+                                          cam.w2c, cam.c2w, near_near, octpath, centers, lengths) > 0
+        close |= seen & (z <= near_near + length)
+    return rate, close
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("near", [0.02, 0.2, 1.5])
 def test_mark_max_samp_rate_and_near_match_per_camera_loop(near):
@@ -103,10 +126,10 @@ def test_mark_max_samp_rate_and_near_match_per_camera_loop(near):
     octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
     cams = [_Cam(c["cam"], t) for c in cases]
     print("mark near", near, flush=True)
-    rate_b = batch.mark_max_samp_rate(cams, octpath, centers, lengths, near=near)
-    rate_r = renderer.mark_max_samp_rate(cams, octpath, centers, lengths, near=near)
-    near_b = batch.mark_near(cams, octpath, centers, lengths, near=near)
-    near_r = renderer.mark_near(cams, octpath, centers, lengths, near=near)
+    rate_b = renderer.mark_max_samp_rate(cams, octpath, centers, lengths, near=near)
+    near_b = renderer.mark_near(cams, octpath, centers, lengths, near=near)
+    assert torch.equal(rate_b, batch.mark_max_samp_rate(cams, octpath, centers, lengths, near=near))
+    rate_r, near_r = _loop_marks(cams, octpath, centers, lengths, near, near)
     torch.cuda.synchronize()
     assert rate_b.shape == rate_r.shape and near_b.dtype == torch.bool and near_b.shape == near_r.shape
     rb, rr = rate_b.cpu().numpy(), rate_r.cpu().numpy()
