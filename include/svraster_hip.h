@@ -142,6 +142,24 @@ int svr_rasterize_backward(int32_t n_samp, int32_t width, int32_t height, float 
                            const float* out_T, const float* dL_dout_color, const float* dL_dout_T, void* workspace,
                            float* dL_dgeos, float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
 
+/* No counterpart in the reference: svr_rasterize_backward for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and outputs
+ * without the workspace: the per-entry gradient rows live in a fixed device
+ * array of the library, sized for SVR_NOALLOC_MAX_ENTRIES entries (192 MiB,
+ * held by every GPU the library runs on until the process ends).  Calls no
+ * allocator; num_rendered above the maximum returns SVR_ERR_CAPACITY.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams or threads) race and must not be issued. */
+#define SVR_NOALLOC_MAX_ENTRIES (1ll << 22)
+int svr_rasterize_backward_noalloc(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy,
+                                   float cx, float cy, const float* c2w, float bg_color, int64_t n_voxels,
+                                   const float* vox_centers, const float* vox_lengths, const float* geos,
+                                   const float* rgbs, const void* geom_state, int64_t num_rendered,
+                                   const void* binning_state, const void* image_state, const float* out_T,
+                                   const float* dL_dout_color, const float* dL_dout_T, float* dL_dgeos,
+                                   float* dL_drgbs, float* subdiv_p_bw, svr_stream_t stream);
+
 /* Replaces `_C.rasterize_voxels` (forward.cu:792-879) with its optional
  * outputs (forward.cu:262-394): need_depth writes the mean depth (channel 0)
  * and the median depth (channel 2) of out_depth [3,H,W], need_distortion the

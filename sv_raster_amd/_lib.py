@@ -51,6 +51,9 @@ SIGNATURES = {
     "svr_rasterize_backward": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
                                        c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                        c_p, c_p]),
+    "svr_rasterize_backward_noalloc": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i64,
+                                               c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                               c_p, c_p]),
     "svr_rasterize_forward_ext": (c_i32, [c_i32, c_i32, c_i32, c_f32, c_f32, c_f32, c_f32, c_p, c_f32, c_i32,
                                           c_i32, c_i32, c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_p,
                                           c_p, c_p, c_p, c_p, c_p, c_p, c_i32, c_p]),

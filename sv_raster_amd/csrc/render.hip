@@ -20,6 +20,8 @@
 // arithmetic.  The backward accumulates the 12 per-entry gradients of a chunk
 // in LDS and flushes them into one partial row per list entry; a per-voxel
 // segmented sum over those rows gives the voxel gradients.
+#include <mutex>
+
 #include "compositor.h"
 #include "svr_host.h"
 
@@ -473,6 +475,40 @@ int backward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64
     return SVR_OK;
 }
 
+// Scratch of svr_rasterize_backward_noalloc: the workspace of
+// svr_rasterize_backward (one 12-float partial row per entry, then the rows of
+// split huge voxels) for up to SVR_NOALLOC_MAX_ENTRIES entries, as a device
+// array of this library.  Every GPU the library runs on holds its own copy
+// from the code object's load until the process ends.  Nothing in it needs
+// clearing: the backward writes every partial row it later reads (zeros for
+// entries no pixel used) and the huge-voxel rows before their sum.
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    ((sizeof(float) * 12 * (size_t)SVR_NOALLOC_MAX_ENTRIES + 255) / 256) * 256 + sizeof(float) * 12 * HUGE_SPLIT_ROWS;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept: the array's
+// name in host code is a host-side stand-in, never a device address.
+int noalloc_scratch(void** out)
+{
+    constexpr int MAX_DEV = 64;
+    static void* ptrs[MAX_DEV] = {};
+    static std::mutex mu;
+    int dev = 0;
+    SVR_CHECK_HIP(hipGetDevice(&dev));
+    SVR_REQUIRE(dev >= 0 && dev < MAX_DEV, SVR_ERR_INVALID, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev])
+    {
+        void* p = nullptr;
+        SVR_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SVR_REQUIRE(p != nullptr, SVR_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        ptrs[dev] = p;
+    }
+    *out = ptrs[dev];
+    return SVR_OK;
+}
+
 }  // namespace svr
 
 extern "C" {
@@ -545,6 +581,24 @@ int svr_rasterize_backward(int32_t n_samp, int32_t W, int32_t H, float tanfovx, 
     default: return launch_bwd<3>(W, H, tanfovx, tanfovy, cx, cy, c2w, bg, N, R, g, b, img, centers, lengths, geos,
                                   rgbs, out_T, dL_dcolor, dL_dT, partial, dgeos, drgbs, dsub, stream);
     }
+}
+
+int svr_rasterize_backward_noalloc(int32_t n_samp, int32_t W, int32_t H, float tanfovx, float tanfovy, float cx,
+                                   float cy, const float* c2w, float bg, int64_t N, const float* centers,
+                                   const float* lengths, const float* geos, const float* rgbs, const void* geom_state,
+                                   int64_t R, const void* binning_state, const void* image_state, const float* out_T,
+                                   const float* dL_dcolor, const float* dL_dT, float* dgeos, float* drgbs,
+                                   float* dsub, svr_stream_t stream)
+{
+    SVR_REQUIRE(R >= 0 && R <= SVR_NOALLOC_MAX_ENTRIES, SVR_ERR_CAPACITY,
+                "svr_rasterize_backward_noalloc: %lld entries exceed its fixed scratch of %lld entries",
+                (long long)R, (long long)SVR_NOALLOC_MAX_ENTRIES);
+    void* ws = nullptr;
+    const int rc = noalloc_scratch(&ws);
+    if (rc != SVR_OK) return rc;
+    return svr_rasterize_backward(n_samp, W, H, tanfovx, tanfovy, cx, cy, c2w, bg, N, centers, lengths, geos, rgbs,
+                                  geom_state, R, binning_state, image_state, out_T, dL_dcolor, dL_dT, ws, dgeos,
+                                  drgbs, dsub, stream);
 }
 
 int svr_unpack_image_state(int32_t W, int32_t H, const void* image_state, int32_t* ranges, int32_t* tile_last,
