@@ -176,7 +176,7 @@ def batch_vs_plain(args, dev):
 
     # batched: all views in one step, binning tile picked by the autotuner
     geos, sh0, shs, opt = fresh()
-    views, tune_ms = batch.autotune(settings, octpath, centers, lengths, geos, sh0, shs)
+    views, tune = batch.autotune(settings, octpath, centers, lengths, geos, sh0, shs)
 
     def batch_step():
         colors, _ = batch.rasterize_voxels_batch(views, octpath, centers, lengths, geos, sh0, shs)
@@ -213,7 +213,8 @@ def batch_vs_plain(args, dev):
             "graph_views_per_s": V / graph_s, "graph_ms_per_step": graph_s * 1e3,
             "graph_over_plain": plain_s / graph_s, "graph_capacity": trainer.capacity,
             "graph_overflow": overflow,
-            "tile_shape": views.tile_shape, "autotune_fwd_bwd_ms": tune_ms,
+            "tile_shape": views.tile_shape, "bwd_window": views.bwd_window, "bwd_sync": views.bwd_sync,
+            "autotune": tune,
             "note": "plain = one drop-in training step (rasterize_voxels + SH_eval, MSE, SparseAdam) per view; "
                     "batched = rasterize_voxels_batch over all views, summed MSE, one SparseAdam step; "
                     "graph = the batched step replayed from a captured HIP graph (GraphTrainStep)"}

@@ -326,12 +326,21 @@ typedef struct svr_view {
     float c2w[12]; /* first 3 rows of the row-major camera-to-world matrix */
     int32_t tile_base;
     int32_t tile_shape; /* SVR_TILE_*: binning tile of the batch, the same in every view */
+    int32_t bwd_window; /* backward LDS depth in 64-entry chunks: 2, 4, 6 or 8 (0: 4); the same in every view */
+    int32_t bwd_sync;   /* backward wave-sum threshold in pixels: 4, 8 or 16 (0: 8); the same in every view */
     int64_t pix_base;
 } svr_view;
 
 /* Binning tiles of a batch (the single-view entry points use the reference's
  * 16x16).  Images and gradients do not depend on the tile; n_contrib and the
- * tile ranges are relative to the batch's tiles. */
+ * tile ranges are relative to the batch's tiles.
+ *
+ * Backward depth of a batch: the adjoint stages bwd_window chunks of 64 list
+ * entries (payload and gradient accumulators) in LDS at a time, and an entry
+ * used by at least bwd_sync pixels of a wave has its gradient terms summed
+ * across the wave at once instead of per pixel.  Pure performance choices:
+ * gradients do not depend on them beyond float summation order.  n_samp 2
+ * and 3 always use the defaults. */
 enum { SVR_TILE_16X16 = 0, SVR_TILE_8X8 = 1, SVR_TILE_32X8 = 2 };
 
 /* Sizes: geometry state of V views (its count runs over V * N entries, so

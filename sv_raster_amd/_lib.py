@@ -20,15 +20,15 @@ class SvrConstants(ctypes.Structure):
 
 
 class SvrView(ctypes.Structure):
-    """svr_view of include/svraster_hip.h (144 bytes)."""
+    """svr_view of include/svraster_hip.h (152 bytes)."""
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("tanfovx", ctypes.c_float),
                 ("tanfovy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
                 ("near_plane", ctypes.c_float), ("bg_color", ctypes.c_float), ("w2c", ctypes.c_float * 12),
                 ("c2w", ctypes.c_float * 12), ("tile_base", ctypes.c_int32), ("tile_shape", ctypes.c_int32),
-                ("pix_base", ctypes.c_int64)]
+                ("bwd_window", ctypes.c_int32), ("bwd_sync", ctypes.c_int32), ("pix_base", ctypes.c_int64)]
 
 
-assert ctypes.sizeof(SvrView) == 144
+assert ctypes.sizeof(SvrView) == 152
 
 # name -> (restype, argtypes)
 SIGNATURES = {

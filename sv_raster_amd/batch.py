@@ -31,6 +31,11 @@ _L = _lib.load()
 
 # binning tiles of a batch (svr_view.tile_shape): name -> (code, width, height)
 TILE_SHAPES = {"16x16": (0, 16, 16), "8x8": (1, 8, 8), "32x8": (2, 32, 8)}
+# backward depth of a batch (svr_view.bwd_window, bwd_sync): LDS window in 64-entry chunks, and the number of a
+# wave's pixels from which an entry's gradient terms are summed across the wave at once
+BWD_WINDOWS = (2, 4, 6, 8)
+BWD_SYNCS = (4, 8, 16)
+BWD_DEFAULT = (4, 8)
 
 
 def _mat12(m):
@@ -51,9 +56,12 @@ class ViewBatch:
     Reading the camera matrices synchronises once, here.
     """
 
-    def __init__(self, settings, device=None, tile_shape="16x16"):
+    def __init__(self, settings, device=None, tile_shape="16x16", bwd_window=BWD_DEFAULT[0], bwd_sync=BWD_DEFAULT[1]):
         if tile_shape not in TILE_SHAPES:
             raise Exception(f"tile_shape must be one of {sorted(TILE_SHAPES)}")
+        if bwd_window not in BWD_WINDOWS or bwd_sync not in BWD_SYNCS:
+            raise Exception(f"bwd_window must be one of {BWD_WINDOWS} and bwd_sync one of {BWD_SYNCS}")
+        self.bwd_window, self.bwd_sync = int(bwd_window), int(bwd_sync)
         code, tw, th = TILE_SHAPES[tile_shape]
         self.tile_shape = tile_shape
         self.settings = list(settings)
@@ -85,6 +93,7 @@ class ViewBatch:
             v.w2c[:] = _mat12(s.w2c_matrix).tolist()
             v.c2w[:] = _mat12(s.c2w_matrix).tolist()
             v.tile_base, v.tile_shape, v.pix_base = tile_base, code, pix_base
+            v.bwd_window, v.bwd_sync = self.bwd_window, self.bwd_sync
             self.sizes.append((W, H))
             tile_base += ((W + tw - 1) // tw) * ((H + th - 1) // th)
             pix_base += W * H
@@ -102,6 +111,18 @@ class ViewBatch:
 
     def host_ptr(self):
         return ctypes.cast(self.host, ctypes.c_void_p)
+
+    def set_backward(self, bwd_window, bwd_sync):
+        """Selects the backward window depth and sync threshold (read on the host at each backward call; images
+        and gradients do not depend on them beyond float summation order)."""
+        if bwd_window not in BWD_WINDOWS or bwd_sync not in BWD_SYNCS:
+            raise Exception(f"bwd_window must be one of {BWD_WINDOWS} and bwd_sync one of {BWD_SYNCS}")
+        self.bwd_window, self.bwd_sync = int(bwd_window), int(bwd_sync)
+        for u in range(self.n_views):
+            self.host[u].bwd_window, self.host[u].bwd_sync = self.bwd_window, self.bwd_sync
+        raw = np.frombuffer(bytes(memoryview(self.host)), dtype=np.uint8).copy()
+        self.dev.copy_(torch.from_numpy(raw).to(self.device))
+        return self
 
     def split(self, color_flat, T_flat):
         """Per-view [3, H, W] colour and [1, H, W] transmittance views of the flat buffers."""
@@ -356,37 +377,98 @@ class GraphTrainStep:
 _TUNED = {}
 
 
+def _spread(ts):
+    ts = sorted(ts)
+    return {"median_ms": ts[len(ts) // 2], "min_ms": ts[0], "max_ms": ts[-1], "runs": len(ts)}
+
+
+def _separated(times):
+    """True when the fastest candidate's slowest run beats every other candidate's fastest run."""
+    med = {k: sorted(v)[len(v) // 2] for k, v in times.items()}
+    best = min(med, key=med.get)
+    return all(max(times[best]) < min(v) for k, v in times.items() if k != best), best
+
+
+def _tune_rounds(cands, run, min_rounds, max_rounds):
+    """Times run(c) for every candidate, min_rounds times each, then more rounds (one run of each candidate per
+    round, candidates interleaved) until the fastest separates from the rest or max_rounds is reached."""
+    times = {c: [] for c in cands}
+    for r in range(max_rounds):
+        for c in cands:
+            times[c].append(run(c))
+        if r + 1 >= min_rounds and _separated(times)[0]:
+            break
+    sep, best = _separated(times)
+    return best, sep, times
+
+
 def autotune(settings, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, candidates=tuple(TILE_SHAPES),
-             repeats=3, active_sh_degree=3):
-    """Picks the binning tile of a multi-view workload by timing a forward + backward of every candidate on
-    the actual scene and cameras (the first call for a workload; later calls with the same voxel count, image
-    sizes and n_samp return the cached choice).  Returns (ViewBatch with the fastest tile shape, {shape: ms}).
+             windows=BWD_WINDOWS, syncs=BWD_SYNCS, min_rounds=3, max_rounds=10, active_sh_degree=3):
+    """Picks the binning tile and the backward depth (LDS window, sync threshold) of a multi-view workload by
+    timing them on the actual scene and cameras (the first call for a workload; later calls with the same voxel
+    count, image sizes and n_samp return the cached choice).  Returns (ViewBatch with the choice, report).
+
+    Stage 1 times a forward + backward per tile shape (default backward depth); stage 2 times the backward
+    alone for every (window, sync) pair on the chosen tile, all from one forward state.  Each stage repeats its
+    candidates (interleaved rounds) until the fastest one's slowest run beats every other candidate's fastest
+    run, or max_rounds; the report gives every candidate's median, min and max and whether that separation was
+    reached.
 
     The tile shape trades sort size against list relevance: small tiles give more list entries (more keys to
     sort) but each wave walks only the entries of its own 8x8 pixels; large tiles sort fewer entries but every
-    wave tests the entries of its neighbours' pixels too."""
+    wave tests the entries of its neighbours' pixels too.  The window sets how many 64-entry chunks the
+    backward stages in LDS at a time (deeper windows share each staging barrier among more entries, shallower
+    ones leave LDS for more workgroups per CU); the sync threshold sets which entries have their gradient terms
+    summed across the wave at once instead of per pixel."""
     settings = list(settings)
     key = (int(vox_centers.shape[0]), tuple((int(s.image_width), int(s.image_height)) for s in settings),
-           int(settings[0].n_samp_per_vox))
+           int(settings[0].n_samp_per_vox), tuple(candidates), tuple(windows), tuple(syncs))
     if key in _TUNED:
-        best, times = _TUNED[key]
-        return ViewBatch(settings, tile_shape=best), dict(times)
+        (best, kw, sync), report = _TUNED[key]
+        return ViewBatch(settings, tile_shape=best, bwd_window=kw, bwd_sync=sync), report
     import time
-    times = {}
+
+    def fwd_bwd(name):
+        views = ViewBatch(settings, tile_shape=name)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
+        batch_backward(views, st, st["color"] - 0.5, None)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
     with torch.no_grad():
-        for name in candidates:
-            views = ViewBatch(settings, tile_shape=name)
-            for r in range(repeats + 1):
-                if r == 1:
-                    torch.cuda.synchronize()
-                    t0 = time.perf_counter()
-                st = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
-                batch_backward(views, st, st["color"] - 0.5, None)
+        for name in candidates:  # first use of each shape (allocations, code loading), untimed
+            fwd_bwd(name)
+            torch.cuda.empty_cache()
+        best, sep1, t1 = _tune_rounds(list(candidates), fwd_bwd, min_rounds, max_rounds)
+        views = ViewBatch(settings, tile_shape=best)
+        st = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
+        dcol = st["color"] - 0.5
+        if views.n_samp != 1:  # n_samp 2 and 3 always run the default backward depth
+            pairs = [BWD_DEFAULT]
+        else:
+            pairs = [(w, s) for w in windows for s in syncs]
+
+        def bwd(pair):
+            views.set_backward(*pair)
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             torch.cuda.synchronize()
-            times[name] = (time.perf_counter() - t0) * 1e3 / repeats
-    best = min(times, key=times.get)
-    _TUNED[key] = (best, dict(times))
-    return ViewBatch(settings, tile_shape=best), times
+            start.record()
+            batch_backward(views, st, dcol, None)
+            end.record()
+            end.synchronize()
+            return start.elapsed_time(end)
+
+        for pair in pairs:
+            bwd(pair)
+        (kw, sync), sep2, t2 = _tune_rounds(pairs, bwd, min_rounds, max_rounds)
+        del st, dcol
+    report = {"tile_shape": best, "bwd_window": kw, "bwd_sync": sync,
+              "fwd_bwd_ms": {k: _spread(v) for k, v in t1.items()}, "tile_separated": sep1,
+              "bwd_ms": {f"{w}x{s}": _spread(v) for (w, s), v in t2.items()}, "bwd_separated": sep2}
+    _TUNED[key] = ((best, kw, sync), report)
+    return ViewBatch(settings, tile_shape=best, bwd_window=kw, bwd_sync=sync), report
 
 
 

@@ -21,90 +21,10 @@
 //   * SH colours of all views are computed from one read of each voxel's 48
 //     coefficients, and the SH backward sums the views' gradients in registers
 //     before one write.
-#include "compositor.h"
-#include "svr_host.h"
+#include "batch_common.h"
 #include "voxel_ops.h"
 
-// Backward window and sync-slot threshold of the batch compositor, measured
-// apart from the single-view kernel's: on the 16-view 2 M-voxel workload 4
-// chunks and 8 pixels beat (6, 8), (5, 12) and (4, 12) (the 8x8 binning tile
-// has one wave per workgroup, so larger windows stage more entries per thread).
-#ifndef BATCH_BWD_WINDOW
-#define BATCH_BWD_WINDOW 4
-#endif
-#ifndef BATCH_BWD_SYNC
-#define BATCH_BWD_SYNC 8
-#endif
-
 namespace svr {
-
-// Batch image state: per global tile ranges / tile_last, per pixel n_contrib,
-// and the tile queue (order, deferred exact-ray tiles, counters).
-struct BatchImageState
-{
-    uint2* ranges;        // [T]
-    uint32_t* tile_last;  // [T]
-    uint32_t* n_contrib;  // [P]
-    uint64_t* qkeys;      // [T] queue sort keys
-    uint64_t* qkeys_alt;  // [T]
-    uint32_t* order;      // [T] tile order (longest list first)
-    uint32_t* order_alt;  // [T]
-    uint32_t* exact_list; // [T] tiles deferred to the exact ray-box launch
-    uint32_t* qcounts;    // radix counts for the queue sort
-    uint32_t* ctr;        // [8]: 0 fwd head, 1 exact count, 2 fwd exact head, 3 bwd head, 4 bwd exact head
-    uint64_t* hit_masks;  // [hit_mask_slots(R, T)][up to 4 waves][64]: as in ImageState, global tile ids
-    static BatchImageState carve(void* p, int64_t T, int64_t P, size_t* bytes, int64_t R = 0)
-    {
-        Carver c(p);
-        BatchImageState s;
-        s.ranges = c.take<uint2>(T);
-        s.tile_last = c.take<uint32_t>(T);
-        s.n_contrib = c.take<uint32_t>(P);
-        s.qkeys = c.take<uint64_t>(T);
-        s.qkeys_alt = c.take<uint64_t>(T);
-        s.order = c.take<uint32_t>(T);
-        s.order_alt = c.take<uint32_t>(T);
-        s.exact_list = c.take<uint32_t>(T);
-        s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
-        s.ctr = c.take<uint32_t>(8);
-        s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
-        if (bytes) *bytes = c.size();
-        return s;
-    }
-};
-
-// Backward workspace: per-entry partial rows, then per-pair density and
-// subdivision-priority gradients before the sum over views.
-struct BatchBwdWork
-{
-    float* partial;   // [R][12]
-    float* pair_geo;  // [V*N][8]
-    float* pair_sub;  // [V*N]
-    float* huge;      // [HUGE_SPLIT_ROWS][12] partial sums of split huge pairs
-    static BatchBwdWork carve(void* p, int64_t VN, int64_t R, size_t* bytes)
-    {
-        Carver c(p);
-        BatchBwdWork w;
-        w.partial = c.take<float>(12 * R);
-        w.pair_geo = c.take<float>(8 * VN);
-        w.pair_sub = c.take<float>(VN);
-        w.huge = c.take<float>(12 * HUGE_SPLIT_ROWS);
-        if (bytes) *bytes = c.size();
-        return w;
-    }
-};
-
-__device__ inline int view_of_tile(const svr_view* __restrict__ views, int V, uint32_t g)
-{
-    int lo = 0, hi = V - 1;
-    while (lo < hi)
-    {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((uint32_t)views[mid].tile_base <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // ---- preprocess and key duplication over (view, voxel) pairs -------------
 
@@ -241,36 +161,6 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(GeomState g, i
     }
 }
 
-// ---- compositor over the tile queue ---------------------------------------
-
-struct TileGeom
-{
-    int u, W, H, tx, ty, sx0, sy0, sx1, sy1, px, py;
-    bool inside;
-};
-
-template <typename TS>
-__device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, uint32_t g, int w, int lane)
-{
-    TileGeom t;
-    t.u = view_of_tile(views, V, g);
-    const svr_view& vw = views[t.u];
-    t.W = vw.width;
-    t.H = vw.height;
-    const int gx = tiles_along(t.W, TS::TX);
-    const uint32_t tile = g - (uint32_t)vw.tile_base;
-    t.tx = (int)(tile % (uint32_t)gx);
-    t.ty = (int)(tile / (uint32_t)gx);
-    t.sx0 = t.tx * TS::TX + (w % TS::WX) * 8;
-    t.sy0 = t.ty * TS::TY + (w / TS::WX) * 8;
-    t.sx1 = min(t.sx0 + 7, t.W - 1);
-    t.sy1 = min(t.sy0 + 7, t.H - 1);
-    t.px = t.sx0 + (lane & 7);
-    t.py = t.sy0 + (lane >> 3);
-    t.inside = t.px < t.W && t.py < t.H;
-    return t;
-}
-
 // One tile of the batch forward.  The fast form (EXACT = false) defers a tile
 // holding a ray whose 1/rd is not finite to exact_list (count ctr[1]).
 // MAX_W: weights-only pass (no colour read or written) that raises
@@ -376,76 +266,6 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
         batch_fwd_tile<NS, true, TS, MAX_W>(st, sv, s_last, g, V, views, N, ctr, exact_list, ranges, vox_list, bboxes, octs,
                                             centers, lengths, geos, rgbs, tile_last, n_contrib, out_color, out_T,
                                             max_w, hit_masks);
-        __syncthreads();
-    }
-}
-
-template <int NS, bool EXACT, typename TS>
-__device__ inline void batch_bwd_tile(WindowLDS<BATCH_BWD_WINDOW>& sh, uint32_t g, int V, const svr_view* __restrict__ views,
-                                      int64_t N, const uint2* __restrict__ ranges,
-                                      const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
-                                      const float* __restrict__ lengths, const float* __restrict__ geos,
-                                      const float* __restrict__ rgbs, const float* __restrict__ out_T,
-                                      const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
-                                      const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-                                      const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
-{
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
-    const svr_view& vw = views[t.u];
-    PlainHit<NS, EXACT> hit;
-    hit.ray = make_ray(t.px, t.py, t.W, t.H, vw.tanfovx, vw.tanfovy, vw.cx, vw.cy, vw.c2w);
-    const bool need_exact = __syncthreads_or(!ray_inv_finite(hit.ray)) != 0;
-    if (!EXACT && need_exact) return;  // rendered by the exact launch
-    const size_t vo = (size_t)t.u * (size_t)N;
-    const size_t pb = (size_t)vw.pix_base;
-    const size_t HW = (size_t)t.H * t.W;
-    const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
-    const uint32_t nc = hit.init(t.inside, pid, HW, vw.bg_color, out_T + pb, n_contrib + pb, dL_dcolor + 3 * pb,
-                                 dL_dT + pb);
-    const uint2 range = ranges[g];
-    const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
-    adjoint_window<BATCH_BWD_SYNC, TS, BATCH_BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths,
-                                                        geos, rgbs + 3 * vo, orig, partial);
-}
-
-template <int NS, typename TS>
-__global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
-    int V, const svr_view* __restrict__ views, int64_t N, const uint32_t* __restrict__ order,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
-    const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
-    const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
-    const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-    float* __restrict__ partial)
-{
-    __shared__ WindowLDS<BATCH_BWD_WINDOW> sh;
-    const uint32_t g = order[blockIdx.x];
-    batch_bwd_tile<NS, false, TS>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T, n_contrib,
-                                  dL_dcolor, dL_dT, orig, hit_masks, partial);
-}
-
-// The forward's deferred tiles (exact_list, count ctr[1]; head ctr[4]).
-template <int NS, typename TS>
-__global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
-    int V, const svr_view* __restrict__ views, int64_t N, uint32_t* __restrict__ ctr,
-    const uint32_t* __restrict__ exact_list, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list,
-    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
-    const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
-    const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
-{
-    __shared__ WindowLDS<BATCH_BWD_WINDOW> sh;
-    __shared__ uint32_t s_i;
-    const uint32_t count = ctr[1];
-    while (true)
-    {
-        if (threadIdx.x == 0) s_i = atomicAdd(&ctr[4], 1u);
-        __syncthreads();
-        const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
-        if (i >= count) break;
-        const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_bwd_tile<NS, true, TS>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
-                                     n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
         __syncthreads();
     }
 }
@@ -604,9 +424,6 @@ __global__ __launch_bounds__(64) void batch_sh_bwd_kernel(int deg, int V, const 
 
 // ---- host side --------------------------------------------------------------
 
-// Workgroups of the launches that loop over the deferred exact-ray tiles.
-constexpr int64_t EXACT_GRID = 256;
-
 // Binning tile of a tile_shape code (SVR_TILE_*).
 inline void tile_dims(int shape, int* tx, int* ty)
 {
@@ -634,6 +451,13 @@ int check_views(int V, const svr_view* views, int64_t* T_out, int64_t* P_out, in
                     SVR_ERR_INVALID, "view %d: image size %dx%d outside [1, %d]", u, vw.width, vw.height,
                     MAX_PACKED_IMAGE_DIM);
         SVR_REQUIRE(vw.tile_shape == shape, SVR_ERR_INVALID, "view %d: all views need the same tile shape", u);
+        SVR_REQUIRE(vw.bwd_window == views[0].bwd_window && vw.bwd_sync == views[0].bwd_sync, SVR_ERR_INVALID,
+                    "view %d: all views need the same backward window and sync threshold", u);
+        SVR_REQUIRE(vw.bwd_window == 0 || vw.bwd_window == 2 || vw.bwd_window == 4 || vw.bwd_window == 6 ||
+                        vw.bwd_window == 8,
+                    SVR_ERR_INVALID, "view %d: backward window %d not in {2, 4, 6, 8}", u, vw.bwd_window);
+        SVR_REQUIRE(vw.bwd_sync == 0 || vw.bwd_sync == 4 || vw.bwd_sync == 8 || vw.bwd_sync == 16, SVR_ERR_INVALID,
+                    "view %d: backward sync threshold %d not in {4, 8, 16}", u, vw.bwd_sync);
         SVR_REQUIRE(vw.tile_base == T && vw.pix_base == P, SVR_ERR_INVALID,
                     "view %d: tile_base / pix_base must be the running sums of the previous views", u);
         T += (int64_t)tiles_along(vw.width, tw) * tiles_along(vw.height, th);
@@ -683,23 +507,6 @@ int batch_fwd_dispatch(int V, const svr_view* views, int64_t N, int64_t T, const
                                               out_T, max_w, stream);
     return batch_fwd_launch<NS, TS, false>(V, views, N, T, b, img, g, centers, lengths, geos, rgbs, out_color, out_T,
                                            nullptr, stream);
-}
-
-template <int NS, typename TS>
-int batch_bwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const BinningState& b,
-                     const BatchImageState& img, const GeomState& g, const float* centers, const float* lengths,
-                     const float* geos, const float* rgbs, const float* out_T, const float* dL_dcolor,
-                     const float* dL_dT, float* partial, hipStream_t stream)
-{
-    batch_bwd_kernel<NS, TS><<<(unsigned)T, TS::THREADS, 0, stream>>>(
-        V, views, N, img.order, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T, img.n_contrib, dL_dcolor,
-        dL_dT, b.orig, img.hit_masks, partial);
-    SVR_CHECK_LAUNCH();
-    batch_bwd_exact_kernel<NS, TS><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
-        V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T,
-        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
-    SVR_CHECK_LAUNCH();
-    return SVR_OK;
 }
 
 }  // namespace svr
@@ -910,15 +717,10 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
     {
         if (fill_u32(img.ctr + 3, 0u, 2, stream) != SVR_OK) return SVR_ERR_HIP;
         prof_begin(PROF_RENDER_BWD, stream);
-        SVR_WITH_TILE(shape, (rc = n_samp == 1   ? batch_bwd_launch<1, TS>(V, views_dev, N, T, b, img, g, centers,
-                                                                              lengths, geos, rgbs, out_T, dL_dcolor,
-                                                                              dL_dT, wk.partial, stream)
-                                   : n_samp == 2 ? batch_bwd_launch<2, TS>(V, views_dev, N, T, b, img, g, centers,
-                                                                              lengths, geos, rgbs, out_T, dL_dcolor,
-                                                                              dL_dT, wk.partial, stream)
-                                                 : batch_bwd_launch<3, TS>(V, views_dev, N, T, b, img, g, centers,
-                                                                              lengths, geos, rgbs, out_T, dL_dcolor,
-                                                                              dL_dT, wk.partial, stream)));
+        const int kw = views_host[0].bwd_window ? views_host[0].bwd_window : BATCH_BWD_WINDOW_DEFAULT;
+        const int sync = views_host[0].bwd_sync ? views_host[0].bwd_sync : BATCH_BWD_SYNC_DEFAULT;
+        SVR_WITH_TILE(shape, (rc = batch_bwd_run<TS>(n_samp, kw, sync, V, views_dev, N, T, b, img, centers, lengths,
+                                                     geos, rgbs, out_T, dL_dcolor, dL_dT, wk.partial, stream)));
         prof_end(PROF_RENDER_BWD, stream);
         if (rc != SVR_OK) return rc;
     }

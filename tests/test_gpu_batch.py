@@ -36,12 +36,13 @@ def _settings(cam, bg=0.0, n_samp=1):
                                    bg_color=bg, near=cam["near"])
 
 
-def _run_batch(cases, bg=0.0, n_samp=1, tile_shape="16x16"):
+def _run_batch(cases, bg=0.0, n_samp=1, tile_shape="16x16", bwd=(4, 8)):
     import torch
     from sv_raster_amd import batch
     sc = cases[0]["scene"]
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
-    views = batch.ViewBatch([_settings(c["cam"], bg, n_samp) for c in cases], tile_shape=tile_shape)
+    views = batch.ViewBatch([_settings(c["cam"], bg, n_samp) for c in cases], tile_shape=tile_shape,
+                            bwd_window=bwd[0], bwd_sync=bwd[1])
     octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
     geos, sh0, shs = t(sc["geos"]), t(sc["sh0"]), t(sc["shs"])
     st = batch.batch_forward(views, 3, octpath, centers, lengths, geos, sh0, shs)
@@ -174,7 +175,44 @@ def test_batch_other_tile_shapes_match_oracle(tile_shape):
 
 
 @pytest.mark.gpu
-def test_autotune_picks_a_shape_and_caches_it():
+@pytest.mark.parametrize("tile_shape", ["16x16", "8x8", "32x8"])
+@pytest.mark.parametrize("window", [2, 4, 6, 8])
+@pytest.mark.parametrize("sync", [4, 8, 16])
+def test_batch_backward_depths_match_oracle(tile_shape, window, sync):
+    """Every backward window depth and sync threshold the autotuner can pick gives the oracle's gradients
+    (images and the forward are unaffected), on four views of the 2k-voxel scene at every binning tile."""
+    cases = _views("C", [((64, 64), None), ((48, 40), None), ((80, 56), None), ((63, 63), (0.0, 0.0, -1.0))],
+                   seed=6)
+    print("batch C x4", tile_shape, window, sync, flush=True)
+    orc = _oracle_cache("C6", cases)
+    views, g = _run_batch(cases, tile_shape=tile_shape, bwd=(window, sync))
+    assert (views.bwd_window, views.bwd_sync) == (window, sync)
+    _compare(cases, views, g, orc, tile16=tile_shape == "16x16")
+
+
+_ORACLE = {}
+
+
+def _oracle_cache(key, cases):
+    if key not in _ORACLE:
+        _ORACLE[key] = [run_oracle(c) for c in cases]
+    return _ORACLE[key]
+
+
+@pytest.mark.gpu
+def test_batch_backward_depths_long_lists_match_oracle():
+    """Deep windows on long lists: config A (100 k voxels) seen by two 200 x 160 views, every window depth at
+    the default sync threshold, against the oracle."""
+    cases = _views("A", [((200, 160), None), ((160, 200), None)], seed=2)
+    orc = _oracle_cache("A2", cases)
+    for window in (2, 4, 6, 8):
+        print("batch A x2 window", window, flush=True)
+        views, g = _run_batch(cases, tile_shape="16x16", bwd=(window, 8))
+        _compare(cases, views, g, orc, tile16=True)
+
+
+@pytest.mark.gpu
+def test_autotune_picks_and_caches():
     import torch
     from sv_raster_amd import batch
     cases = _views("C", [((64, 64), None), ((96, 48), None)], seed=7)
@@ -183,8 +221,16 @@ def test_autotune_picks_a_shape_and_caches_it():
     args = (t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"]), t(sc["geos"]), t(sc["sh0"]),
             t(sc["shs"]))
     settings = [_settings(c["cam"]) for c in cases]
-    views, times = batch.autotune(settings, *args, repeats=1)
-    assert set(times) == set(batch.TILE_SHAPES) and views.tile_shape == min(times, key=times.get)
-    views2, times2 = batch.autotune(settings, *args)
-    assert views2.tile_shape == views.tile_shape and times2 == times
-
+    views, rep = batch.autotune(settings, *args, min_rounds=2, max_rounds=3)
+    assert set(rep["fwd_bwd_ms"]) == set(batch.TILE_SHAPES)
+    assert len(rep["bwd_ms"]) == len(batch.BWD_WINDOWS) * len(batch.BWD_SYNCS)
+    med = {k: v["median_ms"] for k, v in rep["fwd_bwd_ms"].items()}
+    assert views.tile_shape == min(med, key=med.get) == rep["tile_shape"]
+    bmed = {k: v["median_ms"] for k, v in rep["bwd_ms"].items()}
+    assert f"{views.bwd_window}x{views.bwd_sync}" == min(bmed, key=bmed.get)
+    for v in list(rep["fwd_bwd_ms"].values()) + list(rep["bwd_ms"].values()):
+        assert 2 <= v["runs"] <= 3 and v["min_ms"] <= v["median_ms"] <= v["max_ms"]
+    views2, rep2 = batch.autotune(settings, *args, min_rounds=2, max_rounds=3)
+    assert (views2.tile_shape, views2.bwd_window, views2.bwd_sync) == (views.tile_shape, views.bwd_window,
+                                                                       views.bwd_sync)
+    assert rep2 == rep
