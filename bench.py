@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--batch-config", default="D", help="multi-view workload for the batched-vs-plain figure "
                                                           "('none' to skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--large-config", default="E", help="large batched workload: all its views in one batch per "
+                                                         "step, autotuned ('none' to skip)")
+    ap.add_argument("--large-steps", type=int, default=50, help="timed batched training steps of the large workload")
+    ap.add_argument("--large-plain-rounds", type=int, default=1,
+                    help="timed rounds of the plain path (one drop-in step per view) on the large workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the bench, run 3 plain copy kernels (torch.neg) of this many MiB: the known-size "
                          "dispatch that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -220,6 +225,98 @@ def batch_vs_plain(args, dev):
                     "graph = the batched step replayed from a captured HIP graph (GraphTrainStep)"}
 
 
+def batched_large(args, dev):
+    """The large batched training workload: every view of a step in one batch (one launch per stage), binning
+    tile and backward depth picked by the autotuner, eager fwd + bwd (summed MSE) + SparseAdam steps; next to
+    it the plain path (one drop-in training step per view), both in views per second."""
+    import torch
+    from sv_raster_amd import _C, batch, renderer, sparse_adam, synthetic as S
+    name = args.large_config
+    t0 = time.perf_counter()
+    sc = S.make_scene(name, seed=args.seed)
+    gen_s = time.perf_counter() - t0
+    sizes = S.config_views(name, args.seed)
+    cams = [S.make_camera(W, H, seed=args.seed + 1000 + u) for u, (W, H) in enumerate(sizes)]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
+    geos = t(sc["geos"]).requires_grad_()
+    sh0 = t(sc["sh0"]).requires_grad_()
+    shs = t(sc["shs"]).requires_grad_()
+    del sc
+    targets = [t(S.make_target(c["H"], c["W"], args.seed + u)) for u, c in enumerate(cams)]
+    settings = [renderer.RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=c["W"], image_height=c["H"],
+                                        tanfovx=c["tanfovx"], tanfovy=c["tanfovy"], cx=c["cx"], cy=c["cy"],
+                                        w2c_matrix=t(c["w2c"]), c2w_matrix=t(c["c2w"])) for c in cams]
+    N, V = centers.shape[0], len(cams)
+    opt = sparse_adam.SparseAdam([{"params": [geos], "lr": 0.025}, {"params": [sh0], "lr": 0.01},
+                                  {"params": [shs], "lr": 0.01}], betas=(0.1, 0.99), eps=1e-15, sparse=True)
+    _C.order_table_cache.get(octpath)
+    progress(f"{name}: {N} voxels, {V} views; autotune")
+    t0 = time.perf_counter()
+    views, tune = batch.autotune(settings, octpath, centers, lengths, geos, sh0, shs)
+    tune_s = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+
+    def batch_step():
+        colors, _ = batch.rasterize_voxels_batch(views, octpath, centers, lengths, geos, sh0, shs)
+        loss = sum(torch.nn.functional.mse_loss(c, tg) for c, tg in zip(colors, targets))
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return loss
+
+    batch_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.large_steps):
+        loss = batch_step()
+        if (k + 1) % 10 == 0:
+            progress(f"{name}: {k + 1} batched steps")
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / args.large_steps
+    peak = torch.cuda.max_memory_allocated()
+    with torch.no_grad():
+        st = batch.batch_forward(views, 3, octpath, centers, lengths, geos, sh0, shs)
+        R = st["R"]
+        del st
+    progress(f"{name}: {batch_s * 1e3:.1f} ms per batched step; plain path")
+    subdiv = torch.ones((N, 1), device=dev)
+
+    def plain_round():
+        for u in range(V):
+            def vox_fn(idx, cam_pos, color_mode):
+                rgbs = renderer.SH_eval.apply(3, idx, centers, cam_pos, None, sh0, shs)
+                return {"geos": geos, "rgbs": rgbs, "subdiv_p": subdiv}
+            color, _, _, _, _ = renderer.rasterize_voxels(settings[u], octpath, centers, lengths, vox_fn)
+            loss = torch.nn.functional.mse_loss(color, targets[u])
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+
+    plain_s = None
+    if args.large_plain_rounds > 0:
+        plain_round()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.large_plain_rounds):
+            plain_round()
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t0) / args.large_plain_rounds
+    pix = sum(W * H for W, H in sizes)
+    return {"workload": f"{name}: {N} voxels (levels 9-21 with a 5-level outside shell), {V} views of 720p-1440p "
+                        f"({pix / 1e6:.1f} Mpix per step) in one batch, fwd+bwd+sparse Adam",
+            "views": V, "entries_R": R, "steps_timed": args.large_steps,
+            "batched_ms_per_step": batch_s * 1e3, "batched_views_per_s": V / batch_s,
+            "batched_mpix_per_s": pix / batch_s / 1e6,
+            "plain_ms_per_round": None if plain_s is None else plain_s * 1e3,
+            "plain_views_per_s": None if plain_s is None else V / plain_s,
+            "batched_over_plain": None if plain_s is None else plain_s / batch_s,
+            "peak_device_memory_gib": peak / 2**30, "loss_last": float(loss.item()),
+            "tile_shape": views.tile_shape, "bwd_window": views.bwd_window, "bwd_sync": views.bwd_sync,
+            "autotune": tune, "autotune_s": tune_s, "scene_generation_s": gen_s}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -369,6 +466,13 @@ def main():
         progress("multi-view figure")
         batched = batch_vs_plain(args, dev)
 
+    large = None
+    if args.large_config.lower() != "none":
+        torch.cuda.empty_cache()
+        progress("large batched workload")
+        large = batched_large(args, dev)
+        torch.cuda.empty_cache()
+
     cpu = None
     if args.cpu_steps > 0:
         progress("CPU baseline")
@@ -410,6 +514,7 @@ def main():
         "tile_entry_pixel_pairs_per_s_fwd": E * 256 / (stage_ms["render_fwd"] * 1e-3),
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_large": large,
     }
     print(json.dumps(result), flush=True)
 

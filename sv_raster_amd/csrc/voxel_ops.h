@@ -35,8 +35,9 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
     uint2 bb = make_uint2(0u, 0u);
     // octagon of the projection in the pixel frame (pixel (px, py) centred
     // at (px, py)): bounds of x + y and x - y over the projected corners,
-    // widened by OCT_MARGIN pixels; unbounded if a corner is behind the
-    // near plane (the projection of the box is then not their hull)
+    // widened by OCT_MARGIN pixels; if a corner is behind the near plane (the
+    // projection of the box is then not their hull) the candidates are the
+    // bbox's pixels, as in the reference
     float os0 = 1e30f, os1 = -1e30f, od0 = 1e30f, od1 = -1e30f;
     float pmax = 0.f, ox0 = 1e30f, ox1 = -1e30f, oy0 = 1e30f, oy1 = -1e30f;
     bool all_front = true;
@@ -109,13 +110,20 @@ __device__ inline void preprocess_voxel(float c0, float c1, float c2, float r, i
     {
         const auto px16 = [](float x) { return (uint32_t)fminf(fmaxf(x, 0.f), 65535.f); };
         oct_out.o = make_float4(os0 - mg, os1 + mg, od0 - mg, od1 + mg);
-        oct_out.tb = make_uint2((px16(ceilf(ox0 - mg)) << 16) | px16(ceilf(oy0 - mg)),
-                                (px16(floorf(ox1 + mg)) << 16) | px16(floorf(oy1 + mg)));
+        // the pixel box, kept inside the bbox: the candidates never leave the
+        // reference's pix_in_bbox set, whatever the margin
+        const uint32_t x0 = max(px16(ceilf(ox0 - mg)), bb.x >> 16), y0 = max(px16(ceilf(oy0 - mg)), bb.x & 0xFFFFu);
+        const uint32_t x1 = min(px16(floorf(ox1 + mg)), bb.y >> 16), y1 = min(px16(floorf(oy1 + mg)), bb.y & 0xFFFFu);
+        oct_out.tb = make_uint2((x0 << 16) | y0, (x1 << 16) | y1);
     }
     else
     {
+        // a corner behind the near plane: the projection is not the hull of
+        // the projected corners, so only the reference's own per-pixel test
+        // applies, the packed bbox of the corners in front (forward.cu
+        // pix_in_bbox); the octagon does not cut
         oct_out.o = make_float4(-1e30f, 1e30f, -1e30f, 1e30f);
-        oct_out.tb = make_uint2(0u, 0xFFFFFFFFu);
+        oct_out.tb = bb;
     }
     oct_out.pad = make_uint2(0u, 0u);
 }

@@ -7,8 +7,8 @@ contributor counts and tile ranges of the single-view path run on that view
 alone, and gradients equal to the sum over views of the single-view gradients.
 The single-view path is itself held to the oracle up to the 1 M-voxel 1080p
 scene (test_gpu_parity.py).  Workloads: the 2 M-voxel scene with its 16 views
-of uneven resolution, and one GPU's share (8 of the 64 views) of the 8 M-voxel
-scene with the outside shell.
+of uneven resolution, and the 8 M-voxel scene with the outside shell and all
+64 of its views in one batch.
 
 Bar: n_contrib and tile ranges bit-exact, images within 1e-5 relative,
 gradients within 1e-4 (elementwise, absolute floor 1e-3 x max|g|)."""
@@ -86,7 +86,7 @@ def _single_view(dev, cam, dcol, dT):
                 dsh0=dsh0, dshs=dshs)
 
 
-def _check(name, n_views, tile_shape="16x16", case=None):
+def _check(name, n_views, tile_shape="16x16", case=None, bwd=(4, 8)):
     """Batch against the single-view path, view by view.  With the reference's 16x16 binning tile the batch's
     tile ranges and per-pixel n_contrib (positions in a tile's list) are compared bit for bit; with the 8x8
     and 32x8 tiles the lists are those of other tiles, so the per-pixel outputs (image, T) and the gradients
@@ -96,7 +96,7 @@ def _check(name, n_views, tile_shape="16x16", case=None):
     dev, cams, dcol, dT = _full_case(name, n_views) if case is None else case
     N = dev["centers"].shape[0]
     print(f"full {name}: {N} voxels, {n_views} views, {tile_shape} tiles", flush=True)
-    views = batch.ViewBatch([_settings(c) for c in cams], tile_shape=tile_shape)
+    views = batch.ViewBatch([_settings(c) for c in cams], tile_shape=tile_shape, bwd_window=bwd[0], bwd_sync=bwd[1])
     same_tiles = tile_shape == "16x16"
     st = batch.batch_forward(views, 3, dev["octpath"], dev["centers"], dev["lengths"], dev["geos"], dev["sh0"],
                              dev["shs"])
@@ -202,7 +202,39 @@ def test_graph_replays_match_eager_full_multiview_2m(case_d, tile_shape):
     assert not trainer.overflowed()
 
 
+@pytest.fixture(scope="module")
+def case_e():
+    import torch
+    case = _full_case("E", 64)
+    yield case
+    del case
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.gpu
-def test_batch_matches_single_view_full_8m_one_gpu_share():
-    """8 M voxels (levels 4-16 with the 5-level outside shell), 8 of the 64 views of 720p-1440p."""
-    _check("E", 8)
+@pytest.mark.timeout(600)
+def test_batch_matches_single_view_full_8m_64_views(case_e):
+    """The large batched workload at full size on one GPU: 8 M voxels (levels 4-16 with the 5-level outside
+    shell) and all 64 views of 720p-1440p (144 Mpix, about 590 M list entries) in one batch, against the
+    single-view path view by view, with the reference's 16x16 tiles (lists and n_contrib bit-exact)."""
+    import torch
+    _check("E", 64, "16x16", case=case_e)
+    print(f"peak device memory {torch.cuda.max_memory_allocated() / 2**30:.1f} GiB", flush=True)
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_autotuned_batch_matches_single_view_full_8m_64_views(case_e):
+    """The same workload in the configuration the autotuner picks for it (binning tile, backward window depth
+    and sync threshold): images, T and gradients against the single-view path."""
+    import torch
+    from sv_raster_amd import batch
+    dev, cams, _, _ = case_e
+    views, rep = batch.autotune([_settings(c) for c in cams], dev["octpath"], dev["centers"], dev["lengths"],
+                                dev["geos"], dev["sh0"], dev["shs"], min_rounds=2, max_rounds=4)
+    print(f"autotuned: {views.tile_shape}, window {views.bwd_window}, sync {views.bwd_sync}", flush=True)
+    del views
+    torch.cuda.empty_cache()
+    _check("E", 64, rep["tile_shape"], case=case_e, bwd=(rep["bwd_window"], rep["bwd_sync"]))
+    torch.cuda.empty_cache()
