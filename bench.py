@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--config", default="B")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--fwd-iters", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps on all host threads in the CPU "
+                                                             "baseline (median; 0: skip the baseline)")
+    ap.add_argument("--cpu-single-steps", type=int, default=5, help="timed oracle steps on one thread (median)")
     ap.add_argument("--batch-config", default="D", help="multi-view workload for the batched-vs-plain figure "
                                                           "('none' to skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -93,8 +95,20 @@ def collect_stages(L):
     return out
 
 
-def cpu_baseline(case, steps):
-    """The oracle's training step (preprocess, SH, binning, compositor fwd/bwd, SH bwd, Adam) on host cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, runs, single_runs):
+    """The oracle's training step (preprocess, SH, binning, compositor fwd/bwd, SH bwd, Adam) on the host cores:
+    one warm-up step, then the median of `runs` timed steps with every host thread (OpenMP), and the median of
+    `single_runs` steps on one thread."""
     from oracle import oracle as O
     sc, cam = case["scene"], case["cam"]
     cam_pos = cam["c2w"][:3, 3]
@@ -102,8 +116,12 @@ def cpu_baseline(case, steps):
     state = {k: (np.zeros_like(v), np.zeros_like(v)) for k, v in (("g", geos), ("s0", sh0), ("s", shs))}
     target = case["target"]
     H, W = cam["H"], cam["W"]
+    counter = [0]
 
-    def one(step):
+    def one():
+        counter[0] += 1
+        step = counter[0]
+        t0 = time.perf_counter()
         n_dup, bbox, qbits = O.preprocess(cam, sc["vox_centers"], sc["vox_lengths"])
         idx = np.nonzero(n_dup > 0)[0]
         rgbs = O.sh_fwd(3, idx, sc["vox_centers"], cam_pos, sh0, shs)
@@ -116,16 +134,25 @@ def cpu_baseline(case, steps):
         for key, p, g, lr in (("g", geos, dg.astype(np.float32), 0.025), ("s0", sh0, d0, 0.01), ("s", shs, ds, 0.01)):
             m, v = state[key]
             O.adam_step(p, np.ascontiguousarray(g), m, v, step, lr, 0.1, 0.99, 1e-15, sparse=True)
+        return time.perf_counter() - t0
 
-    one(1)  # warm-up (page faults, thread pool)
-    t0 = time.perf_counter()
-    for s in range(steps):
-        one(s + 2)
-    dt = time.perf_counter() - t0
-    return dict(value=steps / dt, unit="steps/s", cores=O.num_threads(), kind="port",
-                sample=f"{steps} full training steps of the same 1M-voxel 1920x1080 workload "
-                       f"(oracle C++ restatement, OpenMP, {O.num_threads()} threads, after 1 warm-up step)",
-                sec_per_step=dt / steps)
+    all_threads = O.num_threads()
+    one()  # warm-up (page faults, thread pool)
+    multi = sorted(one() for _ in range(runs))
+    O.set_num_threads(1)
+    try:  # the data is paged in and warm from the runs above: no second warm-up
+        single = sorted(one() for _ in range(single_runs))
+    finally:
+        O.set_num_threads(all_threads)
+    med = multi[len(multi) // 2]
+    med1 = single[len(single) // 2]
+    return dict(value=1.0 / med, unit="steps/s", cores=all_threads, kind="port",
+                sample=f"full training steps of the same 1M-voxel 1920x1080 workload (the oracle's C++ "
+                       f"restatement): median of {runs} steps on {all_threads} OpenMP threads and of {single_runs} "
+                       f"steps on 1 thread, after one warm-up step",
+                cpu_model=cpu_model(), sec_per_step_median=med, sec_per_step_range=[multi[0], multi[-1]],
+                single_thread={"value": 1.0 / med1, "unit": "steps/s", "cores": 1, "sec_per_step_median": med1,
+                               "sec_per_step_range": [single[0], single[-1]]})
 
 
 def batch_vs_plain(args, dev):
@@ -476,7 +503,7 @@ def main():
     cpu = None
     if args.cpu_steps > 0:
         progress("CPU baseline")
-        cpu = cpu_baseline(case, args.cpu_steps)
+        cpu = cpu_baseline(case, args.cpu_steps, args.cpu_single_steps)
 
     result = {
         "metric": METRIC,

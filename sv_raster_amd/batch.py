@@ -236,15 +236,33 @@ def batch_backward(views, state, dL_dcolor, dL_dT, out=None):
 
 
 class _RasterizeBatch(torch.autograd.Function):
+    """Outputs: the V per-view colour images, then the V per-view T images (views of the flat buffers).  The
+    backward gathers their gradients into the flat layout with one concatenation: had the function returned
+    the flat buffers and left the per-view split to autograd, every view's gradient would be widened to a
+    full-size zero tensor and accumulated, O(V^2) traffic over the batch's pixels."""
+
     @staticmethod
     def forward(ctx, views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, subdiv_p):
         state = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
         ctx.views, ctx.state = views, state
-        return state["color"], state["T"]
+        colors, Ts = views.split(state["color"], state["T"])
+        return (*colors, *Ts)
 
     @staticmethod
-    def backward(ctx, dL_dcolor, dL_dT):
-        dgeos, dsh0, dshs, dsub, _ = batch_backward(ctx.views, ctx.state, dL_dcolor, dL_dT)
+    def backward(ctx, *grads):
+        views, state = ctx.views, ctx.state
+        V = views.n_views
+        dev = state["color"].device
+
+        def flat(gs, chans):
+            if all(g is None for g in gs):
+                return None
+            return torch.cat([(g.reshape(-1) if g is not None else
+                               torch.zeros(chans * W * H, dtype=torch.float32, device=dev))
+                              for g, (W, H) in zip(gs, views.sizes)])
+
+        dL_dcolor, dL_dT = flat(grads[:V], 3), flat(grads[V:], 1)
+        dgeos, dsh0, dshs, dsub, _ = batch_backward(views, state, dL_dcolor, dL_dT)
         ctx.state = None
         return None, None, None, None, None, dgeos, dsh0, dshs, dsub
 
@@ -273,9 +291,9 @@ def rasterize_voxels_batch(views, octree_paths, vox_centers, vox_lengths, geos, 
     _C._require_cuda(octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
     if subdiv_p is None:
         subdiv_p = torch.zeros((N, 1), dtype=torch.float32, device=vox_centers.device)
-    color, T = _RasterizeBatch.apply(views, int(active_sh_degree), octree_paths, vox_centers, vox_lengths, geos,
-                                     sh0, shs, subdiv_p)
-    return views.split(color, T)
+    out = _RasterizeBatch.apply(views, int(active_sh_degree), octree_paths, vox_centers, vox_lengths, geos, sh0, shs,
+                                subdiv_p)
+    return list(out[:views.n_views]), list(out[views.n_views:])
 
 
 def unpack_image_state(views, imageBuffer):
@@ -389,15 +407,34 @@ def _separated(times):
     return all(max(times[best]) < min(v) for k, v in times.items() if k != best), best
 
 
-def _tune_rounds(cands, run, min_rounds, max_rounds):
-    """Times run(c) for every candidate, min_rounds times each, then more rounds (one run of each candidate per
-    round, candidates interleaved) until the fastest separates from the rest or max_rounds is reached."""
+def _tune_rounds(cands, run, min_rounds, max_rounds, setup=None):
+    """Times run(c) for every candidate in blocks of consecutive runs (setup(c), one untimed warm-up, then
+    min_rounds timed runs), then further blocks for the candidates whose fastest run is not beaten by the
+    leader's slowest, until the fastest candidate separates from the rest or max_rounds runs are reached.
+    Consecutive blocks keep each candidate's buffers in the caching allocator between its runs (a batch can
+    hold a large part of the device memory, and interleaving candidates of different sizes would reallocate
+    it between runs)."""
     times = {c: [] for c in cands}
-    for r in range(max_rounds):
-        for c in cands:
+
+    def block(c, n):
+        if setup is not None:
+            setup(c)
+        run(c)
+        for _ in range(n):
             times[c].append(run(c))
-        if r + 1 >= min_rounds and _separated(times)[0]:
+
+    for c in cands:
+        block(c, min_rounds)
+    while True:
+        sep, best = _separated(times)
+        if sep:
             break
+        contenders = [c for c in cands if c != best and min(times[c]) <= max(times[best])]
+        todo = [c for c in [best] + contenders if len(times[c]) < max_rounds]
+        if not todo:
+            break
+        for c in todo:
+            block(c, min(min_rounds, max_rounds - len(times[c])))
     sep, best = _separated(times)
     return best, sep, times
 
@@ -409,10 +446,10 @@ def autotune(settings, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, c
     count, image sizes and n_samp return the cached choice).  Returns (ViewBatch with the choice, report).
 
     Stage 1 times a forward + backward per tile shape (default backward depth); stage 2 times the backward
-    alone for every (window, sync) pair on the chosen tile, all from one forward state.  Each stage repeats its
-    candidates (interleaved rounds) until the fastest one's slowest run beats every other candidate's fastest
-    run, or max_rounds; the report gives every candidate's median, min and max and whether that separation was
-    reached.
+    alone for every (window, sync) pair on the chosen tile, all from one forward state.  Each stage times its
+    candidates in blocks of consecutive runs and repeats the blocks of the candidates still in contention until
+    the fastest one's slowest run beats every other candidate's fastest run, or max_rounds runs; the report
+    gives every candidate's median, min and max and whether that separation was reached.
 
     The tile shape trades sort size against list relevance: small tiles give more list entries (more keys to
     sort) but each wave walks only the entries of its own 8x8 pixels; large tiles sort fewer entries but every
@@ -438,10 +475,9 @@ def autotune(settings, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, c
         return (time.perf_counter() - t0) * 1e3
 
     with torch.no_grad():
-        for name in candidates:  # first use of each shape (allocations, code loading), untimed
-            fwd_bwd(name)
-            torch.cuda.empty_cache()
-        best, sep1, t1 = _tune_rounds(list(candidates), fwd_bwd, min_rounds, max_rounds)
+        best, sep1, t1 = _tune_rounds(list(candidates), fwd_bwd, min_rounds, max_rounds,
+                                      setup=lambda c: torch.cuda.empty_cache())
+        torch.cuda.empty_cache()
         views = ViewBatch(settings, tile_shape=best)
         st = batch_forward(views, active_sh_degree, octree_paths, vox_centers, vox_lengths, geos, sh0, shs)
         dcol = st["color"] - 0.5
@@ -460,8 +496,6 @@ def autotune(settings, octree_paths, vox_centers, vox_lengths, geos, sh0, shs, c
             end.synchronize()
             return start.elapsed_time(end)
 
-        for pair in pairs:
-            bwd(pair)
         (kw, sync), sep2, t2 = _tune_rounds(pairs, bwd, min_rounds, max_rounds)
         del st, dcol
     report = {"tile_shape": best, "bwd_window": kw, "bwd_sync": sync,
