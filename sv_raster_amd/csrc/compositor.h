@@ -105,6 +105,14 @@ __device__ inline uint64_t select_q(const uint64_t* qpix, uint32_t q)
     return s;
 }
 
+// A value that is the same in every lane, moved to scalar registers.
+__device__ inline uint64_t uniform_u64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ inline uint32_t wave_max_u32(uint32_t x)
 {
 #pragma unroll
@@ -303,15 +311,20 @@ struct EntryPay
 // Adjoint of one (pixel, entry) hit (backward.cu:259-527 without the
 // regularizers): recovers T before the entry, updates the running dL/dT and
 // writes the 12 gradient terms (8 densities, 3 colours, subdivision priority).
+// Branch-free: the terms are computed for every lane, and the pixel's state
+// only changes where `active` holds and the entry contributes (the ray meets
+// the box and alpha >= MIN_ALPHA; always true for the entries of the
+// forward's hit masks, which came from the same arithmetic).  Returns whether
+// the terms count.  A lane with nothing to do passes any staged slot and
+// active = false; its terms are garbage and must not be used.
 template <int NS, bool EXACT>
 __device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, float& last_dT, float dp0, float dp1,
-                               float dp2, float* gq)
+                               float dp2, float* gq, bool active)
 {
     const float4 a0 = p.q[0], a1 = p.q[1], a2 = p.q[2], a3 = p.q[3];
     const float c0 = a0.x, c1 = a0.y, c2 = a0.z, len = a0.w;
     float a, b;
     ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
-    if (a > b) return false;
     const float g[8] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
     // NS == 1: dI[i] = dd * w[i] is folded into the gradient scale below
     float dI[8], dd = 1.f;
@@ -319,13 +332,13 @@ __device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, f
                                 : vox_vol<NS, true>(c0, c1, c2, len, a3.w, g, a, b, ray, dI);
     const float e = svr_expf_core(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
-    if (alpha < MIN_ALPHA) return false;
-    T = div_refined(T, 1.f - alpha);
-    const float pt_w = alpha * T;
+    const bool hit = active && !(a > b) && !(alpha < MIN_ALPHA);
+    const float Tn = div_refined(T, 1.f - alpha);
+    const float pt_w = alpha * Tn;
     const float dpt = __builtin_fmaf(dp2, a3.z, __builtin_fmaf(dp1, a3.y, dp0 * a3.x));
     const float diff = dpt - last_dT;
-    const float dalpha = T * diff;
-    last_dT = __builtin_fmaf(alpha, diff, last_dT);
+    const float dalpha = Tn * diff;
+    const float ldn = __builtin_fmaf(alpha, diff, last_dT);
     const float dIs = (dalpha * e) * dd;
 #pragma unroll
     for (int i = 0; i < 8; ++i) gq[i] = dIs * dI[i];
@@ -333,7 +346,9 @@ __device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, f
     gq[9] = pt_w * dp1;
     gq[10] = pt_w * dp2;
     gq[11] = fabsf(dalpha * alpha);
-    return true;
+    T = hit ? Tn : T;
+    last_dT = hit ? ldn : last_dT;
+    return hit;
 }
 
 // A pixel of the plain adjoint: its ray and running state; one call per hit,
@@ -358,9 +373,9 @@ struct PlainHit
         last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
         return n_contrib[pid];
     }
-    __device__ inline bool operator()(const EntryPay& p, float* gq)
+    __device__ inline bool operator()(const EntryPay& p, float* gq, bool active)
     {
-        return bwd_hit<NS, EXACT>(p, ray, T, last_dT, dp0, dp1, dp2, gq);
+        return bwd_hit<NS, EXACT>(p, ray, T, last_dT, dp0, dp1, dp2, gq, active);
     }
 };
 
@@ -379,12 +394,6 @@ struct WindowLDS
 // Chunks per window of the backward (measured with the window prefetch on the
 // 1080p scene: 6 beats 2-5, 7 and 8; 7 and more leave room in LDS for only
 // two workgroups per CU).
-// Backward hit walk: 1 = the merged walk (other lanes take their own hits
-// in a sync slot's step), 0 = segments between sync slots in lock-step.
-#ifndef BWD_MERGED_WALK
-#define BWD_MERGED_WALK 1
-#endif
-
 #ifndef BWD_WINDOW_CHUNKS
 #define BWD_WINDOW_CHUNKS 6
 #endif
@@ -427,7 +436,7 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
     constexpr size_t STRIDE = 64 * TS::NW;
     constexpr int PER = (KW * 64 + TS::THREADS - 1) / TS::THREADS;  // window entries per thread
     const int t = threadIdx.x;
-    const uint32_t my_chunks = (wave_max_u32(nc) + 63u) >> 6;
+    const uint32_t my_chunks = __builtin_amdgcn_readfirstlane((wave_max_u32(nc) + 63u) >> 6);
     if (lane == 0) sh.nch[w] = my_chunks;
     for (int i = t; i < KW * 12 * 64; i += TS::THREADS) (&sh.acc[0][0][0])[i] = 0.f;
     for (int i = t; i < KW * 64; i += TS::THREADS) (&sh.used[0][0])[i] = 0u;
@@ -504,38 +513,33 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
         for (int j = 0; j < KW; ++j)
         {
             if (j >= nj) break;
-            uint64_t hm = m[j], sync = syncm[j];
+            uint64_t hm = m[j], sync = uniform_u64(syncm[j]);
             float (&acc)[12][64] = sh.acc[j];
-#if BWD_MERGED_WALK
-            // One hit per lane and step.  A lane's next hits are those above
-            // its next sync slot ("allowed"); the wave's highest remaining
-            // sync slot s is taken in the step where none of its users has an
-            // allowed hit left, and in that step the other lanes go on with
-            // their own allowed hits, so they do not idle while s is summed.
+            // One hit per lane and step, each lane's hits in descending order.
+            // A lane takes its highest remaining hit h unless h is a sync slot;
+            // the wave's highest remaining sync slot s is taken in the step where
+            // every lane that uses it has reached it (h == s), by all of them at
+            // once (wave_sum12), while the other lanes go on with their own hits.
+            // sync is the same in every lane and stays in scalar registers; the
+            // hit is computed in every lane (a lane with nothing to take passes
+            // slot 0 and active = false) so that no branch splits the step.
             while (__ballot(hm != 0ull) != 0ull)
             {
+                const int h = hm ? 63 - __builtin_clzll(hm) : -1;
+                const bool h_sync = h >= 0 && ((sync >> h) & 1ull);
                 const int s = sync ? 63 - __builtin_clzll(sync) : -1;
-                const uint64_t mine_sync = hm & sync;
-                const int ns = mine_sync ? 63 - __builtin_clzll(mine_sync) : -1;
-                const uint64_t allowed = (ns < 0) ? hm : (ns == 63 ? 0ull : (hm & ~((2ull << ns) - 1ull)));
                 const bool owner = s >= 0 && ((hm >> s) & 1ull);
-                const bool ready = s >= 0 && __ballot(owner && allowed != 0ull) == 0ull;
-                int l = -1;
+                const bool ready = s >= 0 && __ballot(owner && h != s) == 0ull;
                 const bool is_sync = ready && owner;
-                if (is_sync) l = s;
-                else if (allowed) l = 63 - __builtin_clzll(allowed);
+                const int l = is_sync ? s : (h_sync ? -1 : h);
+                const bool act = l >= 0;
+                const int ls = act ? l : 0;
+                hm &= act ? ~(1ull << ls) : ~0ull;
                 float gq[12];
-#pragma unroll
-                for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-                bool hit = false;
-                if (l >= 0)
-                {
-                    hm &= ~(1ull << l);
-                    hit = hit_fn(sh.pay[j][l], gq);
-                }
+                const bool hit = hit_fn(sh.pay[j][ls], gq, act);
                 if (ready)
                 {
-                    sync &= ~(1ull << s);
+                    sync = uniform_u64(sync & ~(1ull << s));
                     if (__ballot(is_sync && hit) != 0ull)
                     {
                         float gs[12];
@@ -552,45 +556,6 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
                     for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][l], gq[i]);
                 }
             }
-#else
-            while (true)
-            {
-                const int s = sync ? 63 - __builtin_clzll(sync) : -1;
-                // this lane's hits above the next sync slot, highest first
-                uint64_t seg = (s < 0) ? hm : (hm & ~((2ull << s) - 1ull));
-                if (s == 63) seg = 0ull;
-                hm &= ~seg;
-                while (seg)
-                {
-                    const int l = 63 - __builtin_clzll(seg);
-                    seg &= ~(1ull << l);
-                    float gq[12];
-                    if (hit_fn(sh.pay[j][l], gq))
-                    {
-#pragma unroll
-                        for (int i = 0; i < 12; ++i) atomicAdd(&acc[i][l], gq[i]);
-                    }
-                }
-                if (s < 0) break;
-                sync &= ~(1ull << s);
-                const bool mine = (hm >> s) & 1ull;
-                hm &= ~(1ull << s);
-                float gq[12];
-#pragma unroll
-                for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-                bool hit = false;
-                if (mine) hit = hit_fn(sh.pay[j][s], gq);
-                if (__ballot(hit) == 0ull) continue;
-                if (!hit)
-                {
-#pragma unroll
-                    for (int i = 0; i < 12; ++i) gq[i] = 0.f;
-                }
-                const float r = wave_sum12(gq, lane);
-                const int ri = (lane >> 2) & 15;
-                if ((lane & 3) == 0 && ri < 12) atomicAdd(&acc[ri][s], r);
-            }
-#endif
         }
         __syncthreads();
         // 4. one row per entry (accumulators of unused entries are zero), then reset the window

@@ -514,8 +514,9 @@ struct ExtHit
         }
         return n_contrib[pid];
     }
-    __device__ inline bool operator()(const EntryPay& pe, float* gq)
+    __device__ inline bool operator()(const EntryPay& pe, float* gq, bool active)
     {
+        if (!active) return false;
         return bwd_hit_ext<NS, EXACT>(pe, ray, k, p, T, last_dT, dp0, dp1, dp2, gq);
     }
 };
