@@ -34,12 +34,15 @@ struct TileShape
 using Tile16 = TileShape<16, 16>;
 constexpr int NWAVE = Tile16::NW;
 
-// Staged payload of the current chunk, one 64-entry slot row per wave.
+// Staged payload of the current chunk, one 64-entry slot row per wave: the
+// view-constant slab bounds and sample origin of the entry (entry_bounds),
+// 1 / length, the 8 densities and the colour.
 template <int NW>
 struct StageT
 {
-    float c[3][NW][64];
-    float l[NW][64];
+    float lo[3][NW][64];
+    float hi[3][NW][64];
+    float cm[3][NW][64];
     float linv[NW][64];
     float g[8][NW][64];
     float rgb[3][NW][64];
@@ -126,7 +129,7 @@ struct EntryRegs
 {
     uint32_t v;
     uint64_t pm;
-    float c0, c1, c2, l, linv;
+    float lo[3], hi[3], cm[3], linv;
     float4 g0, g1;
     float r0, r1, r2;
 };
@@ -138,11 +141,12 @@ struct EntryRegs
 // with a face in the ray's plane and counts them as hits (forward.cu's
 // !(a > b)), whether or not the ray meets the box, so those pixels keep the
 // bbox candidates.
+// ro: the camera centre (the view's ray origin).
 template <bool RGB = true, bool OCT = true>
 __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
                                   const float* __restrict__ centers, const float* __restrict__ lengths,
                                   const float* __restrict__ geos, const float* __restrict__ rgbs, int sx0, int sy0,
-                                  int sx1, int sy1, const uint64_t* qpix, const uint64_t* diag)
+                                  int sx1, int sy1, const uint64_t* qpix, const uint64_t* diag, const float* ro)
 {
     e.pm = 0ull;
     e.v = val & VOX_ID_MASK;
@@ -159,11 +163,9 @@ __device__ inline void load_entry(EntryRegs& e, bool valid, uint32_t val, const 
     }
     if (!e.pm) return;
     const size_t v = e.v;
-    e.c0 = centers[3 * v + 0];
-    e.c1 = centers[3 * v + 1];
-    e.c2 = centers[3 * v + 2];
-    e.l = lengths[v];
-    e.linv = 1.f / e.l;
+    const float l = lengths[v];
+    entry_bounds(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], l, ro, e.lo, e.hi, e.cm);
+    e.linv = 1.f / l;
     e.g0 = *reinterpret_cast<const float4*>(geos + 8 * v);
     e.g1 = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
     if constexpr (RGB)
@@ -182,7 +184,13 @@ template <int NW>
 __device__ inline void stage_entry(StageT<NW>& st, int w, int lane, const EntryRegs& e)
 {
     if (!e.pm) return;
-    st.c[0][w][lane] = e.c0; st.c[1][w][lane] = e.c1; st.c[2][w][lane] = e.c2; st.l[w][lane] = e.l;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        st.lo[k][w][lane] = e.lo[k];
+        st.hi[k][w][lane] = e.hi[k];
+        st.cm[k][w][lane] = e.cm[k];
+    }
     st.linv[w][lane] = e.linv;
     st.g[0][w][lane] = e.g0.x; st.g[1][w][lane] = e.g0.y; st.g[2][w][lane] = e.g0.z; st.g[3][w][lane] = e.g0.w;
     st.g[4][w][lane] = e.g1.x; st.g[5][w][lane] = e.g1.y; st.g[6][w][lane] = e.g1.z; st.g[7][w][lane] = e.g1.w;
@@ -232,7 +240,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
         const uint32_t j = base + (uint32_t)lane;
         const bool valid = j < range.y;
         load_entry<!MAX_W, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                           sy1, qpix, st.diag[w]);
+                           sy1, qpix, st.diag[w], ray.ro);
     }
     while (base < range.y)
     {
@@ -256,7 +264,7 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
             const uint32_t j = nbase + (uint32_t)lane;
             const bool valid = j < range.y;
             load_entry<!MAX_W, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0,
-                               sx1, sy1, qpix, st.diag[w]);
+                               sx1, sy1, qpix, st.diag[w], ray.ro);
         }
 
         uint64_t hits = 0ull;
@@ -264,14 +272,15 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
         {
             const int l = __builtin_ctzll(hm);
             hm &= hm - 1ull;
-            const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
+            ray_aabb_pre<EXACT>(st.lo[0][w][l], st.lo[1][w][l], st.lo[2][w][l], st.hi[0][w][l], st.hi[1][w][l],
+                                st.hi[2][w][l], ray, a, b);
             if (a > b) continue;
             float g[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-            const float vol = vox_vol<NS, false>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, nullptr);
+            const float vol = vox_vol_cm<NS, false>(st.cm[0][w][l], st.cm[1][w][l], st.cm[2][w][l], st.linv[w][l], g,
+                                                    a, b, ray, nullptr);
             const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf_core(-vol));
             if (alpha < MIN_ALPHA) continue;
             if constexpr (MASK) hits |= 1ull << l;
@@ -301,12 +310,24 @@ __device__ inline uint32_t composite_subtile(StageT<NW>& st, int w, int lane, co
     return last;
 }
 
-// Staged payload of one entry for the backward, 4 float4 (one 16-byte LDS
-// read each): centre + length, densities 0-3, densities 4-7, colour + 1/length.
+// Staged payload of one entry for the backward, 6 float4 (one 16-byte LDS
+// read each): slab bounds lo (3) and hi (3), sample origin cm (3),
+// 1 / length, colour (3), densities 0-3, densities 4-7 (entry_bounds).
 struct EntryPay
 {
-    float4 q[4];
+    float4 q[6];
 };
+
+__device__ inline void make_pay(EntryPay& p, const float* lo, const float* hi, const float* cm, float linv, float r0,
+                                float r1, float r2, float4 g0, float4 g1)
+{
+    p.q[0] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+    p.q[1] = make_float4(hi[1], hi[2], cm[0], cm[1]);
+    p.q[2] = make_float4(cm[2], linv, r0, r1);
+    p.q[3] = make_float4(r2, 0.f, 0.f, 0.f);
+    p.q[4] = g0;
+    p.q[5] = g1;
+}
 
 // Adjoint of one (pixel, entry) hit (backward.cu:259-527 without the
 // regularizers): recovers T before the entry, updates the running dL/dT and
@@ -321,21 +342,21 @@ template <int NS, bool EXACT>
 __device__ inline bool bwd_hit(const EntryPay& p, const PixRay& ray, float& T, float& last_dT, float dp0, float dp1,
                                float dp2, float* gq, bool active)
 {
-    const float4 a0 = p.q[0], a1 = p.q[1], a2 = p.q[2], a3 = p.q[3];
-    const float c0 = a0.x, c1 = a0.y, c2 = a0.z, len = a0.w;
+    const float4 q0 = p.q[0], q1 = p.q[1], q2 = p.q[2], g0 = p.q[4], g1 = p.q[5];
+    const float r2 = p.q[3].x;
     float a, b;
-    ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
-    const float g[8] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+    ray_aabb_pre<EXACT>(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, ray, a, b);
+    const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
     // NS == 1: dI[i] = dd * w[i] is folded into the gradient scale below
     float dI[8], dd = 1.f;
-    const float vol = (NS == 1) ? vox_vol1_w(c0, c1, c2, len, a3.w, g, a, b, ray, dI, dd)
-                                : vox_vol<NS, true>(c0, c1, c2, len, a3.w, g, a, b, ray, dI);
+    const float vol = (NS == 1) ? vox_vol1_w_cm(q1.z, q1.w, q2.x, q2.y, g, a, b, ray, dI, dd)
+                                : vox_vol_cm<NS, true>(q1.z, q1.w, q2.x, q2.y, g, a, b, ray, dI);
     const float e = svr_expf_core(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     const bool hit = active && !(a > b) && !(alpha < MIN_ALPHA);
     const float Tn = div_refined(T, 1.f - alpha);
     const float pt_w = alpha * Tn;
-    const float dpt = __builtin_fmaf(dp2, a3.z, __builtin_fmaf(dp1, a3.y, dp0 * a3.x));
+    const float dpt = __builtin_fmaf(dp2, r2, __builtin_fmaf(dp1, q2.w, dp0 * q2.z));
     const float diff = dpt - last_dT;
     const float dalpha = Tn * diff;
     const float ldn = __builtin_fmaf(alpha, diff, last_dT);
@@ -395,7 +416,7 @@ struct WindowLDS
 // 1080p scene: 6 beats 2-5, 7 and 8; 7 and more leave room in LDS for only
 // two workgroups per CU).
 #ifndef BWD_WINDOW_CHUNKS
-#define BWD_WINDOW_CHUNKS 6
+#define BWD_WINDOW_CHUNKS 4
 #endif
 constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 
@@ -499,11 +520,11 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
             {
                 const size_t v = val_pf[k] & VOX_ID_MASK;
                 const float len = lengths[v];
-                EntryPay& p = sh.pay[j][l];
-                p.q[0] = make_float4(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], len);
-                p.q[1] = *reinterpret_cast<const float4*>(geos + 8 * v);
-                p.q[2] = *reinterpret_cast<const float4*>(geos + 8 * v + 4);
-                p.q[3] = make_float4(rgbs[3 * v + 0], rgbs[3 * v + 1], rgbs[3 * v + 2], 1.f / len);
+                float lo[3], hi[3], cm[3];
+                entry_bounds(centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2], len, hit_fn.ray.ro, lo, hi, cm);
+                make_pay(sh.pay[j][l], lo, hi, cm, 1.f / len, rgbs[3 * v + 0], rgbs[3 * v + 1], rgbs[3 * v + 2],
+                         *reinterpret_cast<const float4*>(geos + 8 * v),
+                         *reinterpret_cast<const float4*>(geos + 8 * v + 4));
             }
         }
         if (c0 - KW >= 0) prefetch(c0 - KW);

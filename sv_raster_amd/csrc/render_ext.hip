@@ -133,7 +133,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
         const uint32_t j = base + (uint32_t)lane;
         const bool valid = j < range.y;
         load_entry<true, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0, sx1, sy1,
-                   qpix, st.diag[w]);
+                   qpix, st.diag[w], ray.ro);
     }
     while (base < range.y)
     {
@@ -152,7 +152,7 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
             const uint32_t j = nbase + (uint32_t)lane;
             const bool valid = j < range.y;
             load_entry<true, !EXACT>(cur, valid, valid ? vox_list[j] : 0u, bboxes, octs, centers, lengths, geos, rgbs, sx0, sy0, sx1,
-                       sy1, qpix, st.diag[w]);
+                       sy1, qpix, st.diag[w], ray.ro);
         }
 
         uint64_t hits = 0ull;
@@ -160,15 +160,15 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
         {
             const int l = __builtin_ctzll(hm);
             hm &= hm - 1ull;
-            const float c0 = st.c[0][w][l], c1 = st.c[1][w][l], c2 = st.c[2][w][l], len = st.l[w][l];
             float a, b;
-            ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
+            ray_aabb_pre<EXACT>(st.lo[0][w][l], st.lo[1][w][l], st.lo[2][w][l], st.hi[0][w][l], st.hi[1][w][l],
+                                st.hi[2][w][l], ray, a, b);
             if (a > b) continue;
             float g[8], lv[NS];
 #pragma unroll
             for (int i = 0; i < 8; ++i) g[i] = st.g[i][w][l];
-            const float vol = vox_samples<NS, false>(c0, c1, c2, len, st.linv[w][l], g, a, b, ray, lv, nullptr,
-                                                     nullptr);
+            const float vol = vox_samples_cm<NS, false>(st.cm[0][w][l], st.cm[1][w][l], st.cm[2][w][l],
+                                                        st.linv[w][l], g, a, b, ray, lv, nullptr, nullptr);
             const float alpha = fminf(MAX_ALPHA, 1.f - svr_expf_core(-vol));
             if (alpha < MIN_ALPHA) continue;
             hits |= 1ull << l;
@@ -352,21 +352,21 @@ template <int NS, bool EXACT>
 __device__ inline bool bwd_hit_ext(const EntryPay& pe, const PixRay& ray, const ExtCoef& k, ExtPix& p, float& T,
                                    float& last_dT, float dp0, float dp1, float dp2, float* gq)
 {
-    const float4 a0 = pe.q[0], a1 = pe.q[1], a2 = pe.q[2], a3 = pe.q[3];
-    const float c0 = a0.x, c1 = a0.y, c2 = a0.z, len = a0.w;
+    const float4 q0 = pe.q[0], q1 = pe.q[1], q2 = pe.q[2], g0 = pe.q[4], g1 = pe.q[5];
     float a, b;
-    ray_aabb<EXACT>(c0, c1, c2, len, ray, a, b);
+    ray_aabb_pre<EXACT>(q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, ray, a, b);
     if (a > b) return false;
-    const float g[8] = {a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+    const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
     float dI[8], lv[NS], edI[NS][8];
-    const float linv = a3.w;
-    const float vol = vox_samples<NS, true>(c0, c1, c2, len, linv, g, a, b, ray, lv, edI, dI);
+    const float cm[3] = {q1.z, q1.w, q2.x};
+    const float linv = q2.y;
+    const float vol = vox_samples_cm<NS, true>(cm[0], cm[1], cm[2], linv, g, a, b, ray, lv, edI, dI);
     const float e = svr_expf_core(-vol);
     const float alpha = fminf(MAX_ALPHA, 1.f - e);
     if (alpha < MIN_ALPHA) return false;
     T = T / (1.f - alpha);
     const float pt_w = alpha * T;
-    const float col[3] = {a3.x, a3.y, a3.z};
+    const float col[3] = {q2.z, q2.w, pe.q[3].x};
     float dpt = __builtin_fmaf(dp2, col[2], __builtin_fmaf(dp1, col[1], dp0 * col[0]));
     float dg[8];
 #pragma unroll
@@ -436,14 +436,12 @@ __device__ inline bool bwd_hit_ext(const EntryPay& pe, const PixRay& ray, const 
     {
         // backward.cu:476-498: penalise density decreasing along the ray
         // (weights and the segment ends held fixed)
-        const float r = 0.5f * len;
-        const float cc[3] = {c0, c1, c2};
         float qa[3], qb[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            qa[i] = (__builtin_fmaf(a, ray.rd[i], ray.ro[i]) - (cc[i] - r)) * linv;
-            qb[i] = (__builtin_fmaf(b, ray.rd[i], ray.ro[i]) - (cc[i] - r)) * linv;
+            qa[i] = (__builtin_fmaf(a, ray.rd[i], ray.ro[i]) - cm[i]) * linv;
+            qb[i] = (__builtin_fmaf(b, ray.rd[i], ray.ro[i]) - cm[i]) * linv;
         }
         float wa[8], wb[8];
         const float da = density_at(g, qa, wa);

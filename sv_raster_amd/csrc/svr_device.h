@@ -175,6 +175,48 @@ __device__ inline void ray_aabb(float cx_, float cy_, float cz_, float l, const 
     }
 }
 
+// The view-constant part of ray_aabb and of the sample positions for one
+// entry: every ray of a view starts at the camera centre ro, so
+// lo_k = (c_k - ro_k) - r, hi_k = (c_k - ro_k) + r and cm_k = c_k - r
+// (r = l / 2) can be computed once per entry.  The roundings are those of
+// ray_aabb and vox_vol, so ray_aabb_pre and the *_cm forms below give the
+// same bits as the forms that take the centre and length.
+__device__ inline void entry_bounds(float c0, float c1, float c2, float l, const float* ro, float* lo, float* hi,
+                                    float* cm)
+{
+    const float r = 0.5f * l;
+    const float d0 = c0 - ro[0], d1 = c1 - ro[1], d2 = c2 - ro[2];
+    lo[0] = d0 - r; lo[1] = d1 - r; lo[2] = d2 - r;
+    hi[0] = d0 + r; hi[1] = d1 + r; hi[2] = d2 + r;
+    cm[0] = c0 - r; cm[1] = c1 - r; cm[2] = c2 - r;
+}
+
+template <bool EXACT>
+__device__ inline void ray_aabb_pre(float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
+                                    const PixRay& ray, float& a, float& b)
+{
+    const float a0 = lo0 * ray.rd_inv[0], b0 = hi0 * ray.rd_inv[0];
+    const float a1 = lo1 * ray.rd_inv[1], b1 = hi1 * ray.rd_inv[1];
+    const float a2 = lo2 * ray.rd_inv[2], b2 = hi2 * ray.rd_inv[2];
+    if constexpr (EXACT)
+    {
+        const float lo0_ = smin(a0, b0), hi0_ = smax(a0, b0);
+        const float lo1_ = smin(a1, b1), hi1_ = smax(a1, b1);
+        const float lo2_ = smin(a2, b2), hi2_ = smax(a2, b2);
+        a = smax(smax(lo0_, lo1_), lo2_);
+        b = smin(smin(hi0_, hi1_), hi2_);
+    }
+    else
+    {
+        const float ninf = -__builtin_inff(), pinf = __builtin_inff();
+        const float lo0_ = __builtin_amdgcn_fmed3f(a0, b0, ninf), hi0_ = __builtin_amdgcn_fmed3f(a0, b0, pinf);
+        const float lo1_ = __builtin_amdgcn_fmed3f(a1, b1, ninf), hi1_ = __builtin_amdgcn_fmed3f(a1, b1, pinf);
+        const float lo2_ = __builtin_amdgcn_fmed3f(a2, b2, ninf), hi2_ = __builtin_amdgcn_fmed3f(a2, b2, pinf);
+        a = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(lo0_, lo1_, pinf), lo2_, pinf);
+        b = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(hi0_, hi1_, ninf), hi2_, ninf);
+    }
+}
+
 // True when every component of 1/rd is finite (the ray-box slabs cannot be NaN).
 __device__ inline bool ray_inv_finite(const PixRay& r)
 {
@@ -184,16 +226,16 @@ __device__ inline bool ray_inv_finite(const PixRay& r)
 
 // Volume integral of one voxel segment (forward.cu:224-253, backward.cu:283-328).
 // When DI is true, dI[k] receives sum_s 100*step*exp_linear_11'(d_s)*w_k(s).
+// cm_k = c_k - r (entry_bounds).
 template <int NS, bool DI>
-__device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_inv, const float* geo, float a,
-                                float b, const PixRay& ray, float* dI)
+__device__ inline float vox_vol_cm(float cm0, float cm1, float cm2, float l_inv, const float* geo, float a, float b,
+                                   const PixRay& ray, float* dI)
 {
     const float step_sz = (b - a) * inv_nsamp(NS);
     const float t0 = __builtin_fmaf(0.5f, step_sz, a);
-    const float r = 0.5f * l;
-    float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - (c0 - r)) * l_inv;
-    float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - (c1 - r)) * l_inv;
-    float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - (c2 - r)) * l_inv;
+    float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - cm0) * l_inv;
+    float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - cm1) * l_inv;
+    float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - cm2) * l_inv;
     const float qs0 = (step_sz * ray.rd[0]) * l_inv;
     const float qs1 = (step_sz * ray.rd[1]) * l_inv;
     const float qs2 = (step_sz * ray.rd[2]) * l_inv;
@@ -233,18 +275,25 @@ __device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_i
     return vol;
 }
 
+template <int NS, bool DI>
+__device__ inline float vox_vol(float c0, float c1, float c2, float l, float l_inv, const float* geo, float a,
+                                float b, const PixRay& ray, float* dI)
+{
+    const float r = 0.5f * l;
+    return vox_vol_cm<NS, DI>(c0 - r, c1 - r, c2 - r, l_inv, geo, a, b, ray, dI);
+}
+
 // vox_vol<1, true> with the density derivative kept apart: w receives the 8
 // trilinear weights and dd the factor 100*step*exp_linear_11'(d), so that
 // dI[i] = dd * w[i] (the same vol; the backward scales w by dL/dvol * dd once).
-__device__ inline float vox_vol1_w(float c0, float c1, float c2, float l, float l_inv, const float* geo, float a,
-                                   float b, const PixRay& ray, float* w, float& dd)
+__device__ inline float vox_vol1_w_cm(float cm0, float cm1, float cm2, float l_inv, const float* geo, float a,
+                                      float b, const PixRay& ray, float* w, float& dd)
 {
     const float step_sz = b - a;
     const float t0 = __builtin_fmaf(0.5f, step_sz, a);
-    const float r = 0.5f * l;
-    const float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - (c0 - r)) * l_inv;
-    const float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - (c1 - r)) * l_inv;
-    const float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - (c2 - r)) * l_inv;
+    const float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - cm0) * l_inv;
+    const float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - cm1) * l_inv;
+    const float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - cm2) * l_inv;
     const float scale = STEP_SZ_SCALE * step_sz;
     const float wx[2] = {1.f - qt0, qt0};
     const float wy[2] = {1.f - qt1, qt1};
@@ -275,15 +324,14 @@ __device__ inline float div_refined(float x, float y)
 // term and, with DI, edI[k][i] = dd_k * w_k[i] its share of dI[i]
 // (forward.cu:235-253 / backward.cu:300-320 keep them for the depth output).
 template <int NS, bool DI>
-__device__ inline float vox_samples(float c0, float c1, float c2, float l, float l_inv, const float* geo, float a,
-                                    float b, const PixRay& ray, float* lv, float (*edI)[8], float* dI)
+__device__ inline float vox_samples_cm(float cm0, float cm1, float cm2, float l_inv, const float* geo, float a,
+                                       float b, const PixRay& ray, float* lv, float (*edI)[8], float* dI)
 {
     const float step_sz = (b - a) * inv_nsamp(NS);
     const float t0 = __builtin_fmaf(0.5f, step_sz, a);
-    const float r = 0.5f * l;
-    float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - (c0 - r)) * l_inv;
-    float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - (c1 - r)) * l_inv;
-    float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - (c2 - r)) * l_inv;
+    float qt0 = (__builtin_fmaf(t0, ray.rd[0], ray.ro[0]) - cm0) * l_inv;
+    float qt1 = (__builtin_fmaf(t0, ray.rd[1], ray.ro[1]) - cm1) * l_inv;
+    float qt2 = (__builtin_fmaf(t0, ray.rd[2], ray.ro[2]) - cm2) * l_inv;
     const float qs0 = (step_sz * ray.rd[0]) * l_inv;
     const float qs1 = (step_sz * ray.rd[1]) * l_inv;
     const float qs2 = (step_sz * ray.rd[2]) * l_inv;
