@@ -300,48 +300,23 @@ __global__ __launch_bounds__(256) void batch_combine_kernel(int V, int64_t N, co
 constexpr int BSH_PAD = 49;
 
 // One wave per 64 consecutive voxels: their 48 coefficients are staged in LDS
-// with contiguous loads, then each lane evaluates its voxel for every view
-// where it has entries (sh_compute.cu:55-106 arithmetic).
+// with contiguous (16-byte where aligned) loads, then each lane evaluates its
+// voxel for every view where it has entries (sh_compute.cu:55-106 arithmetic).
 __global__ __launch_bounds__(64) void batch_sh_fwd_kernel(int deg, int V, const svr_view* __restrict__ views,
                                                           int64_t N, const float* __restrict__ centers,
                                                           const uint32_t* __restrict__ n_dup,
                                                           const float* __restrict__ sh0,
                                                           const float* __restrict__ shs, float* __restrict__ rgbs)
 {
-    __shared__ float buf[64 * BSH_PAD];
+    __shared__ ShStage st;
     const int lane = threadIdx.x;
     const int64_t v0 = (int64_t)blockIdx.x * 64;
     const int nv = (int)min((int64_t)64, N - v0);
-    // the block's rows are contiguous: all loads in flight before the LDS stores
-    float r0[3], r1[45];
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-    {
-        const int e = lane + 64 * j;
-        r0[j] = e < nv * 3 ? sh0[3 * v0 + e] : 0.f;
-    }
-#pragma unroll
-    for (int j = 0; j < 45; ++j)
-    {
-        const int e = lane + 64 * j;
-        r1[j] = e < nv * 45 ? shs[45 * v0 + e] : 0.f;
-    }
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-    {
-        const int e = lane + 64 * j, k = e / 3, f = e - 3 * k;
-        if (e < nv * 3) buf[k * BSH_PAD + f] = r0[j];
-    }
-#pragma unroll
-    for (int j = 0; j < 45; ++j)
-    {
-        const int e = lane + 64 * j, k = e / 45, f = e - 45 * k;
-        if (e < nv * 45) buf[k * BSH_PAD + 3 + f] = r1[j];
-    }
+    float c[3];
+    sh_stage_rows(st, sh0, shs, centers, v0, nv, lane, c);
     wave_sync();
     if (lane >= nv) return;
     const int64_t v = v0 + lane;
-    const float* row = buf + lane * BSH_PAD;
     for (int u = 0; u < V; ++u)
     {
         const size_t p = (size_t)u * (size_t)N + (size_t)v;
@@ -353,18 +328,11 @@ __global__ __launch_bounds__(64) void batch_sh_fwd_kernel(int deg, int V, const 
         }
         const svr_view& vw = views[u];
         const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
-        float x, y, z, b[16];
-        sh_dir(centers, v, cam, x, y, z);
+        float x, y, z, b[16], res[3];
+        sh_dir_c(c, cam, x, y, z);
         const int nb = sh_basis(deg, x, y, z, b);
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch)
-        {
-            float res = SH_C0 * row[ch];
-            if (nb > 1) res = res - (SH_C1 * y) * row[3 + ch] + (SH_C1 * z) * row[6 + ch] - (SH_C1 * x) * row[9 + ch];
-            for (int k = 4; k < nb; ++k) res = res + b[k] * row[3 * k + ch];
-            res = res + 0.5f;
-            out[ch] = res > 0.f ? res : 0.f;
-        }
+        sh_eval_rows(nb, b, x, y, z, st.s0 + 3 * lane, st.s1 + 45 * lane, res);
+        out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
     }
 }
 

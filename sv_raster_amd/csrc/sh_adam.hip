@@ -64,15 +64,6 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int deg, int64_t cnt, const
 constexpr int SH_F = 48;
 constexpr int SH_PAD = 49;
 
-// Linear staging of one block's 64 rows: s0 holds the 64 x 3 sh0 values,
-// s1 the 64 x 45 shs values, row k at 3 k and 45 k (odd strides: lane k
-// touching row k is bank-conflict free), so that the block's contiguous
-// output spans are plain 16-byte copies out of LDS.
-struct ShStage
-{
-    alignas(16) float s0[64 * 3 + 4];
-    alignas(16) float s1[64 * 45 + 4];
-};
 
 __global__ __launch_bounds__(64) void sh_fwd16_kernel(int deg, int64_t cnt, const int64_t* __restrict__ idx,
                                                       const float* __restrict__ centers,
@@ -175,13 +166,59 @@ __global__ __launch_bounds__(64) void sh_bwd16_kernel(int deg, int64_t cnt, cons
 // 16-byte stores.  The same values as sh_bwd16_kernel over idx after zeroing
 // both outputs (rows listed twice are written once: the reference writes, it
 // does not accumulate), without the separate 12 N + 180 N byte zero fill.
-constexpr uint32_t MARK_BITS = 0x7FC0DEADu;  // a NaN payload no gradient produces
+constexpr uint32_t MARK_BITS = 0x7FC0DEADu;  // a NaN payload no gradient or colour produces
 
 __global__ __launch_bounds__(256) void sh_mark_kernel(int64_t cnt, const int64_t* __restrict__ idx,
                                                       float* __restrict__ dsh0)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cnt) reinterpret_cast<uint32_t*>(dsh0)[3 * idx[i]] = MARK_BITS;
+}
+
+// The degree-3 forward over every voxel, in blocks of 64 consecutive voxels
+// (for lists that cover most of the voxels): each block loads its contiguous
+// 64 x 45 and 64 x 3 coefficient spans with 16-byte loads (all in flight
+// before the LDS stores), each lane evaluates its voxel if it is listed
+// (marked by sh_mark_kernel in the first word of its rgbs row; every voxel
+// when ALL) and the block writes its 64 x 3 output span with 16-byte stores,
+// zeros for the rows not listed.  Same arithmetic as sh_fwd16_kernel.
+template <bool ALL>
+__global__ __launch_bounds__(64) void sh_fwd16_dense_kernel(int deg, int64_t N, const float* __restrict__ centers,
+                                                            const float* __restrict__ cam,
+                                                            const float* __restrict__ sh0,
+                                                            const float* __restrict__ shs, float* __restrict__ rgbs)
+{
+    __shared__ ShStage st;
+    __shared__ alignas(16) float out[64 * 3 + 4];
+    const int lane = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * 64;
+    const int nv = (int)min((int64_t)64, N - v0);
+    const int64_t v = v0 + lane;
+    // the marks (first word of each rgbs row, stride 3) and the coefficients in one batch of loads
+    float c[3];
+    const uint32_t mark = sh_stage_rows(st, sh0, shs, centers, v0, nv, lane, c,
+                                        ALL ? nullptr : reinterpret_cast<const uint32_t*>(rgbs + 3 * v0));
+    const bool listed = lane < nv && (ALL || mark == MARK_BITS);
+    wave_sync();
+    float res[3] = {0.f, 0.f, 0.f};
+    if (listed)
+    {
+        float x, y, z, b[16];
+        sh_dir_c(c, cam, x, y, z);
+        const int nb = sh_basis(deg, x, y, z, b);
+        sh_eval_rows(nb, b, x, y, z, st.s0 + 3 * lane, st.s1 + 45 * lane, res);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) out[3 * lane + ch] = res[ch];
+    wave_sync();
+    if (nv == 64)
+    {
+        if (lane < 48) reinterpret_cast<float4*>(rgbs + 3 * v0)[lane] = reinterpret_cast<const float4*>(out)[lane];
+    }
+    else
+    {
+        for (int e = lane; e < nv * 3; e += 64) rgbs[3 * v0 + e] = out[e];
+    }
 }
 
 __global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, const float* __restrict__ centers,
@@ -449,6 +486,29 @@ int svr_sh_compute(int32_t deg, int64_t n_idx, const int64_t* idx, int64_t N, co
     SVR_REQUIRE(deg >= 0 && deg <= 3 && M >= (deg + 1) * (deg + 1), SVR_ERR_INVALID,
                 "sh degree %d needs %d coefficients, got %d", deg, (deg + 1) * (deg + 1), M);
     if (N == 0) return SVR_OK;
+    const bool aligned = ((((uintptr_t)sh0) | ((uintptr_t)shs) | ((uintptr_t)rgbs)) & 15u) == 0;
+    if (M == 16 && aligned && (n_idx == 0 || 2 * n_idx >= N))
+    {
+        // a list covering at least half the voxels: every voxel's coefficients are read
+        // contiguously, the listed rows marked first
+        prof_begin(PROF_SH_FWD, stream);
+        if (n_idx == 0)
+        {
+            sh_fwd16_dense_kernel<true><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, sh0,
+                                                                                      shs, rgbs);
+        }
+        else
+        {
+            if (fill_u32(rgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
+            sh_mark_kernel<<<(unsigned)((n_idx + 255) / 256), 256, 0, stream>>>(n_idx, idx, rgbs);
+            SVR_CHECK_LAUNCH();
+            sh_fwd16_dense_kernel<false><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, sh0,
+                                                                                       shs, rgbs);
+        }
+        SVR_CHECK_LAUNCH();
+        prof_end(PROF_SH_FWD, stream);
+        return SVR_OK;
+    }
     if (fill_u32(rgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
     const int64_t cnt = n_idx ? n_idx : N;
     if (cnt == 0) return SVR_OK;

@@ -174,6 +174,84 @@ __device__ inline int sh_basis(int deg, float x, float y, float z, float* b)
     return 16;
 }
 
+// Linear staging of a block's 64 coefficient rows (degree 3): s0 holds the
+// 64 x 3 sh0 values, s1 the 64 x 45 shs values, row k at 3 k and 45 k (odd
+// strides: lane k touching row k is bank-conflict free), so that a block's
+// contiguous spans move between global memory and LDS as 16-byte copies.
+struct ShStage
+{
+    alignas(16) float s0[64 * 3 + 4];
+    alignas(16) float s1[64 * 45 + 4];
+};
+
+// Loads the coefficient rows of voxels v0 .. v0 + nv - 1 into st: 16-byte
+// loads (all in flight before the LDS stores) for a full block of 64 with
+// 16-byte aligned arrays, single loads otherwise.  One wave.  In the same
+// batch of loads lane l (l < nv) reads the centre of voxel v0 + l into c (so
+// that no second memory round trip follows) and, with `word` (one word per 3:
+// the first word of each voxel's rgb row), word[3 l], which it returns.
+__device__ inline uint32_t sh_stage_rows(ShStage& st, const float* __restrict__ sh0, const float* __restrict__ shs,
+                                         const float* __restrict__ centers, int64_t v0, int nv, int lane, float* c,
+                                         const uint32_t* __restrict__ word = nullptr)
+{
+    uint32_t wv = 0u;
+    const int64_t vc = v0 + min(lane, nv - 1);
+    c[0] = centers[3 * vc + 0];
+    c[1] = centers[3 * vc + 1];
+    c[2] = centers[3 * vc + 2];
+    if (nv == 64 && ((((uintptr_t)sh0) | ((uintptr_t)shs)) & 15u) == 0)
+    {
+        // unconditional loads (indices clamped into the span), so that the
+        // compiler issues all of them before the first wait; named registers,
+        // not an array, so that nothing goes through private memory
+        const float4* i1 = reinterpret_cast<const float4*>(shs + 45 * v0);
+#define SVR_SH_LD(j) const float4 r##j = i1[min(lane + 64 * j, 719)];
+        SVR_SH_LD(0) SVR_SH_LD(1) SVR_SH_LD(2) SVR_SH_LD(3) SVR_SH_LD(4) SVR_SH_LD(5)
+        SVR_SH_LD(6) SVR_SH_LD(7) SVR_SH_LD(8) SVR_SH_LD(9) SVR_SH_LD(10) SVR_SH_LD(11)
+#undef SVR_SH_LD
+        const float4 q0 = reinterpret_cast<const float4*>(sh0 + 3 * v0)[min(lane, 47)];
+        if (word) wv = word[3 * lane];
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the stores
+        // 720 = 11 x 64 + 16: eleven full rounds and a 16-lane tail
+        float4* s1 = reinterpret_cast<float4*>(st.s1);
+#define SVR_SH_ST(j) s1[lane + 64 * j] = r##j;
+        SVR_SH_ST(0) SVR_SH_ST(1) SVR_SH_ST(2) SVR_SH_ST(3) SVR_SH_ST(4) SVR_SH_ST(5)
+        SVR_SH_ST(6) SVR_SH_ST(7) SVR_SH_ST(8) SVR_SH_ST(9) SVR_SH_ST(10)
+#undef SVR_SH_ST
+        if (lane < 16) s1[lane + 704] = r11;
+        if (lane < 48) reinterpret_cast<float4*>(st.s0)[lane] = q0;
+    }
+    else
+    {
+        if (word && lane < nv) wv = word[3 * lane];
+        for (int e = lane; e < nv * 45; e += 64) st.s1[e] = shs[45 * v0 + e];
+        for (int e = lane; e < nv * 3; e += 64) st.s0[e] = sh0[3 * v0 + e];
+    }
+    return wv;
+}
+
+// rgb of one voxel from its staged rows (sh_compute.cu:55-106 arithmetic).
+__device__ inline void sh_eval_rows(int nb, const float* b, float x, float y, float z, const float* r0,
+                                    const float* r1, float* res)
+{
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+    {
+        float a = SH_C0 * r0[ch];
+        if (nb > 1) a = a - (SH_C1 * y) * r1[ch] + (SH_C1 * z) * r1[3 + ch] - (SH_C1 * x) * r1[6 + ch];
+        for (int k = 4; k < nb; ++k) a = a + b[k] * r1[3 * (k - 1) + ch];
+        a = a + 0.5f;
+        res[ch] = a > 0.f ? a : 0.f;
+    }
+}
+
+__device__ inline void sh_dir_c(const float* c, const float* __restrict__ cam, float& x, float& y, float& z)
+{
+    const float dx = c[0] - cam[0], dy = c[1] - cam[1], dz = c[2] - cam[2];
+    const float rn = 1.f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    x = dx * rn; y = dy * rn; z = dz * rn;
+}
+
 __device__ inline void sh_dir(const float* __restrict__ centers, int64_t v, const float* __restrict__ cam, float& x,
                               float& y, float& z)
 {

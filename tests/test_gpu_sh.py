@@ -2,7 +2,9 @@
 (sh_compute.cu restated), for index lists that exercise every staging path of
 the degree-3 kernels: runs of 64 consecutive ids at aligned and misaligned
 starts (16-byte span copies), runs ending at the last voxel, gaps, unsorted
-ids and duplicates (row gathers), and all voxels (no index list)."""
+ids and duplicates (row gathers), lists of most voxels (the dense pass over
+marked rows, sorted and unsorted with duplicates), and all voxels (no index
+list)."""
 import numpy as np
 import pytest
 
@@ -19,6 +21,9 @@ def _idx_cases(N, rng):
         "gaps": np.sort(rng.choice(N, N // 3, replace=False)),
         "unsorted": perm[: N // 2],
         "duplicates": dup,
+        # lists covering at least half of the voxels take the dense path (listed rows marked)
+        "most_sorted": np.sort(rng.choice(N, (4 * N) // 5, replace=False)),
+        "most_unsorted_duplicates": np.concatenate([perm[: (3 * N) // 4], perm[:50]]),
         "all": None,
     }
 
