@@ -602,6 +602,8 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     SVR_REQUIRE(fill_u32(img.ctr, 0u, 8, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
     const int pos_bits = (int)required_bits((uint64_t)8 * (N > 0 ? N : 1));
     const int tile_bits = (int)required_bits((uint64_t)T);
+    SVR_REQUIRE(radix_passes(tile_bits) <= SORT_FRAME_PASSES, SVR_ERR_INVALID, "%lld tiles are too many to sort",
+                (long long)T);
     const bool capacity_mode = overflow != nullptr;
     const uint32_t cap = capacity_mode ? (uint32_t)R : 0xFFFFFFFFu;
     const uint32_t* R_dev = capacity_mode ? g.counters + 4 : nullptr;

@@ -50,7 +50,7 @@ struct BatchImageState
         s.order = c.take<uint32_t>(T);
         s.order_alt = c.take<uint32_t>(T);
         s.exact_list = c.take<uint32_t>(T);
-        s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
+        s.qcounts = c.take<uint32_t>(sort_scratch_words(T, 1));
         s.ctr = c.take<uint32_t>(8);
         s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
         if (bytes) *bytes = c.size();

@@ -11,23 +11,48 @@
 // pair, ties by (v, q) exactly as the reference's stable sort orders them).
 // Per frame the key is (tile << pos_bits) | pos, which sorts into exactly the
 // reference order with 30..40 significant bits instead of 76.
+#include <algorithm>
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include "voxel_ops.h"
 #include "svr_host.h"
 
 namespace svr {
 
 
+// Lanes of `active` whose digit d (of DBITS bits) equals this lane's.
+template <int DBITS>
 __device__ inline uint64_t match_digit(uint32_t d, uint64_t active)
 {
     uint64_t m = active;
 #pragma unroll
-    for (int b = 0; b < RADIX_BITS; ++b)
+    for (int b = 0; b < DBITS; ++b)
     {
         const uint32_t bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
         m &= bit ? bal : ~bal;
     }
     return m;
+}
+
+// Adds one count per lane to h[d] with one LDS atomic per run of equal
+// digits on consecutive lanes (the keys arrive with long runs of one tile,
+// which as single-lane atomics would all queue on one LDS address).  The
+// valid lanes must be a prefix of the wave.
+__device__ inline void add_digit_runs(uint32_t* h, uint32_t d, bool valid, int lane)
+{
+    const uint32_t prev = __shfl_up(d, 1, 64);
+    const bool head = valid && (lane == 0 || prev != d);
+    const uint64_t heads = __ballot(head);
+    const uint64_t vmask = __ballot(valid);
+    if (head)
+    {
+        const uint64_t above = heads & ~((2ull << lane) - 1ull);
+        const int end = above ? __ffsll((unsigned long long)above) - 1 : __popcll(vmask);
+        atomicAdd(&h[d], (uint32_t)(end - lane));
+    }
 }
 
 // Number of keys to sort: n, or the smaller count a previous kernel left at
@@ -39,199 +64,430 @@ __device__ inline int64_t sort_count(int64_t n, const uint32_t* __restrict__ n_d
     return d < n ? d : n;
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void radix_upsweep(const uint64_t* __restrict__ keys, int64_t n_cap,
-                                                              const uint32_t* __restrict__ n_dev, int shift,
-                                                              uint32_t dmask, uint32_t* __restrict__ counts,
-                                                              int64_t nb)
-{
-    const int64_t n = sort_count(n_cap, n_dev);
-    __shared__ uint32_t hist[SORT_THREADS / 64][RADIX];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-#pragma unroll
-    for (int i = 0; i < SORT_THREADS / 64; ++i) hist[i][t] = 0;
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * SORT_TILE + (int64_t)w * (SORT_ITEMS * 64);
-    for (int r = 0; r < SORT_ITEMS; ++r)
-    {
-        const int64_t idx = base + r * 64 + lane;
-        const bool valid = idx < n;
-        const uint64_t active = __ballot(valid);
-        if (active == 0) break;
-        const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & dmask : 0u;
-        const uint64_t m = match_digit(d, active);
-        if (valid && lane == 63 - __clzll(m)) hist[w][d] += (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-    uint32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < SORT_THREADS / 64; ++i) c += hist[i][t];
-    counts[(int64_t)t * nb + blockIdx.x] = c;
-}
+// ---- onesweep LSD radix sort ----------------------------------------------
+//
+// One launch counts the digits of every pass (one read of the keys); then
+// each pass is one launch in which a workgroup takes tiles of SORT_TILE keys,
+// counts a tile's digits and publishes the counts, finds the counts of the
+// tiles before it by a two-level decoupled look-back, ranks the tile's keys
+// by digit (stable) and scatters them.  Tiles form groups of SORT_LB_GROUP:
+// a tile sums the counts of the tiles before it in its group (all published
+// at about the same time, one round of loads), and the last tile of a group
+// publishes the group's total (FLAG_AGG), then its running total over all
+// groups (FLAG_PREFIX) once it has looked back over the groups before it;
+// every tile looks back over the groups the same way, newest first, until a
+// group with a running total.  A chained look-back over single tiles would
+// make the last of a few hundred tiles wait some tens of dependent rounds of
+// cross-XCD loads; over groups it is a few.
+//
+// The pass grid is no larger than the number of its workgroups the device
+// holds at once, and workgroup b takes tiles b, b + grid, b + 2 grid, ... in
+// that order.  A tile waits only on tiles with smaller ids (its group's
+// earlier tiles, and the last tiles of earlier groups), so the unfinished
+// tile with the smallest id never waits and runs to its end: every wait
+// ends.  (No tile counter is needed for that, and a counter every workgroup
+// adds to would be a serial chain of device-scope atomics on one address.)
+// Each wait is also bounded (LOOKBACK_SPIN_LIMIT polls, then the error word
+// of the scratch is set and the wait gives up) so that a broken invariant
+// cannot stall the device.
 
-// Exclusive scan of each digit row of counts[RADIX][nb]; row totals -> counts[RADIX*nb + d].
-__global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* __restrict__ counts, int64_t nb)
+constexpr uint32_t FLAG_AGG = 1u, FLAG_PREFIX = 2u;
+constexpr uint32_t LOOKBACK_SPIN_LIMIT = 1u << 22;
+constexpr int LOOKBACK_WINDOW = 4;
+constexpr int SORT_HIST_GRID = 2048;
+constexpr int SORT_RANK_CHAINS = 1;
+
+struct SortScratch
 {
-    __shared__ uint32_t lds[4];
-    __shared__ uint32_t carry_s;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t* row = counts + (int64_t)blockIdx.x * nb;
-    if (t == 0) carry_s = 0;
-    __syncthreads();
-    for (int64_t off = 0; off < nb; off += 256)
+    uint32_t* err;      // [1]
+    uint32_t* hist;     // [passes][SORT_HIST_COPIES][RADIX]
+    uint64_t* status;   // [passes][nb][RADIX]
+    uint64_t* gstatus;  // [passes][ngroups][RADIX]
+    int64_t words;      // words in use (cleared before the sort)
+    static SortScratch carve(uint32_t* p, int64_t n, int passes)
     {
-        const int64_t i = off + t;
-        const uint32_t x = i < nb ? row[i] : 0u;
-        uint32_t s = x;
+        SortScratch s;
+        const int64_t nb = sort_blocks(n > 0 ? n : 1);
+        const int64_t hist_words = (int64_t)passes * SORT_HIST_COPIES * RADIX;
+        s.err = p;
+        s.hist = p + SORT_HEADER_WORDS;
+        s.status = reinterpret_cast<uint64_t*>(p + SORT_HEADER_WORDS + hist_words);
+        s.gstatus = s.status + (int64_t)passes * nb * RADIX;
+        s.words = SORT_HEADER_WORDS + hist_words + 2 * (int64_t)passes * (nb + sort_lb_groups(n)) * RADIX;
+        return s;
+    }
+};
+
+// Digit counts of every pass in one read of the keys: pass p covers bits
+// [lo_bit + p * step, lo_bit + min((p + 1) * step, nbits)).  Workgroup b
+// counts tiles b, b + grid, ... in LDS and adds them into histogram copy
+// b % SORT_HIST_COPIES.
+__global__ __launch_bounds__(SORT_THREADS) void onesweep_hist_kernel(const uint64_t* __restrict__ keys, int64_t n_cap,
+                                                                     const uint32_t* __restrict__ n_dev, int lo_bit,
+                                                                     int step, int nbits, int passes,
+                                                                     uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[SORT_MAX_PASSES][RADIX];
+    const int64_t n = sort_count(n_cap, n_dev);
+    const int t = threadIdx.x;
+    for (int p = 0; p < passes; ++p) h[p][t] = 0u;
+    __syncthreads();
+    for (int64_t b0 = (int64_t)blockIdx.x * SORT_TILE; b0 < n; b0 += (int64_t)gridDim.x * SORT_TILE)
+    {
+        // every load of the tile in flight at once, then the counting
+        uint64_t k[SORT_ITEMS];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1)
+        for (int r = 0; r < SORT_ITEMS; ++r)
         {
-            const uint32_t y = __shfl_up(s, o, 64);
-            if (lane >= o) s += y;
+            const int64_t i = b0 + r * SORT_THREADS + t;
+            k[r] = i < n ? keys[i] : 0ull;
         }
-        if (lane == 63) lds[w] = s;
-        __syncthreads();
-        uint32_t pre = 0;
-        for (int k = 0; k < w; ++k) pre += lds[k];
-        const uint32_t tot = lds[0] + lds[1] + lds[2] + lds[3];
-        const uint32_t carry = carry_s;
-        if (i < nb) row[i] = carry + pre + s - x;
-        __syncthreads();
-        if (t == 0) carry_s = carry + tot;
-        __syncthreads();
+        for (int p = 0; p < passes; ++p)
+        {
+            const int sh = lo_bit + p * step;
+            const uint32_t mask = (1u << min(step, nbits - p * step)) - 1u;
+#pragma unroll
+            for (int r = 0; r < SORT_ITEMS; ++r)
+                add_digit_runs(h[p], (uint32_t)(k[r] >> sh) & mask, b0 + r * SORT_THREADS + t < n, t & 63);
+        }
     }
-    if (t == 0) counts[(int64_t)RADIX * nb + blockIdx.x] = carry_s;
+    __syncthreads();
+    uint32_t* out = hist + (blockIdx.x % SORT_HIST_COPIES) * RADIX;
+    for (int p = 0; p < passes; ++p)
+        if (h[p][t]) atomicAdd(&out[p * SORT_HIST_COPIES * RADIX + t], h[p][t]);
 }
 
-// Ranks each key of the block (stable within its digit), reorders the block's
-// keys in LDS by digit, then writes each digit's run to its global position
-// with consecutive threads on consecutive addresses (coalesced stores instead
-// of one scattered 12-byte store per key).
-template <bool HAS_VALS>
-__global__ __launch_bounds__(SORT_THREADS) void radix_downsweep(
+// A status word carries its flag and its count together, and nothing else is
+// read on the strength of it, so relaxed device-scope atomics suffice: they
+// are coherent across the XCDs' L2s without the L2 write-back / invalidate
+// that release / acquire ordering would add to every poll.
+__device__ inline uint64_t status_load(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void status_store(uint64_t* p, uint32_t flag, uint32_t value)
+{
+    __hip_atomic_store(p, ((uint64_t)flag << 32) | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of digit t's counts of tiles [first, first + r) (r < SORT_LB_GROUP),
+// every load in flight at once; words not published yet are polled again.
+__device__ inline uint32_t group_sum(const uint64_t* __restrict__ status, int64_t first, int r, int t,
+                                     uint32_t* __restrict__ err)
+{
+    uint32_t sum = 0;
+    uint32_t pending = (1u << r) - 1u;
+    uint32_t spins = 0;
+    while (pending)
+    {
+        uint64_t sw[SORT_LB_GROUP - 1];
+#pragma unroll
+        for (int q = 0; q < SORT_LB_GROUP - 1; ++q)
+            sw[q] = (pending >> q) & 1u ? status_load(status + (first + q) * RADIX + t) : 0ull;
+#pragma unroll
+        for (int q = 0; q < SORT_LB_GROUP - 1; ++q)
+            if (((pending >> q) & 1u) && (sw[q] >> 32) != 0u)
+            {
+                sum += (uint32_t)sw[q];
+                pending &= ~(1u << q);
+            }
+        if (pending)
+        {
+            if (++spins > LOOKBACK_SPIN_LIMIT)
+            {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return sum;
+}
+
+// Exclusive count of digit t over the entries before `tile` (tile > 0) of a
+// status array whose entry 0 publishes FLAG_PREFIX: newest first,
+// LOOKBACK_WINDOW predecessors per round (their loads in flight together); a
+// round consumes words until a prefix (done), an entry that has not
+// published yet (poll again from it) or the end of the window.
+__device__ inline uint32_t look_back(const uint64_t* __restrict__ status, int64_t tile, int t,
+                                     uint32_t* __restrict__ err)
+{
+    uint32_t excl = 0;
+    int64_t j = tile - 1;
+    uint32_t spins = 0;
+    bool done = false;
+    while (!done)
+    {
+        uint64_t sw[LOOKBACK_WINDOW];
+#pragma unroll
+        for (int q = 0; q < LOOKBACK_WINDOW; ++q)
+            sw[q] = j - q >= 0 ? status_load(status + (j - q) * RADIX + t) : 0ull;
+        int used = 0;
+        bool stalled = false;
+#pragma unroll
+        for (int q = 0; q < LOOKBACK_WINDOW; ++q)
+        {
+            if (done || stalled || j - q < 0) continue;
+            const uint32_t flag = (uint32_t)(sw[q] >> 32);
+            if (flag == 0u)
+            {
+                stalled = true;
+                continue;
+            }
+            excl += (uint32_t)sw[q];
+            ++used;
+            done = flag == FLAG_PREFIX || j - q == 0;
+        }
+        j -= used;
+        if (!done && stalled)
+        {
+            if (++spins > LOOKBACK_SPIN_LIMIT)
+            {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return excl;
+}
+
+// One pass: for each of the workgroup's tiles, rank the tile's keys by digit
+// (stable; wave w ranks its 64 x SORT_ITEMS keys in order, then the waves in
+// order), find the digit's global offset (the pass's exclusive digit prefix +
+// the counts of earlier tiles by look-back), reorder the tile's keys by digit
+// in LDS and write each digit's run with consecutive threads on consecutive
+// addresses.
+template <bool HAS_VALS, int DBITS>
+__global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift, uint32_t dmask,
-    const uint32_t* __restrict__ counts, int64_t nb)
+    uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
+    const uint32_t* __restrict__ hist, uint64_t* __restrict__ status, uint64_t* __restrict__ gstatus,
+    uint32_t* __restrict__ err)
 {
     const int64_t n = sort_count(n_cap, n_dev);
+    constexpr uint32_t dmask = (1u << DBITS) - 1u;
     constexpr int NW = SORT_THREADS / 64;
-    __shared__ uint32_t wcnt[NW][RADIX];
-    __shared__ uint32_t gbase[RADIX];  // global start of this block's run of digit d
-    __shared__ uint32_t lbase[RADIX];  // start of digit d in the block's reordered keys
-    __shared__ uint32_t wsum[NW];
+    constexpr int NC = SORT_RANK_CHAINS;
+    // per wave, SORT_RANK_CHAINS running digit counts: the wave ranks
+    // consecutive slices of its keys as independent chains of LDS
+    // read-modify-writes (interleaved, so that their latencies overlap)
+    __shared__ uint32_t wcnt[NW][NC][RADIX];
+    __shared__ uint32_t gbase[RADIX];  // global start of this tile's run of digit d
+    __shared__ uint32_t lbase[RADIX];  // start of digit d in the tile's reordered keys
+    __shared__ uint32_t wsum[NW], wsum2[NW];
+    __shared__ uint32_t tcnt[RADIX];  // the tile's digit counts
     __shared__ uint64_t lkeys[SORT_TILE];  // reordered keys, then (as 32-bit words) values
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-
-    // digit base = exclusive prefix over digit totals + this block's row offset
-    {
-        const uint32_t tot = counts[(int64_t)RADIX * nb + t];
-        uint32_t s = tot;
+    // the pass's digit count (digit = thread) over the histogram copies
+    uint32_t tot = 0;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1)
-        {
-            const uint32_t y = __shfl_up(s, o, 64);
-            if (lane >= o) s += y;
-        }
-        if (lane == 63) wsum[w] = s;
-        __syncthreads();
-        uint32_t pre = 0;
-        for (int k = 0; k < w; ++k) pre += wsum[k];
-        gbase[t] = pre + s - tot + counts[(int64_t)t * nb + blockIdx.x];
-#pragma unroll
-        for (int i = 0; i < NW; ++i) wcnt[i][t] = 0;
-    }
-    __syncthreads();
-
-    const int64_t blk0 = (int64_t)blockIdx.x * SORT_TILE;
-    const int64_t base = blk0 + (int64_t)w * (SORT_ITEMS * 64);
-    uint64_t k[SORT_ITEMS];
-    uint32_t v[SORT_ITEMS];
-    uint32_t rk[SORT_ITEMS];
-#pragma unroll
-    for (int r = 0; r < SORT_ITEMS; ++r)
+    for (int c = 0; c < SORT_HIST_COPIES; ++c) tot += hist[c * RADIX + t];
+    const int64_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
     {
-        const int64_t idx = base + r * 64 + lane;
-        const bool valid = idx < n;
-        k[r] = valid ? keys_in[idx] : 0ull;
-        if (HAS_VALS) v[r] = valid ? vals_in[idx] : 0u;
-        const uint64_t active = __ballot(valid);
-        const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
-        const uint64_t m = match_digit(d, active);
-        const uint32_t before = (uint32_t)__popcll(m & lanemask_lt(lane));
-        uint32_t cnt = 0;
-        if (valid) cnt = wcnt[w][d];
-        rk[r] = cnt + before;
-        if (valid && lane == 63 - __clzll(m)) wcnt[w][d] = cnt + (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-    // per digit: exclusive offsets of the waves, and the block's digit total
-    uint32_t dtot = 0;
+        const int64_t blk0 = tile * SORT_TILE;
+        __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
-    for (int i = 0; i < NW; ++i)
-    {
-        const uint32_t c = wcnt[i][t];
-        wcnt[i][t] = dtot;
-        dtot += c;
-    }
-    // exclusive scan of the digit totals over the block (digit = thread)
-    {
-        uint32_t s = dtot;
+        for (int i = 0; i < NW; ++i)
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1)
-        {
-            const uint32_t y = __shfl_up(s, o, 64);
-            if (lane >= o) s += y;
-        }
-        __syncthreads();
-        if (lane == 63) wsum[w] = s;
-        __syncthreads();
-        uint32_t pre = 0;
-        for (int q = 0; q < w; ++q) pre += wsum[q];
-        lbase[t] = pre + s - dtot;
-    }
-    __syncthreads();
-    uint32_t lp[SORT_ITEMS];
-#pragma unroll
-    for (int r = 0; r < SORT_ITEMS; ++r)
-    {
-        const int64_t idx = base + r * 64 + lane;
-        const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
-        lp[r] = lbase[d] + wcnt[w][d] + rk[r];
-        if (idx < n) lkeys[lp[r]] = k[r];
-    }
-    __syncthreads();
-    const int nk = (int)min((int64_t)SORT_TILE, n - blk0);
-    // the digit of reordered position i, kept for the values pass
-    uint32_t dd[SORT_ITEMS];
-#pragma unroll
-    for (int r = 0; r < SORT_ITEMS; ++r)
-    {
-        const int i = t + r * SORT_THREADS;
-        dd[r] = 0;
-        if (i < nk)
-        {
-            const uint64_t key = lkeys[i];
-            const uint32_t d = (uint32_t)(key >> shift) & dmask;
-            dd[r] = d;
-            keys_out[gbase[d] + (uint32_t)i - lbase[d]] = key;
-        }
-    }
-    if (HAS_VALS)
-    {
-        uint32_t* lv = reinterpret_cast<uint32_t*>(lkeys);
-        __syncthreads();
+            for (int h = 0; h < NC; ++h) wcnt[i][h][t] = 0;
+        tcnt[t] = 0;
+        const int64_t base = blk0 + (int64_t)w * (SORT_ITEMS * 64);
+        uint64_t k[SORT_ITEMS];
+        uint32_t v[SORT_ITEMS];
+        uint32_t rk[SORT_ITEMS];
 #pragma unroll
         for (int r = 0; r < SORT_ITEMS; ++r)
         {
             const int64_t idx = base + r * 64 + lane;
-            if (idx < n) lv[lp[r]] = v[r];
+            const bool valid = idx < n;
+            k[r] = valid ? keys_in[idx] : 0ull;
+            if (HAS_VALS) v[r] = valid ? vals_in[idx] : 0u;
         }
         __syncthreads();
+        // the tile's digit counts first, so that they are published (and
+        // successors can look back past this tile) before the ranking
+#pragma unroll
+        for (int r = 0; r < SORT_ITEMS; ++r)
+            add_digit_runs(tcnt, (uint32_t)(k[r] >> shift) & dmask, base + r * 64 + lane < n, lane);
+        __syncthreads();
+        const uint32_t dtot = tcnt[t];
+        // digits above dmask never occur: their counts stay zero and take no
+        // part in the look-back (a 6-bit pass polls a quarter of the words)
+        const bool live = (uint32_t)t <= dmask;
+        uint32_t excl = 0;
+        if (live)
+        {
+            status_store(status + tile * RADIX + t, FLAG_AGG, dtot);
+            const int64_t g = tile / SORT_LB_GROUP;
+            const int r = (int)(tile - g * SORT_LB_GROUP);
+            const uint32_t in_group = r > 0 ? group_sum(status, tile - r, r, t, err) : 0u;
+            uint32_t before = 0;
+            if (r == SORT_LB_GROUP - 1)
+            {
+                const uint32_t total = in_group + dtot;
+                status_store(gstatus + g * RADIX + t, g == 0 ? FLAG_PREFIX : FLAG_AGG, total);
+                if (g > 0)
+                {
+                    before = look_back(gstatus, g, t, err);
+                    status_store(gstatus + g * RADIX + t, FLAG_PREFIX, before + total);
+                }
+            }
+            else if (g > 0)
+                before = look_back(gstatus, g, t, err);
+            excl = before + in_group;
+        }
+        static_assert(SORT_ITEMS % NC == 0, "ranking chains of equal length");
+        constexpr int SLICE = SORT_ITEMS / NC;
+#pragma unroll
+        for (int r2 = 0; r2 < SORT_ITEMS; ++r2)
+        {
+            const int h = r2 % NC;                // chain
+            const int r = (r2 / NC) + h * SLICE;  // item
+            const int64_t idx = base + r * 64 + lane;
+            const bool valid = idx < n;
+            const uint64_t active = __ballot(valid);
+            const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
+            const uint64_t m = match_digit<DBITS>(d, active);
+            const uint32_t before = (uint32_t)__popcll(m & lanemask_lt(lane));
+            uint32_t cnt = 0;
+            if (valid) cnt = wcnt[w][h][d];
+            rk[r] = cnt + before;
+            if (valid && lane == 63 - __clzll(m)) wcnt[w][h][d] = cnt + (uint32_t)__popcll(m);
+        }
+        // a chain's keys follow all keys of the chains before it
+#pragma unroll
+        for (int r = SLICE; r < SORT_ITEMS; ++r)
+        {
+            const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
+#pragma unroll
+            for (int h = 0; h < r / SLICE; ++h) rk[r] += wcnt[w][h][d];
+        }
+        __syncthreads();
+        // per digit (thread t = digit): exclusive offsets of the waves
+        {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+            {
+                uint32_t c = 0;
+#pragma unroll
+                for (int h = 0; h < NC; ++h) c += wcnt[i][h][t];
+                wcnt[i][0][t] = acc;
+                acc += c;
+            }
+        }
+        // the pass's exclusive digit prefix and the tile's digit starts
+        {
+            uint32_t s = tot, l = dtot;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const uint32_t y = __shfl_up(s, o, 64);
+                const uint32_t z = __shfl_up(l, o, 64);
+                if (lane >= o)
+                {
+                    s += y;
+                    l += z;
+                }
+            }
+            if (lane == 63)
+            {
+                wsum[w] = s;
+                wsum2[w] = l;
+            }
+            __syncthreads();
+            uint32_t pre = 0, pre_l = 0;
+            for (int q = 0; q < w; ++q)
+            {
+                pre += wsum[q];
+                pre_l += wsum2[q];
+            }
+            gbase[t] = pre + s - tot + excl;
+            lbase[t] = pre_l + l - dtot;
+        }
+        __syncthreads();
+        uint32_t lp[SORT_ITEMS];
+#pragma unroll
+        for (int r = 0; r < SORT_ITEMS; ++r)
+        {
+            const int64_t idx = base + r * 64 + lane;
+            const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
+            lp[r] = lbase[d] + wcnt[w][0][d] + rk[r];
+            if (idx < n) lkeys[lp[r]] = k[r];
+        }
+        __syncthreads();
+        const int nk = (int)min((int64_t)SORT_TILE, n - blk0);
+        // the digit of reordered position i, kept for the values pass
+        uint32_t dd[SORT_ITEMS];
 #pragma unroll
         for (int r = 0; r < SORT_ITEMS; ++r)
         {
             const int i = t + r * SORT_THREADS;
-            if (i < nk) vals_out[gbase[dd[r]] + (uint32_t)i - lbase[dd[r]]] = lv[i];
+            dd[r] = 0;
+            if (i < nk)
+            {
+                const uint64_t key = lkeys[i];
+                const uint32_t d = (uint32_t)(key >> shift) & dmask;
+                dd[r] = d;
+                keys_out[gbase[d] + (uint32_t)i - lbase[d]] = key;
+            }
         }
+        if (HAS_VALS)
+        {
+            uint32_t* lv = reinterpret_cast<uint32_t*>(lkeys);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < SORT_ITEMS; ++r)
+            {
+                const int64_t idx = base + r * 64 + lane;
+                if (idx < n) lv[lp[r]] = v[r];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < SORT_ITEMS; ++r)
+            {
+                const int i = t + r * SORT_THREADS;
+                if (i < nk) vals_out[gbase[dd[r]] + (uint32_t)i - lbase[dd[r]]] = lv[i];
+            }
+        }
+    }
+}
+
+// Workgroups of `kernel` (SORT_THREADS threads each) the current device
+// holds at once, queried once per (device, kernel).
+static int64_t resident_blocks(const void* kernel)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<int, const void*>, int64_t>> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& e : cache)
+        if (e.first.first == dev && e.first.second == kernel) return e.second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, SORT_THREADS, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    cache.push_back({{dev, kernel}, (int64_t)per_cu * cus});
+    return (int64_t)per_cu * cus;
+}
+
+template <bool HAS_VALS>
+static const void* pass_kernel(int bits)
+{
+    switch (bits)
+    {
+        case 1: return (const void*)onesweep_pass_kernel<HAS_VALS, 1>;
+        case 2: return (const void*)onesweep_pass_kernel<HAS_VALS, 2>;
+        case 3: return (const void*)onesweep_pass_kernel<HAS_VALS, 3>;
+        case 4: return (const void*)onesweep_pass_kernel<HAS_VALS, 4>;
+        case 5: return (const void*)onesweep_pass_kernel<HAS_VALS, 5>;
+        case 6: return (const void*)onesweep_pass_kernel<HAS_VALS, 6>;
+        case 7: return (const void*)onesweep_pass_kernel<HAS_VALS, 7>;
+        default: return (const void*)onesweep_pass_kernel<HAS_VALS, 8>;
     }
 }
 
@@ -248,28 +504,35 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
     SVR_REQUIRE(lo_bit >= 0 && lo_bit + nbits <= 64, SVR_ERR_INVALID, "radix sort bits [%d, %d) out of range", lo_bit,
                 lo_bit + nbits);
     const int64_t nb = sort_blocks(n);
-    SVR_REQUIRE(nb < (1ll << 31), SVR_ERR_INVALID, "radix sort of %lld keys too large", (long long)n);
+    SVR_REQUIRE(n <= (int64_t)UINT32_MAX, SVR_ERR_INVALID, "radix sort of %lld keys too large", (long long)n);
+    const int passes = radix_passes(nbits);
+    SVR_REQUIRE(passes <= SORT_MAX_PASSES, SVR_ERR_INVALID, "radix sort of %d bits needs too many passes", nbits);
+    const int step = (nbits + passes - 1) / passes;
+    SortScratch sc = SortScratch::carve(counts, n, passes);
+    if (fill_u32(counts, 0u, sc.words, stream) != SVR_OK) return SVR_ERR_HIP;
+    onesweep_hist_kernel<<<(unsigned)std::min<int64_t>(nb, SORT_HIST_GRID), SORT_THREADS, 0, stream>>>(
+        keys, n, n_dev, lo_bit, step, nbits, passes, sc.hist);
+    SVR_CHECK_LAUNCH();
     uint64_t* kin = keys; uint64_t* kout = keys_alt;
     uint32_t* vin = vals; uint32_t* vout = vals_alt;
-    // radix_passes(nbits) passes with the bits spread evenly over them
-    const int passes = radix_passes(nbits);
-    const int step = (nbits + passes - 1) / passes;
-    for (int done = 0; done < nbits; done += step)
+    for (int p = 0; p < passes; ++p)
     {
+        const int done = p * step;
         const int shift = lo_bit + done;
         const int bits = (nbits - done) < step ? (nbits - done) : step;
-        const uint32_t dmask = (1u << bits) - 1u;
-        radix_upsweep<<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, n, n_dev, shift, dmask, counts, nb);
-        SVR_CHECK_LAUNCH();
-        radix_scan_rows<<<RADIX, 256, 0, stream>>>(counts, nb);
-        SVR_CHECK_LAUNCH();
-        if (vals)
-            radix_downsweep<true><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, vin, vout, n, n_dev, shift,
-                                                                             dmask, counts, nb);
-        else
-            radix_downsweep<false><<<(unsigned)nb, SORT_THREADS, 0, stream>>>(kin, kout, nullptr, nullptr, n, n_dev,
-                                                                              shift, dmask, counts, nb);
-        SVR_CHECK_LAUNCH();
+        uint64_t* status = sc.status + (int64_t)p * nb * RADIX;
+        uint64_t* gstatus = sc.gstatus + (int64_t)p * sort_lb_groups(n) * RADIX;
+        const uint32_t* hist = sc.hist + (int64_t)p * SORT_HIST_COPIES * RADIX;
+        const void* kernel = vals ? pass_kernel<true>(bits) : pass_kernel<false>(bits);
+        const int64_t resident = resident_blocks(kernel);
+        SVR_REQUIRE(resident > 0, SVR_ERR_HIP, "radix sort occupancy query failed");
+        const unsigned grid = (unsigned)std::min(nb, resident);
+        const uint64_t* kin_c = kin;
+        const uint32_t* vin_c = vin;
+        void* args[] = {(void*)&kin_c, (void*)&kout,  (void*)&vin_c,  (void*)&vout,    (void*)&n,
+                        (void*)&n_dev, (void*)&shift, (void*)&hist,   (void*)&status,  (void*)&gstatus,
+                        (void*)&sc.err};
+        SVR_CHECK_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(SORT_THREADS), args, 0, stream));
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }
@@ -333,7 +596,7 @@ struct OrderScratch
         s.keys_alt = c.take<uint64_t>(n8);
         s.vals = c.take<uint32_t>(n8);
         s.vals_alt = c.take<uint32_t>(n8);
-        s.counts = c.take<uint32_t>(RADIX * sort_blocks(n8 > 0 ? n8 : 1) + RADIX);
+        s.counts = c.take<uint32_t>(sort_scratch_words(n8, SORT_MAX_PASSES));
         s.misc = c.take<uint32_t>(4);
         if (bytes) *bytes = c.size();
         return s;
@@ -635,6 +898,8 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     }
     const int pos_bits = (int)required_bits((uint64_t)8 * N);
     const int tile_bits = (int)required_bits((uint64_t)T);
+    SVR_REQUIRE(radix_passes(tile_bits) <= SORT_FRAME_PASSES, SVR_ERR_INVALID, "%lld tiles are too many to sort",
+                (long long)T);
     int rc = pairs_ready ? SVR_OK : pair_prepare(N, g, ot, stream);
     if (rc != SVR_OK) return rc;
     // entries are emitted in rank order, so the sort only orders the tile
@@ -702,6 +967,10 @@ int svr_build_order_table(int64_t N, const int64_t* paths, void* order_table_buf
     uint32_t* vo = nullptr;
     int rc = radix_sort_ex(s.keys, s.keys_alt, s.vals, s.vals_alt, n8, key_bits, s.counts, stream, &ko, &vo);
     if (rc != SVR_OK) return rc;
+    uint32_t sort_err = 0;
+    SVR_CHECK_HIP(hipMemcpyAsync(&sort_err, s.counts, sizeof(sort_err), hipMemcpyDeviceToHost, stream));
+    SVR_CHECK_HIP(hipStreamSynchronize(stream));
+    SVR_REQUIRE(sort_err == 0u, SVR_ERR_HIP, "order sort look-back timed out");
     order_scatter_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, stream>>>(vo, n8, ot.pos, ot.pair);
     SVR_CHECK_LAUNCH();
     const uint32_t meta[4] = {required_bits((uint64_t)n8), (uint32_t)key_bits, (uint32_t)finest, 0u};

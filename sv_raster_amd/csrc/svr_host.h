@@ -188,6 +188,26 @@ constexpr int RADIX = 1 << RADIX_BITS;
 
 inline int64_t sort_blocks(int64_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
 
+// Scratch of the onesweep radix sort (radix_sort_ex) of n keys in at most
+// max_passes passes, in 32-bit words: a header (word 0: look-back error
+// flag), SORT_HIST_COPIES partial digit histograms of every pass (workgroups
+// add into copy blockIdx % SORT_HIST_COPIES, so that no counter takes every
+// workgroup's atomic), then the look-back status of every (pass, tile, digit)
+// and of every (pass, group of SORT_LB_GROUP tiles, digit), one 64-bit word
+// each (flag << 32 | count).  Cleared by radix_sort_ex itself.
+constexpr int SORT_HEADER_WORDS = 64;
+constexpr int SORT_MAX_PASSES = 8;
+constexpr int SORT_HIST_COPIES = 8;
+constexpr int SORT_LB_GROUP = 16;
+inline int64_t sort_lb_groups(int64_t n) { return (sort_blocks(n > 0 ? n : 1) + SORT_LB_GROUP - 1) / SORT_LB_GROUP; }
+inline int64_t sort_scratch_words(int64_t n, int max_passes)
+{
+    return SORT_HEADER_WORDS + (int64_t)max_passes * RADIX *
+                                   (SORT_HIST_COPIES + 2 * sort_blocks(n > 0 ? n : 1) + 2 * sort_lb_groups(n));
+}
+// Passes of the per-frame sorts: tile ids of at most 32 bits in 8-bit digits.
+constexpr int SORT_FRAME_PASSES = 4;
+
 // Per-frame binning state: sort keys (double buffer), the sorted entry list.
 struct BinningState
 {
@@ -196,18 +216,17 @@ struct BinningState
     uint32_t* vox_list; // [R] (q << 29) | vox_id in sorted order
     uint32_t* orig;     // [R] emission-order position of each sorted entry
     uint32_t* orig_alt; // [R]
-    uint32_t* counts;   // [RADIX * blocks + RADIX] radix-sort digit counts
+    uint32_t* counts;   // [sort_scratch_words(R, SORT_FRAME_PASSES)] radix-sort scratch
     static BinningState carve(void* p, int64_t R, size_t* bytes)
     {
         Carver c(p);
         BinningState b;
-        const int64_t nb = sort_blocks(R > 0 ? R : 1);
         b.keys = c.take<uint64_t>(R);
         b.keys_alt = c.take<uint64_t>(R);
         b.vox_list = c.take<uint32_t>(R);
         b.orig = c.take<uint32_t>(R);
         b.orig_alt = c.take<uint32_t>(R);
-        b.counts = c.take<uint32_t>(RADIX * nb + RADIX);
+        b.counts = c.take<uint32_t>(sort_scratch_words(R, SORT_FRAME_PASSES));
         if (bytes) *bytes = c.size();
         return b;
     }
@@ -249,7 +268,7 @@ struct ImageState
         s.order_alt = c.take<uint32_t>(T);
         s.qkeys = c.take<uint64_t>(T);
         s.qkeys_alt = c.take<uint64_t>(T);
-        s.qcounts = c.take<uint32_t>(RADIX * sort_blocks(T > 0 ? T : 1) + RADIX);
+        s.qcounts = c.take<uint32_t>(sort_scratch_words(T, 1));
         s.ctr = c.take<uint32_t>(8);
         s.exact_list = c.take<uint32_t>(T);
         s.hit_masks = c.take<uint64_t>(hit_mask_slots(R, T) * 256);
@@ -263,10 +282,12 @@ __host__ __device__ inline int tiles_y(int H) { return (H + 15) / 16; }
 
 // Stable LSD radix sort of 64-bit keys (optionally with 32-bit values) over
 // bits [lo_bit, lo_bit + nbits), in radix_passes(nbits) passes of at most
-// RADIX_BITS bits.  The sorted data ends in the buffers returned through
-// out_keys / out_vals; counts needs RADIX * sort_blocks(n) + RADIX words.
-// With n_dev, the kernels sort min(n, *n_dev) keys (a 64-bit count stored as
-// two 32-bit words on the device); n then only sizes the launch.
+// RADIX_BITS bits: one histogram launch for every pass, then one onesweep
+// launch per pass (decoupled look-back).  The sorted data ends in the buffers
+// returned through out_keys / out_vals; counts needs
+// sort_scratch_words(n, radix_passes(nbits)) words.  With n_dev, the kernels
+// sort min(n, *n_dev) keys (a 64-bit count stored as two 32-bit words on the
+// device); n then only sizes the launch.
 inline int radix_passes(int nbits) { return (nbits + RADIX_BITS - 1) / RADIX_BITS; }
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
