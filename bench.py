@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--config", default="B")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--fwd-iters", type=int, default=50)
+    ap.add_argument("--kernel-iters", type=int, default=20,
+                    help="frames per compositor in the back-to-back launch timing (the roofline's launch duration)")
     ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps on all host threads in the CPU "
                                                              "baseline (median; 0: skip the baseline)")
     ap.add_argument("--cpu-single-steps", type=int, default=5, help="timed oracle steps on one thread (median)")
@@ -59,24 +61,35 @@ def parse():
     return ap.parse_args()
 
 
-def load_traffic(kernel, config, n_vox, width, height):
-    """HBM bytes per dispatch of `kernel` on this workload, from the committed rocprofv3 PMC summary
-    (profiles/<kernel>_<config>_traffic.txt, written from FETCH_SIZE and WRITE_SIZE passes over this same
-    bench command and scaled by a copy kernel of known size).  None when no summary matches the workload."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_traffic.txt")
+def load_profile(kernel, config, n_vox, width, height, entries):
+    """The committed rocprofv3 counter summary of `kernel` on this workload
+    (profiles/<kernel>_<config>_counters.txt: FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known
+    size, and the SQ counter passes, over the bench command it names).  Returns (traffic bytes per dispatch,
+    extra fields, note).  The traffic is used only when the summary's frame traversed the same number of
+    tile entries as this one to within 1 %; otherwise it is None and the note says why (the per-entry figure is
+    still reported)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_counters.txt")
     if not os.path.exists(path):
-        return None
+        return None, {}, "no counter summary for this kernel and workload"
     fields = {}
     for line in open(path):
         if ":" in line and not line.startswith((" ", "#", "|")):
             k, v = line.split(":", 1)
             fields[k.strip()] = v.strip()
     if fields.get("workload") != f"{config} {n_vox} {width}x{height}":
-        return None
+        return None, {}, "counter summary is for another workload"
     try:
-        return int(fields["traffic_bytes_per_dispatch"])
+        traffic = int(fields["traffic_bytes_per_dispatch"])
+        e_prof = int(fields["entries_traversed_E"])
     except (KeyError, ValueError):
-        return None
+        return None, {}, "counter summary lacks traffic or E"
+    extra = {"traffic_bytes_per_entry": traffic / e_prof, "profile_E": e_prof}
+    for k in ("valu_busy_frac", "lds_bank_conflict_frac", "occupancy_waves_per_simd"):
+        if k in fields:
+            extra[k] = float(fields[k])
+    if abs(e_prof - entries) > 0.01 * entries:
+        return None, extra, f"counter frame traversed {e_prof} entries, this one {entries} (> 1 % apart)"
+    return traffic, extra, None
 
 
 def progress(msg):
@@ -468,6 +481,22 @@ def main():
                                                False, False, None, None)
         N_h = int((dg != 0).any(1).sum().item())
         N_v = int((n_dup > 0).sum().item())
+        # launch durations of the compositors with the queue kept full (frames back to back with no host
+        # wait in between), so that a stage's events bracket its kernels only: the duration the roofline uses,
+        # and the one a kernel trace of this command reports for these kernels
+        _lib.check(L.svr_profile_enable(1), "profile")
+        for _ in range(args.kernel_iters):
+            _C.rasterize_voxels(1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w, 0.0, False,
+                                False, False, False, octpath, centers, lengths, geos, rgbs, geom)
+        for _ in range(args.kernel_iters):
+            _C.rasterize_voxels_backward(R, 1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w,
+                                         0.0, octpath, centers, lengths, geos, rgbs, geom, binning, img, T_img, dcol,
+                                         None, None, torch.zeros_like(T_img), 0.0, None, 0.0, 0.0, False, False,
+                                         None, None)
+        torch.cuda.synchronize()
+        kst = collect_stages(L)
+        _lib.check(L.svr_profile_enable(0), "profile")
+    kernel_ms = {k: kst[k][0] / kst[k][1] for k in ("render_fwd", "render_bwd")}
     P = W * H
     T_tiles = ((W + 15) // 16) * ((H + 15) // 16)
     bytes_fwd = 72 * E + 20 * P + 12 * T_tiles
@@ -483,10 +512,11 @@ def main():
     stages_gbs = {k: stage_bytes[k] / (stage_ms_step[k] * 1e-3) / 1e9 if stage_ms_step[k] else None
                   for k in STAGES}
     step_bytes = sum(stage_bytes.values())
-    dom = "render_bwd" if stage_ms["render_bwd"] >= stage_ms["render_fwd"] else "render_fwd"
+    dom = "render_bwd" if kernel_ms["render_bwd"] >= kernel_ms["render_fwd"] else "render_fwd"
+    other = "render_fwd" if dom == "render_bwd" else "render_bwd"
     dom_bytes = bytes_bwd if dom == "render_bwd" else bytes_fwd
-    achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    traffic = load_traffic(dom, args.config, N, W, H)
+    achieved = dom_bytes / (kernel_ms[dom] * 1e-3) / 1e9
+    traffic, prof_extra, traffic_note = load_profile(dom, args.config, N, W, H, E)
 
     batched = None
     if args.batch_config.lower() != "none":
@@ -532,13 +562,17 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes,
-                     "avg_launch_ms": stage_ms[dom],
+                     "avg_launch_ms": kernel_ms[dom],
+                     "launches_timed": args.kernel_iters,
+                     "entries_traversed_E": E, "voxels_hit_N_h": N_h,
+                     "traffic_note": traffic_note, **prof_extra,
+                     "other_compositor": other,
+                     "other_compositor_ms": kernel_ms[other],
                      "other_compositor_gbs": (bytes_fwd if dom == "render_bwd" else bytes_bwd) /
-                                             (stage_ms["render_fwd" if dom == "render_bwd" else "render_bwd"]
-                                              * 1e-3) / 1e9},
+                                             (kernel_ms[other] * 1e-3) / 1e9},
         "stages_gbs_algorithmic": stages_gbs,
         "step_gbs_algorithmic": step_bytes / (ms_per_step * 1e-3) / 1e9,
-        "tile_entry_pixel_pairs_per_s_fwd": E * 256 / (stage_ms["render_fwd"] * 1e-3),
+        "tile_entry_pixel_pairs_per_s_fwd": E * 256 / (kernel_ms["render_fwd"] * 1e-3),
         "cpu_baseline": cpu,
         "batched": batched,
         "batched_large": large,
