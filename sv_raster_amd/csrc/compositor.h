@@ -13,7 +13,8 @@ namespace svr {
 // Sync-slot threshold of the backward: an entry used by at least this many
 // pixels of a wave is summed across the wave by a butterfly instead of
 // per-lane LDS atomics (measured on the 1080p scene with 6-chunk windows:
-// 8 beats 6, 10, 12 and 16).
+// 8 beats 6, 10, 12 and 16; in the batch kernel 8 also beats 4 and 16 with
+// windows of 2 and 4 chunks on 16 views).
 #ifndef BWD_SYNC_MIN_MASKED
 #define BWD_SYNC_MIN_MASKED 8
 #endif
@@ -412,9 +413,11 @@ struct WindowLDS
     uint32_t orow[KW][64];
     uint32_t nch[16];  // per wave: chunks that hold its pixels' contributors
 };
-// Chunks per window of the backward (measured with the window prefetch on the
-// 1080p scene: 6 beats 2-5, 7 and 8; 7 and more leave room in LDS for only
-// two workgroups per CU).
+// Chunks per window of the single-view backward.  With the branch-free hit
+// walk and view-constant staging, 4 was the fastest of 2, 3, 4 and 5 on the
+// 1080p scene (sync threshold 8); before them, 6 beat 2-5, 7 and 8.  Windows of
+// 7 and more leave room in LDS for only two workgroups per CU.  The batch
+// kernels take their depth from the batch (autotuned).
 #ifndef BWD_WINDOW_CHUNKS
 #define BWD_WINDOW_CHUNKS 4
 #endif
