@@ -270,29 +270,97 @@ __global__ __launch_bounds__(TS::THREADS) void batch_fwd_exact_kernel(
     }
 }
 
-// Per voxel: gradients of the shared parameters summed over the views in
-// view order (the order in which a per-view training loop accumulates them).
-__global__ __launch_bounds__(256) void batch_combine_kernel(int V, int64_t N, const float* __restrict__ pair_geo,
-                                                            const float* __restrict__ pair_sub,
-                                                            float* __restrict__ dgeos, float* __restrict__ dsub)
+__device__ inline void add_row_batch(float (&a)[12], const float* __restrict__ partial, uint32_t row)
+{
+    const float4* r = reinterpret_cast<const float4*>(partial + 12 * (size_t)row);
+    const float4 p0 = r[0], p1 = r[1], p2 = r[2];
+    a[0] += p0.x; a[1] += p0.y; a[2] += p0.z; a[3] += p0.w;
+    a[4] += p1.x; a[5] += p1.y; a[6] += p1.z; a[7] += p1.w;
+    a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
+}
+
+// Per voxel, over the views in view order (the order in which a per-view
+// training loop accumulates them): each (view, voxel) pair's gradient is the
+// sum of its entries' partial rows (quadrant runs ascending, each run in
+// emission order; pairs with more than BIG_DUP entries were summed by
+// grad_reduce_big_launch into pair_geo / drgbs / pair_sub), its colour
+// gradient goes to drgbs (zeros for a pair without entries), and the density
+// gradient and subdivision priority are summed over the views.  One pass:
+// no per-pair density rows are written and read back.  A run ending beyond
+// `cap` was never emitted and counts as zero.
+__global__ __launch_bounds__(256) void batch_reduce_kernel(int V, GeomState g, int64_t N, uint32_t cap,
+                                                           const float* __restrict__ partial,
+                                                           const float* __restrict__ pair_geo,
+                                                           const float* __restrict__ pair_sub,
+                                                           float* __restrict__ drgbs, float* __restrict__ dgeos,
+                                                           float* __restrict__ dsub)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
-    float4 a0 = reinterpret_cast<const float4*>(pair_geo + 8 * v)[0];
-    float4 a1 = reinterpret_cast<const float4*>(pair_geo + 8 * v)[1];
-    float s = pair_sub[v];
-    for (int u = 1; u < V; ++u)
+    float s[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s[i] = 0.f;
+    // views in batches of VB: the counts, quadrant bits and first run offsets
+    // of a batch are loaded together (each round's loads in flight at once),
+    // so a voxel's chain of dependent loads is per batch, not per view
+    constexpr int VB = 8;
+    for (int u0 = 0; u0 < V; u0 += VB)
     {
-        const size_t p = (size_t)u * (size_t)N + (size_t)v;
-        const float4 b0 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[0];
-        const float4 b1 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[1];
-        a0.x += b0.x; a0.y += b0.y; a0.z += b0.z; a0.w += b0.w;
-        a1.x += b1.x; a1.y += b1.y; a1.z += b1.z; a1.w += b1.w;
-        s += pair_sub[p];
+        uint32_t nd[VB], bits[VB], off0[VB];
+#pragma unroll
+        for (int j = 0; j < VB; ++j)
+        {
+            const size_t p = (size_t)min(u0 + j, V - 1) * (size_t)N + (size_t)v;
+            nd[j] = u0 + j < V ? g.n_dup[p] : 0u;
+            bits[j] = g.qbits[p];
+        }
+#pragma unroll
+        for (int j = 0; j < VB; ++j)
+        {
+            const size_t p = (size_t)(u0 + j) * (size_t)N + (size_t)v;
+            off0[j] = (nd[j] > 0 && nd[j] <= BIG_DUP) ? run_offset(g, p, bits[j], (uint32_t)__builtin_ctz(bits[j]))
+                                                       : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < VB; ++j)
+        {
+            if (u0 + j >= V) break;
+            const size_t p = (size_t)(u0 + j) * (size_t)N + (size_t)v;
+            if (nd[j] > BIG_DUP)
+            {
+                const float4 b0 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[0];
+                const float4 b1 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[1];
+                s[0] += b0.x; s[1] += b0.y; s[2] += b0.z; s[3] += b0.w;
+                s[4] += b1.x; s[5] += b1.y; s[6] += b1.z; s[7] += b1.w;
+                s[8] += pair_sub[p];
+                continue;
+            }
+            float a[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) a[i] = 0.f;
+            if (nd[j] > 0)
+            {
+                const uint32_t tiles = nd[j] / (uint32_t)__popc(bits[j]);
+                bool first = true;
+                for (uint32_t b = bits[j]; b; b &= b - 1u)
+                {
+                    const uint32_t off = first ? off0[j] : run_offset(g, p, bits[j], (uint32_t)__builtin_ctz(b));
+                    first = false;
+                    if ((uint64_t)off + tiles > cap) continue;
+                    for (uint32_t e = 0; e < tiles; ++e) add_row_batch(a, partial, off + e);
+                }
+            }
+            drgbs[3 * p + 0] = a[8];
+            drgbs[3 * p + 1] = a[9];
+            drgbs[3 * p + 2] = a[10];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s[i] += a[i];
+            s[8] += a[11];
+        }
     }
-    reinterpret_cast<float4*>(dgeos + 8 * v)[0] = a0;
-    reinterpret_cast<float4*>(dgeos + 8 * v)[1] = a1;
-    dsub[v] = s;
+    reinterpret_cast<float4*>(dgeos + 8 * v)[0] = make_float4(s[0], s[1], s[2], s[3]);
+    reinterpret_cast<float4*>(dgeos + 8 * v)[1] = make_float4(s[4], s[5], s[6], s[7]);
+    dsub[v] = s[8];
 }
 
 // ---- SH colour of all views ----------------------------------------------
@@ -317,22 +385,33 @@ __global__ __launch_bounds__(64) void batch_sh_fwd_kernel(int deg, int V, const 
     wave_sync();
     if (lane >= nv) return;
     const int64_t v = v0 + lane;
-    for (int u = 0; u < V; ++u)
+    // the counts of a batch of views are loaded together (one round trip per batch)
+    constexpr int VB = 16;
+    for (int u0 = 0; u0 < V; u0 += VB)
     {
-        const size_t p = (size_t)u * (size_t)N + (size_t)v;
-        float* out = rgbs + 3 * p;
-        if (n_dup[p] == 0)
+        uint32_t nd[VB];
+#pragma unroll
+        for (int j = 0; j < VB; ++j) nd[j] = n_dup[(size_t)min(u0 + j, V - 1) * (size_t)N + (size_t)v];
+#pragma unroll
+        for (int j = 0; j < VB; ++j)
         {
-            out[0] = 0.f; out[1] = 0.f; out[2] = 0.f;
-            continue;
+            const int u = u0 + j;
+            if (u >= V) break;
+            const size_t p = (size_t)u * (size_t)N + (size_t)v;
+            float* out = rgbs + 3 * p;
+            if (nd[j] == 0)
+            {
+                out[0] = 0.f; out[1] = 0.f; out[2] = 0.f;
+                continue;
+            }
+            const svr_view& vw = views[u];
+            const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
+            float x, y, z, b[16], res[3];
+            sh_dir_c(c, cam, x, y, z);
+            const int nb = sh_basis(deg, x, y, z, b);
+            sh_eval_rows(nb, b, x, y, z, st.s0 + 3 * lane, st.s1 + 45 * lane, res);
+            out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
         }
-        const svr_view& vw = views[u];
-        const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
-        float x, y, z, b[16], res[3];
-        sh_dir_c(c, cam, x, y, z);
-        const int nb = sh_basis(deg, x, y, z, b);
-        sh_eval_rows(nb, b, x, y, z, st.s0 + 3 * lane, st.s1 + 45 * lane, res);
-        out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
     }
 }
 
@@ -354,23 +433,41 @@ __global__ __launch_bounds__(64) void batch_sh_bwd_kernel(int deg, int V, const 
         float a[48];
 #pragma unroll
         for (int k = 0; k < 48; ++k) a[k] = 0.f;
-        for (int u = 0; u < V; ++u)
+        const float c[3] = {centers[3 * v + 0], centers[3 * v + 1], centers[3 * v + 2]};
+        // the counts and colour gradients of a batch of views are loaded
+        // together (one round trip per batch)
+        constexpr int VB = 8;
+        for (int u0 = 0; u0 < V; u0 += VB)
         {
-            const size_t p = (size_t)u * (size_t)N + (size_t)v;
-            if (n_dup[p] == 0) continue;
-            const svr_view& vw = views[u];
-            const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
-            float x, y, z, b[16];
-            sh_dir(centers, v, cam, x, y, z);
-            const int nb = sh_basis(deg, x, y, z, b);
+            uint32_t nd[VB];
+            float gr[VB][3];
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch)
+            for (int j = 0; j < VB; ++j)
             {
-                const float g = dL_drgbs[3 * p + ch];
-                a[ch] += SH_C0 * g;
+                const size_t p = (size_t)min(u0 + j, V - 1) * (size_t)N + (size_t)v;
+                nd[j] = u0 + j < V ? n_dup[p] : 0u;
+                gr[j][0] = dL_drgbs[3 * p + 0];
+                gr[j][1] = dL_drgbs[3 * p + 1];
+                gr[j][2] = dL_drgbs[3 * p + 2];
+            }
 #pragma unroll
-                for (int k = 1; k < 16; ++k)
-                    if (k < nb) a[3 * k + ch] += b[k] * g;
+            for (int j = 0; j < VB; ++j)
+            {
+                if (nd[j] == 0) continue;
+                const svr_view& vw = views[u0 + j];
+                const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
+                float x, y, z, b[16];
+                sh_dir_c(c, cam, x, y, z);
+                const int nb = sh_basis(deg, x, y, z, b);
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch)
+                {
+                    const float g = gr[j][ch];
+                    a[ch] += SH_C0 * g;
+#pragma unroll
+                    for (int k = 1; k < 16; ++k)
+                        if (k < nb) a[3 * k + ch] += b[k] * g;
+                }
             }
         }
         float* row = buf + lane * BSH_PAD;
@@ -697,19 +794,19 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
     prof_begin(PROF_GRAD_REDUCE, stream);
     if (R > 0)
     {
-        rc = grad_reduce_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, wk.huge, stream,
-                                capacity_mode ? (uint32_t)R : 0xFFFFFFFFu);
+        const uint32_t cap = capacity_mode ? (uint32_t)R : 0xFFFFFFFFu;
+        rc = grad_reduce_big_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, wk.huge, stream, cap);
         if (rc != SVR_OK) return rc;
+        batch_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(V, g, N, cap, wk.partial, wk.pair_geo,
+                                                                             wk.pair_sub, drgbs, dgeos, dsub);
+        SVR_CHECK_LAUNCH();
     }
     else
     {
-        if (fill_u32(wk.pair_geo, 0u, 8 * VN, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(dgeos, 0u, 8 * N, stream) != SVR_OK) return SVR_ERR_HIP;
         if (fill_u32(drgbs, 0u, 3 * VN, stream) != SVR_OK) return SVR_ERR_HIP;
-        if (fill_u32(wk.pair_sub, 0u, VN, stream) != SVR_OK) return SVR_ERR_HIP;
+        if (fill_u32(dsub, 0u, N, stream) != SVR_OK) return SVR_ERR_HIP;
     }
-    batch_combine_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(V, N, wk.pair_geo, wk.pair_sub, dgeos,
-                                                                          dsub);
-    SVR_CHECK_LAUNCH();
     prof_end(PROF_GRAD_REDUCE, stream);
     return SVR_OK;
 }

@@ -357,16 +357,22 @@ __global__ __launch_bounds__(64) void grad_reduce_huge_kernel(GeomState g, int64
     }
 }
 
-int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs, float* dsub,
-                       float* huge_rows, hipStream_t stream, uint32_t cap)
+int grad_reduce_big_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
+                           float* dsub, float* huge_rows, hipStream_t stream, uint32_t cap)
 {
-    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
-    SVR_CHECK_LAUNCH();
     grad_reduce_big_kernel<<<HUGE_SPLIT_ROWS, 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub, huge_rows);
     SVR_CHECK_LAUNCH();
     grad_reduce_huge_kernel<<<64, 64, 0, stream>>>(g, N, huge_rows, dgeos, drgbs, dsub);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
+}
+
+int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs, float* dsub,
+                       float* huge_rows, hipStream_t stream, uint32_t cap)
+{
+    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
+    SVR_CHECK_LAUNCH();
+    return grad_reduce_big_launch(N, g, partial, dgeos, drgbs, dsub, huge_rows, stream, cap);
 }
 
 template <int NS>

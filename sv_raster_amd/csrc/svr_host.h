@@ -350,6 +350,10 @@ inline float* huge_rows(float* partial, int64_t R)
 
 int grad_reduce_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
                        float* dsub, float* huge_rows, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
+// The items with more than BIG_DUP entries only (the queue of the duplication
+// pass): their sums go to dgeos / drgbs / dsub at the item's index.
+int grad_reduce_big_launch(int64_t n_pairs, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
+                           float* dsub, float* huge_rows, hipStream_t stream, uint32_t cap = 0xFFFFFFFFu);
 
 // Argument checks, state carving and (forward) binning shared by the plain and
 // the extended compositor entry points.  backward_setup returns 1 after
