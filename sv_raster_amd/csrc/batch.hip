@@ -285,9 +285,16 @@ __device__ inline void add_row_batch(float (&a)[12], const float* __restrict__ p
 // emission order; pairs with more than BIG_DUP entries were summed by
 // grad_reduce_big_launch into pair_geo / drgbs / pair_sub), its colour
 // gradient goes to drgbs (zeros for a pair without entries), and the density
-// gradient and subdivision priority are summed over the views.  One pass:
-// no per-pair density rows are written and read back.  A run ending beyond
-// `cap` was never emitted and counts as zero.
+// gradient and subdivision priority are summed over the views.  A workgroup
+// takes RED_VOX consecutive voxels and RED_VIEWS views at a time, one pair
+// per thread (so every pair's chain of dependent loads runs in parallel); the
+// pairs' density terms meet in LDS, where the first RED_VOX threads add them
+// to their voxel's running sums in view order.  No per-pair density rows are
+// written and read back.  A run ending beyond `cap` was never emitted and
+// counts as zero.
+constexpr int RED_VOX = 32, RED_VIEWS = 8;
+static_assert(RED_VOX * RED_VIEWS == 256, "one pair per thread");
+
 __global__ __launch_bounds__(256) void batch_reduce_kernel(int V, GeomState g, int64_t N, uint32_t cap,
                                                            const float* __restrict__ partial,
                                                            const float* __restrict__ pair_geo,
@@ -295,123 +302,117 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(int V, GeomState g, i
                                                            float* __restrict__ drgbs, float* __restrict__ dgeos,
                                                            float* __restrict__ dsub)
 {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
-    float s[9];
+    __shared__ float part[RED_VIEWS][9][RED_VOX];
+    const int t = threadIdx.x, xi = t % RED_VOX, vj = t / RED_VOX;
+    const int64_t v = (int64_t)blockIdx.x * RED_VOX + xi;
+    float run[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) s[i] = 0.f;
-    // views in batches of VB: the counts, quadrant bits and first run offsets
-    // of a batch are loaded together (each round's loads in flight at once),
-    // so a voxel's chain of dependent loads is per batch, not per view
-    constexpr int VB = 8;
-    for (int u0 = 0; u0 < V; u0 += VB)
+    for (int i = 0; i < 9; ++i) run[i] = 0.f;
+    for (int u0 = 0; u0 < V; u0 += RED_VIEWS)
     {
-        uint32_t nd[VB], bits[VB], off0[VB];
+        const int u = u0 + vj;
+        float a[12];
 #pragma unroll
-        for (int j = 0; j < VB; ++j)
+        for (int i = 0; i < 12; ++i) a[i] = 0.f;
+        if (v < N && u < V)
         {
-            const size_t p = (size_t)min(u0 + j, V - 1) * (size_t)N + (size_t)v;
-            nd[j] = u0 + j < V ? g.n_dup[p] : 0u;
-            bits[j] = g.qbits[p];
-        }
-#pragma unroll
-        for (int j = 0; j < VB; ++j)
-        {
-            const size_t p = (size_t)(u0 + j) * (size_t)N + (size_t)v;
-            off0[j] = (nd[j] > 0 && nd[j] <= BIG_DUP) ? run_offset(g, p, bits[j], (uint32_t)__builtin_ctz(bits[j]))
-                                                       : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < VB; ++j)
-        {
-            if (u0 + j >= V) break;
-            const size_t p = (size_t)(u0 + j) * (size_t)N + (size_t)v;
-            if (nd[j] > BIG_DUP)
+            const size_t p = (size_t)u * (size_t)N + (size_t)v;
+            const uint32_t nd = g.n_dup[p];
+            if (nd > BIG_DUP)
             {
                 const float4 b0 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[0];
                 const float4 b1 = reinterpret_cast<const float4*>(pair_geo + 8 * p)[1];
-                s[0] += b0.x; s[1] += b0.y; s[2] += b0.z; s[3] += b0.w;
-                s[4] += b1.x; s[5] += b1.y; s[6] += b1.z; s[7] += b1.w;
-                s[8] += pair_sub[p];
-                continue;
+                a[0] = b0.x; a[1] = b0.y; a[2] = b0.z; a[3] = b0.w;
+                a[4] = b1.x; a[5] = b1.y; a[6] = b1.z; a[7] = b1.w;
+                a[11] = pair_sub[p];
             }
-            float a[12];
-#pragma unroll
-            for (int i = 0; i < 12; ++i) a[i] = 0.f;
-            if (nd[j] > 0)
+            else
             {
-                const uint32_t tiles = nd[j] / (uint32_t)__popc(bits[j]);
-                bool first = true;
-                for (uint32_t b = bits[j]; b; b &= b - 1u)
+                if (nd > 0)
                 {
-                    const uint32_t off = first ? off0[j] : run_offset(g, p, bits[j], (uint32_t)__builtin_ctz(b));
-                    first = false;
-                    if ((uint64_t)off + tiles > cap) continue;
-                    for (uint32_t e = 0; e < tiles; ++e) add_row_batch(a, partial, off + e);
+                    const uint32_t bits = g.qbits[p];
+                    const uint32_t tiles = nd / (uint32_t)__popc(bits);
+                    for (uint32_t b = bits; b; b &= b - 1u)
+                    {
+                        const uint32_t off = run_offset(g, p, bits, (uint32_t)__builtin_ctz(b));
+                        if ((uint64_t)off + tiles > cap) continue;
+                        for (uint32_t e = 0; e < tiles; ++e) add_row_batch(a, partial, off + e);
+                    }
                 }
+                drgbs[3 * p + 0] = a[8];
+                drgbs[3 * p + 1] = a[9];
+                drgbs[3 * p + 2] = a[10];
             }
-            drgbs[3 * p + 0] = a[8];
-            drgbs[3 * p + 1] = a[9];
-            drgbs[3 * p + 2] = a[10];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) s[i] += a[i];
-            s[8] += a[11];
         }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) part[vj][c][xi] = a[c];
+        part[vj][8][xi] = a[11];
+        __syncthreads();
+        if (vj == 0)
+        {
+            const int nk = min(RED_VIEWS, V - u0);
+            for (int k = 0; k < nk; ++k)
+#pragma unroll
+                for (int c = 0; c < 9; ++c) run[c] += part[k][c][xi];
+        }
+        __syncthreads();
     }
-    reinterpret_cast<float4*>(dgeos + 8 * v)[0] = make_float4(s[0], s[1], s[2], s[3]);
-    reinterpret_cast<float4*>(dgeos + 8 * v)[1] = make_float4(s[4], s[5], s[6], s[7]);
-    dsub[v] = s[8];
+    if (vj == 0 && v < N)
+    {
+        reinterpret_cast<float4*>(dgeos + 8 * v)[0] = make_float4(run[0], run[1], run[2], run[3]);
+        reinterpret_cast<float4*>(dgeos + 8 * v)[1] = make_float4(run[4], run[5], run[6], run[7]);
+        dsub[v] = run[8];
+    }
 }
 
 // ---- SH colour of all views ----------------------------------------------
 
 constexpr int BSH_PAD = 49;
 
-// One wave per 64 consecutive voxels: their 48 coefficients are staged in LDS
-// with contiguous (16-byte where aligned) loads, then each lane evaluates its
-// voxel for every view where it has entries (sh_compute.cu:55-106 arithmetic).
-__global__ __launch_bounds__(64) void batch_sh_fwd_kernel(int deg, int V, const svr_view* __restrict__ views,
-                                                          int64_t N, const float* __restrict__ centers,
-                                                          const uint32_t* __restrict__ n_dup,
-                                                          const float* __restrict__ sh0,
-                                                          const float* __restrict__ shs, float* __restrict__ rgbs)
+// A workgroup takes SHF_VOX consecutive voxels: their coefficients and
+// centres are staged in LDS with contiguous loads, then each thread evaluates
+// one (view, voxel) pair at a time, SHF_VIEWS views per round, where the pair
+// has entries (sh_compute.cu:55-106 arithmetic); zeros elsewhere.
+constexpr int SHF_VOX = 32, SHF_VIEWS = 8;
+static_assert(SHF_VOX * SHF_VIEWS == 256, "one pair per thread");
+
+__global__ __launch_bounds__(256) void batch_sh_fwd_kernel(int deg, int V, const svr_view* __restrict__ views,
+                                                           int64_t N, const float* __restrict__ centers,
+                                                           const uint32_t* __restrict__ n_dup,
+                                                           const float* __restrict__ sh0,
+                                                           const float* __restrict__ shs, float* __restrict__ rgbs)
 {
-    __shared__ ShStage st;
-    const int lane = threadIdx.x;
-    const int64_t v0 = (int64_t)blockIdx.x * 64;
-    const int nv = (int)min((int64_t)64, N - v0);
-    float c[3];
-    sh_stage_rows(st, sh0, shs, centers, v0, nv, lane, c);
-    wave_sync();
-    if (lane >= nv) return;
-    const int64_t v = v0 + lane;
-    // the counts of a batch of views are loaded together (one round trip per batch)
-    constexpr int VB = 16;
-    for (int u0 = 0; u0 < V; u0 += VB)
+    __shared__ float s0[SHF_VOX * 3], s1[SHF_VOX * 45], cen[SHF_VOX * 3];
+    const int t = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.x * SHF_VOX;
+    const int nv = (int)min((int64_t)SHF_VOX, N - v0);
+    for (int e = t; e < nv * 45; e += 256) s1[e] = shs[45 * v0 + e];
+    if (t < nv * 3)
     {
-        uint32_t nd[VB];
-#pragma unroll
-        for (int j = 0; j < VB; ++j) nd[j] = n_dup[(size_t)min(u0 + j, V - 1) * (size_t)N + (size_t)v];
-#pragma unroll
-        for (int j = 0; j < VB; ++j)
+        s0[t] = sh0[3 * v0 + t];
+        cen[t] = centers[3 * v0 + t];
+    }
+    __syncthreads();
+    const int xi = t % SHF_VOX, vj = t / SHF_VOX;
+    if (xi >= nv) return;
+    const int64_t v = v0 + xi;
+    const float c[3] = {cen[3 * xi + 0], cen[3 * xi + 1], cen[3 * xi + 2]};
+    for (int u = vj; u < V; u += SHF_VIEWS)
+    {
+        const size_t p = (size_t)u * (size_t)N + (size_t)v;
+        float* out = rgbs + 3 * p;
+        if (n_dup[p] == 0)
         {
-            const int u = u0 + j;
-            if (u >= V) break;
-            const size_t p = (size_t)u * (size_t)N + (size_t)v;
-            float* out = rgbs + 3 * p;
-            if (nd[j] == 0)
-            {
-                out[0] = 0.f; out[1] = 0.f; out[2] = 0.f;
-                continue;
-            }
-            const svr_view& vw = views[u];
-            const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
-            float x, y, z, b[16], res[3];
-            sh_dir_c(c, cam, x, y, z);
-            const int nb = sh_basis(deg, x, y, z, b);
-            sh_eval_rows(nb, b, x, y, z, st.s0 + 3 * lane, st.s1 + 45 * lane, res);
-            out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
+            out[0] = 0.f; out[1] = 0.f; out[2] = 0.f;
+            continue;
         }
+        const svr_view& vw = views[u];
+        const float cam[3] = {vw.c2w[3], vw.c2w[7], vw.c2w[11]};
+        float x, y, z, b[16], res[3];
+        sh_dir_c(c, cam, x, y, z);
+        const int nb = sh_basis(deg, x, y, z, b);
+        sh_eval_rows(nb, b, x, y, z, s0 + 3 * xi, s1 + 45 * xi, res);
+        out[0] = res[0]; out[1] = res[1]; out[2] = res[2];
     }
 }
 
@@ -635,7 +636,8 @@ int svr_batch_sh_compute(int32_t deg, int32_t V, const svr_view* views_dev, int6
                 "svr_batch_sh_compute: null pointer");
     GeomState g = GeomState::carve((void*)geom_state, (int64_t)V * N, nullptr);
     prof_begin(PROF_SH_FWD, stream);
-    batch_sh_fwd_kernel<<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, V, views_dev, N, centers, g.n_dup, sh0,
+    batch_sh_fwd_kernel<<<(unsigned)((N + SHF_VOX - 1) / SHF_VOX), 256, 0, stream>>>(deg, V, views_dev, N, centers,
+                                                                                    g.n_dup, sh0,
                                                                      shs, rgbs);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_SH_FWD, stream);
@@ -797,8 +799,8 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
         const uint32_t cap = capacity_mode ? (uint32_t)R : 0xFFFFFFFFu;
         rc = grad_reduce_big_launch(VN, g, wk.partial, wk.pair_geo, drgbs, wk.pair_sub, wk.huge, stream, cap);
         if (rc != SVR_OK) return rc;
-        batch_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(V, g, N, cap, wk.partial, wk.pair_geo,
-                                                                             wk.pair_sub, drgbs, dgeos, dsub);
+        batch_reduce_kernel<<<(unsigned)((N + RED_VOX - 1) / RED_VOX), 256, 0, stream>>>(
+            V, g, N, cap, wk.partial, wk.pair_geo, wk.pair_sub, drgbs, dgeos, dsub);
         SVR_CHECK_LAUNCH();
     }
     else
