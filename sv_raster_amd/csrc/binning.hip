@@ -101,8 +101,8 @@ struct SortScratch
 {
     uint32_t* err;      // [1]
     uint32_t* hist;     // [passes][SORT_HIST_COPIES][RADIX]
-    uint64_t* status;   // [passes][nb][RADIX]
-    uint64_t* gstatus;  // [passes][ngroups][RADIX]
+    uint32_t* status;   // [passes][nb][RADIX] a tile's digit count | TILE_PUBLISHED
+    uint64_t* gstatus;  // [passes][ngroups][RADIX] flag << 32 | count
     int64_t words;      // words in use (cleared before the sort)
     static SortScratch carve(uint32_t* p, int64_t n, int passes)
     {
@@ -111,9 +111,9 @@ struct SortScratch
         const int64_t hist_words = (int64_t)passes * SORT_HIST_COPIES * RADIX;
         s.err = p;
         s.hist = p + SORT_HEADER_WORDS;
-        s.status = reinterpret_cast<uint64_t*>(p + SORT_HEADER_WORDS + hist_words);
-        s.gstatus = s.status + (int64_t)passes * nb * RADIX;
-        s.words = SORT_HEADER_WORDS + hist_words + 2 * (int64_t)passes * (nb + sort_lb_groups(n)) * RADIX;
+        s.gstatus = reinterpret_cast<uint64_t*>(p + SORT_HEADER_WORDS + hist_words);
+        s.status = reinterpret_cast<uint32_t*>(s.gstatus + (int64_t)passes * sort_lb_groups(n) * RADIX);
+        s.words = SORT_HEADER_WORDS + hist_words + (int64_t)passes * (nb + 2 * sort_lb_groups(n)) * RADIX;
         return s;
     }
 };
@@ -171,9 +171,23 @@ __device__ inline void status_store(uint64_t* p, uint32_t flag, uint32_t value)
     __hip_atomic_store(p, ((uint64_t)flag << 32) | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A tile's status word: its digit count (at most SORT_TILE) with the top bit
+// set once published (zero before).
+constexpr uint32_t TILE_PUBLISHED = 0x80000000u;
+
+__device__ inline uint32_t tile_load(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void tile_store(uint32_t* p, uint32_t count)
+{
+    __hip_atomic_store(p, count | TILE_PUBLISHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Sum of digit t's counts of tiles [first, first + r) (r < SORT_LB_GROUP),
 // every load in flight at once; words not published yet are polled again.
-__device__ inline uint32_t group_sum(const uint64_t* __restrict__ status, int64_t first, int r, int t,
+__device__ inline uint32_t group_sum(const uint32_t* __restrict__ status, int64_t first, int r, int t,
                                      uint32_t* __restrict__ err)
 {
     uint32_t sum = 0;
@@ -181,15 +195,15 @@ __device__ inline uint32_t group_sum(const uint64_t* __restrict__ status, int64_
     uint32_t spins = 0;
     while (pending)
     {
-        uint64_t sw[SORT_LB_GROUP - 1];
+        uint32_t sw[SORT_LB_GROUP - 1];
 #pragma unroll
         for (int q = 0; q < SORT_LB_GROUP - 1; ++q)
-            sw[q] = (pending >> q) & 1u ? status_load(status + (first + q) * RADIX + t) : 0ull;
+            sw[q] = (pending >> q) & 1u ? tile_load(status + (first + q) * RADIX + t) : 0u;
 #pragma unroll
         for (int q = 0; q < SORT_LB_GROUP - 1; ++q)
-            if (((pending >> q) & 1u) && (sw[q] >> 32) != 0u)
+            if (((pending >> q) & 1u) && (sw[q] & TILE_PUBLISHED))
             {
-                sum += (uint32_t)sw[q];
+                sum += sw[q] & ~TILE_PUBLISHED;
                 pending &= ~(1u << q);
             }
         if (pending)
@@ -263,7 +277,7 @@ template <bool HAS_VALS, int DBITS>
 __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
-    const uint32_t* __restrict__ hist, uint64_t* __restrict__ status, uint64_t* __restrict__ gstatus,
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ status, uint64_t* __restrict__ gstatus,
     uint32_t* __restrict__ err)
 {
     const int64_t n = sort_count(n_cap, n_dev);
@@ -274,16 +288,32 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
     // consecutive slices of its keys as independent chains of LDS
     // read-modify-writes (interleaved, so that their latencies overlap)
     __shared__ uint32_t wcnt[NW][NC][RADIX];
-    __shared__ uint32_t gbase[RADIX];  // global start of this tile's run of digit d
-    __shared__ uint32_t lbase[RADIX];  // start of digit d in the tile's reordered keys
-    __shared__ uint32_t wsum[NW], wsum2[NW];
-    __shared__ uint32_t tcnt[RADIX];  // the tile's digit counts
+    __shared__ uint32_t gdelta[RADIX];  // global start of this tile's run of digit d minus lbase[d]
+    __shared__ uint32_t lbase[RADIX];   // start of digit d in the tile's reordered keys
+    __shared__ uint32_t wsum[NW];
+    __shared__ uint8_t ldig[SORT_TILE];  // digit of each reordered key
     __shared__ uint64_t lkeys[SORT_TILE];  // reordered keys, then (as 32-bit words) values
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // the pass's digit count (digit = thread) over the histogram copies
-    uint32_t tot = 0;
+    // the pass's exclusive digit starts (digit = thread), once per workgroup:
+    // the digit's count over the histogram copies, scanned across the digits
+    uint32_t gstart = 0;
+    {
+        uint32_t tot = 0;
 #pragma unroll
-    for (int c = 0; c < SORT_HIST_COPIES; ++c) tot += hist[c * RADIX + t];
+        for (int c = 0; c < SORT_HIST_COPIES; ++c) tot += hist[c * RADIX + t];
+        uint32_t sc = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t y = __shfl_up(sc, o, 64);
+            if (lane >= o) sc += y;
+        }
+        if (lane == 63) wsum[w] = sc;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int q = 0; q < w; ++q) pre += wsum[q];
+        gstart = pre + sc - tot;
+    }
     const int64_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
     {
@@ -293,7 +323,6 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
         for (int i = 0; i < NW; ++i)
 #pragma unroll
             for (int h = 0; h < NC; ++h) wcnt[i][h][t] = 0;
-        tcnt[t] = 0;
         const int64_t base = blk0 + (int64_t)w * (SORT_ITEMS * 64);
         uint64_t k[SORT_ITEMS];
         uint32_t v[SORT_ITEMS];
@@ -307,38 +336,6 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             if (HAS_VALS) v[r] = valid ? vals_in[idx] : 0u;
         }
         __syncthreads();
-        // the tile's digit counts first, so that they are published (and
-        // successors can look back past this tile) before the ranking
-#pragma unroll
-        for (int r = 0; r < SORT_ITEMS; ++r)
-            add_digit_runs(tcnt, (uint32_t)(k[r] >> shift) & dmask, base + r * 64 + lane < n, lane);
-        __syncthreads();
-        const uint32_t dtot = tcnt[t];
-        // digits above dmask never occur: their counts stay zero and take no
-        // part in the look-back (a 6-bit pass polls a quarter of the words)
-        const bool live = (uint32_t)t <= dmask;
-        uint32_t excl = 0;
-        if (live)
-        {
-            status_store(status + tile * RADIX + t, FLAG_AGG, dtot);
-            const int64_t g = tile / SORT_LB_GROUP;
-            const int r = (int)(tile - g * SORT_LB_GROUP);
-            const uint32_t in_group = r > 0 ? group_sum(status, tile - r, r, t, err) : 0u;
-            uint32_t before = 0;
-            if (r == SORT_LB_GROUP - 1)
-            {
-                const uint32_t total = in_group + dtot;
-                status_store(gstatus + g * RADIX + t, g == 0 ? FLAG_PREFIX : FLAG_AGG, total);
-                if (g > 0)
-                {
-                    before = look_back(gstatus, g, t, err);
-                    status_store(gstatus + g * RADIX + t, FLAG_PREFIX, before + total);
-                }
-            }
-            else if (g > 0)
-                before = look_back(gstatus, g, t, err);
-            excl = before + in_group;
-        }
         static_assert(SORT_ITEMS % NC == 0, "ranking chains of equal length");
         constexpr int SLICE = SORT_ITEMS / NC;
 #pragma unroll
@@ -366,49 +363,35 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             for (int h = 0; h < r / SLICE; ++h) rk[r] += wcnt[w][h][d];
         }
         __syncthreads();
-        // per digit (thread t = digit): exclusive offsets of the waves
+        // per digit (thread t = digit): exclusive offsets of the waves, the
+        // tile's count and its start in the tile's reordered keys
+        uint32_t dtot = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i)
         {
-            uint32_t acc = 0;
+            uint32_t c = 0;
 #pragma unroll
-            for (int i = 0; i < NW; ++i)
-            {
-                uint32_t c = 0;
-#pragma unroll
-                for (int h = 0; h < NC; ++h) c += wcnt[i][h][t];
-                wcnt[i][0][t] = acc;
-                acc += c;
-            }
+            for (int h = 0; h < NC; ++h) c += wcnt[i][h][t];
+            wcnt[i][0][t] = dtot;
+            dtot += c;
         }
-        // the pass's exclusive digit prefix and the tile's digit starts
         {
-            uint32_t s = tot, l = dtot;
+            uint32_t l = dtot;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1)
             {
-                const uint32_t y = __shfl_up(s, o, 64);
                 const uint32_t z = __shfl_up(l, o, 64);
-                if (lane >= o)
-                {
-                    s += y;
-                    l += z;
-                }
+                if (lane >= o) l += z;
             }
-            if (lane == 63)
-            {
-                wsum[w] = s;
-                wsum2[w] = l;
-            }
+            if (lane == 63) wsum[w] = l;
             __syncthreads();
-            uint32_t pre = 0, pre_l = 0;
-            for (int q = 0; q < w; ++q)
-            {
-                pre += wsum[q];
-                pre_l += wsum2[q];
-            }
-            gbase[t] = pre + s - tot + excl;
+            uint32_t pre_l = 0;
+            for (int q = 0; q < w; ++q) pre_l += wsum[q];
             lbase[t] = pre_l + l - dtot;
         }
         __syncthreads();
+        // the keys in digit order in LDS (so that neither they nor their
+        // ranks stay in registers through the look-back)
         uint32_t lp[SORT_ITEMS];
 #pragma unroll
         for (int r = 0; r < SORT_ITEMS; ++r)
@@ -416,24 +399,47 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             const int64_t idx = base + r * 64 + lane;
             const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
             lp[r] = lbase[d] + wcnt[w][0][d] + rk[r];
-            if (idx < n) lkeys[lp[r]] = k[r];
+            if (idx < n)
+            {
+                lkeys[lp[r]] = k[r];
+                ldig[lp[r]] = (uint8_t)d;
+            }
         }
+        // digits above dmask never occur: their counts stay zero and take no
+        // part in the look-back (a 6-bit pass polls a quarter of the words)
+        const bool live = (uint32_t)t <= dmask;
+        uint32_t excl = 0;
+        if (live)
+        {
+            tile_store(status + tile * RADIX + t, dtot);
+            const int64_t g = tile / SORT_LB_GROUP;
+            const int r = (int)(tile - g * SORT_LB_GROUP);
+            const uint32_t in_group = r > 0 ? group_sum(status, tile - r, r, t, err) : 0u;
+            uint32_t before = 0;
+            if (r == SORT_LB_GROUP - 1)
+            {
+                const uint32_t total = in_group + dtot;
+                status_store(gstatus + g * RADIX + t, g == 0 ? FLAG_PREFIX : FLAG_AGG, total);
+                if (g > 0)
+                {
+                    before = look_back(gstatus, g, t, err);
+                    status_store(gstatus + g * RADIX + t, FLAG_PREFIX, before + total);
+                }
+            }
+            else if (g > 0)
+                before = look_back(gstatus, g, t, err);
+            excl = before + in_group;
+        }
+        // destination of the tile's run of digit t, as an offset from its
+        // reordered position
+        gdelta[t] = gstart + excl - lbase[t];
         __syncthreads();
         const int nk = (int)min((int64_t)SORT_TILE, n - blk0);
-        // the digit of reordered position i, kept for the values pass
-        uint32_t dd[SORT_ITEMS];
 #pragma unroll
         for (int r = 0; r < SORT_ITEMS; ++r)
         {
             const int i = t + r * SORT_THREADS;
-            dd[r] = 0;
-            if (i < nk)
-            {
-                const uint64_t key = lkeys[i];
-                const uint32_t d = (uint32_t)(key >> shift) & dmask;
-                dd[r] = d;
-                keys_out[gbase[d] + (uint32_t)i - lbase[d]] = key;
-            }
+            if (i < nk) keys_out[gdelta[ldig[i]] + (uint32_t)i] = lkeys[i];
         }
         if (HAS_VALS)
         {
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             for (int r = 0; r < SORT_ITEMS; ++r)
             {
                 const int i = t + r * SORT_THREADS;
-                if (i < nk) vals_out[gbase[dd[r]] + (uint32_t)i - lbase[dd[r]]] = lv[i];
+                if (i < nk) vals_out[gdelta[ldig[i]] + (uint32_t)i] = lv[i];
             }
         }
     }
@@ -520,7 +526,7 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
         const int done = p * step;
         const int shift = lo_bit + done;
         const int bits = (nbits - done) < step ? (nbits - done) : step;
-        uint64_t* status = sc.status + (int64_t)p * nb * RADIX;
+        uint32_t* status = sc.status + (int64_t)p * nb * RADIX;
         uint64_t* gstatus = sc.gstatus + (int64_t)p * sort_lb_groups(n) * RADIX;
         const uint32_t* hist = sc.hist + (int64_t)p * SORT_HIST_COPIES * RADIX;
         const void* kernel = vals ? pass_kernel<true>(bits) : pass_kernel<false>(bits);

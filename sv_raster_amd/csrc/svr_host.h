@@ -192,9 +192,10 @@ inline int64_t sort_blocks(int64_t n) { return (n + SORT_TILE - 1) / SORT_TILE; 
 // max_passes passes, in 32-bit words: a header (word 0: look-back error
 // flag), SORT_HIST_COPIES partial digit histograms of every pass (workgroups
 // add into copy blockIdx % SORT_HIST_COPIES, so that no counter takes every
-// workgroup's atomic), then the look-back status of every (pass, tile, digit)
-// and of every (pass, group of SORT_LB_GROUP tiles, digit), one 64-bit word
-// each (flag << 32 | count).  Cleared by radix_sort_ex itself.
+// workgroup's atomic), the look-back status of every (pass, group of
+// SORT_LB_GROUP tiles, digit), one 64-bit word each (flag << 32 | count), and
+// of every (pass, tile, digit), one 32-bit word each (published bit | count).
+// Cleared by radix_sort_ex itself.
 constexpr int SORT_HEADER_WORDS = 64;
 constexpr int SORT_MAX_PASSES = 8;
 constexpr int SORT_HIST_COPIES = 8;
@@ -203,7 +204,7 @@ inline int64_t sort_lb_groups(int64_t n) { return (sort_blocks(n > 0 ? n : 1) + 
 inline int64_t sort_scratch_words(int64_t n, int max_passes)
 {
     return SORT_HEADER_WORDS + (int64_t)max_passes * RADIX *
-                                   (SORT_HIST_COPIES + 2 * sort_blocks(n > 0 ? n : 1) + 2 * sort_lb_groups(n));
+                                   (SORT_HIST_COPIES + sort_blocks(n > 0 ? n : 1) + 2 * sort_lb_groups(n));
 }
 // Passes of the per-frame sorts: tile ids of at most 32 bits in 8-bit digits.
 constexpr int SORT_FRAME_PASSES = 4;
