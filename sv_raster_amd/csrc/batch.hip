@@ -60,6 +60,9 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(const svr_view* _
     const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
     out_n_dup[p] = (int32_t)nd;
     n_dup[p] = nd;
+    // every reader of a pair's box, quadrant bits and octagon first finds
+    // entries in n_dup, so a pair without entries leaves them unwritten
+    if (nd == 0) return;
     bboxes[p] = bb;
     qbits_out[p] = bits;
     octs[p] = oc;
@@ -73,8 +76,7 @@ template <typename TS>
 __global__ __launch_bounds__(256) void batch_duplicate_kernel(GeomState g, const svr_view* __restrict__ views,
                                                               int64_t N, int pos_bits,
                                                               const uint32_t* __restrict__ order_pos,
-                                                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                              uint32_t cap)
+                                                              uint64_t* __restrict__ keys, uint32_t cap)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -101,7 +103,6 @@ __global__ __launch_bounds__(256) void batch_duplicate_kernel(GeomState g, const
             {
                 const uint32_t g = (uint32_t)vw.tile_base + (uint32_t)y * (uint32_t)gx + (uint32_t)x;
                 keys[off] = ((uint64_t)g << pos_bits) | pos;
-                vals[off] = off;
                 ++off;
             }
     }
@@ -112,8 +113,7 @@ template <typename TS>
 __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(GeomState g, int V, const svr_view* __restrict__ views,
                                                                   int64_t N, int pos_bits,
                                                                   const uint32_t* __restrict__ order_pos,
-                                                                  uint64_t* __restrict__ keys,
-                                                                  uint32_t* __restrict__ vals, uint32_t cap)
+                                                                  uint64_t* __restrict__ keys, uint32_t cap)
 {
     // large pairs one workgroup each, then the huge ones (queued from the end
     // of big_list) spread over every workgroup
@@ -154,7 +154,6 @@ __global__ __launch_bounds__(256) void batch_duplicate_big_kernel(GeomState g, i
                 const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
                 const uint32_t gt = (uint32_t)vw.tile_base + y * (uint32_t)gx + x;
                 keys[off + rem] = ((uint64_t)gt << pos_bits) | pos;
-                vals[off + rem] = off + rem;
             }
         }
     }
@@ -723,15 +722,15 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
         uint64_t* k0 = odd ? b.keys_alt : b.keys;
         uint32_t* o0 = odd ? b.orig_alt : b.orig;
         SVR_WITH_TILE(shape, (batch_duplicate_kernel<TS><<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0,
-                                                           stream>>>(g, views_dev, N, pos_bits, ot.pos, k0, o0, cap)));
+                                                           stream>>>(g, views_dev, N, pos_bits, ot.pos, k0, cap)));
         SVR_CHECK_LAUNCH();
         SVR_WITH_TILE(shape, (batch_duplicate_big_kernel<TS><<<1024, 256, 0, stream>>>(g, V, views_dev, N, pos_bits,
-                                                                                       ot.pos, k0, o0, cap)));
+                                                                                       ot.pos, k0, cap)));
         SVR_CHECK_LAUNCH();
         uint64_t* sorted = nullptr;
         uint32_t* sorted_orig = nullptr;
         rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, tile_bits, b.counts,
-                           stream, &sorted, &sorted_orig, R_dev, pos_bits);
+                           stream, &sorted, &sorted_orig, R_dev, pos_bits, true);
         if (rc != SVR_OK) return rc;
         SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
         rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream, R_dev);

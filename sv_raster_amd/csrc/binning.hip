@@ -276,7 +276,7 @@ __device__ inline uint32_t look_back(const uint64_t* __restrict__ status, int64_
 template <bool HAS_VALS, int DBITS>
 __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift,
+    uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift, int ident,
     const uint32_t* __restrict__ hist, uint32_t* __restrict__ status, uint64_t* __restrict__ gstatus,
     uint32_t* __restrict__ err)
 {
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             const int64_t idx = base + r * 64 + lane;
             const bool valid = idx < n;
             k[r] = valid ? keys_in[idx] : 0ull;
-            if (HAS_VALS) v[r] = valid ? vals_in[idx] : 0u;
+            if (HAS_VALS) v[r] = valid ? (ident ? (uint32_t)idx : vals_in[idx]) : 0u;
         }
         __syncthreads();
         static_assert(SORT_ITEMS % NC == 0, "ranking chains of equal length");
@@ -502,7 +502,7 @@ static const void* pass_kernel(int bits)
 // out_keys / out_vals.
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
-                  const uint32_t* n_dev, int lo_bit)
+                  const uint32_t* n_dev, int lo_bit, bool identity_vals)
 {
     *out_keys = keys;
     if (out_vals) *out_vals = vals;
@@ -535,9 +535,10 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
         const unsigned grid = (unsigned)std::min(nb, resident);
         const uint64_t* kin_c = kin;
         const uint32_t* vin_c = vin;
-        void* args[] = {(void*)&kin_c, (void*)&kout,  (void*)&vin_c,  (void*)&vout,    (void*)&n,
-                        (void*)&n_dev, (void*)&shift, (void*)&hist,   (void*)&status,  (void*)&gstatus,
-                        (void*)&sc.err};
+        int ident = (identity_vals && p == 0) ? 1 : 0;
+        void* args[] = {(void*)&kin_c, (void*)&kout,  (void*)&vin_c, (void*)&vout,   (void*)&n,
+                        (void*)&n_dev, (void*)&shift, (void*)&ident, (void*)&hist,   (void*)&status,
+                        (void*)&gstatus, (void*)&sc.err};
         SVR_CHECK_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(SORT_THREADS), args, 0, stream));
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
@@ -569,14 +570,13 @@ __global__ void or_reduce_kernel(const int64_t* __restrict__ paths, int64_t n, u
 }
 
 __global__ void order_keys_kernel(const int64_t* __restrict__ paths, int64_t n8, int lshift, int with_q,
-                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                                  uint64_t* __restrict__ keys)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n8) return;
     const uint32_t q = (uint32_t)(i & 7);
     const uint64_t rank = (uint64_t)paths[i >> 3] ^ order_table(q);
     keys[i] = with_q ? (((rank >> lshift) << 3) | q) : rank;
-    vals[i] = (uint32_t)i;
 }
 
 __global__ void order_scatter_kernel(const uint32_t* __restrict__ sorted_pairs, int64_t n8, uint32_t* __restrict__ pos,
@@ -648,7 +648,7 @@ int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* or
 // run at its rank-ordered offset (recorded in qoff for the later passes).
 __global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
                                                         const uint32_t* __restrict__ order_pos,
-                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                                                        uint64_t* __restrict__ keys)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
@@ -669,7 +669,6 @@ __global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, 
             for (int x = txmin; x <= txmax; ++x)
             {
                 keys[off] = ((uint64_t)((uint32_t)y * (uint32_t)gx + (uint32_t)x) << pos_bits) | pos;
-                vals[off] = off;
                 ++off;
             }
     }
@@ -682,8 +681,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, 
 // that quadrant's run.
 __global__ __launch_bounds__(256) void duplicate_big_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
                                                             const uint32_t* __restrict__ order_pos,
-                                                            uint64_t* __restrict__ keys,
-                                                            uint32_t* __restrict__ vals)
+                                                            uint64_t* __restrict__ keys)
 {
     // pass 0: one workgroup per large voxel; pass 1: each huge voxel split into
     // K entry ranges taken by K different workgroups (item = chunk * huge + voxel)
@@ -718,7 +716,6 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(GeomState g, int64_t
                     const uint32_t rem = e - qi * tiles;
                     const uint32_t y = (uint32_t)tymin + rem / w, x = (uint32_t)txmin + rem % w;
                     keys[off + rem] = ((uint64_t)(y * (uint32_t)gx + x) << pos_bits) | pos;
-                    vals[off + rem] = off + rem;
                 }
             }
         }
@@ -797,14 +794,12 @@ int clamp_count_launch(uint32_t* counters, uint64_t cap, uint32_t* overflow, hip
 // Tile queue keys: 255 - length_bucket (an ascending sort puts the longest
 // lists first), values the tile ids.
 __global__ __launch_bounds__(256) void tile_queue_keys_kernel(int64_t T, const uint2* __restrict__ ranges,
-                                                              uint64_t* __restrict__ qkeys,
-                                                              uint32_t* __restrict__ order)
+                                                              uint64_t* __restrict__ qkeys)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= T) return;
     const uint2 r = ranges[g];
     qkeys[g] = 255u - length_bucket(r.y - r.x);
-    order[g] = (uint32_t)g;
 }
 
 // The whole queue in one workgroup (frames of at most QUEUE_ONE_BLOCK_MAX
@@ -872,11 +867,11 @@ int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t*
         return SVR_OK;
     }
     if (clear8 && fill_u32(clear8, 0u, 8, stream) != SVR_OK) return SVR_ERR_HIP;
-    tile_queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, ranges, qkeys_alt, order_alt);
+    tile_queue_keys_kernel<<<(unsigned)((T + 255) / 256), 256, 0, stream>>>(T, ranges, qkeys_alt);
     SVR_CHECK_LAUNCH();
     uint64_t* qk = nullptr;
     uint32_t* qo = nullptr;
-    const int rc = radix_sort_ex(qkeys_alt, qkeys, order_alt, order, T, 8, qcounts, stream, &qk, &qo);
+    const int rc = radix_sort_ex(qkeys_alt, qkeys, order_alt, order, T, 8, qcounts, stream, &qk, &qo, nullptr, 0, true);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(qo == order, SVR_ERR_HIP, "tile queue sort ended in the wrong buffer");
     return SVR_OK;
@@ -914,14 +909,14 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     const bool odd = radix_passes(tile_bits) & 1;
     uint64_t* k0 = odd ? b.keys_alt : b.keys;
     uint32_t* o0 = odd ? b.orig_alt : b.orig;
-    duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0, o0);
+    duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0);
     SVR_CHECK_LAUNCH();
-    duplicate_big_kernel<<<1024, 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0, o0);
+    duplicate_big_kernel<<<1024, 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0);
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
     uint32_t* sorted_orig = nullptr;
     rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, tile_bits, b.counts, stream,
-                       &sorted, &sorted_orig, nullptr, pos_bits);
+                       &sorted, &sorted_orig, nullptr, pos_bits, true);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
     rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);
@@ -967,11 +962,12 @@ int svr_build_order_table(int64_t N, const int64_t* paths, void* order_table_buf
     const int with_q = finest < MAX_NUM_LEVELS;
     const int lshift = 3 * (MAX_NUM_LEVELS - finest);
     const int key_bits = with_q ? 3 * finest + 3 : ORDER_RANK_BITS;
-    order_keys_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, stream>>>(paths, n8, lshift, with_q, s.keys, s.vals);
+    order_keys_kernel<<<(unsigned)((n8 + 255) / 256), 256, 0, stream>>>(paths, n8, lshift, with_q, s.keys);
     SVR_CHECK_LAUNCH();
     uint64_t* ko = nullptr;
     uint32_t* vo = nullptr;
-    int rc = radix_sort_ex(s.keys, s.keys_alt, s.vals, s.vals_alt, n8, key_bits, s.counts, stream, &ko, &vo);
+    int rc = radix_sort_ex(s.keys, s.keys_alt, s.vals, s.vals_alt, n8, key_bits, s.counts, stream, &ko, &vo, nullptr,
+                           0, true);
     if (rc != SVR_OK) return rc;
     uint32_t sort_err = 0;
     SVR_CHECK_HIP(hipMemcpyAsync(&sort_err, s.counts, sizeof(sort_err), hipMemcpyDeviceToHost, stream));

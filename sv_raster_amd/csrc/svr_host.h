@@ -288,11 +288,13 @@ __host__ __device__ inline int tiles_y(int H) { return (H + 15) / 16; }
 // returned through out_keys / out_vals; counts needs
 // sort_scratch_words(n, radix_passes(nbits)) words.  With n_dev, the kernels
 // sort min(n, *n_dev) keys (a 64-bit count stored as two 32-bit words on the
-// device); n then only sizes the launch.
+// device); n then only sizes the launch.  identity_vals: the values are the
+// keys' input positions (0, 1, 2, ...); the first pass generates them, so
+// `vals` need not be written beforehand (it is still the ping-pong buffer).
 inline int radix_passes(int nbits) { return (nbits + RADIX_BITS - 1) / RADIX_BITS; }
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
-                  const uint32_t* n_dev = nullptr, int lo_bit = 0);
+                  const uint32_t* n_dev = nullptr, int lo_bit = 0, bool identity_vals = false);
 
 // Entry offsets of the (item, quadrant) pairs in rank order (GeomState's
 // emission order): pos_off cleared, filled with each active pair's tile count
