@@ -468,7 +468,8 @@ def main():
     n_dup, geom = _C.rasterize_preprocess(W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w,
                                           cam["near"], octpath, centers, lengths)
     with torch.no_grad():
-        rgbs = _C.sh_compute(3, torch.where(n_dup > 0)[0], centers, c2w[:3, 3], sh0, shs)
+        vis_idx = torch.where(n_dup > 0)[0]
+        rgbs = _C.sh_compute(3, vis_idx, centers, c2w[:3, 3], sh0, shs)
         R, binning, img, color, _, _, T_img, _ = _C.rasterize_voxels(
             1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w, 0.0, False, False, False, False,
             octpath, centers, lengths, geos, rgbs, geom)
@@ -489,6 +490,8 @@ def main():
             _C.rasterize_voxels(1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w, 0.0, False,
                                 False, False, False, octpath, centers, lengths, geos, rgbs, geom)
         for _ in range(args.kernel_iters):
+            _C.sh_compute(3, vis_idx, centers, c2w[:3, 3], sh0, shs)
+        for _ in range(args.kernel_iters):
             _C.rasterize_voxels_backward(R, 1, W, H, cam["tanfovx"], cam["tanfovy"], cam["cx"], cam["cy"], w2c, c2w,
                                          0.0, octpath, centers, lengths, geos, rgbs, geom, binning, img, T_img, dcol,
                                          None, None, torch.zeros_like(T_img), 0.0, None, 0.0, 0.0, False, False,
@@ -496,7 +499,7 @@ def main():
         torch.cuda.synchronize()
         kst = collect_stages(L)
         _lib.check(L.svr_profile_enable(0), "profile")
-    kernel_ms = {k: kst[k][0] / kst[k][1] for k in ("render_fwd", "render_bwd")}
+    kernel_ms = {k: kst[k][0] / kst[k][1] for k in ("render_fwd", "render_bwd", "sh_fwd")}
     P = W * H
     T_tiles = ((W + 15) // 16) * ((H + 15) // 16)
     bytes_fwd = 72 * E + 20 * P + 12 * T_tiles
@@ -571,6 +574,10 @@ def main():
                      "other_compositor_gbs": (bytes_fwd if dom == "render_bwd" else bytes_bwd) /
                                              (kernel_ms[other] * 1e-3) / 1e9},
         "stages_gbs_algorithmic": stages_gbs,
+        # the SH forward (fill, list marks, dense pass) timed like the compositors, back to back: inside the
+        # training step it also pays for the write-back of dirty cache lines the binning left behind
+        "sh_fwd_back_to_back": {"ms": kernel_ms["sh_fwd"],
+                                "gbs_algorithmic": stage_bytes["sh_fwd"] / (kernel_ms["sh_fwd"] * 1e-3) / 1e9},
         "step_gbs_algorithmic": step_bytes / (ms_per_step * 1e-3) / 1e9,
         "tile_entry_pixel_pairs_per_s_fwd": E * 256 / (kernel_ms["render_fwd"] * 1e-3),
         "cpu_baseline": cpu,
