@@ -67,10 +67,10 @@ __device__ inline int64_t sort_count(int64_t n, const uint32_t* __restrict__ n_d
 // ---- onesweep LSD radix sort ----------------------------------------------
 //
 // One launch counts the digits of every pass (one read of the keys); then
-// each pass is one launch in which a workgroup takes tiles of SORT_TILE keys,
-// counts a tile's digits and publishes the counts, finds the counts of the
-// tiles before it by a two-level decoupled look-back, ranks the tile's keys
-// by digit (stable) and scatters them.  Tiles form groups of SORT_LB_GROUP:
+// each pass is one launch in which a workgroup takes tiles of
+// 256 x sort_items(n) keys, ranks the tile's keys by digit (stable),
+// publishes the tile's digit counts, finds the counts of the tiles before it
+// by a two-level decoupled look-back and scatters the keys.  Tiles form groups of SORT_LB_GROUP:
 // a tile sums the counts of the tiles before it in its group (all published
 // at about the same time, one round of loads), and the last tile of a group
 // publishes the group's total (FLAG_AGG), then its running total over all
@@ -171,7 +171,7 @@ __device__ inline void status_store(uint64_t* p, uint32_t flag, uint32_t value)
     __hip_atomic_store(p, ((uint64_t)flag << 32) | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// A tile's status word: its digit count (at most SORT_TILE) with the top bit
+// A tile's status word: its digit count (at most 256 x ITEMS) with the top bit
 // set once published (zero before).
 constexpr uint32_t TILE_PUBLISHED = 0x80000000u;
 
@@ -268,12 +268,12 @@ __device__ inline uint32_t look_back(const uint64_t* __restrict__ status, int64_
 }
 
 // One pass: for each of the workgroup's tiles, rank the tile's keys by digit
-// (stable; wave w ranks its 64 x SORT_ITEMS keys in order, then the waves in
+// (stable; wave w ranks its 64 x ITEMS keys in order, then the waves in
 // order), find the digit's global offset (the pass's exclusive digit prefix +
 // the counts of earlier tiles by look-back), reorder the tile's keys by digit
 // in LDS and write each digit's run with consecutive threads on consecutive
 // addresses.
-template <bool HAS_VALS, int DBITS>
+template <bool HAS_VALS, int DBITS, int ITEMS>
 __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, uint64_t* __restrict__ keys_out, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ vals_out, int64_t n_cap, const uint32_t* __restrict__ n_dev, int shift, int ident,
@@ -282,6 +282,7 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
 {
     const int64_t n = sort_count(n_cap, n_dev);
     constexpr uint32_t dmask = (1u << DBITS) - 1u;
+    constexpr int TILE = SORT_THREADS * ITEMS;
     constexpr int NW = SORT_THREADS / 64;
     constexpr int NC = SORT_RANK_CHAINS;
     // per wave, SORT_RANK_CHAINS running digit counts: the wave ranks
@@ -291,8 +292,8 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
     __shared__ uint32_t gdelta[RADIX];  // global start of this tile's run of digit d minus lbase[d]
     __shared__ uint32_t lbase[RADIX];   // start of digit d in the tile's reordered keys
     __shared__ uint32_t wsum[NW];
-    __shared__ uint8_t ldig[SORT_TILE];  // digit of each reordered key
-    __shared__ uint64_t lkeys[SORT_TILE];  // reordered keys, then (as 32-bit words) values
+    __shared__ uint8_t ldig[TILE];  // digit of each reordered key
+    __shared__ uint64_t lkeys[TILE];  // reordered keys, then (as 32-bit words) values
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     // the pass's exclusive digit starts (digit = thread), once per workgroup:
     // the digit's count over the histogram copies, scanned across the digits
@@ -314,21 +315,21 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
         for (int q = 0; q < w; ++q) pre += wsum[q];
         gstart = pre + sc - tot;
     }
-    const int64_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+    const int64_t ntiles = (n + TILE - 1) / TILE;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
     {
-        const int64_t blk0 = tile * SORT_TILE;
+        const int64_t blk0 = tile * TILE;
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
         for (int i = 0; i < NW; ++i)
 #pragma unroll
             for (int h = 0; h < NC; ++h) wcnt[i][h][t] = 0;
-        const int64_t base = blk0 + (int64_t)w * (SORT_ITEMS * 64);
-        uint64_t k[SORT_ITEMS];
-        uint32_t v[SORT_ITEMS];
-        uint32_t rk[SORT_ITEMS];
+        const int64_t base = blk0 + (int64_t)w * (ITEMS * 64);
+        uint64_t k[ITEMS];
+        uint32_t v[ITEMS];
+        uint32_t rk[ITEMS];
 #pragma unroll
-        for (int r = 0; r < SORT_ITEMS; ++r)
+        for (int r = 0; r < ITEMS; ++r)
         {
             const int64_t idx = base + r * 64 + lane;
             const bool valid = idx < n;
@@ -336,10 +337,10 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             if (HAS_VALS) v[r] = valid ? (ident ? (uint32_t)idx : vals_in[idx]) : 0u;
         }
         __syncthreads();
-        static_assert(SORT_ITEMS % NC == 0, "ranking chains of equal length");
-        constexpr int SLICE = SORT_ITEMS / NC;
+        static_assert(ITEMS % NC == 0, "ranking chains of equal length");
+        constexpr int SLICE = ITEMS / NC;
 #pragma unroll
-        for (int r2 = 0; r2 < SORT_ITEMS; ++r2)
+        for (int r2 = 0; r2 < ITEMS; ++r2)
         {
             const int h = r2 % NC;                // chain
             const int r = (r2 / NC) + h * SLICE;  // item
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
         }
         // a chain's keys follow all keys of the chains before it
 #pragma unroll
-        for (int r = SLICE; r < SORT_ITEMS; ++r)
+        for (int r = SLICE; r < ITEMS; ++r)
         {
             const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
 #pragma unroll
@@ -392,9 +393,9 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
         __syncthreads();
         // the keys in digit order in LDS (so that neither they nor their
         // ranks stay in registers through the look-back)
-        uint32_t lp[SORT_ITEMS];
+        uint32_t lp[ITEMS];
 #pragma unroll
-        for (int r = 0; r < SORT_ITEMS; ++r)
+        for (int r = 0; r < ITEMS; ++r)
         {
             const int64_t idx = base + r * 64 + lane;
             const uint32_t d = (uint32_t)(k[r] >> shift) & dmask;
@@ -434,9 +435,9 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
         // reordered position
         gdelta[t] = gstart + excl - lbase[t];
         __syncthreads();
-        const int nk = (int)min((int64_t)SORT_TILE, n - blk0);
+        const int nk = (int)min((int64_t)TILE, n - blk0);
 #pragma unroll
-        for (int r = 0; r < SORT_ITEMS; ++r)
+        for (int r = 0; r < ITEMS; ++r)
         {
             const int i = t + r * SORT_THREADS;
             if (i < nk) keys_out[gdelta[ldig[i]] + (uint32_t)i] = lkeys[i];
@@ -446,14 +447,14 @@ __global__ __launch_bounds__(SORT_THREADS) void onesweep_pass_kernel(
             uint32_t* lv = reinterpret_cast<uint32_t*>(lkeys);
             __syncthreads();
 #pragma unroll
-            for (int r = 0; r < SORT_ITEMS; ++r)
+            for (int r = 0; r < ITEMS; ++r)
             {
                 const int64_t idx = base + r * 64 + lane;
                 if (idx < n) lv[lp[r]] = v[r];
             }
             __syncthreads();
 #pragma unroll
-            for (int r = 0; r < SORT_ITEMS; ++r)
+            for (int r = 0; r < ITEMS; ++r)
             {
                 const int i = t + r * SORT_THREADS;
                 if (i < nk) vals_out[gdelta[ldig[i]] + (uint32_t)i] = lv[i];
@@ -481,20 +482,27 @@ static int64_t resident_blocks(const void* kernel)
     return (int64_t)per_cu * cus;
 }
 
-template <bool HAS_VALS>
-static const void* pass_kernel(int bits)
+template <bool HAS_VALS, int ITEMS>
+static const void* pass_kernel_items(int bits)
 {
     switch (bits)
     {
-        case 1: return (const void*)onesweep_pass_kernel<HAS_VALS, 1>;
-        case 2: return (const void*)onesweep_pass_kernel<HAS_VALS, 2>;
-        case 3: return (const void*)onesweep_pass_kernel<HAS_VALS, 3>;
-        case 4: return (const void*)onesweep_pass_kernel<HAS_VALS, 4>;
-        case 5: return (const void*)onesweep_pass_kernel<HAS_VALS, 5>;
-        case 6: return (const void*)onesweep_pass_kernel<HAS_VALS, 6>;
-        case 7: return (const void*)onesweep_pass_kernel<HAS_VALS, 7>;
-        default: return (const void*)onesweep_pass_kernel<HAS_VALS, 8>;
+        case 1: return (const void*)onesweep_pass_kernel<HAS_VALS, 1, ITEMS>;
+        case 2: return (const void*)onesweep_pass_kernel<HAS_VALS, 2, ITEMS>;
+        case 3: return (const void*)onesweep_pass_kernel<HAS_VALS, 3, ITEMS>;
+        case 4: return (const void*)onesweep_pass_kernel<HAS_VALS, 4, ITEMS>;
+        case 5: return (const void*)onesweep_pass_kernel<HAS_VALS, 5, ITEMS>;
+        case 6: return (const void*)onesweep_pass_kernel<HAS_VALS, 6, ITEMS>;
+        case 7: return (const void*)onesweep_pass_kernel<HAS_VALS, 7, ITEMS>;
+        default: return (const void*)onesweep_pass_kernel<HAS_VALS, 8, ITEMS>;
     }
+}
+
+template <bool HAS_VALS>
+static const void* pass_kernel(int bits, int items)
+{
+    return items == SORT_ITEMS_SMALL ? pass_kernel_items<HAS_VALS, SORT_ITEMS_SMALL>(bits)
+                                     : pass_kernel_items<HAS_VALS, SORT_ITEMS>(bits);
 }
 
 // Stable LSD radix sort over bits [lo_bit, lo_bit + nbits) in digits of at
@@ -529,7 +537,7 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
         uint32_t* status = sc.status + (int64_t)p * nb * RADIX;
         uint64_t* gstatus = sc.gstatus + (int64_t)p * sort_lb_groups(n) * RADIX;
         const uint32_t* hist = sc.hist + (int64_t)p * SORT_HIST_COPIES * RADIX;
-        const void* kernel = vals ? pass_kernel<true>(bits) : pass_kernel<false>(bits);
+        const void* kernel = vals ? pass_kernel<true>(bits, sort_items(n)) : pass_kernel<false>(bits, sort_items(n));
         const int64_t resident = resident_blocks(kernel);
         SVR_REQUIRE(resident > 0, SVR_ERR_HIP, "radix sort occupancy query failed");
         const unsigned grid = (unsigned)std::min(nb, resident);

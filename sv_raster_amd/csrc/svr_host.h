@@ -186,7 +186,18 @@ constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 
-inline int64_t sort_blocks(int64_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+// Keys per thread of a sort pass: 8 up to SORT_SMALL_MAX keys (more, shorter
+// tiles: the tile latency and the look-back chain dominate small sorts), 16
+// beyond (fewer tiles per workgroup in the persistent loop).
+constexpr int SORT_ITEMS_SMALL = 8;
+constexpr int64_t SORT_SMALL_MAX = 1ll << 22;
+inline int sort_items(int64_t n) { return n <= SORT_SMALL_MAX ? SORT_ITEMS_SMALL : SORT_ITEMS; }
+// Tiles of a sort pass over n keys.
+inline int64_t sort_blocks(int64_t n)
+{
+    const int64_t tile = (int64_t)SORT_THREADS * sort_items(n);
+    return (n + tile - 1) / tile;
+}
 
 // Scratch of the onesweep radix sort (radix_sort_ex) of n keys in at most
 // max_passes passes, in 32-bit words: a header (word 0: look-back error
