@@ -129,7 +129,8 @@ __device__ inline void batch_bwd_tile(WindowLDS<KW>& sh, uint32_t g, int V, cons
                                       const float* __restrict__ rgbs, const float* __restrict__ out_T,
                                       const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
                                       const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-                                      const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+                                      const uint64_t* __restrict__ hit_masks, const uint32_t* __restrict__ tile_last,
+                                      float* __restrict__ partial)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const TileGeom t = tile_geom<TS>(views, V, g, w, lane);
@@ -142,12 +143,15 @@ __device__ inline void batch_bwd_tile(WindowLDS<KW>& sh, uint32_t g, int V, cons
     const size_t pb = (size_t)vw.pix_base;
     const size_t HW = (size_t)t.H * t.W;
     const size_t pid = t.inside ? (size_t)t.py * t.W + t.px : 0;
-    const uint32_t nc = hit.init(t.inside, pid, HW, vw.bg_color, out_T + pb, n_contrib + pb, dL_dcolor + 3 * pb,
-                                 dL_dT + pb);
     const uint2 range = ranges[g];
+    const uint32_t C = tile_chunks(range, tile_last[g]);
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + g) * TS::NW + w) * 64 + lane;
-    adjoint_window<SYNC, TS, KW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths,
-                                                        geos, rgbs + 3 * vo, orig, partial);
+    adjoint_window<SYNC, TS, KW>(
+        sh, w, lane, C,
+        [&]() {
+            return hit.init(t.inside, pid, HW, vw.bg_color, out_T + pb, n_contrib + pb, dL_dcolor + 3 * pb, dL_dT + pb);
+        },
+        hit, range, masks, vox_list, centers, lengths, geos, rgbs + 3 * vo, orig, partial);
 }
 
 template <int NS, typename TS, int KW, int SYNC>
@@ -157,12 +161,12 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
     const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
     const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
     const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-    float* __restrict__ partial)
+    const uint32_t* __restrict__ tile_last, float* __restrict__ partial)
 {
     __shared__ WindowLDS<KW> sh;
     const uint32_t g = order[blockIdx.x];
-    batch_bwd_tile<NS, false, TS, KW, SYNC>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T, n_contrib,
-                                  dL_dcolor, dL_dT, orig, hit_masks, partial);
+    batch_bwd_tile<NS, false, TS, KW, SYNC>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
+                                            n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
 }
 
 // The forward's deferred tiles (exact_list, count ctr[1]; head ctr[4]).
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+    const uint64_t* __restrict__ hit_masks, const uint32_t* __restrict__ tile_last, float* __restrict__ partial)
 {
     __shared__ WindowLDS<KW> sh;
     __shared__ uint32_t s_i;
@@ -185,8 +189,8 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
         const uint32_t i = __builtin_amdgcn_readfirstlane(s_i);
         if (i >= count) break;
         const uint32_t g = __builtin_amdgcn_readfirstlane(exact_list[i]);
-        batch_bwd_tile<NS, true, TS, KW, SYNC>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
-                                     n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+        batch_bwd_tile<NS, true, TS, KW, SYNC>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs,
+                                               out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
         __syncthreads();
     }
 }
@@ -210,11 +214,11 @@ int batch_bwd_launch(int V, const svr_view* views, int64_t N, int64_t T, const B
 {
     batch_bwd_kernel<NS, TS, KW, SYNC><<<(unsigned)T, TS::THREADS, 0, stream>>>(
         V, views, N, img.order, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T, img.n_contrib, dL_dcolor,
-        dL_dT, b.orig, img.hit_masks, partial);
+        dL_dT, b.orig, img.hit_masks, img.tile_last, partial);
     SVR_CHECK_LAUNCH();
     batch_bwd_exact_kernel<NS, TS, KW, SYNC><<<(unsigned)std::min<int64_t>(EXACT_GRID, T), TS::THREADS, 0, stream>>>(
         V, views, N, img.ctr, img.exact_list, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T,
-        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
+        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, img.tile_last, partial);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }

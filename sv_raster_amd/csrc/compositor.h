@@ -402,17 +402,26 @@ struct PlainHit
 };
 
 // Workgroup-shared state of the windowed adjoint: for the KW chunks of a
-// window, the staged payload, the per-entry gradient accumulators, whether
-// some pixel of the tile used the entry, and the entry's emission-order row.
+// window, the staged payload, the per-entry gradient accumulators, each
+// wave's mask of the entries its pixels used, and the entries' emission-order
+// rows.
 template <int KW>
 struct WindowLDS
 {
     EntryPay pay[KW][64];
     float acc[KW][12][64];
-    uint32_t used[KW][64];
+    uint64_t usedw[16][KW];
     uint32_t orow[KW][64];
-    uint32_t nch[16];  // per wave: chunks that hold its pixels' contributors
 };
+
+// Chunks of 64 list entries that hold a tile's contributors, from the
+// forward's tile_last (range.x + the largest n_contrib of the tile's pixels),
+// never more than the tile's list has.
+__device__ inline uint32_t tile_chunks(uint2 range, uint32_t tile_last)
+{
+    const uint32_t n = min(tile_last - range.x, range.y - range.x);
+    return (n + 63u) >> 6;
+}
 // Chunks per window of the single-view backward.  With the branch-free hit
 // walk and view-constant staging, 4 was the fastest of 2, 3, 4 and 5 on the
 // 1080p scene (sync threshold 8); before them, 6 beat 2-5, 7 and 8.  Windows of
@@ -429,8 +438,11 @@ constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 // pixel, so it walks only real hits (no bbox, quadrant, ray-box or alpha
 // rejections, no n_contrib cut).
 //
-// The tile's waves go through the chunks together, KW at a time (a window),
-// from the tile's last contributor down:
+// The tile's waves go through its C chunks (tile_chunks) together, KW at a
+// time (a window), from the tile's last contributor down.  The first window's
+// hit masks and list values are requested before the pixels' state (init,
+// which loads it and returns the pixel's n_contrib), so that both round trips
+// overlap:
 //   1. each wave reads its masks for the window's chunks and marks the
 //      entries its pixels used;
 //   2. the workgroup stages the payload of the used entries, one entry per
@@ -449,37 +461,33 @@ constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 // zero rows, so partial needs no clearing beforehand.
 // HIT holds the pixel's ray and running state and computes one hit's 12
 // terms (PlainHit here; the adjoint with the optional terms has its own).
-template <int SYNC_MIN, typename TS, int KW, typename HIT>
-__device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32_t nc, HIT& hit_fn, uint2 range,
-                                      const uint64_t* __restrict__ masks, const uint32_t* __restrict__ vox_list,
-                                      const float* __restrict__ centers, const float* __restrict__ lengths,
-                                      const float* __restrict__ geos, const float* __restrict__ rgbs,
-                                      const uint32_t* __restrict__ orig, float* __restrict__ partial)
+template <int SYNC_MIN, typename TS, int KW, typename HIT, typename INIT>
+__device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32_t C, INIT&& init, HIT& hit_fn,
+                                      uint2 range, const uint64_t* __restrict__ masks,
+                                      const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
+                                      const float* __restrict__ lengths, const float* __restrict__ geos,
+                                      const float* __restrict__ rgbs, const uint32_t* __restrict__ orig,
+                                      float* __restrict__ partial)
 {
-    static_assert(TS::NW <= 16, "per-wave chunk counts are kept for at most 16 waves");
+    static_assert(TS::NW <= 16, "per-wave entry masks are kept for at most 16 waves");
     constexpr size_t STRIDE = 64 * TS::NW;
     constexpr int PER = (KW * 64 + TS::THREADS - 1) / TS::THREADS;  // window entries per thread
     const int t = threadIdx.x;
-    const uint32_t my_chunks = __builtin_amdgcn_readfirstlane((wave_max_u32(nc) + 63u) >> 6);
-    if (lane == 0) sh.nch[w] = my_chunks;
-    for (int i = t; i < KW * 12 * 64; i += TS::THREADS) (&sh.acc[0][0][0])[i] = 0.f;
-    for (int i = t; i < KW * 64; i += TS::THREADS) (&sh.used[0][0])[i] = 0u;
-    __syncthreads();
-    uint32_t C = 0;
-#pragma unroll
-    for (int i = 0; i < TS::NW; ++i) C = max(C, sh.nch[i]);
 
     // Software pipeline over windows: the hit masks of the next window and the
     // list values / emission rows of its entries are loaded while the current
     // window is staged and walked, so a window starts with its masks in
-    // registers and its payload loads need one round trip.
+    // registers and its payload loads need one round trip.  Masks of chunks
+    // past the wave's last contributor were never written (lim: the wave's
+    // chunk count once known); they are loaded for the first window, whose
+    // loads go out before the count, and dropped when the window starts.
     uint64_t m_pf[KW];
     uint32_t val_pf[PER], orow_pf[PER];
-    auto prefetch = [&](int c0) {
+    auto prefetch = [&](int c0, uint32_t lim) {
         const int nj = min(KW, c0 + 1);
 #pragma unroll
         for (int j = 0; j < KW; ++j)
-            m_pf[j] = (j < nj && (uint32_t)(c0 - j) < my_chunks) ? masks[(size_t)(c0 - j) * STRIDE] : 0ull;
+            m_pf[j] = (j < nj && (uint32_t)(c0 - j) < lim) ? masks[(size_t)(c0 - j) * STRIDE] : 0ull;
 #pragma unroll
         for (int k = 0; k < PER; ++k)
         {
@@ -491,7 +499,11 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
             orow_pf[k] = ok ? orig[e] : 0u;
         }
     };
-    if (C > 0) prefetch((int)C - 1);
+    if (C > 0) prefetch((int)C - 1, ~0u);
+    const uint32_t nc = init();
+    const uint32_t my_chunks = __builtin_amdgcn_readfirstlane((wave_max_u32(nc) + 63u) >> 6);
+    // cleared before the first walk (two barriers away)
+    for (int i = t; i < KW * 12 * 64; i += TS::THREADS) (&sh.acc[0][0][0])[i] = 0.f;
 
     for (int c0 = (int)C - 1; c0 >= 0; c0 -= KW)
     {
@@ -501,14 +513,17 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
 #pragma unroll
         for (int j = 0; j < KW; ++j)
         {
-            m[j] = m_pf[j];
+            const bool mine = j < nj && (uint32_t)(c0 - j) < my_chunks;
+            m[j] = mine ? m_pf[j] : 0ull;
             syncm[j] = 0ull;
-            if (j < nj && (uint32_t)(c0 - j) < my_chunks)
+            uint64_t um = 0ull;
+            if (mine)
             {
                 const uint64_t er = wave_transpose64(m[j], lane);
-                if (er) sh.used[j][lane] = 1u;
+                um = __ballot(er != 0ull);
                 syncm[j] = __ballot(__popcll(er) >= SYNC_MIN);
             }
+            if (lane == 0) sh.usedw[w][j] = um;
         }
         __syncthreads();
         // 2. stage the used entries' payload
@@ -517,9 +532,15 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
         {
             const int i = t + k * TS::THREADS;
             const int j = i >> 6, l = i & 63;
-            if (i < KW * 64 && j < nj && range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l < range.y)
-                sh.orow[j][l] = orow_pf[k];
-            if (i < KW * 64 && j < nj && sh.used[j][l])
+            const bool inwin = i < KW * 64 && j < nj;
+            if (inwin && range.x + ((uint32_t)(c0 - j) << 6) + (uint32_t)l < range.y) sh.orow[j][l] = orow_pf[k];
+            uint64_t um = 0ull;
+            if (inwin)
+            {
+#pragma unroll
+                for (int v = 0; v < TS::NW; ++v) um |= sh.usedw[v][j];
+            }
+            if ((um >> l) & 1ull)
             {
                 const size_t v = val_pf[k] & VOX_ID_MASK;
                 const float len = lengths[v];
@@ -530,7 +551,7 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
                          *reinterpret_cast<const float4*>(geos + 8 * v + 4));
             }
         }
-        if (c0 - KW >= 0) prefetch(c0 - KW);
+        if (c0 - KW >= 0) prefetch(c0 - KW, my_chunks);
         __syncthreads();
         // 3. walk this wave's hits
 #pragma unroll
@@ -599,7 +620,6 @@ __device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32
                 row[0] = make_float4(a[0], a[1], a[2], a[3]);
                 row[1] = make_float4(a[4], a[5], a[6], a[7]);
                 row[2] = make_float4(a[8], a[9], a[10], a[11]);
-                sh.used[j][l] = 0u;
             }
         }
         __syncthreads();

@@ -138,7 +138,7 @@ __device__ inline void bwd_tile(WindowLDS<BWD_WINDOW>& sh, int tile, const PixRa
                                 const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
                                 const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
                                 const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-                                float* __restrict__ partial)
+                                const uint32_t* __restrict__ tile_last, float* __restrict__ partial)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % gx, ty = tile / gx;
@@ -147,12 +147,13 @@ __device__ inline void bwd_tile(WindowLDS<BWD_WINDOW>& sh, int tile, const PixRa
     PlainHit<NS, EXACT> hit;
     hit.ray = ray;
     const uint2 range = ranges[tile];
+    const uint32_t C = tile_chunks(range, tile_last[tile]);
     const size_t HW = (size_t)H * W;
     const size_t pid = inside ? (size_t)py * W + px : 0;
-    const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT);
     const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
-    adjoint_window<SYNC_MIN, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers, lengths, geos,
-                                                 rgbs, orig, partial);
+    adjoint_window<SYNC_MIN, Tile16, BWD_WINDOW>(
+        sh, w, lane, C, [&]() { return hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT); }, hit, range,
+        masks, vox_list, centers, lengths, geos, rgbs, orig, partial);
 }
 
 template <int NS, int SYNC_MIN>
@@ -162,14 +163,14 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
     const float* __restrict__ rgbs, const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
-    const uint64_t* __restrict__ hit_masks, float* __restrict__ partial)
+    const uint64_t* __restrict__ hit_masks, const uint32_t* __restrict__ tile_last, float* __restrict__ partial)
 {
     __shared__ WindowLDS<BWD_WINDOW> sh;
     const int tile = (int)tile_order[blockIdx.x];
     const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
     if (__syncthreads_or(!ray_inv_finite(ray)) != 0) return;  // the forward listed it for the exact launch
     bwd_tile<NS, SYNC_MIN, false>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
-                                  n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+                                  n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
 }
 
 template <int NS, int SYNC_MIN>
@@ -179,7 +180,8 @@ __global__ __launch_bounds__(256) void render_bwd_exact_kernel(
     const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
     const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
     const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-    float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr, const uint32_t* __restrict__ exact_list)
+    const uint32_t* __restrict__ tile_last, float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr,
+    const uint32_t* __restrict__ exact_list)
 {
     __shared__ WindowLDS<BWD_WINDOW> sh;
     const uint32_t count = exact_ctr[0];
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(256) void render_bwd_exact_kernel(
         const int tile = (int)exact_list[i];
         const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
         bwd_tile<NS, SYNC_MIN, true>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs,
-                                     out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, partial);
+                                     out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
         __syncthreads();
     }
 }
@@ -408,11 +410,11 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     prof_begin(PROF_RENDER_BWD, stream);
     render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
-        out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial);
+        out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, img.tile_last, partial);
     SVR_CHECK_LAUNCH();
     render_bwd_exact_kernel<NS, BWD_SYNC_MIN_MASKED><<<EXACT_GRID, 256, 0, stream>>>(
         W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T,
-        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, partial, img.ctr, img.exact_list);
+        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, img.tile_last, partial, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);

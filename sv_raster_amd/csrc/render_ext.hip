@@ -528,7 +528,8 @@ __global__ __launch_bounds__(256) void render_bwd_ext_kernel(
     const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT,
     const float* __restrict__ dL_ddepth, const float* __restrict__ dL_dnormal, const float* __restrict__ gt,
     const float* __restrict__ out_depth, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-    float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr, const uint32_t* __restrict__ exact_list)
+    const uint32_t* __restrict__ tile_last, float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr,
+    const uint32_t* __restrict__ exact_list)
 {
     __shared__ WindowLDS<BWD_WINDOW> sh;
     // tiles split between the two launches as in render_fwd_ext_kernel (the
@@ -548,11 +549,15 @@ __global__ __launch_bounds__(256) void render_bwd_ext_kernel(
         const size_t HW = (size_t)H * W;
         const size_t pid = inside ? (size_t)py * W + px : 0;
         if (!EXACT && __syncthreads_or(!ray_inv_finite(hit.ray)) != 0) return;
-        const uint32_t nc = hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt,
-                                     out_depth);
+        const uint32_t C = tile_chunks(range, tile_last[tile]);
         const uint64_t* masks = hit_masks + (((size_t)(range.x >> 6) + tile) * NWAVE + w) * 64 + lane;
-        adjoint_window<BWD_SYNC_MIN_MASKED, Tile16, BWD_WINDOW>(sh, w, lane, nc, hit, range, masks, vox_list, centers,
-                                                                lengths, geos, rgbs, orig, partial);
+        adjoint_window<BWD_SYNC_MIN_MASKED, Tile16, BWD_WINDOW>(
+            sh, w, lane, C,
+            [&]() {
+                return hit.init(inside, pid, HW, bg, out_T, n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt,
+                                out_depth);
+            },
+            hit, range, masks, vox_list, centers, lengths, geos, rgbs, orig, partial);
         __syncthreads();
     }
 }
@@ -593,13 +598,13 @@ int launch_bwd_ext(int W, int H, float tanfovx, float tanfovy, float cx, float c
     prof_begin(PROF_RENDER_BWD, stream);
     render_bwd_ext_kernel<NS, false><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
-        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial,
-        img.ctr, img.exact_list);
+        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks,
+        img.tile_last, partial, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     render_bwd_ext_kernel<NS, true><<<EXT_EXACT_GRID, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, k, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
-        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks, partial,
-        img.ctr, img.exact_list);
+        out_T, img.n_contrib, dL_dcolor, dL_dT, dL_ddepth, dL_dnormal, gt, out_depth, b.orig, img.hit_masks,
+        img.tile_last, partial, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);

@@ -328,6 +328,12 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict_
 // of zero gradients is skipped whole, and inside a mixed vector the elements
 // with a zero gradient keep their values (they are stored back unchanged).
 // The last n % 4 elements are updated one per thread.
+// 16-byte form: each thread takes ADAM_UNROLL vectors per pass (a grid
+// stride apart), reads all their gradients, then the parameters and moments of
+// those with a nonzero gradient, so that a wave keeps several round trips of
+// loads in flight instead of one gradient load and one row load at a time.
+constexpr int ADAM_UNROLL = 4;
+
 template <bool SPARSE, bool BIASED>
 __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
                                                     float* __restrict__ m, float* __restrict__ v, float step_size,
@@ -347,18 +353,42 @@ __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict
     const float4* __restrict__ G = reinterpret_cast<const float4*>(g);
     float4* __restrict__ M = reinterpret_cast<float4*>(m);
     float4* __restrict__ V = reinterpret_cast<float4*>(v);
-    for (int64_t i = gid; i < n4; i += stride)
+    for (int64_t base = gid; base < n4; base += ADAM_UNROLL * stride)
     {
-        const float4 g4 = G[i];
-        if (SPARSE && g4.x == 0.f && g4.y == 0.f && g4.z == 0.f && g4.w == 0.f) continue;
-        float4 p4 = P[i], m4 = M[i], v4 = V[i];
-        adam_elem<SPARSE, BIASED>(p4.x, m4.x, v4.x, g4.x, step_size, beta1, beta2, rb2, eps);
-        adam_elem<SPARSE, BIASED>(p4.y, m4.y, v4.y, g4.y, step_size, beta1, beta2, rb2, eps);
-        adam_elem<SPARSE, BIASED>(p4.z, m4.z, v4.z, g4.z, step_size, beta1, beta2, rb2, eps);
-        adam_elem<SPARSE, BIASED>(p4.w, m4.w, v4.w, g4.w, step_size, beta1, beta2, rb2, eps);
-        P[i] = p4;
-        M[i] = m4;
-        V[i] = v4;
+        float4 g4[ADAM_UNROLL], p4[ADAM_UNROLL], m4[ADAM_UNROLL], v4[ADAM_UNROLL];
+        bool on[ADAM_UNROLL];
+#pragma unroll
+        for (int u = 0; u < ADAM_UNROLL; ++u)
+        {
+            const int64_t i = base + u * stride;
+            g4[u] = i < n4 ? G[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            on[u] = i < n4;
+        }
+#pragma unroll
+        for (int u = 0; u < ADAM_UNROLL; ++u)
+        {
+            if (SPARSE) on[u] = on[u] && !(g4[u].x == 0.f && g4[u].y == 0.f && g4[u].z == 0.f && g4[u].w == 0.f);
+            if (on[u])
+            {
+                const int64_t i = base + u * stride;
+                p4[u] = P[i];
+                m4[u] = M[i];
+                v4[u] = V[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ADAM_UNROLL; ++u)
+        {
+            if (!on[u]) continue;
+            const int64_t i = base + u * stride;
+            adam_elem<SPARSE, BIASED>(p4[u].x, m4[u].x, v4[u].x, g4[u].x, step_size, beta1, beta2, rb2, eps);
+            adam_elem<SPARSE, BIASED>(p4[u].y, m4[u].y, v4[u].y, g4[u].y, step_size, beta1, beta2, rb2, eps);
+            adam_elem<SPARSE, BIASED>(p4[u].z, m4[u].z, v4[u].z, g4[u].z, step_size, beta1, beta2, rb2, eps);
+            adam_elem<SPARSE, BIASED>(p4[u].w, m4[u].w, v4[u].w, g4[u].w, step_size, beta1, beta2, rb2, eps);
+            P[i] = p4[u];
+            M[i] = m4[u];
+            V[i] = v4[u];
+        }
     }
     const int64_t t = 4 * n4 + gid;
     if (t < n)
@@ -378,7 +408,8 @@ void adam_launch(bool sparse, int64_t n, float* p, const float* g, float* m, flo
     const bool vec = ((((uintptr_t)p) | ((uintptr_t)g) | ((uintptr_t)m) | ((uintptr_t)v)) & 15u) == 0;
     if (vec)
     {
-        const unsigned grid = grid_for((n >> 2) > 0 ? (n >> 2) : 1, 8192);
+        const int64_t n4 = n >> 2;
+        const unsigned grid = grid_for(n4 > 0 ? (n4 + ADAM_UNROLL - 1) / ADAM_UNROLL : 1, 16384);
         if (sparse)
             adam4_kernel<true, BIASED><<<grid, 256, 0, stream>>>(n, p, g, m, v, step_size, beta1, beta2, rb2, eps,
                                                                  coef);
