@@ -122,7 +122,7 @@ __device__ inline TileGeom tile_geom(const svr_view* __restrict__ views, int V, 
 }
 
 template <int NS, bool EXACT, typename TS, int KW, int SYNC>
-__device__ inline void batch_bwd_tile(WindowLDS<KW>& sh, uint32_t g, int V, const svr_view* __restrict__ views,
+__device__ inline void batch_bwd_tile(WindowLDS<KW, TS::NW>& sh, uint32_t g, int V, const svr_view* __restrict__ views,
                                       int64_t N, const uint2* __restrict__ ranges,
                                       const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
                                       const float* __restrict__ lengths, const float* __restrict__ geos,
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_kernel(
     const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
     const uint32_t* __restrict__ tile_last, float* __restrict__ partial)
 {
-    __shared__ WindowLDS<KW> sh;
+    __shared__ WindowLDS<KW, TS::NW> sh;
     const uint32_t g = order[blockIdx.x];
     batch_bwd_tile<NS, false, TS, KW, SYNC>(sh, g, V, views, N, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
                                             n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(TS::THREADS) void batch_bwd_exact_kernel(
     const float* __restrict__ dL_dcolor, const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig,
     const uint64_t* __restrict__ hit_masks, const uint32_t* __restrict__ tile_last, float* __restrict__ partial)
 {
-    __shared__ WindowLDS<KW> sh;
+    __shared__ WindowLDS<KW, TS::NW> sh;
     __shared__ uint32_t s_i;
     const uint32_t count = ctr[1];
     while (true)

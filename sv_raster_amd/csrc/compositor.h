@@ -405,12 +405,12 @@ struct PlainHit
 // window, the staged payload, the per-entry gradient accumulators, each
 // wave's mask of the entries its pixels used, and the entries' emission-order
 // rows.
-template <int KW>
+template <int KW, int NW = 4>
 struct WindowLDS
 {
     EntryPay pay[KW][64];
     float acc[KW][12][64];
-    uint64_t usedw[16][KW];
+    uint64_t usedw[NW][KW];
     uint32_t orow[KW][64];
 };
 
@@ -422,13 +422,15 @@ __device__ inline uint32_t tile_chunks(uint2 range, uint32_t tile_last)
     const uint32_t n = min(tile_last - range.x, range.y - range.x);
     return (n + 63u) >> 6;
 }
-// Chunks per window of the single-view backward.  With the branch-free hit
-// walk and view-constant staging, 4 was the fastest of 2, 3, 4 and 5 on the
-// 1080p scene (sync threshold 8); before them, 6 beat 2-5, 7 and 8.  Windows of
-// 7 and more leave room in LDS for only two workgroups per CU.  The batch
-// kernels take their depth from the batch (autotuned).
+// Chunks per window of the single-view backward.  On the 1 M-voxel 1080p
+// scene after 410 training steps (sync threshold 8) render_bwd took 0.697,
+// 0.653, 0.615, 0.601, 0.728, 0.725 and 0.720 ms with windows of 2 to 8
+// chunks: 5 chunks (44 KB of LDS, three workgroups per CU) cover most tiles'
+// contributors in one window; 4 (four workgroups per CU) need a second window
+// more often; from 6 on only two workgroups fit on a CU.  The batch kernels
+// take their depth from the batch (autotuned).
 #ifndef BWD_WINDOW_CHUNKS
-#define BWD_WINDOW_CHUNKS 4
+#define BWD_WINDOW_CHUNKS 5
 #endif
 constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 
@@ -462,14 +464,13 @@ constexpr int BWD_WINDOW = BWD_WINDOW_CHUNKS;
 // HIT holds the pixel's ray and running state and computes one hit's 12
 // terms (PlainHit here; the adjoint with the optional terms has its own).
 template <int SYNC_MIN, typename TS, int KW, typename HIT, typename INIT>
-__device__ inline void adjoint_window(WindowLDS<KW>& sh, int w, int lane, uint32_t C, INIT&& init, HIT& hit_fn,
+__device__ inline void adjoint_window(WindowLDS<KW, TS::NW>& sh, int w, int lane, uint32_t C, INIT&& init, HIT& hit_fn,
                                       uint2 range, const uint64_t* __restrict__ masks,
                                       const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
                                       const float* __restrict__ lengths, const float* __restrict__ geos,
                                       const float* __restrict__ rgbs, const uint32_t* __restrict__ orig,
                                       float* __restrict__ partial)
 {
-    static_assert(TS::NW <= 16, "per-wave entry masks are kept for at most 16 waves");
     constexpr size_t STRIDE = 64 * TS::NW;
     constexpr int PER = (KW * 64 + TS::THREADS - 1) / TS::THREADS;  // window entries per thread
     const int t = threadIdx.x;
