@@ -250,6 +250,16 @@ int svr_unbiased_adam_step_device(int32_t sparse, int64_t numel, float* param, c
                                   float* exp_avg_sq, int32_t* step_dev, const void* bias_table,
                                   const uint32_t* skip_dev, float* coef_dev, double lr, double beta1, double beta2,
                                   float eps, svr_stream_t stream);
+/* SparseAdam.step over several tensors in one launch (sparse_adam.py:43-72
+ * calls the per-tensor step once per parameter): tensor i of `count` takes
+ * the update of svr_unbiased_adam_step (biased = 0; step count steps[i]) or
+ * svr_biased_adam_step (biased = 1) with its own lr, betas and eps, bit-equal
+ * to those calls.  The arrays are host arrays of `count` entries; the tensors
+ * are device pointers. */
+int svr_adam_step_multi(int32_t biased, int32_t sparse, int32_t count, const int64_t* numels, float* const* params,
+                        const float* const* grads, float* const* exp_avgs, float* const* exp_avg_sqs,
+                        const double* steps, const double* lrs, const double* beta1s, const double* beta2s,
+                        const float* eps, svr_stream_t stream);
 /* Replaces `_C.biased_adam_step` (adam_step.cu:139-162). */
 int svr_biased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,
                          float* exp_avg_sq, float lr, float beta1, float beta2, float eps, svr_stream_t stream);

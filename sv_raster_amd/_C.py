@@ -363,6 +363,37 @@ def unbiased_adam_step(sparse, param, grad, exp_avg, exp_avg_sq, step, lr, beta1
                                          float(eps), _stream(param.device)), "unbiased_adam_step")
 
 
+def adam_step_multi(biased, sparse, items):
+    """Several tensors' Adam updates in one launch (svr_adam_step_multi; no reference counterpart, the reference
+    calls unbiased_adam_step / biased_adam_step once per tensor): items is a list of (param, grad, exp_avg,
+    exp_avg_sq, step, lr, beta1, beta2, eps) with every tensor on one device; each update is bit-equal to the
+    per-tensor call."""
+    if not items:
+        return
+    dev = items[0][0].device
+    for it in items:
+        _check_adam(*it[:4])
+        if it[0].device != dev:
+            raise RuntimeError("adam_step_multi: tensors on different devices")
+        if not (it[0].numel() == it[1].numel() == it[2].numel() == it[3].numel()):
+            raise RuntimeError("adam_step_multi: param, grad and moments differ in size")
+    n = len(items)
+    arr = lambda ct, vals: (ct * n)(*vals)  # noqa: E731
+    _lib.check(_L.svr_adam_step_multi(
+        int(bool(biased)), int(bool(sparse)), n,
+        arr(ctypes.c_int64, [it[0].numel() for it in items]),
+        arr(ctypes.c_void_p, [_ptr(it[0]) for it in items]),
+        arr(ctypes.c_void_p, [_ptr(it[1]) for it in items]),
+        arr(ctypes.c_void_p, [_ptr(it[2]) for it in items]),
+        arr(ctypes.c_void_p, [_ptr(it[3]) for it in items]),
+        arr(ctypes.c_double, [float(it[4]) for it in items]),
+        arr(ctypes.c_double, [float(it[5]) for it in items]),
+        arr(ctypes.c_double, [float(it[6]) for it in items]),
+        arr(ctypes.c_double, [float(it[7]) for it in items]),
+        arr(ctypes.c_float, [float(it[8]) for it in items]),
+        _stream(dev)), "adam_step_multi")
+
+
 def adam_bias_table(first_step, count, lr, beta1, beta2, device):
     """Device table of the host-form bias terms (svr_adam_bias_terms) of steps first_step .. first_step + count - 1:
     int32 [first_step, count] followed by count (step_size, rb2) float32 pairs, in one int32 tensor."""

@@ -72,6 +72,7 @@ SIGNATURES = {
     "svr_unbiased_adam_step": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_f64, c_f64, c_f64, c_f64, c_f32,
                                        c_p]),
     "svr_biased_adam_step": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_f32, c_f32, c_f32, c_f32, c_p]),
+    "svr_adam_step_multi": (c_i32, [c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_voxel_order_rank": (c_i32, [c_i64, c_p, c_p, c_p]),
     "svr_ijk_2_octpath": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),
     "svr_octpath_2_ijk": (c_i32, [c_i64, c_p, c_p, c_p, c_p]),

@@ -401,6 +401,91 @@ __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict
     }
 }
 
+// Several parameter tensors in one launch (SparseAdam.step over its groups):
+// workgroups [wg0[k], wg0[k + 1]) update tensor k with the body of
+// adam4_kernel and the tensor's own step size, bias term, betas and eps.  All
+// arrays 16-byte aligned (the caller sends other tensors through adam_launch).
+constexpr int ADAM_MULTI_MAX = 8;
+struct AdamMulti
+{
+    int count;
+    int64_t n[ADAM_MULTI_MAX];
+    int64_t wg0[ADAM_MULTI_MAX + 1];
+    float* p[ADAM_MULTI_MAX];
+    const float* g[ADAM_MULTI_MAX];
+    float* m[ADAM_MULTI_MAX];
+    float* v[ADAM_MULTI_MAX];
+    float step_size[ADAM_MULTI_MAX], rb2[ADAM_MULTI_MAX], beta1[ADAM_MULTI_MAX], beta2[ADAM_MULTI_MAX],
+        eps[ADAM_MULTI_MAX];
+};
+
+template <bool SPARSE, bool BIASED>
+__global__ __launch_bounds__(256) void adam4_multi_kernel(AdamMulti a)
+{
+    int k = 0;
+    while (k + 1 < a.count && (int64_t)blockIdx.x >= a.wg0[k + 1]) ++k;
+    const int64_t b = (int64_t)blockIdx.x - a.wg0[k];
+    const int64_t n = a.n[k];
+    const int64_t n4 = n >> 2;
+    const int64_t stride = (a.wg0[k + 1] - a.wg0[k]) * 256;
+    const int64_t gid = b * 256 + threadIdx.x;
+    const float step_size = a.step_size[k], rb2 = a.rb2[k], beta1 = a.beta1[k], beta2 = a.beta2[k], eps = a.eps[k];
+    float* __restrict__ p = a.p[k];
+    const float* __restrict__ g = a.g[k];
+    float* __restrict__ m = a.m[k];
+    float* __restrict__ v = a.v[k];
+    float4* __restrict__ P = reinterpret_cast<float4*>(p);
+    const float4* __restrict__ G = reinterpret_cast<const float4*>(g);
+    float4* __restrict__ M = reinterpret_cast<float4*>(m);
+    float4* __restrict__ V = reinterpret_cast<float4*>(v);
+    for (int64_t base = gid; base < n4; base += ADAM_UNROLL * stride)
+    {
+        float4 g4[ADAM_UNROLL], p4[ADAM_UNROLL], m4[ADAM_UNROLL], v4[ADAM_UNROLL];
+        bool on[ADAM_UNROLL];
+#pragma unroll
+        for (int u = 0; u < ADAM_UNROLL; ++u)
+        {
+            const int64_t i = base + u * stride;
+            g4[u] = i < n4 ? G[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            on[u] = i < n4;
+        }
+#pragma unroll
+        for (int u = 0; u < ADAM_UNROLL; ++u)
+        {
+            if (SPARSE) on[u] = on[u] && !(g4[u].x == 0.f && g4[u].y == 0.f && g4[u].z == 0.f && g4[u].w == 0.f);
+            if (on[u])
+            {
+                const int64_t i = base + u * stride;
+                p4[u] = P[i];
+                m4[u] = M[i];
+                v4[u] = V[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ADAM_UNROLL; ++u)
+        {
+            if (!on[u]) continue;
+            const int64_t i = base + u * stride;
+            adam_elem<SPARSE, BIASED>(p4[u].x, m4[u].x, v4[u].x, g4[u].x, step_size, beta1, beta2, rb2, eps);
+            adam_elem<SPARSE, BIASED>(p4[u].y, m4[u].y, v4[u].y, g4[u].y, step_size, beta1, beta2, rb2, eps);
+            adam_elem<SPARSE, BIASED>(p4[u].z, m4[u].z, v4[u].z, g4[u].z, step_size, beta1, beta2, rb2, eps);
+            adam_elem<SPARSE, BIASED>(p4[u].w, m4[u].w, v4[u].w, g4[u].w, step_size, beta1, beta2, rb2, eps);
+            P[i] = p4[u];
+            M[i] = m4[u];
+            V[i] = v4[u];
+        }
+    }
+    const int64_t t = 4 * n4 + gid;
+    if (t < n)
+    {
+        float pv = p[t], mv = m[t], vv = v[t];
+        adam_elem<SPARSE, BIASED>(pv, mv, vv, g[t], step_size, beta1, beta2, rb2, eps);
+        p[t] = pv;
+        m[t] = mv;
+        v[t] = vv;
+    }
+}
+
 template <bool BIASED>
 void adam_launch(bool sparse, int64_t n, float* p, const float* g, float* m, float* v, float step_size, float beta1,
                  float beta2, float rb2, float eps, const float* coef, hipStream_t stream)
@@ -628,6 +713,73 @@ int svr_unbiased_adam_step_device(int32_t sparse, int64_t n, float* p, const flo
         adam_launch<false>(sparse != 0, n, p, g, m, v, 0.f, (float)beta1, (float)beta2, 0.f, eps, coef_dev, stream);
         SVR_CHECK_LAUNCH();
     }
+    prof_end(PROF_ADAM, stream);
+    return SVR_OK;
+}
+
+int svr_adam_step_multi(int32_t biased, int32_t sparse, int32_t count, const int64_t* numels, float* const* params,
+                        const float* const* grads, float* const* exp_avgs, float* const* exp_avg_sqs,
+                        const double* steps, const double* lrs, const double* beta1s, const double* beta2s,
+                        const float* eps, svr_stream_t stream)
+{
+    SVR_REQUIRE(count >= 0, SVR_ERR_INVALID, "svr_adam_step_multi: negative tensor count %d", (int)count);
+    if (count == 0) return SVR_OK;
+    SVR_REQUIRE(numels && params && grads && exp_avgs && exp_avg_sqs && steps && lrs && beta1s && beta2s && eps,
+                SVR_ERR_INVALID, "svr_adam_step_multi: null argument array");
+    prof_begin(PROF_ADAM, stream);
+    AdamMulti a{};
+    auto flush = [&]() {
+        if (a.count == 0) return;
+        const unsigned grid = (unsigned)a.wg0[a.count];
+        if (biased)
+        {
+            if (sparse) adam4_multi_kernel<true, true><<<grid, 256, 0, stream>>>(a);
+            else adam4_multi_kernel<false, true><<<grid, 256, 0, stream>>>(a);
+        }
+        else
+        {
+            if (sparse) adam4_multi_kernel<true, false><<<grid, 256, 0, stream>>>(a);
+            else adam4_multi_kernel<false, false><<<grid, 256, 0, stream>>>(a);
+        }
+        a = AdamMulti{};
+    };
+    for (int32_t i = 0; i < count; ++i)
+    {
+        const int64_t n = numels[i];
+        if (n <= 0) continue;
+        float* p = params[i];
+        const float* g = grads[i];
+        float* m = exp_avgs[i];
+        float* v = exp_avg_sqs[i];
+        SVR_REQUIRE(p && g && m && v, SVR_ERR_INVALID, "svr_adam_step_multi: null array of tensor %d", (int)i);
+        float step_size = 0.f, rb2 = 1.f;
+        if (biased)
+            step_size = (float)lrs[i];
+        else
+            svr_adam_bias_terms(steps[i], lrs[i], beta1s[i], beta2s[i], &step_size, &rb2);
+        const bool vec = ((((uintptr_t)p) | ((uintptr_t)g) | ((uintptr_t)m) | ((uintptr_t)v)) & 15u) == 0;
+        if (!vec)
+        {
+            if (biased)
+                adam_launch<true>(sparse != 0, n, p, g, m, v, step_size, (float)beta1s[i], (float)beta2s[i], rb2,
+                                  eps[i], nullptr, stream);
+            else
+                adam_launch<false>(sparse != 0, n, p, g, m, v, step_size, (float)beta1s[i], (float)beta2s[i], rb2,
+                                   eps[i], nullptr, stream);
+            continue;
+        }
+        const int k = a.count;
+        const int64_t n4 = n >> 2;
+        a.n[k] = n;
+        a.p[k] = p; a.g[k] = g; a.m[k] = m; a.v[k] = v;
+        a.step_size[k] = step_size; a.rb2[k] = rb2;
+        a.beta1[k] = (float)beta1s[i]; a.beta2[k] = (float)beta2s[i]; a.eps[k] = eps[i];
+        a.wg0[k + 1] = a.wg0[k] + grid_for(n4 > 0 ? (n4 + ADAM_UNROLL - 1) / ADAM_UNROLL : 1, 16384);
+        a.count = k + 1;
+        if (a.count == ADAM_MULTI_MAX) flush();
+    }
+    flush();
+    SVR_CHECK_LAUNCH();
     prof_end(PROF_ADAM, stream);
     return SVR_OK;
 }

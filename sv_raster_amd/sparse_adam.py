@@ -92,6 +92,8 @@ class SparseAdam(torch.optim.Optimizer):
             if not torch.cuda.is_current_stream_capturing():
                 self.note_device_steps(1)
             return
+        # every tensor's update (the reference's per-tensor step, adam_step.cu:106-162) in one launch per device
+        items = {}
         for group in self.param_groups:
             lr = group['lr']
             beta1, beta2 = group['betas']
@@ -105,9 +107,7 @@ class SparseAdam(torch.optim.Optimizer):
                     state['exp_avg'] = torch.zeros_like(param, memory_format=torch.preserve_format)
                     state['exp_avg_sq'] = torch.zeros_like(param, memory_format=torch.preserve_format)
                 state['step'] += 1
-                if self.biased:
-                    _C.biased_adam_step(self.sparse, param, param.grad, state['exp_avg'], state['exp_avg_sq'],
-                                        lr, beta1, beta2, eps)
-                else:
-                    _C.unbiased_adam_step(self.sparse, param, param.grad, state['exp_avg'], state['exp_avg_sq'],
-                                          state['step'], lr, beta1, beta2, eps)
+                items.setdefault(param.device, []).append(
+                    (param, param.grad, state['exp_avg'], state['exp_avg_sq'], state['step'], lr, beta1, beta2, eps))
+        for its in items.values():
+            _C.adam_step_multi(self.biased, self.sparse, its)

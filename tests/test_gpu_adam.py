@@ -115,3 +115,66 @@ def test_adam_device_step_skip_changes_nothing(sparse):
     oh.step()
     torch.cuda.synchronize()
     assert torch.equal(pd, ph)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sparse,biased", [(True, False), (False, False), (True, True)])
+def test_adam_multi_matches_oracle(sparse, biased):
+    """One launch over several tensors (SparseAdam.step's path): each tensor, with its own step count, lr,
+    betas and eps, equals the oracle bit for bit.  Eleven tensors (two launches of at most eight), sizes on
+    and off multiples of 4, one view at a 4-byte offset (sent through the per-tensor kernel)."""
+    import torch
+    from oracle import oracle as O
+    from sv_raster_amd import _C
+    sizes = [1, 7, 4096, 100003, 64, 5, 12, 333, 8, 100000, 3]
+    print(f"adam multi sparse={sparse} biased={biased}", flush=True)
+    host, dev, hyper = [], [], []
+    for i, n in enumerate(sizes):
+        p, g, m, v = _case(n, seed=100 + i, density=0.4)
+        offset = 1 if i == 3 else 0
+        ts = []
+        for a in (p, g, m, v):
+            buf = torch.zeros(a.size + offset, dtype=torch.float32, device="cuda")
+            buf[offset:] = torch.from_numpy(a).cuda()
+            ts.append(buf[offset:])
+        host.append([p, g, m, v])
+        dev.append(ts)
+        hyper.append((0.01 * (1 + i), 0.1 + 0.02 * i, 0.99 - 0.001 * i, 1e-15 * (1 + i)))
+    for k in range(1, 4):
+        steps = [k + i for i in range(len(sizes))]
+        _C.adam_step_multi(biased, sparse, [(t[0], t[1], t[2], t[3], s, *h) for t, s, h in zip(dev, steps, hyper)])
+        for (p, g, m, v), s, (lr, b1, b2, eps) in zip(host, steps, hyper):
+            O.adam_step(p, g, m, v, s, lr, b1, b2, eps, biased=biased, sparse=sparse)
+    torch.cuda.synchronize()
+    for i, (ts, hs) in enumerate(zip(dev, host)):
+        for name, a, b in zip(("param", "grad", "exp_avg", "exp_avg_sq"), ts, hs):
+            assert np.array_equal(a.cpu().numpy().view(np.uint32), b.view(np.uint32)), (i, name)
+
+
+@pytest.mark.gpu
+def test_sparse_adam_step_matches_oracle():
+    """SparseAdam.step (three parameter groups with their own lr, sparse) against the oracle, bit for bit."""
+    import torch
+    from oracle import oracle as O
+    from sv_raster_amd import sparse_adam
+    sizes, lrs = [8000, 3000, 45000], [0.025, 0.01, 0.01]
+    params, host = [], []
+    for i, n in enumerate(sizes):
+        p, g, _, _ = _case(n, seed=200 + i, density=0.6)
+        t = torch.from_numpy(p).cuda().requires_grad_()
+        params.append(t)
+        host.append([p, g, np.zeros_like(p), np.zeros_like(p)])
+    opt = sparse_adam.SparseAdam([{"params": [t], "lr": lr} for t, lr in zip(params, lrs)], betas=(0.1, 0.99),
+                                 eps=1e-15, sparse=True)
+    for step in (1, 2, 3):
+        for t, h in zip(params, host):
+            t.grad = torch.from_numpy(h[1]).cuda()
+        opt.step()
+        for h, lr in zip(host, lrs):
+            O.adam_step(h[0], h[1], h[2], h[3], step, lr, 0.1, 0.99, 1e-15, sparse=True)
+    torch.cuda.synchronize()
+    for t, h in zip(params, host):
+        assert np.array_equal(t.detach().cpu().numpy().view(np.uint32), h[0].view(np.uint32))
+        st = opt.state[t]
+        assert np.array_equal(st["exp_avg"].cpu().numpy().view(np.uint32), h[2].view(np.uint32))
+        assert np.array_equal(st["exp_avg_sq"].cpu().numpy().view(np.uint32), h[3].view(np.uint32))
