@@ -134,7 +134,8 @@ size_t svr_backward_workspace_size(int64_t num_rendered);
  * are summed per list entry in `workspace`, then per voxel over its entries.
  * Writes every row of dL_dgeos [N,8], dL_drgbs [N,3], subdiv_p_bw [N,1]
  * (zero for voxels with no entry).  geos, dL_dgeos and workspace must be
- * 16-byte aligned. */
+ * 16-byte aligned.  dL_dout_T may be NULL (no loss on T: the same as zeros;
+ * the batch backward below takes NULL the same way). */
 int svr_rasterize_backward(int32_t n_samp, int32_t width, int32_t height, float tanfovx, float tanfovy, float cx,
                            float cy, const float* c2w, float bg_color, int64_t n_voxels, const float* vox_centers,
                            const float* vox_lengths, const float* geos, const float* rgbs, const void* geom_state,

@@ -126,7 +126,7 @@ def rasterize_preprocess(image_width, image_height, tan_fovx, tan_fovy, cx, cy, 
     _require_cuda(w2c_matrix, c2w_matrix, vox_centers, vox_lengths)
     dev = vox_centers.device
     N = vox_centers.size(0)
-    out_n_dup = torch.zeros(N, dtype=torch.int32, device=dev)
+    out_n_dup = torch.empty(N, dtype=torch.int32, device=dev)  # the preprocess writes every voxel's count
     geom = torch.empty(int(_L.svr_geom_state_size(N)) if N > 0 else 0, dtype=torch.uint8, device=dev)
     if N == 0:
         return out_n_dup, geom
@@ -251,7 +251,7 @@ def rasterize_voxels_backward(R, n_samp_per_vox, image_width, image_height, tan_
     dsub = torch.empty((N, 1), dtype=torch.float32, device=dev)
     centers, lengths, geos_c, rgbs_c = _f32(vox_centers), _f32(vox_lengths), _f32_a16(geos), _f32(rgbs)
     dcol = _f32(dL_dout_color)
-    dT = _f32(dL_dout_T) if dL_dout_T is not None else torch.zeros((1, H, W), device=dev)
+    dT = _f32(dL_dout_T) if dL_dout_T is not None else None  # the plain backward takes no T loss as NULL
     work = torch.empty(int(_L.svr_backward_workspace_size(int(R))), dtype=torch.uint8, device=dev)
     if need_depth or need_normal or lambda_R_concen > 0 or lambda_ascending > 0 or lambda_dist > 0:
         # backward.cu:160-510: depth / normal gradients and the regularizers
@@ -267,7 +267,8 @@ def rasterize_voxels_backward(R, n_samp_per_vox, image_width, image_height, tan_
             int(n_samp_per_vox), W, H, float(tan_fovx), float(tan_fovy), float(cx), float(cy),
             _ptr(_f32(c2w_matrix)), float(bg_color), N, _ptr(centers), _ptr(lengths), _ptr(geos_c), _ptr(rgbs_c),
             _ptr(geomBuffer), int(R), _ptr(binningBuffer), _ptr(imageBuffer), _ptr(_f32(out_T)), _ptr(dcol),
-            _ptr(dD), _ptr(dN), _ptr(dT), float(lambda_R_concen), _ptr(gt), float(lambda_ascending),
+            _ptr(dD), _ptr(dN), _ptr(dT if dT is not None else torch.zeros((1, H, W), device=dev)),
+            float(lambda_R_concen), _ptr(gt), float(lambda_ascending),
             float(lambda_dist), int(bool(need_depth)), int(bool(need_normal)), _ptr(oD), _ptr(work), _ptr(dgeos),
             _ptr(drgbs), _ptr(dsub), _stream(dev)), "rasterize_voxels_backward")
         _debug(debug, dev, "rasterize_voxels_backward")

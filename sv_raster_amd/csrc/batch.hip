@@ -769,7 +769,7 @@ static int batch_backward_impl(int32_t n_samp, int32_t V, const svr_view* views_
     int rc = check_views(V, views_host, &T, &P, &shape);
     if (rc != SVR_OK) return rc;
     if (N == 0) return SVR_OK;
-    SVR_REQUIRE(views_dev && geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub &&
+    SVR_REQUIRE(views_dev && geom_state && image_state && out_T && dL_dcolor && dgeos && drgbs && dsub &&
                     workspace,
                 SVR_ERR_INVALID, "svr_batch_rasterize_backward: null pointer");
     SVR_REQUIRE(((uintptr_t)geos & 15u) == 0 && ((uintptr_t)dgeos & 15u) == 0 && ((uintptr_t)workspace & 15u) == 0,

@@ -149,7 +149,8 @@ __device__ inline void batch_bwd_tile(WindowLDS<KW, TS::NW>& sh, uint32_t g, int
     adjoint_window<SYNC, TS, KW>(
         sh, w, lane, C,
         [&]() {
-            return hit.init(t.inside, pid, HW, vw.bg_color, out_T + pb, n_contrib + pb, dL_dcolor + 3 * pb, dL_dT + pb);
+            return hit.init(t.inside, pid, HW, vw.bg_color, out_T + pb, n_contrib + pb, dL_dcolor + 3 * pb,
+                            dL_dT ? dL_dT + pb : nullptr);
         },
         hit, range, masks, vox_list, centers, lengths, geos, rgbs + 3 * vo, orig, partial);
 }

@@ -392,7 +392,7 @@ struct PlainHit
         dp0 = dL_dcolor[pid];
         dp1 = dL_dcolor[HW + pid];
         dp2 = dL_dcolor[2 * HW + pid];
-        last_dT = dL_dT[pid] + bg * (dp0 + dp1 + dp2);
+        last_dT = (dL_dT ? dL_dT[pid] : 0.f) + bg * (dp0 + dp1 + dp2);  // dL_dT null: no loss on T
         return n_contrib[pid];
     }
     __device__ inline bool operator()(const EntryPay& p, float* gq, bool active)

@@ -463,7 +463,7 @@ int backward_setup(int32_t n_samp, int32_t W, int32_t H, const float* c2w, int64
     SVR_REQUIRE(n_samp >= 1 && n_samp <= MAX_N_SAMP, SVR_ERR_INVALID, "n_samp_per_vox should be in range [1, %d].",
                 MAX_N_SAMP);
     if (N == 0) return 1;
-    SVR_REQUIRE(geom_state && image_state && out_T && dL_dcolor && dL_dT && dgeos && drgbs && dsub && c2w,
+    SVR_REQUIRE(geom_state && image_state && out_T && dL_dcolor && dgeos && drgbs && dsub && c2w,
                 SVR_ERR_INVALID, "svr_rasterize_backward: null pointer");
     SVR_REQUIRE(((uintptr_t)geos & 15u) == 0 && ((uintptr_t)dgeos & 15u) == 0, SVR_ERR_INVALID,
                 "geos and dL_dgeos must be 16-byte aligned");

@@ -166,6 +166,9 @@ class _RasterizeVoxels(torch.autograd.Function):
             )
         ctx.raster_settings = raster_settings
         ctx.num_rendered = num_rendered
+        # outputs the loss does not use arrive as None (no zero tensors materialised): the backward takes a
+        # missing T gradient as none at all
+        ctx.set_materialize_grads(False)
         ctx.save_for_backward(
             octree_paths, vox_centers, vox_lengths,
             geos, rgbs,
@@ -180,6 +183,13 @@ class _RasterizeVoxels(torch.autograd.Function):
         octree_paths, vox_centers, vox_lengths, \
             geos, rgbs, \
             geomBuffer, binningBuffer, imgBuffer, out_T, out_depth, out_normal = ctx.saved_tensors
+        shape = (3, raster_settings.image_height, raster_settings.image_width)
+        if dL_dout_color is None:
+            dL_dout_color = torch.zeros(shape, device=geos.device)
+        if raster_settings.need_depth and dL_dout_depth is None:
+            dL_dout_depth = torch.zeros(shape, device=geos.device)
+        if raster_settings.need_normal and dL_dout_normal is None:
+            dL_dout_normal = torch.zeros(shape, device=geos.device)
 
         dL_dgeos, dL_drgbs, subdiv_p_bw = _C.rasterize_voxels_backward(
             num_rendered,

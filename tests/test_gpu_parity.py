@@ -179,3 +179,40 @@ def test_renderer_prepared_binning_matches_oracle(name, seed):
         assert rel_err(geos.grad.cpu().numpy(), o["dgeos"]) <= GRAD_RTOL
         assert rel_err(sh0.grad.cpu().numpy(), o["dsh0"]) <= GRAD_RTOL
         assert rel_err(shs.grad.cpu().numpy(), o["dshs"]) <= GRAD_RTOL
+
+
+@pytest.mark.gpu
+def test_backward_without_T_loss_equals_zero_T_gradient():
+    """A loss on the colour only: the drop-in autograd no longer materialises a zero T gradient, the C ABI
+    takes dL_dout_T = NULL; the gradients equal those of an explicit zero dL_dout_T, through
+    rasterize_voxels (config A, 800x800); within the gradient bar, since the LDS float atomics may add in a
+    different order from one run to the next."""
+    import torch
+    from oracle import oracle as O
+    from sv_raster_amd import renderer, synthetic as S
+    sc = S.make_scene("A", seed=2)
+    cam = S.make_camera(800, 800, seed=2)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
+    w2c, c2w = t(cam["w2c"]), t(cam["c2w"])
+    rgbs = t(O.sh_fwd(3, None, sc["vox_centers"], cam["c2w"][:3, 3], sc["sh0"], sc["shs"]).astype(np.float32))
+    target = t(S.make_target(800, 800, 2))
+    settings = renderer.RasterSettings(color_mode="sh", n_samp_per_vox=1, image_width=800, image_height=800,
+                                       tanfovx=cam["tanfovx"], tanfovy=cam["tanfovy"], cx=cam["cx"], cy=cam["cy"],
+                                       w2c_matrix=w2c, c2w_matrix=c2w)
+    grads = []
+    for use_T in (False, True):
+        geos = t(sc["geos"]).requires_grad_()
+        rg = rgbs.clone().requires_grad_()
+
+        def vox_fn(idx, cam_pos, color_mode):
+            return {"geos": geos, "rgbs": rg, "subdiv_p": torch.ones((centers.shape[0], 1), device="cuda")}
+
+        color, _, _, T, _ = renderer.rasterize_voxels(settings, octpath, centers, lengths, vox_fn)
+        loss = torch.nn.functional.mse_loss(color, target) + (0.0 * T.sum() if use_T else 0.0)
+        loss.backward()
+        grads.append((geos.grad.clone(), rg.grad.clone()))
+    torch.cuda.synchronize()
+    for a, b in zip(*grads):
+        assert a.abs().max() > 0
+        assert rel_err(a.cpu().numpy(), b.cpu().numpy()) <= GRAD_RTOL
