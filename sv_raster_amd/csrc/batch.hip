@@ -712,7 +712,6 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     }
     if (R > 0)
     {
-        SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
         rc = pair_offsets_launch(V, N, g, ot.pos, stream);
         if (rc != SVR_OK) return rc;
         // entries are emitted view by view in rank order, so the sort only

@@ -510,11 +510,11 @@ static const void* pass_kernel(int bits, int items)
 // out_keys / out_vals.
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
-                  const uint32_t* n_dev, int lo_bit, bool identity_vals)
+                  const uint32_t* n_dev, int lo_bit, bool identity_vals, void* also_zero, int64_t also_zero_n)
 {
     *out_keys = keys;
     if (out_vals) *out_vals = vals;
-    if (n <= 0 || nbits <= 0) return SVR_OK;
+    if (n <= 0 || nbits <= 0) return also_zero ? fill_u32(also_zero, 0u, also_zero_n, stream) : SVR_OK;
     SVR_REQUIRE(lo_bit >= 0 && lo_bit + nbits <= 64, SVR_ERR_INVALID, "radix sort bits [%d, %d) out of range", lo_bit,
                 lo_bit + nbits);
     const int64_t nb = sort_blocks(n);
@@ -523,7 +523,8 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
     SVR_REQUIRE(passes <= SORT_MAX_PASSES, SVR_ERR_INVALID, "radix sort of %d bits needs too many passes", nbits);
     const int step = (nbits + passes - 1) / passes;
     SortScratch sc = SortScratch::carve(counts, n, passes);
-    if (fill_u32(counts, 0u, sc.words, stream) != SVR_OK) return SVR_ERR_HIP;
+    if (fill_u32_pair(counts, sc.words, also_zero, also_zero ? also_zero_n : 0, 0u, stream) != SVR_OK)
+        return SVR_ERR_HIP;
     onesweep_hist_kernel<<<(unsigned)std::min<int64_t>(nb, SORT_HIST_GRID), SORT_THREADS, 0, stream>>>(
         keys, n, n_dev, lo_bit, step, nbits, passes, sc.hist);
     SVR_CHECK_LAUNCH();
@@ -644,8 +645,8 @@ __global__ __launch_bounds__(256) void pair_count_kernel(int64_t N, const uint32
 int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream)
 {
     const int64_t n8 = 8 * (int64_t)V * N;
-    if (n8 == 0) return SVR_OK;
-    if (fill_u32(g.pos_off, 0u, n8, stream) != SVR_OK) return SVR_ERR_HIP;
+    if (n8 == 0) return fill_u32(g.counters + 2, 0u, 2, stream);
+    if (fill_u32_pair(g.pos_off, n8, g.counters + 2, 2, 0u, stream) != SVR_OK) return SVR_ERR_HIP;
     pair_count_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(N, g.n_dup, g.qbits,
                                                                                          order_pos, g.pos_off);
     SVR_CHECK_LAUNCH();
@@ -887,7 +888,6 @@ int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t*
 
 int pair_prepare(int64_t N, const GeomState& g, const OrderTable& ot, hipStream_t stream)
 {
-    SVR_REQUIRE(fill_u32(g.counters + 2, 0u, 2, stream) == SVR_OK, SVR_ERR_HIP, "fill counters failed");
     return pair_offsets_launch(1, N, g, ot.pos, stream);
 }
 
@@ -897,9 +897,9 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     const int gx = tiles_x(W), gy = tiles_y(H);
     const int64_t T = (int64_t)gx * gy;
     prof_begin(PROF_BINNING, stream);
-    SVR_REQUIRE(fill_u32(img.ranges, 0u, 2 * T, stream) == SVR_OK, SVR_ERR_HIP, "fill ranges failed");
     if (R == 0)
     {
+        SVR_REQUIRE(fill_u32(img.ranges, 0u, 2 * T, stream) == SVR_OK, SVR_ERR_HIP, "fill ranges failed");
         const int rc0 = tile_queue_launch(T, img.ranges, img.qkeys, img.qkeys_alt, img.order, img.order_alt,
                                           img.qcounts, stream, img.ctr);
         prof_end(PROF_BINNING, stream);
@@ -923,8 +923,9 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
     uint32_t* sorted_orig = nullptr;
+    // the tile ranges (empty tiles stay (0, 0)) are cleared with the sort's scratch
     rc = radix_sort_ex(k0, odd ? b.keys : b.keys_alt, o0, odd ? b.orig : b.orig_alt, R, tile_bits, b.counts, stream,
-                       &sorted, &sorted_orig, nullptr, pos_bits, true);
+                       &sorted, &sorted_orig, nullptr, pos_bits, true, img.ranges, 2 * T);
     if (rc != SVR_OK) return rc;
     SVR_REQUIRE(sorted == b.keys && sorted_orig == b.orig, SVR_ERR_HIP, "sort ended in the wrong buffer");
     rc = ranges_launch(b.keys, R, pos_bits, ot.pair, img.ranges, b.vox_list, stream);

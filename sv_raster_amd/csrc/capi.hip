@@ -34,18 +34,60 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value, int6
     if (t0 < count - tail0) dst[tail0 + t0] = value;
 }
 
+// Two ranges in one launch: row blockIdx.y of the grid fills range y.
+__global__ void fill_u32_pair_kernel(uint32_t* __restrict__ d0, int64_t n0, int64_t h0, uint32_t* __restrict__ d1,
+                                     int64_t n1, int64_t h1, uint32_t value)
+{
+    uint32_t* dst = blockIdx.y ? d1 : d0;
+    const int64_t count = blockIdx.y ? n1 : n0, head = blockIdx.y ? h1 : h0;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nvec = (count - head) >> 2;
+    uint4* __restrict__ body = reinterpret_cast<uint4*>(dst + head);
+    const uint4 v4 = make_uint4(value, value, value, value);
+    for (int64_t i = t0; i < nvec; i += stride) body[i] = v4;
+    if (t0 < head) dst[t0] = value;
+    const int64_t tail0 = head + 4 * nvec;
+    if (t0 < count - tail0) dst[tail0 + t0] = value;
+}
+
+// Elements before the first 16-byte boundary of dst (at most count), and the
+// workgroups a fill of the rest takes (at least one: head and tail need the
+// first threads only; at most 4096, a grid-stride loop does the rest).
+static int64_t fill_head(const void* dst, int64_t count)
+{
+    const int64_t head = (int64_t)(((16u - ((uintptr_t)dst & 15u)) & 15u) >> 2);
+    return head > count ? count : head;
+}
+static int64_t fill_blocks(int64_t count, int64_t head)
+{
+    const int64_t b = ((count - head) / 4 + 255) / 256;
+    return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+}
+
 // Clears device memory with a kernel instead of hipMemsetAsync so that a
 // captured graph never records a memset node.
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return SVR_OK;
     SVR_REQUIRE(((uintptr_t)dst & 3u) == 0, SVR_ERR_INVALID, "fill: destination not 4-byte aligned");
-    int64_t head = (int64_t)(((16u - ((uintptr_t)dst & 15u)) & 15u) >> 2);
-    if (head > count) head = count;
-    int64_t blocks = ((count - head) / 4 + 255) / 256;
-    if (blocks < 1) blocks = 1;  // head and tail (at most 3 elements each) need the first threads only
-    if (blocks > 4096) blocks = 4096;
-    fill_u32_kernel<<<(unsigned)blocks, 256, 0, stream>>>((uint32_t*)dst, value, count, head);
+    const int64_t head = fill_head(dst, count);
+    fill_u32_kernel<<<(unsigned)fill_blocks(count, head), 256, 0, stream>>>((uint32_t*)dst, value, count, head);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
+}
+
+// fill_u32 of two ranges with one launch.
+int fill_u32_pair(void* dst0, int64_t n0, void* dst1, int64_t n1, uint32_t value, hipStream_t stream)
+{
+    if (n0 <= 0) return fill_u32(dst1, value, n1, stream);
+    if (n1 <= 0) return fill_u32(dst0, value, n0, stream);
+    SVR_REQUIRE((((uintptr_t)dst0 | (uintptr_t)dst1) & 3u) == 0, SVR_ERR_INVALID,
+                "fill: destination not 4-byte aligned");
+    const int64_t h0 = fill_head(dst0, n0), h1 = fill_head(dst1, n1);
+    const int64_t b0 = fill_blocks(n0, h0), b1 = fill_blocks(n1, h1);
+    fill_u32_pair_kernel<<<dim3((unsigned)(b0 > b1 ? b0 : b1), 2), 256, 0, stream>>>(
+        (uint32_t*)dst0, n0, h0, (uint32_t*)dst1, n1, h1, value);
     SVR_CHECK_LAUNCH();
     return SVR_OK;
 }

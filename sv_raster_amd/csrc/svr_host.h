@@ -303,14 +303,18 @@ __host__ __device__ inline int tiles_y(int H) { return (H + 15) / 16; }
 // keys' input positions (0, 1, 2, ...); the first pass generates them, so
 // `vals` need not be written beforehand (it is still the ping-pong buffer).
 inline int radix_passes(int nbits) { return (nbits + RADIX_BITS - 1) / RADIX_BITS; }
+// also_zero / also_zero_n: another buffer (uint32 words) cleared by the same
+// launch that clears the sort's scratch.
 int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int nbits,
                   uint32_t* counts, hipStream_t stream, uint64_t** out_keys, uint32_t** out_vals,
-                  const uint32_t* n_dev = nullptr, int lo_bit = 0, bool identity_vals = false);
+                  const uint32_t* n_dev = nullptr, int lo_bit = 0, bool identity_vals = false,
+                  void* also_zero = nullptr, int64_t also_zero_n = 0);
 
 // Entry offsets of the (item, quadrant) pairs in rank order (GeomState's
 // emission order): pos_off cleared, filled with each active pair's tile count
 // at view u's block [8 N u, 8 N (u + 1)) indexed by rank position, then
-// exclusive-scanned in place (only the active positions are written).
+// exclusive-scanned in place (only the active positions are written).  The
+// two words at g.counters + 2 are cleared by the same launch as pos_off.
 int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream);
 int pair_scan_launch(uint32_t* c, int64_t n, uint32_t* sums, hipStream_t stream);
 
@@ -325,6 +329,7 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
 int pair_prepare(int64_t N, const GeomState& g, const OrderTable& ot, hipStream_t stream);
 
 int fill_u32(void* dst, uint32_t value, int64_t count, hipStream_t stream);
+int fill_u32_pair(void* dst0, int64_t n0, void* dst1, int64_t n1, uint32_t value, hipStream_t stream);
 
 // A timing-disabled event of the calling thread for the current device,
 // created on first use and kept until the process ends.
