@@ -334,6 +334,14 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t n, float* __restrict_
 // loads in flight instead of one gradient load and one row load at a time.
 constexpr int ADAM_UNROLL = 4;
 
+// Parameters and moments are written once per step and not read again in it:
+// non-temporal stores (3-4 % faster on the 1 M-voxel Adam than plain ones).
+typedef float adam_v4f __attribute__((ext_vector_type(4)));
+__device__ inline void adam_store(float4* dst, const float4& x)
+{
+    __builtin_nontemporal_store(*reinterpret_cast<const adam_v4f*>(&x), reinterpret_cast<adam_v4f*>(dst));
+}
+
 template <bool SPARSE, bool BIASED>
 __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
                                                     float* __restrict__ m, float* __restrict__ v, float step_size,
@@ -385,9 +393,9 @@ __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict
             adam_elem<SPARSE, BIASED>(p4[u].y, m4[u].y, v4[u].y, g4[u].y, step_size, beta1, beta2, rb2, eps);
             adam_elem<SPARSE, BIASED>(p4[u].z, m4[u].z, v4[u].z, g4[u].z, step_size, beta1, beta2, rb2, eps);
             adam_elem<SPARSE, BIASED>(p4[u].w, m4[u].w, v4[u].w, g4[u].w, step_size, beta1, beta2, rb2, eps);
-            P[i] = p4[u];
-            M[i] = m4[u];
-            V[i] = v4[u];
+            adam_store(&P[i], p4[u]);
+            adam_store(&M[i], m4[u]);
+            adam_store(&V[i], v4[u]);
         }
     }
     const int64_t t = 4 * n4 + gid;
@@ -470,9 +478,9 @@ __global__ __launch_bounds__(256) void adam4_multi_kernel(AdamMulti a)
             adam_elem<SPARSE, BIASED>(p4[u].y, m4[u].y, v4[u].y, g4[u].y, step_size, beta1, beta2, rb2, eps);
             adam_elem<SPARSE, BIASED>(p4[u].z, m4[u].z, v4[u].z, g4[u].z, step_size, beta1, beta2, rb2, eps);
             adam_elem<SPARSE, BIASED>(p4[u].w, m4[u].w, v4[u].w, g4[u].w, step_size, beta1, beta2, rb2, eps);
-            P[i] = p4[u];
-            M[i] = m4[u];
-            V[i] = v4[u];
+            adam_store(&P[i], p4[u]);
+            adam_store(&M[i], m4[u]);
+            adam_store(&V[i], v4[u]);
         }
     }
     const int64_t t = 4 * n4 + gid;
