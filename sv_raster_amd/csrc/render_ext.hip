@@ -158,11 +158,17 @@ __device__ inline uint32_t composite_ext(StageT<NW>& st, uint32_t (&sv)[NW][64],
         uint64_t hits = 0ull;
         while (hm)
         {
-            const int l = __builtin_ctzll(hm);
-            hm &= hm - 1ull;
+            // a lane whose candidate misses the box tests its next ones in the same step, as in the
+            // plain forward
+            int l;
             float a, b;
-            ray_aabb_pre<EXACT>(st.lo[0][w][l], st.lo[1][w][l], st.lo[2][w][l], st.hi[0][w][l], st.hi[1][w][l],
-                                st.hi[2][w][l], ray, a, b);
+            do
+            {
+                l = __builtin_ctzll(hm);
+                hm &= hm - 1ull;
+                ray_aabb_pre<EXACT>(st.lo[0][w][l], st.lo[1][w][l], st.lo[2][w][l], st.hi[0][w][l],
+                                    st.hi[1][w][l], st.hi[2][w][l], ray, a, b);
+            } while ((a > b) && hm);
             if (a > b) continue;
             float g[8], lv[NS];
 #pragma unroll
