@@ -589,7 +589,10 @@ __device__ inline void adjoint_window(WindowLDS<KW, TS::NW>& sh, int w, int lane
                 const int ls = act ? l : 0;
                 hm &= act ? ~(1ull << ls) : ~0ull;
                 float gq[12];
-                const bool hit = hit_fn(sh.pay[j][ls], gq, act);
+                // the payload's byte offset by a 24-bit multiply (ls < 64), a full-rate instruction
+                const EntryPay& pay = *reinterpret_cast<const EntryPay*>(
+                    reinterpret_cast<const char*>(&sh.pay[j][0]) + __umul24((unsigned)ls, (unsigned)sizeof(EntryPay)));
+                const bool hit = hit_fn(pay, gq, act);
                 if (ready)
                 {
                     sync = uniform_u64(sync & ~(1ull << s));
@@ -599,8 +602,8 @@ __device__ inline void adjoint_window(WindowLDS<KW, TS::NW>& sh, int w, int lane
 #pragma unroll
                         for (int i = 0; i < 12; ++i) gs[i] = (is_sync && hit) ? gq[i] : 0.f;
                         const float r = wave_sum12(gs, lane);
-                        const int ri = (lane >> 2) & 15;
-                        if ((lane & 3) == 0 && ri < 12) atomicAdd(&acc[ri][s], r);
+                        const int ri = wave_sum12_index(lane);
+                        if ((lane & 3) == 0 && ri >= 0) atomicAdd(&acc[ri][s], r);
                     }
                 }
                 if (hit && !is_sync)

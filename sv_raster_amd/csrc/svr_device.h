@@ -401,49 +401,51 @@ __device__ inline float dpp_f32(float v)
 
 // Sums of 12 values over the 64 lanes of a wave (all lanes must be active) by
 // a halving butterfly: each exchange step keeps half of the remaining values
-// in each lane and adds its partner's copy of them (lanes i and i ^ 32 with
-// v_permlane32_swap, i ^ 16 with v_permlane16_swap, i ^ 8 with a DPP row
-// rotation, i ^ 7 with a half-row mirror), then two DPP steps finish the sum.
-// About 35 instructions for all 12 sums instead of 12 separate reductions.
-// Lane i returns the total of value (i >> 2) & 15 (indices 12..15 are zero).
+// in each lane and adds its partner's copy of them.  Lanes i and i ^ 32
+// (v_permlane32_swap) pair value k with value k + 6, leaving 6 registers of 2
+// values; lanes i and i ^ 16 (v_permlane16_swap) pair those registers two by
+// two, leaving 3 registers whose 16-lane row r holds value
+// 2k + (r & 1) + 6 (r >> 1) of register k.  Within the rows, a DPP row rotation
+// (i ^ 8) adds registers 0 and 1 into one register (lanes with bit 3 clear
+// keep register 0) and register 2 into another, a half-row mirror (i ^ 7)
+// adds those two (lanes with bit 2 clear keep the first), and two DPP steps
+// finish the sum.  About 28 instructions for all 12 sums instead of 12
+// separate reductions.  Lane i returns the total of value
+// wave_sum12_index(i), or a copy of another lane's total where that is -1.
 __device__ inline float wave_sum12(const float (&g)[12], int lane)
 {
-    float v[16];
+    float u[6];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = g[k];
-#pragma unroll
-    for (int k = 12; k < 16; ++k) v[k] = 0.f;
-    float u[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
+    for (int k = 0; k < 6; ++k)
     {
-        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[k]), __float_as_uint(v[k + 8]), false,
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(g[k]), __float_as_uint(g[k + 6]), false,
                                                         false);
         u[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
-    float w[4];
+    float w[3];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 3; ++k)
     {
-        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(u[k]), __float_as_uint(u[k + 4]), false,
-                                                        false);
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(u[2 * k]), __float_as_uint(u[2 * k + 1]),
+                                                        false, false);
         w[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
     const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
-    float x[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-    {
-        const float keep = b3 ? w[k + 2] : w[k];
-        const float send = b3 ? w[k] : w[k + 2];
-        x[k] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x128, 0xF, 0xF, false));
-    }
-    const float keep = b2 ? x[1] : x[0];
-    const float send = b2 ? x[0] : x[1];
-    float y = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), 0x141, 0xF, 0xF, false));
-    y = y + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(y), 0xB1, 0xF, 0xF, false));
-    y = y + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(y), 0x4E, 0xF, 0xF, false));
+    const float x0 = (b3 ? w[1] : w[0]) + dpp_f32<0x128>(b3 ? w[0] : w[1]);  // row_ror:8
+    const float x1 = w[2] + dpp_f32<0x128>(w[2]);
+    float y = (b2 ? x1 : x0) + dpp_f32<0x141>(b2 ? x0 : x1);  // row_half_mirror
+    y = y + dpp_f32<0xB1>(y);  // quad_perm [1,0,3,2]
+    y = y + dpp_f32<0x4E>(y);  // quad_perm [2,3,0,1]
     return y;
+}
+
+// The value whose total wave_sum12 returns in lane i (lanes 4g .. 4g + 3 hold
+// the same one), or -1 where the lane holds a copy of register 2's total that
+// the lanes 8 below it hold as well.
+__device__ inline int wave_sum12_index(int lane)
+{
+    const int r = lane >> 4, h = (lane >> 3) & 1, q = (lane >> 2) & 1;
+    return (q && h) ? -1 : (q ? 4 : 2 * h) + (r & 1) + 6 * (r >> 1);
 }
 
 // Sum over the 64 lanes of a wave (all lanes must be active); every lane gets
