@@ -227,6 +227,20 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
                       const float* vox_centers, const float* cam_pos, const float* dL_drgbs, float* dL_dsh0,
                       float* dL_dshs, svr_stream_t stream);
 
+/* svr_sh_compute / svr_sh_compute_bw over the in-frustum voxels of a
+ * preprocess geometry state (n_dup > 0: the list svr_visible_voxels returns
+ * for it), without the index list: the same rgbs / gradients as the calls with
+ * that list (sh_compute.cu:244-299 with idx = renderer.py:81's in-frustum
+ * ids), minus the zero fill and marking pass they need.  M must be 16 and
+ * sh0, shs, rgbs (dL_dsh0, dL_dshs) 16-byte aligned; SVR_ERR_INVALID
+ * otherwise. */
+int svr_sh_compute_visible(int32_t deg, int64_t n_voxels, const void* geom_state, const float* vox_centers,
+                           const float* cam_pos, const float* sh0, const float* shs, int32_t M, float* rgbs,
+                           svr_stream_t stream);
+int svr_sh_compute_bw_visible(int32_t deg, int32_t M, int64_t n_voxels, const void* geom_state,
+                              const float* vox_centers, const float* cam_pos, const float* dL_drgbs, float* dL_dsh0,
+                              float* dL_dshs, svr_stream_t stream);
+
 /* ---- optimizer -------------------------------------------------------- */
 /* Replaces `_C.unbiased_adam_step` (adam_step.cu:106-137); sparse skips grad == 0. */
 int svr_unbiased_adam_step(int32_t sparse, int64_t numel, float* param, const float* grad, float* exp_avg,

@@ -140,6 +140,35 @@ def rasterize_preprocess(image_width, image_height, tan_fovx, tan_fovy, cx, cy, 
     return out_n_dup, geom
 
 
+class _VisibleLists:
+    """The in-frustum index lists visible_voxels returned lately, each with the geometry state it came from: for
+    such a list the SH passes take the listed rows from the state's entry counts (n_dup > 0 lists exactly those
+    voxels) instead of zero-filling and marking them first.  A list is recognised by identity, in-place version,
+    storage and length; the entries hold weak references, so they keep nothing alive."""
+
+    def __init__(self, capacity=4):
+        self.capacity = capacity
+        self.entries = []
+
+    def add(self, idx, geom, n_voxels):
+        self.entries.insert(0, (weakref.ref(idx), idx._version, idx.data_ptr(), idx.numel(), weakref.ref(geom),
+                                int(n_voxels)))
+        del self.entries[self.capacity:]
+
+    def geom_of(self, idx, n_voxels):
+        """The geometry state whose in-frustum list idx is (for n_voxels voxels), or None."""
+        if idx is None or not torch.is_tensor(idx):
+            return None
+        for ref, ver, ptr, n, gref, nv in self.entries:
+            if ref() is idx and idx._version == ver and idx.data_ptr() == ptr and idx.numel() == n \
+                    and nv == int(n_voxels):
+                return gref()
+        return None
+
+
+visible_lists = _VisibleLists()
+
+
 def visible_voxels(geomBuffer, n_voxels, device, order_table=None):
     """In-frustum voxel ids (int64, ascending; what renderer.py:81 gets from torch.where(n_duplicates > 0)[0])
     and the number of rendered entries R, from the geometry state of rasterize_preprocess, with one host
@@ -152,7 +181,9 @@ def visible_voxels(geomBuffer, n_voxels, device, order_table=None):
     R, nv = ctypes.c_int64(0), ctypes.c_int64(0)
     _lib.check(_L.svr_visible_voxels(N, _ptr(geomBuffer), _ptr(order_table), _ptr(idx), _stream(device),
                                      ctypes.byref(R), ctypes.byref(nv)), "visible_voxels")
-    return idx[:int(nv.value)], int(R.value)
+    idx = idx[:int(nv.value)]
+    visible_lists.add(idx, geomBuffer, N)
+    return idx, int(R.value)
 
 
 def rasterize_binning(image_width, image_height, octree_paths, geomBuffer, num_rendered, pairs_ready=False):
@@ -323,12 +354,19 @@ def export_sort_keys(image_width, image_height, R, binningBuffer, octree_paths):
 
 
 # binding.cpp:36-37 -> sh_compute.cu:244-299
-def sh_compute(D, indices, vox_centers, cam_pos, sh0, shs):
+def sh_compute(D, indices, vox_centers, cam_pos, sh0, shs, visible=None):
+    """visible: the geometry state whose in-frustum list `indices` is (visible_lists.geom_of), or None; with it
+    the 16-coefficient layout takes the listed rows from the state (svr_sh_compute_visible, same rgbs)."""
     _require_cuda(vox_centers, cam_pos, sh0, shs)
     dev = vox_centers.device
     N = vox_centers.size(0)
     M = 1 + shs.size(1)
     rgbs = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    if visible is not None and M == 16 and N > 0:
+        _lib.check(_L.svr_sh_compute_visible(int(D), N, _ptr(visible), _ptr(_f32(vox_centers)), _ptr(_f32(cam_pos)),
+                                             _ptr(_f32_a16(sh0)), _ptr(_f32_a16(shs)), M, _ptr(rgbs), _stream(dev)),
+                   "sh_compute")
+        return rgbs
     idx = indices.to(device=dev, dtype=torch.int64).contiguous() if indices is not None else None
     n_idx = idx.numel() if idx is not None else 0
     _lib.check(_L.svr_sh_compute(int(D), n_idx, _ptr(idx), N, _ptr(_f32(vox_centers)), _ptr(_f32(cam_pos)),
@@ -336,11 +374,17 @@ def sh_compute(D, indices, vox_centers, cam_pos, sh0, shs):
     return rgbs
 
 
-def sh_compute_bw(D, M, indices, vox_centers, cam_pos, rgbs, dL_drgbs):
+def sh_compute_bw(D, M, indices, vox_centers, cam_pos, rgbs, dL_drgbs, visible=None):
+    """visible: as for sh_compute (svr_sh_compute_bw_visible, same gradients)."""
     dev = vox_centers.device
     N = vox_centers.size(0)
     dsh0 = torch.empty((N, 3), dtype=torch.float32, device=dev)
     dshs = torch.empty((N, M - 1, 3), dtype=torch.float32, device=dev)
+    if visible is not None and M == 16 and N > 0:
+        _lib.check(_L.svr_sh_compute_bw_visible(int(D), int(M), N, _ptr(visible), _ptr(_f32(vox_centers)),
+                                                _ptr(_f32(cam_pos)), _ptr(_f32(dL_drgbs)), _ptr(dsh0), _ptr(dshs),
+                                                _stream(dev)), "sh_compute_bw")
+        return dsh0, dshs
     idx = indices.to(device=dev, dtype=torch.int64).contiguous() if indices is not None else None
     n_idx = idx.numel() if idx is not None else 0
     _lib.check(_L.svr_sh_compute_bw(int(D), int(M), n_idx, _ptr(idx), N, _ptr(_f32(vox_centers)),

@@ -69,6 +69,8 @@ SIGNATURES = {
     "svr_export_sort_keys": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p]),
     "svr_sh_compute": (c_i32, [c_i32, c_i64, c_p, c_i64, c_p, c_p, c_p, c_p, c_i32, c_p, c_p]),
     "svr_sh_compute_bw": (c_i32, [c_i32, c_i32, c_i64, c_p, c_i64, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "svr_sh_compute_visible": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_i32, c_p, c_p]),
+    "svr_sh_compute_bw_visible": (c_i32, [c_i32, c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "svr_unbiased_adam_step": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_f64, c_f64, c_f64, c_f64, c_f32,
                                        c_p]),
     "svr_biased_adam_step": (c_i32, [c_i32, c_i64, c_p, c_p, c_p, c_p, c_f32, c_f32, c_f32, c_f32, c_p]),

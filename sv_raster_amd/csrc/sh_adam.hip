@@ -175,18 +175,28 @@ __global__ __launch_bounds__(256) void sh_mark_kernel(int64_t cnt, const int64_t
     if (i < cnt) reinterpret_cast<uint32_t*>(dsh0)[3 * idx[i]] = MARK_BITS;
 }
 
+// Which rows a dense SH pass evaluates: every voxel, the rows sh_mark_kernel
+// marked, or the voxels with entries in a preprocess geometry state (n_dup > 0:
+// exactly the in-frustum list svr_visible_voxels returns).
+enum ShList
+{
+    SH_LIST_ALL = 0,
+    SH_LIST_MARKS = 1,
+    SH_LIST_NDUP = 2,
+};
+
 // The degree-3 forward over every voxel, in blocks of 64 consecutive voxels
 // (for lists that cover most of the voxels): each block loads its contiguous
 // 64 x 45 and 64 x 3 coefficient spans with 16-byte loads (all in flight
-// before the LDS stores), each lane evaluates its voxel if it is listed
-// (marked by sh_mark_kernel in the first word of its rgbs row; every voxel
-// when ALL) and the block writes its 64 x 3 output span with 16-byte stores,
-// zeros for the rows not listed.  Same arithmetic as sh_fwd16_kernel.
-template <bool ALL>
+// before the LDS stores), each lane evaluates its voxel if it is listed (LIST)
+// and the block writes its 64 x 3 output span with 16-byte stores, zeros for
+// the rows not listed.  Same arithmetic as sh_fwd16_kernel.
+template <int LIST>
 __global__ __launch_bounds__(64) void sh_fwd16_dense_kernel(int deg, int64_t N, const float* __restrict__ centers,
                                                             const float* __restrict__ cam,
                                                             const float* __restrict__ sh0,
-                                                            const float* __restrict__ shs, float* __restrict__ rgbs)
+                                                            const float* __restrict__ shs, float* __restrict__ rgbs,
+                                                            const uint32_t* __restrict__ n_dup)
 {
     __shared__ ShStage st;
     __shared__ alignas(16) float out[64 * 3 + 4];
@@ -194,11 +204,13 @@ __global__ __launch_bounds__(64) void sh_fwd16_dense_kernel(int deg, int64_t N, 
     const int64_t v0 = (int64_t)blockIdx.x * 64;
     const int nv = (int)min((int64_t)64, N - v0);
     const int64_t v = v0 + lane;
-    // the marks (first word of each rgbs row, stride 3) and the coefficients in one batch of loads
+    // the listed words (the marks in the first word of each rgbs row, or the
+    // entry counts) and the coefficients in one batch of loads
     float c[3];
-    const uint32_t mark = sh_stage_rows(st, sh0, shs, centers, v0, nv, lane, c,
-                                        ALL ? nullptr : reinterpret_cast<const uint32_t*>(rgbs + 3 * v0));
-    const bool listed = lane < nv && (ALL || mark == MARK_BITS);
+    const uint32_t* word = LIST == SH_LIST_MARKS ? reinterpret_cast<const uint32_t*>(rgbs + 3 * v0)
+                           : LIST == SH_LIST_NDUP ? n_dup + v0 : nullptr;
+    const uint32_t mark = sh_stage_rows(st, sh0, shs, centers, v0, nv, lane, c, word, LIST == SH_LIST_NDUP ? 1 : 3);
+    const bool listed = lane < nv && (LIST == SH_LIST_ALL || (LIST == SH_LIST_MARKS ? mark == MARK_BITS : mark != 0u));
     wave_sync();
     float res[3] = {0.f, 0.f, 0.f};
     if (listed)
@@ -221,10 +233,13 @@ __global__ __launch_bounds__(64) void sh_fwd16_dense_kernel(int deg, int64_t N, 
     }
 }
 
+// n_dup null: the rows sh_mark_kernel marked; otherwise the voxels with
+// n_dup > 0 (SH_LIST_NDUP).
 __global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, const float* __restrict__ centers,
                                                             const float* __restrict__ cam,
                                                             const float* __restrict__ dL_drgbs,
-                                                            float* __restrict__ dsh0, float* __restrict__ dshs)
+                                                            float* __restrict__ dsh0, float* __restrict__ dshs,
+                                                            const uint32_t* __restrict__ n_dup)
 {
     __shared__ ShStage st;
     const int lane = threadIdx.x;
@@ -235,7 +250,7 @@ __global__ __launch_bounds__(64) void sh_bwd16_dense_kernel(int deg, int64_t N, 
     float* r1 = st.s1 + 45 * lane;
     if (lane < nv)
     {
-        const bool marked = reinterpret_cast<const uint32_t*>(dsh0)[3 * v] == MARK_BITS;
+        const bool marked = n_dup ? n_dup[v] != 0u : reinterpret_cast<const uint32_t*>(dsh0)[3 * v] == MARK_BITS;
         if (marked)
         {
             float x, y, z, b[16];
@@ -618,16 +633,16 @@ int svr_sh_compute(int32_t deg, int64_t n_idx, const int64_t* idx, int64_t N, co
         prof_begin(PROF_SH_FWD, stream);
         if (n_idx == 0)
         {
-            sh_fwd16_dense_kernel<true><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, sh0,
-                                                                                      shs, rgbs);
+            sh_fwd16_dense_kernel<SH_LIST_ALL><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos,
+                                                                                             sh0, shs, rgbs, nullptr);
         }
         else
         {
             if (fill_u32(rgbs, 0u, 3 * N, stream) != SVR_OK) return SVR_ERR_HIP;
             sh_mark_kernel<<<(unsigned)((n_idx + 255) / 256), 256, 0, stream>>>(n_idx, idx, rgbs);
             SVR_CHECK_LAUNCH();
-            sh_fwd16_dense_kernel<false><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, sh0,
-                                                                                       shs, rgbs);
+            sh_fwd16_dense_kernel<SH_LIST_MARKS><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos,
+                                                                                               sh0, shs, rgbs, nullptr);
         }
         SVR_CHECK_LAUNCH();
         prof_end(PROF_SH_FWD, stream);
@@ -661,7 +676,7 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
         sh_mark_kernel<<<(unsigned)((n_idx + 255) / 256), 256, 0, stream>>>(n_idx, idx, dsh0);
         SVR_CHECK_LAUNCH();
         sh_bwd16_dense_kernel<<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, dL_drgbs, dsh0,
-                                                                           dshs);
+                                                                           dshs, nullptr);
         SVR_CHECK_LAUNCH();
         prof_end(PROF_SH_BWD, stream);
         return SVR_OK;
@@ -677,6 +692,47 @@ int svr_sh_compute_bw(int32_t deg, int32_t M, int64_t n_idx, const int64_t* idx,
     else
         sh_bwd_kernel<<<grid_for(cnt, 1ll << 30), 256, 0, stream>>>(deg, cnt, n_idx ? idx : nullptr, centers,
                                                                      cam_pos, dL_drgbs, M, dsh0, dshs);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_SH_BWD, stream);
+    return SVR_OK;
+}
+
+// The listed voxels are those with entries in a preprocess geometry state
+// (n_dup > 0), i.e. exactly svr_visible_voxels' list: no index list, no zero
+// fill and no marking pass before the dense kernel.  Degree-3 layout
+// (M == 16) and 16-byte aligned coefficient and output arrays only.
+int svr_sh_compute_visible(int32_t deg, int64_t N, const void* geom_state, const float* centers, const float* cam_pos,
+                           const float* sh0, const float* shs, int32_t M, float* rgbs, svr_stream_t stream)
+{
+    SVR_REQUIRE(deg >= 0 && deg <= 3 && M == 16, SVR_ERR_INVALID,
+                "svr_sh_compute_visible: degree %d with %d coefficients (needs the 16-coefficient layout)", deg, M);
+    SVR_REQUIRE(((((uintptr_t)sh0) | ((uintptr_t)shs) | ((uintptr_t)rgbs)) & 15u) == 0, SVR_ERR_INVALID,
+                "svr_sh_compute_visible: coefficient and output arrays must be 16-byte aligned");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(geom_state, SVR_ERR_INVALID, "svr_sh_compute_visible: null geometry state");
+    const GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    prof_begin(PROF_SH_FWD, stream);
+    sh_fwd16_dense_kernel<SH_LIST_NDUP><<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, sh0,
+                                                                                      shs, rgbs, g.n_dup);
+    SVR_CHECK_LAUNCH();
+    prof_end(PROF_SH_FWD, stream);
+    return SVR_OK;
+}
+
+int svr_sh_compute_bw_visible(int32_t deg, int32_t M, int64_t N, const void* geom_state, const float* centers,
+                              const float* cam_pos, const float* dL_drgbs, float* dsh0, float* dshs,
+                              svr_stream_t stream)
+{
+    SVR_REQUIRE(deg >= 0 && deg <= 3 && M == 16, SVR_ERR_INVALID,
+                "svr_sh_compute_bw_visible: degree %d with %d coefficients (needs the 16-coefficient layout)", deg, M);
+    SVR_REQUIRE(((((uintptr_t)dsh0) | ((uintptr_t)dshs)) & 15u) == 0, SVR_ERR_INVALID,
+                "svr_sh_compute_bw_visible: gradient arrays must be 16-byte aligned");
+    if (N == 0) return SVR_OK;
+    SVR_REQUIRE(geom_state, SVR_ERR_INVALID, "svr_sh_compute_bw_visible: null geometry state");
+    const GeomState g = GeomState::carve((void*)geom_state, N, nullptr);
+    prof_begin(PROF_SH_BWD, stream);
+    sh_bwd16_dense_kernel<<<(unsigned)((N + 63) / 64), 64, 0, stream>>>(deg, N, centers, cam_pos, dL_drgbs, dsh0,
+                                                                       dshs, g.n_dup);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_SH_BWD, stream);
     return SVR_OK;

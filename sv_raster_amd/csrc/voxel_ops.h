@@ -188,11 +188,12 @@ struct ShStage
 // loads (all in flight before the LDS stores) for a full block of 64 with
 // 16-byte aligned arrays, single loads otherwise.  One wave.  In the same
 // batch of loads lane l (l < nv) reads the centre of voxel v0 + l into c (so
-// that no second memory round trip follows) and, with `word` (one word per 3:
-// the first word of each voxel's rgb row), word[3 l], which it returns.
+// that no second memory round trip follows) and, with `word`, the voxel's
+// word[word_stride l] (the first word of its rgb row with stride 3, its entry
+// count with stride 1), which it returns.
 __device__ inline uint32_t sh_stage_rows(ShStage& st, const float* __restrict__ sh0, const float* __restrict__ shs,
                                          const float* __restrict__ centers, int64_t v0, int nv, int lane, float* c,
-                                         const uint32_t* __restrict__ word = nullptr)
+                                         const uint32_t* __restrict__ word = nullptr, int word_stride = 3)
 {
     uint32_t wv = 0u;
     const int64_t vc = v0 + min(lane, nv - 1);
@@ -210,7 +211,7 @@ __device__ inline uint32_t sh_stage_rows(ShStage& st, const float* __restrict__ 
         SVR_SH_LD(6) SVR_SH_LD(7) SVR_SH_LD(8) SVR_SH_LD(9) SVR_SH_LD(10) SVR_SH_LD(11)
 #undef SVR_SH_LD
         const float4 q0 = reinterpret_cast<const float4*>(sh0 + 3 * v0)[min(lane, 47)];
-        if (word) wv = word[3 * lane];
+        if (word) wv = word[word_stride * lane];
         __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the stores
         // 720 = 11 x 64 + 16: eleven full rounds and a 16-lane tail
         float4* s1 = reinterpret_cast<float4*>(st.s1);
@@ -223,7 +224,7 @@ __device__ inline uint32_t sh_stage_rows(ShStage& st, const float* __restrict__ 
     }
     else
     {
-        if (word && lane < nv) wv = word[3 * lane];
+        if (word && lane < nv) wv = word[word_stride * lane];
         for (int e = lane; e < nv * 45; e += 64) st.s1[e] = shs[45 * v0 + e];
         for (int e = lane; e < nv * 3; e += 64) st.s0[e] = sh0[3 * v0 + e];
     }

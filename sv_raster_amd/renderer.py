@@ -247,15 +247,18 @@ class SH_eval(torch.autograd.Function):
         # a given view direction is evaluated as the direction from the origin to it
         points, origin = (vox_centers, cam_pos) if viewdir is None else (viewdir, torch.zeros_like(cam_pos))
         sel = torch.empty(0, dtype=torch.int64) if idx is None else idx
-        rgbs = _C.sh_compute(active_sh_degree, sel, points, origin, sh0, shs)
-        ctx.active_sh_degree, ctx.M = active_sh_degree, shs.shape[1] + 1
+        # an in-frustum list from rasterize_voxels: its rows come from the geometry state (no marking pass)
+        visible = _C.visible_lists.geom_of(idx, points.shape[0])
+        rgbs = _C.sh_compute(active_sh_degree, sel, points, origin, sh0, shs, visible=visible)
+        ctx.active_sh_degree, ctx.M, ctx.visible = active_sh_degree, shs.shape[1] + 1, visible
         ctx.save_for_backward(sel, points, origin, rgbs)
         return rgbs
 
     @staticmethod
     def backward(ctx, dL_drgbs):
         sel, points, origin, rgbs = ctx.saved_tensors
-        grads = _C.sh_compute_bw(ctx.active_sh_degree, ctx.M, sel, points, origin, rgbs, dL_drgbs)
+        grads = _C.sh_compute_bw(ctx.active_sh_degree, ctx.M, sel, points, origin, rgbs, dL_drgbs,
+                                 visible=ctx.visible)
         return (None,) * 5 + tuple(grads)
 
 

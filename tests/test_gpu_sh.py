@@ -53,3 +53,54 @@ def test_sh_forward_backward_match_oracle(deg):
         r0, rs = O.sh_bwd(deg, 16, idx, centers, cam_pos, dL)
         assert np.abs(d0.cpu().numpy() - r0).max() <= RTOL * max(1.0, np.abs(r0).max()), name
         assert np.abs(ds.cpu().numpy() - rs.reshape(ds.shape)).max() <= RTOL * max(1.0, np.abs(rs).max()), name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("deg", [1, 3])
+def test_sh_in_frustum_list_from_geometry_state(deg):
+    """The SH passes over a geometry state's in-frustum voxels (svr_sh_compute_visible / _bw_visible: rows with
+    n_dup > 0) equal the calls with the list svr_visible_voxels returns for that state, bit for bit, and the
+    oracle within the bar; visible_lists recognises that list object only (not a copy, not after an in-place
+    change), and the drop-in SH_eval takes the geometry-state path for it with the same values and gradients."""
+    import torch
+    from oracle import oracle as O
+    from sv_raster_amd import _C, renderer, synthetic as S
+    sc = S.make_scene("A", seed=3)
+    cam = S.make_camera(800, 800, seed=3)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    octpath, centers, lengths = t(sc["octpath"]), t(sc["vox_centers"]), t(sc["vox_lengths"])
+    sh0, shs = t(sc["sh0"]), t(sc["shs"])
+    N = centers.shape[0]
+    w2c, c2w = t(cam["w2c"]), t(cam["c2w"])
+    print(f"sh in-frustum list deg {deg}", flush=True)
+    # a narrowed field of view leaves part of the voxels outside the frustum (unlisted rows)
+    _, geom = _C.rasterize_preprocess(800, 800, 0.3 * cam["tanfovx"], 0.3 * cam["tanfovy"], cam["cx"], cam["cy"], w2c,
+                                      c2w, 0.02, octpath, centers, lengths)
+    idx, _ = _C.visible_voxels(geom, N, centers.device, _C.order_table_cache.get(octpath))
+    assert 0 < idx.numel() < N
+    assert _C.visible_lists.geom_of(idx, N) is geom
+    assert _C.visible_lists.geom_of(idx.clone(), N) is None
+    assert _C.visible_lists.geom_of(idx, N + 1) is None
+    cam_pos = c2w[:3, 3].contiguous()
+    rgbs_list = _C.sh_compute(deg, idx.clone(), centers, cam_pos, sh0, shs)
+    rgbs_vis = _C.sh_compute(deg, idx, centers, cam_pos, sh0, shs, visible=geom)
+    assert torch.equal(rgbs_list, rgbs_vis)
+    ref = O.sh_fwd(deg, idx.cpu().numpy(), sc["vox_centers"], cam["c2w"][:3, 3], sc["sh0"], sc["shs"])
+    assert np.abs(rgbs_vis.cpu().numpy() - ref).max() <= RTOL * max(1.0, np.abs(ref).max())
+    dL = t(np.random.default_rng(deg).normal(0, 1, (N, 3)).astype(np.float32))
+    g_list = _C.sh_compute_bw(deg, 16, idx.clone(), centers, cam_pos, rgbs_list, dL)
+    g_vis = _C.sh_compute_bw(deg, 16, idx, centers, cam_pos, rgbs_vis, dL, visible=geom)
+    for a, b in zip(g_list, g_vis):
+        assert torch.equal(a, b)
+    # the drop-in: the recognised list takes the geometry-state path, a copy the index path; same results
+    outs = []
+    for sel in (idx, idx.clone()):
+        s0, s1 = sh0.clone().requires_grad_(), shs.clone().requires_grad_()
+        rgbs = renderer.SH_eval.apply(deg, sel, centers, cam_pos, None, s0, s1)
+        (rgbs * dL).sum().backward()
+        outs.append((rgbs.detach(), s0.grad, s1.grad))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    # an in-place change of the list makes it unknown
+    idx[0] = idx[0]
+    assert _C.visible_lists.geom_of(idx, N) is None
