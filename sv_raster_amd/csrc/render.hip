@@ -223,14 +223,13 @@ __device__ inline void add_row(float (&a)[12], const float* __restrict__ partial
     a[8] += p2.x; a[9] += p2.y; a[10] += p2.z; a[11] += p2.w;
 }
 
-// One thread per item with at most BIG_DUP entries.  A run ending beyond
-// `cap` was never emitted (a frame over a fixed capacity) and counts as zero.
-__global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N, uint32_t cap,
-                                                          const float* __restrict__ partial,
-                                                          float* __restrict__ dgeos, float* __restrict__ drgbs,
-                                                          float* __restrict__ dsub)
+// One thread per item with at most BIG_DUP entries (item v).  A run ending
+// beyond `cap` was never emitted (a frame over a fixed capacity) and counts as
+// zero.
+__device__ inline void grad_reduce_small(const GeomState& g, int64_t N, uint32_t cap, int64_t v,
+                                         const float* __restrict__ partial, float* __restrict__ dgeos,
+                                         float* __restrict__ drgbs, float* __restrict__ dsub)
 {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
     const uint32_t nd = g.n_dup[v];
     if (nd > BIG_DUP) return;
@@ -269,10 +268,11 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(GeomState g, int64_t N
 __device__ inline uint32_t huge_split(uint32_t n_huge) { return n_huge ? max(1u, HUGE_SPLIT_ROWS / n_huge) : 1u; }
 
 // Launched with HUGE_SPLIT_ROWS workgroups.
-__global__ __launch_bounds__(256) void grad_reduce_big_kernel(GeomState g, int64_t N, uint32_t cap,
-                                                              const float* __restrict__ partial,
-                                                              float* __restrict__ dgeos, float* __restrict__ drgbs,
-                                                              float* __restrict__ dsub, float* __restrict__ huge_rows)
+// Workgroup bid of nblk (the large and huge items; see grad_reduce_big_kernel).
+__device__ inline void grad_reduce_big(const GeomState& g, int64_t N, uint32_t cap, uint32_t bid, uint32_t nblk,
+                                       const float* __restrict__ partial, float* __restrict__ dgeos,
+                                       float* __restrict__ drgbs, float* __restrict__ dsub,
+                                       float* __restrict__ huge_rows)
 {
     __shared__ float red[12][4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void grad_reduce_big_kernel(GeomState g, int64
     const uint32_t counts[2] = {g.counters[2], g.counters[3]};
     const uint32_t K = huge_split(counts[1]);
     for (uint32_t pass = 0; pass < 2; ++pass)
-    for (uint32_t it = blockIdx.x; it < (pass == 0 ? counts[0] : counts[1] * K); it += gridDim.x)
+    for (uint32_t it = bid; it < (pass == 0 ? counts[0] : counts[1] * K); it += nblk)
     {
         const uint32_t i = pass == 0 ? it : it % counts[1];
         const uint32_t v = pass == 0 ? g.big_list[i] : g.big_list[N - 1 - i];
@@ -359,6 +359,31 @@ __global__ __launch_bounds__(64) void grad_reduce_huge_kernel(GeomState g, int64
     }
 }
 
+// Launched with HUGE_SPLIT_ROWS workgroups.
+__global__ __launch_bounds__(256) void grad_reduce_big_kernel(GeomState g, int64_t N, uint32_t cap,
+                                                              const float* __restrict__ partial,
+                                                              float* __restrict__ dgeos, float* __restrict__ drgbs,
+                                                              float* __restrict__ dsub, float* __restrict__ huge_rows)
+{
+    grad_reduce_big(g, N, cap, blockIdx.x, gridDim.x, partial, dgeos, drgbs, dsub, huge_rows);
+}
+
+// Both in one launch: the first HUGE_SPLIT_ROWS workgroups take the large and
+// huge items (they run longest, so they start first), the rest one small item
+// per thread; the two sets share nothing, so the large items' sums overlap the
+// small ones instead of following them.
+__global__ __launch_bounds__(256) void grad_reduce_all_kernel(GeomState g, int64_t N, uint32_t cap,
+                                                              const float* __restrict__ partial,
+                                                              float* __restrict__ dgeos, float* __restrict__ drgbs,
+                                                              float* __restrict__ dsub, float* __restrict__ huge_rows)
+{
+    if (blockIdx.x < HUGE_SPLIT_ROWS)
+        grad_reduce_big(g, N, cap, blockIdx.x, HUGE_SPLIT_ROWS, partial, dgeos, drgbs, dsub, huge_rows);
+    else
+        grad_reduce_small(g, N, cap, (int64_t)(blockIdx.x - HUGE_SPLIT_ROWS) * blockDim.x + threadIdx.x, partial,
+                          dgeos, drgbs, dsub);
+}
+
 int grad_reduce_big_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs,
                            float* dsub, float* huge_rows, hipStream_t stream, uint32_t cap)
 {
@@ -372,9 +397,12 @@ int grad_reduce_big_launch(int64_t N, const GeomState& g, const float* partial, 
 int grad_reduce_launch(int64_t N, const GeomState& g, const float* partial, float* dgeos, float* drgbs, float* dsub,
                        float* huge_rows, hipStream_t stream, uint32_t cap)
 {
-    grad_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, cap, partial, dgeos, drgbs, dsub);
+    grad_reduce_all_kernel<<<(unsigned)(HUGE_SPLIT_ROWS + (N + 255) / 256), 256, 0, stream>>>(
+        g, N, cap, partial, dgeos, drgbs, dsub, huge_rows);
     SVR_CHECK_LAUNCH();
-    return grad_reduce_big_launch(N, g, partial, dgeos, drgbs, dsub, huge_rows, stream, cap);
+    grad_reduce_huge_kernel<<<64, 64, 0, stream>>>(g, N, huge_rows, dgeos, drgbs, dsub);
+    SVR_CHECK_LAUNCH();
+    return SVR_OK;
 }
 
 template <int NS>
