@@ -712,7 +712,7 @@ static int batch_forward_impl(int32_t n_samp, int32_t V, const svr_view* views_h
     }
     if (R > 0)
     {
-        rc = pair_offsets_launch(V, N, g, ot.pos, stream);
+        rc = pair_offsets_launch(V, N, g, ot.pos, stream, false);
         if (rc != SVR_OK) return rc;
         // entries are emitted view by view in rank order, so the sort only
         // orders the (global) tile bits; emit into the buffer that makes the

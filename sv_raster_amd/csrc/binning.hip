@@ -622,16 +622,22 @@ struct OrderScratch
 
 // Each active (item, quadrant) pair's tile count at its rank position, in
 // view blockIdx.y's block of pos_off (pos_off cleared beforehand).
+// With big_list (one view), items with more than BIG_DUP entries are also
+// queued there (counted in counters[2], huge ones from the end in
+// counters[3]) for the duplication's workgroup-per-item part.
 __global__ __launch_bounds__(256) void pair_count_kernel(int64_t N, const uint32_t* __restrict__ n_dup,
                                                          const uint32_t* __restrict__ qbits,
                                                          const uint32_t* __restrict__ order_pos,
-                                                         uint32_t* __restrict__ pos_off)
+                                                         uint32_t* __restrict__ pos_off,
+                                                         uint32_t* __restrict__ big_list,
+                                                         uint32_t* __restrict__ big_counts)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
     const size_t p = (size_t)blockIdx.y * (size_t)N + (size_t)v;
     const uint32_t nd = n_dup[p];
     if (nd == 0) return;
+    if (big_list && nd > BIG_DUP) queue_large(big_list, big_counts, N, nd, (uint32_t)v);
     const uint32_t bits = qbits[p];
     const uint32_t tiles = nd / (uint32_t)__popc(bits);
     uint32_t* blk = pos_off + 8 * (size_t)blockIdx.y * (size_t)N;
@@ -642,24 +648,25 @@ __global__ __launch_bounds__(256) void pair_count_kernel(int64_t N, const uint32
     }
 }
 
-int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream)
+int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream,
+                        bool queue_large_items)
 {
     const int64_t n8 = 8 * (int64_t)V * N;
     if (n8 == 0) return fill_u32(g.counters + 2, 0u, 2, stream);
     if (fill_u32_pair(g.pos_off, n8, g.counters + 2, 2, 0u, stream) != SVR_OK) return SVR_ERR_HIP;
-    pair_count_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(N, g.n_dup, g.qbits,
-                                                                                         order_pos, g.pos_off);
+    pair_count_kernel<<<dim3((unsigned)((N + 255) / 256), (unsigned)V), 256, 0, stream>>>(
+        N, g.n_dup, g.qbits, order_pos, g.pos_off, queue_large_items ? g.big_list : nullptr, g.counters + 2);
     SVR_CHECK_LAUNCH();
     return pair_scan_launch(g.pos_off, n8, g.pos_sums, stream);
 }
 
 // forward.cu:523-577, emitting (tile << pos_bits | pos) keys: each quadrant's
 // run at its rank-ordered offset (recorded in qoff for the later passes).
-__global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
-                                                        const uint32_t* __restrict__ order_pos,
-                                                        uint64_t* __restrict__ keys)
+// Item v (items with more than BIG_DUP entries only get their run offsets
+// here; pair_count_kernel queued them for duplicate_big).
+__device__ inline void duplicate_small(const GeomState& g, int64_t N, int gx, int gy, int pos_bits, int64_t v,
+                                       const uint32_t* __restrict__ order_pos, uint64_t* __restrict__ keys)
 {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
     const uint32_t nd = g.n_dup[v];
     if (nd == 0) return;
@@ -681,24 +688,24 @@ __global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, 
                 ++off;
             }
     }
-    if (large) queue_large(g.big_list, g.counters + 2, N, nd, (uint32_t)v);
 }
 
 // Large voxels: one workgroup each; then the huge ones with every workgroup
 // taking a strided share of each.  Entry e of a voxel: quadrant (the qi-th set
 // bit of its bits) major, then the tile rows of its rectangle, written into
-// that quadrant's run.
-__global__ __launch_bounds__(256) void duplicate_big_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
-                                                            const uint32_t* __restrict__ order_pos,
-                                                            uint64_t* __restrict__ keys)
+// that quadrant's run (its offset read from pos_off, not from the run offsets
+// duplicate_small records in the same launch).  Workgroup bid of nblk.
+__device__ inline void duplicate_big(const GeomState& g, int64_t N, int gx, int gy, int pos_bits, uint32_t bid,
+                                     uint32_t nblk, const uint32_t* __restrict__ order_pos,
+                                     uint64_t* __restrict__ keys)
 {
     // pass 0: one workgroup per large voxel; pass 1: each huge voxel split into
     // K entry ranges taken by K different workgroups (item = chunk * huge + voxel)
     const uint32_t count = g.counters[2], huge = g.counters[3];
-    const uint32_t K = huge ? max(1u, gridDim.x / huge) : 1u;
+    const uint32_t K = huge ? max(1u, nblk / huge) : 1u;
     for (uint32_t pass = 0; pass < 2; ++pass)
     {
-        for (uint32_t it = blockIdx.x; it < (pass == 0 ? count : huge * K); it += gridDim.x)
+        for (uint32_t it = bid; it < (pass == 0 ? count : huge * K); it += nblk)
         {
             const uint32_t i = pass == 0 ? it : it % huge;
             const uint32_t v = pass == 0 ? g.big_list[i] : g.big_list[N - 1 - i];
@@ -718,8 +725,8 @@ __global__ __launch_bounds__(256) void duplicate_big_kernel(GeomState g, int64_t
                 const uint32_t q = (uint32_t)__builtin_ctz(b);
                 const uint32_t r_lo = max(e_lo, qi * tiles), r_hi = min(e_hi, (qi + 1) * tiles);
                 if (r_lo >= r_hi) continue;
-                const uint32_t off = run_offset(g, v, bits, q);
                 const uint64_t pos = order_pos[8 * (size_t)v + q];
+                const uint32_t off = g.pos_off[pos];
                 for (uint32_t e = r_lo + threadIdx.x; e < r_hi; e += blockDim.x)
                 {
                     const uint32_t rem = e - qi * tiles;
@@ -771,6 +778,21 @@ __global__ void export_keys_kernel(const uint64_t* __restrict__ keys, const uint
     hi[i] = tile >> 1;
     lo[i] = (tile << 63) | rank;
     vals[i] = val;
+}
+
+// The duplication in one launch: the first DUP_BIG_BLOCKS workgroups take the
+// items pair_count_kernel queued as large (they run longest, so they start
+// first), the rest one item per thread.
+constexpr uint32_t DUP_BIG_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void duplicate_kernel(GeomState g, int64_t N, int gx, int gy, int pos_bits,
+                                                        const uint32_t* __restrict__ order_pos,
+                                                        uint64_t* __restrict__ keys)
+{
+    if (blockIdx.x < DUP_BIG_BLOCKS)
+        duplicate_big(g, N, gx, gy, pos_bits, blockIdx.x, DUP_BIG_BLOCKS, order_pos, keys);
+    else
+        duplicate_small(g, N, gx, gy, pos_bits, (int64_t)(blockIdx.x - DUP_BIG_BLOCKS) * blockDim.x + threadIdx.x,
+                        order_pos, keys);
 }
 
 int ranges_launch(const uint64_t* keys, int64_t R, int pos_bits, const uint32_t* order_pair, uint2* ranges,
@@ -888,7 +910,7 @@ int tile_queue_launch(int64_t T, const uint2* ranges, uint64_t* qkeys, uint64_t*
 
 int pair_prepare(int64_t N, const GeomState& g, const OrderTable& ot, hipStream_t stream)
 {
-    return pair_offsets_launch(1, N, g, ot.pos, stream);
+    return pair_offsets_launch(1, N, g, ot.pos, stream, true);
 }
 
 int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTable& ot, int64_t R,
@@ -917,9 +939,8 @@ int binning_forward(int W, int H, int64_t N, const GeomState& g, const OrderTabl
     const bool odd = radix_passes(tile_bits) & 1;
     uint64_t* k0 = odd ? b.keys_alt : b.keys;
     uint32_t* o0 = odd ? b.orig_alt : b.orig;
-    duplicate_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0);
-    SVR_CHECK_LAUNCH();
-    duplicate_big_kernel<<<1024, 256, 0, stream>>>(g, N, gx, gy, pos_bits, ot.pos, k0);
+    duplicate_kernel<<<(unsigned)(DUP_BIG_BLOCKS + (N + 255) / 256), 256, 0, stream>>>(g, N, gx, gy, pos_bits,
+                                                                                      ot.pos, k0);
     SVR_CHECK_LAUNCH();
     uint64_t* sorted = nullptr;
     uint32_t* sorted_orig = nullptr;

@@ -119,13 +119,21 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scan_sums_kernel(uint32_t* __restr
 }
 
 // Workgroup 0 scans the entry-count block sums (total -> counters[0..1]),
-// workgroup 1 the in-frustum block counts (total -> counters[6..7]).
+// workgroup 1 the in-frustum block counts (total -> counters[6..7]).  With
+// host_words (pinned host memory, device-accessible), both totals also go
+// there at the same offsets, so the host reads them without a copy.
 __global__ __launch_bounds__(SCAN_BLOCK) void scan_sums2_kernel(uint32_t* __restrict__ block_sums,
                                                                 uint32_t* __restrict__ vis_sums, int64_t nb,
-                                                                uint32_t* __restrict__ counters)
+                                                                uint32_t* __restrict__ counters,
+                                                                uint32_t* __restrict__ host_words)
 {
-    if (blockIdx.x == 0) scan_sums_block(block_sums, nb, counters);
-    else scan_sums_block(vis_sums, nb, counters + 6);
+    const int o = blockIdx.x == 0 ? 0 : 6;
+    scan_sums_block(blockIdx.x == 0 ? block_sums : vis_sums, nb, counters + o);
+    if (host_words && threadIdx.x == 0)
+    {
+        host_words[o] = counters[o];
+        host_words[o + 1] = counters[o + 1];
+    }
 }
 
 // Per scan block: the entry total (block_sums) and the number of in-frustum
@@ -338,32 +346,28 @@ int svr_visible_voxels(int64_t N, void* geom_state, const void* order_table, int
     prof_begin(PROF_SCAN, stream);
     count_reduce_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.block_sums, g.vis_sums);
     SVR_CHECK_LAUNCH();
-    scan_sums2_kernel<<<2, SCAN_BLOCK, 0, stream>>>(g.block_sums, g.vis_sums, nb, g.counters);
+    // the two totals go straight to pinned host memory (no copy); the host
+    // waits for them only, while the list, the pair offsets and whatever the
+    // caller queues next keep the device busy
+    uint32_t* cnt = nullptr;
+    SVR_CHECK_HIP(thread_pinned_words(&cnt));
+    uint32_t* cnt_dev = nullptr;
+    SVR_CHECK_HIP(hipHostGetDevicePointer((void**)&cnt_dev, cnt, 0));
+    scan_sums2_kernel<<<2, SCAN_BLOCK, 0, stream>>>(g.block_sums, g.vis_sums, nb, g.counters, cnt_dev);
     SVR_CHECK_LAUNCH();
+    hipEvent_t ev = nullptr;
+    SVR_CHECK_HIP(thread_event(&ev));
+    SVR_CHECK_HIP(hipEventRecord(ev, stream));
     visible_write_kernel<<<(unsigned)nb, SCAN_BLOCK, 0, stream>>>(g.n_dup, N, g.vis_sums, idx_out);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_SCAN, stream);
-    // pinned staging: the copy is truly asynchronous, so the pair offsets
-    // below are queued at once behind it
-    uint32_t* cnt = nullptr;
-    SVR_CHECK_HIP(thread_pinned_words(&cnt));
-    SVR_CHECK_HIP(hipMemcpyAsync(cnt, g.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     if (order_table)
     {
-        // wait for the copy only: the pair offsets queued behind it keep the
-        // device busy while the caller evaluates the colours of the list
-        hipEvent_t ev = nullptr;
-        SVR_CHECK_HIP(thread_event(&ev));
-        SVR_CHECK_HIP(hipEventRecord(ev, stream));
         OrderTable ot = OrderTable::carve((void*)order_table, N, nullptr);
         const int rc = pair_prepare(N, g, ot, stream);
         if (rc != SVR_OK) return rc;
-        SVR_CHECK_HIP(hipEventSynchronize(ev));
     }
-    else
-    {
-        SVR_CHECK_HIP(hipStreamSynchronize(stream));
-    }
+    SVR_CHECK_HIP(hipEventSynchronize(ev));
     const uint64_t R = ((uint64_t)cnt[1] << 32) | cnt[0];
     SVR_REQUIRE(R < (1ull << 32), SVR_ERR_CAPACITY, "number of rendered entries %llu exceeds 2^32 - 1",
                 (unsigned long long)R);

@@ -315,7 +315,10 @@ int radix_sort_ex(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* 
 // at view u's block [8 N u, 8 N (u + 1)) indexed by rank position, then
 // exclusive-scanned in place (only the active positions are written).  The
 // two words at g.counters + 2 are cleared by the same launch as pos_off.
-int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream);
+// queue_large_items (one view): the same launch queues the items with more
+// than BIG_DUP entries in big_list for the duplication.
+int pair_offsets_launch(int V, int64_t N, const GeomState& g, const uint32_t* order_pos, hipStream_t stream,
+                        bool queue_large_items);
 int pair_scan_launch(uint32_t* c, int64_t n, uint32_t* sums, hipStream_t stream);
 
 // Key duplication, sort, tile ranges and the sorted entry list of one frame.
