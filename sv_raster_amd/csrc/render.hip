@@ -27,13 +27,15 @@
 
 namespace svr {
 
-// Two launches per frame.  render_fwd_kernel takes the tiles in queue order
-// (workgroup b: tile order[b]) and composites those whose 256 rays all have
-// finite 1/rd (hardware min/max in the ray-box test); a tile with another
-// ray is appended to exact_list.  render_fwd_exact_kernel then loops a few
-// workgroups over that list with the reference's ternary min/max (whose NaN
-// handling differs).  The backward launches walk the same list, so neither
-// direction spends a launch of T workgroups on the rare exact tiles.
+// Two forward launches per frame.  render_fwd_kernel takes the tiles in
+// queue order (workgroup b: tile order[b]) and composites those whose 256 rays
+// all have finite 1/rd (hardware min/max in the ray-box test); a tile with
+// another ray is appended to exact_list.  render_fwd_exact_kernel then loops a
+// few workgroups over that list with the reference's ternary min/max (whose
+// NaN handling differs): the exact form needs more registers, which would
+// lower the fast kernel's occupancy.  The backward's occupancy is bounded by
+// its LDS window instead, so render_bwd_kernel composites an exact tile itself
+// (a workgroup-uniform branch) and needs no second launch.
 constexpr unsigned EXACT_GRID = 64;
 
 template <int NS, bool EXACT>
@@ -168,31 +170,15 @@ __global__ __launch_bounds__(256) void render_bwd_kernel(
     __shared__ WindowLDS<BWD_WINDOW> sh;
     const int tile = (int)tile_order[blockIdx.x];
     const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
-    if (__syncthreads_or(!ray_inv_finite(ray)) != 0) return;  // the forward listed it for the exact launch
-    bwd_tile<NS, SYNC_MIN, false>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs, out_T,
-                                  n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
-}
-
-template <int NS, int SYNC_MIN>
-__global__ __launch_bounds__(256) void render_bwd_exact_kernel(
-    int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy, const float* __restrict__ c2w, float bg,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const float* __restrict__ centers,
-    const float* __restrict__ lengths, const float* __restrict__ geos, const float* __restrict__ rgbs,
-    const float* __restrict__ out_T, const uint32_t* __restrict__ n_contrib, const float* __restrict__ dL_dcolor,
-    const float* __restrict__ dL_dT, const uint32_t* __restrict__ orig, const uint64_t* __restrict__ hit_masks,
-    const uint32_t* __restrict__ tile_last, float* __restrict__ partial, const uint32_t* __restrict__ exact_ctr,
-    const uint32_t* __restrict__ exact_list)
-{
-    __shared__ WindowLDS<BWD_WINDOW> sh;
-    const uint32_t count = exact_ctr[0];
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
-    {
-        const int tile = (int)exact_list[i];
-        const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
+    // a tile with a non-finite 1/rd takes the reference's min/max here (the
+    // same workgroup-uniform choice as the forward's exact launch); its larger
+    // register count costs no occupancy, which the LDS window bounds
+    if (__syncthreads_or(!ray_inv_finite(ray)) != 0)
         bwd_tile<NS, SYNC_MIN, true>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs,
                                      out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
-        __syncthreads();
-    }
+    else
+        bwd_tile<NS, SYNC_MIN, false>(sh, tile, ray, W, H, gx, bg, ranges, vox_list, centers, lengths, geos, rgbs,
+                                      out_T, n_contrib, dL_dcolor, dL_dT, orig, hit_masks, tile_last, partial);
 }
 
 // Per-voxel gradient = sum of the partial rows of its entries: one run of
@@ -439,10 +425,6 @@ int launch_bwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     render_bwd_kernel<NS, BWD_SYNC_MIN_MASKED><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs,
         out_T, img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, img.tile_last, partial);
-    SVR_CHECK_LAUNCH();
-    render_bwd_exact_kernel<NS, BWD_SYNC_MIN_MASKED><<<EXACT_GRID, 256, 0, stream>>>(
-        W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, centers, lengths, geos, rgbs, out_T,
-        img.n_contrib, dL_dcolor, dL_dT, b.orig, img.hit_masks, img.tile_last, partial, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_BWD, stream);
     prof_begin(PROF_GRAD_REDUCE, stream);
