@@ -27,16 +27,13 @@
 
 namespace svr {
 
-// Two forward launches per frame.  render_fwd_kernel takes the tiles in
-// queue order (workgroup b: tile order[b]) and composites those whose 256 rays
-// all have finite 1/rd (hardware min/max in the ray-box test); a tile with
-// another ray is appended to exact_list.  render_fwd_exact_kernel then loops a
-// few workgroups over that list with the reference's ternary min/max (whose
-// NaN handling differs): the exact form needs more registers, which would
-// lower the fast kernel's occupancy.  The backward's occupancy is bounded by
-// its LDS window instead, so render_bwd_kernel composites an exact tile itself
-// (a workgroup-uniform branch) and needs no second launch.
-constexpr unsigned EXACT_GRID = 64;
+// One launch per direction and frame.  render_fwd_kernel takes the tiles in
+// queue order (workgroup b: tile order[b]) and composites with the hardware
+// min/max in the ray-box test when all 256 rays of the tile have finite 1/rd;
+// a tile with another ray takes the reference's ternary min/max (whose NaN
+// handling differs) through a workgroup-uniform branch, at no register cost,
+// and is also appended to exact_list for the extended backward.
+// render_bwd_kernel makes the same choice per tile.
 
 template <int NS, bool EXACT>
 __device__ inline void fwd_tile(Stage& st, uint32_t (&s_last)[NWAVE], int tile, const PixRay& ray, int W, int H,
@@ -103,33 +100,15 @@ __global__ __launch_bounds__(256) void render_fwd_kernel(
     const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
     if (__syncthreads_or(!ray_inv_finite(ray)) != 0)
     {
+        // still listed: the extended backward (after a plain forward when only
+        // its regularizers are on) walks the list in its exact launch
         if (threadIdx.x == 0) exact_list[atomicAdd(exact_ctr, 1u)] = (uint32_t)tile;
-        return;
+        fwd_tile<NS, true>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, octs, centers, lengths, geos,
+                           rgbs, tile_last, n_contrib, out_color, out_T, hit_masks);
     }
-    fwd_tile<NS, false>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, octs, centers, lengths, geos, rgbs,
-                        tile_last, n_contrib, out_color, out_T, hit_masks);
-}
-
-template <int NS>
-__global__ __launch_bounds__(256) void render_fwd_exact_kernel(
-    int W, int H, int gx, float tanfovx, float tanfovy, float cx, float cy, const float* __restrict__ c2w, float bg,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vox_list, const uint2* __restrict__ bboxes, const OctBox* __restrict__ octs,
-    const float* __restrict__ centers, const float* __restrict__ lengths, const float* __restrict__ geos,
-    const float* __restrict__ rgbs, uint32_t* __restrict__ tile_last, uint32_t* __restrict__ n_contrib,
-    float* __restrict__ out_color, float* __restrict__ out_T, uint64_t* __restrict__ hit_masks,
-    const uint32_t* __restrict__ exact_ctr, const uint32_t* __restrict__ exact_list)
-{
-    __shared__ Stage st;
-    __shared__ uint32_t s_last[NWAVE];
-    const uint32_t count = exact_ctr[0];
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x)
-    {
-        const int tile = (int)exact_list[i];
-        const PixRay ray = tile_ray(tile, W, H, gx, tanfovx, tanfovy, cx, cy, c2w);
-        fwd_tile<NS, true>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, octs, centers, lengths, geos, rgbs,
-                           tile_last, n_contrib, out_color, out_T, hit_masks);
-        __syncthreads();
-    }
+    else
+        fwd_tile<NS, false>(st, s_last, tile, ray, W, H, gx, bg, ranges, vox_list, bboxes, octs, centers, lengths, geos,
+                            rgbs, tile_last, n_contrib, out_color, out_T, hit_masks);
 }
 
 template <int NS, int SYNC_MIN, bool EXACT>
@@ -403,10 +382,6 @@ int launch_fwd(int W, int H, float tanfovx, float tanfovy, float cx, float cy, c
     render_fwd_kernel<NS><<<(unsigned)T, 256, 0, stream>>>(
         W, H, gx, img.order, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos,
         rgbs, img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks, img.ctr, img.exact_list);
-    SVR_CHECK_LAUNCH();
-    render_fwd_exact_kernel<NS><<<EXACT_GRID, 256, 0, stream>>>(
-        W, H, gx, tanfovx, tanfovy, cx, cy, c2w, bg, img.ranges, b.vox_list, g.bbox, g.oct, centers, lengths, geos, rgbs,
-        img.tile_last, img.n_contrib, out_color, out_T, img.hit_masks, img.ctr, img.exact_list);
     SVR_CHECK_LAUNCH();
     prof_end(PROF_RENDER_FWD, stream);
     return SVR_OK;
