@@ -429,6 +429,9 @@ __global__ __launch_bounds__(256) void adam4_kernel(int64_t n, float* __restrict
 // adam4_kernel and the tensor's own step size, bias term, betas and eps.  All
 // arrays 16-byte aligned (the caller sends other tensors through adam_launch).
 constexpr int ADAM_MULTI_MAX = 8;
+// below 2^31 vectors, so that a thread's indices (at most the tensor's vector
+// count plus ADAM_UNROLL grid strides of at most 2^22) stay below 2^32
+constexpr int64_t ADAM_MULTI_MAX_VECTORS = 1ll << 31;
 struct AdamMulti
 {
     int count;
@@ -461,16 +464,20 @@ __global__ __launch_bounds__(256) void adam4_multi_kernel(AdamMulti a)
     const float4* __restrict__ G = reinterpret_cast<const float4*>(g);
     float4* __restrict__ M = reinterpret_cast<float4*>(m);
     float4* __restrict__ V = reinterpret_cast<float4*>(v);
-    for (int64_t base = gid; base < n4; base += ADAM_UNROLL * stride)
+    // 32-bit vector indices (fewer address registers: 83 VGPRs instead of 112,
+    // five waves per SIMD instead of four; tensors of ADAM_MULTI_MAX_VECTORS
+    // vectors or more go to adam4_kernel): 0.2456 -> 0.2416 ms at 1 M voxels
+    const uint32_t n4u = (uint32_t)n4, strideu = (uint32_t)stride;
+    for (uint32_t base = (uint32_t)gid; base < n4u; base += ADAM_UNROLL * strideu)
     {
         float4 g4[ADAM_UNROLL], p4[ADAM_UNROLL], m4[ADAM_UNROLL], v4[ADAM_UNROLL];
         bool on[ADAM_UNROLL];
 #pragma unroll
         for (int u = 0; u < ADAM_UNROLL; ++u)
         {
-            const int64_t i = base + u * stride;
-            g4[u] = i < n4 ? G[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            on[u] = i < n4;
+            const uint32_t i = base + u * strideu;
+            g4[u] = i < n4u ? G[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            on[u] = i < n4u;
         }
 #pragma unroll
         for (int u = 0; u < ADAM_UNROLL; ++u)
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(256) void adam4_multi_kernel(AdamMulti a)
             if (SPARSE) on[u] = on[u] && !(g4[u].x == 0.f && g4[u].y == 0.f && g4[u].z == 0.f && g4[u].w == 0.f);
             if (on[u])
             {
-                const int64_t i = base + u * stride;
+                const uint32_t i = base + u * strideu;
                 p4[u] = P[i];
                 m4[u] = M[i];
                 v4[u] = V[i];
@@ -488,7 +495,7 @@ __global__ __launch_bounds__(256) void adam4_multi_kernel(AdamMulti a)
         for (int u = 0; u < ADAM_UNROLL; ++u)
         {
             if (!on[u]) continue;
-            const int64_t i = base + u * stride;
+            const uint32_t i = base + u * strideu;
             adam_elem<SPARSE, BIASED>(p4[u].x, m4[u].x, v4[u].x, g4[u].x, step_size, beta1, beta2, rb2, eps);
             adam_elem<SPARSE, BIASED>(p4[u].y, m4[u].y, v4[u].y, g4[u].y, step_size, beta1, beta2, rb2, eps);
             adam_elem<SPARSE, BIASED>(p4[u].z, m4[u].z, v4[u].z, g4[u].z, step_size, beta1, beta2, rb2, eps);
@@ -822,7 +829,8 @@ int svr_adam_step_multi(int32_t biased, int32_t sparse, int32_t count, const int
         else
             svr_adam_bias_terms(steps[i], lrs[i], beta1s[i], beta2s[i], &step_size, &rb2);
         const bool vec = ((((uintptr_t)p) | ((uintptr_t)g) | ((uintptr_t)m) | ((uintptr_t)v)) & 15u) == 0;
-        if (!vec)
+        // the multi-tensor kernel indexes vectors with 32 bits
+        if (!vec || (n >> 2) >= ADAM_MULTI_MAX_VECTORS)
         {
             if (biased)
                 adam_launch<true>(sparse != 0, n, p, g, m, v, step_size, (float)beta1s[i], (float)beta2s[i], rb2,
